@@ -1,0 +1,315 @@
+/*
+ * mspms_gpu.h -- C ABI of the MI355X (gfx950) force engine for mspms2.
+ *
+ * The engine evaluates mspms2's two force routines on one GPU:
+ *   frclong  : inter-molecular LJ + Ewald (real, reciprocal, self, exclusion,
+ *              vacuum) or Wolf summation, plus the solid-fluid terms
+ *              (reference: src/frclong.c:20-1233)
+ *   frcshort : bonds, angles, dihedrals, impropers
+ *              (reference: src/frcshort.c:20-97)
+ *
+ * Two ways in:
+ *   1. Drop-in symbols `int frclong(void)` / `int frcshort(void)` with the
+ *      reference's global-state contract (declared in mspms_dropin.h, built
+ *      into libmspms_dropin.a); they replace src/frclong.c and src/frcshort.c
+ *      at link time.
+ *   2. The context API below: plain pointers and sizes, no global state,
+ *      several independent contexts per process, batched replicas.
+ *
+ * Units are the reference's reduced units (lengths / sigma_base, energies /
+ * epsilon_base; src/readins.c:33-59).  All host arrays are FP64 SoA, exactly
+ * the reference's xx/yy/zz and fxl/fyl/fzl layout (src/molevars.h:79-83).
+ *
+ * Every function returns 0 on success or a negative errno-style code
+ * (MSPMS_E*); the reference instead prints and calls exit(1)
+ * (src/bndfrc.c:60-62, src/aglfrc.c:186-188, src/dihfrc.c:278-280).
+ * mspms_last_error() returns a message for the last failure on the thread.
+ */
+#ifndef MSPMS_GPU_H_
+#define MSPMS_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define MSPMS_OK       0
+#define MSPMS_EINVAL (-22) /* unknown term type, bad argument, unsupported mode */
+#define MSPMS_EDOM   (-33) /* solid-fluid grid lookup out of range            */
+#define MSPMS_ERANGE (-34) /* cutoff larger than half the box                 */
+#define MSPMS_ENOMEM (-12) /* device allocation failed                        */
+#define MSPMS_EIO    (-5)  /* HIP runtime failure                             */
+#define MSPMS_ENOTFINITE (-74) /* non-finite position or force                 */
+
+/* ---- enumerations: values identical to src/defs.h:24-75 ------------------- */
+#define MSPMS_GHOST_NONE 0
+#define MSPMS_GHOST_LJ   1
+#define MSPMS_GHOST_FULL 2
+
+#define MSPMS_BOND_NONE     0
+#define MSPMS_BOND_HARMONIC 1
+#define MSPMS_BOND_MORSE    2
+#define MSPMS_BOND_FENE     3
+
+#define MSPMS_ANGLE_NONE     0
+#define MSPMS_ANGLE_HARMONIC 1
+#define MSPMS_ANGLE_TR_WATER 2
+
+#define MSPMS_DIH_NONE       0
+#define MSPMS_DIH_OPLS_COSIN 1
+#define MSPMS_DIH_CHARMM     2
+
+#define MSPMS_IMP_NONE   0
+#define MSPMS_IMP_CHARMM 1
+
+#define MSPMS_SF_NONE              0
+#define MSPMS_SF_HYPERGEO          1
+#define MSPMS_SF_ATOM_EXPLICIT     2
+#define MSPMS_SF_TASOS             3 /* tricubic Hermite grid (mylibtasos) */
+#define MSPMS_SF_MY_INTERP         4 /* not supported: no generator exists */
+
+#define MSPMS_ELEC_NONE           0
+#define MSPMS_ELEC_EWALD          1
+#define MSPMS_ELEC_WOLF           2
+#define MSPMS_ELEC_SIMPLE_COULOMB 3
+
+#define MSPMS_EWALD_BC_TINFOIL 1
+#define MSPMS_EWALD_BC_VACUUM  2
+
+/* precision modes */
+#define MSPMS_PREC_FP64  0 /* reference operation order, FP64 everywhere      */
+#define MSPMS_PREC_MIXED 1 /* FP32 pair arithmetic, FP64 accumulation,
+                              FP64 Ewald reciprocal and bonded terms          */
+
+/* ---- molecule template (one per species) ------------------------------------
+ * Mirrors SAMPLE_MOLECULE (src/molevars.h:40-73) as flat arrays.  Parameters
+ * are already reduced as src/readins.c:118-231 leaves them.  Index arrays are
+ * row-major: bnd_idx[2*b+k], agl_idx[3*a+k], dih_idx[4*d+k], imp_idx[4*m+k].
+ * agl_unique / dih_unique are the isAngle_unique / isDih_unique flags that
+ * InitCheckUniques (src/initvars.c:13-146) computes; pass NULL to let the
+ * engine compute them with that routine's rules. */
+typedef struct mspms_species {
+    int natom;                 /* sites per molecule                      */
+    int nmole;                 /* molecules of this species               */
+    const int*    ghost_type;  /* [natom]                                 */
+    const int*    interp_type; /* [natom] grid table index (SF_TASOS)     */
+    const double* aw;          /* [natom] reduced site mass               */
+    const double* sigma;       /* [natom]                                 */
+    const double* epsilon;     /* [natom]                                 */
+    const double* charge;      /* [natom] (e)                             */
+
+    int nbond;
+    const int*    bnd_idx;     /* [2*nbond]                               */
+    const int*    bnd_type;    /* [nbond]                                 */
+    const double* bnd_para;    /* [3*nbond]: Kb, Req, alpha               */
+
+    int nangle;
+    const int*    agl_idx;     /* [3*nangle]                              */
+    const int*    agl_type;    /* [nangle]                                */
+    const double* agl_para;    /* [5*nangle]: para_1..para_5              */
+    const uint8_t* agl_unique; /* [nangle] or NULL                        */
+
+    int ndih;
+    const int*    dih_idx;     /* [4*ndih]                                */
+    const int*    dih_type;    /* [ndih]                                  */
+    const double* dih_para;    /* [4*ndih]: c1..c4                        */
+    const uint8_t* dih_unique; /* [ndih] or NULL                          */
+
+    int nimp;
+    const int*    imp_idx;     /* [4*nimp]                                */
+    const int*    imp_type;    /* [nimp]                                  */
+    const double* imp_para;    /* [2*nimp]: komega, omega0                */
+
+    int nnbp;                  /* must be 0: src/frclong.c:793 reads
+                                  nbp_idx with the dihedral counter       */
+} mspms_species;
+
+/* ---- solid-fluid tables -------------------------------------------------- */
+/* Tricubic Hermite grid, the data of mylibtasos Read_Grids
+ * (mylibtasos/grid.f90:54-129, global.f90:214-227) in SoA node order
+ * node = (iz*ny + iy)*nx + ix.  8 fields per node and table type:
+ * F, dF/dx, dF/dy, dF/dz, d2F/dxdy, d2F/dxdz, d2F/dydz, d3F/dxdydz, in
+ * Kelvin and Angstrom (the Fortran's units; the engine converts as
+ * src/sffrc.c:1322-1337 does).  Grid spans [x0, x0+(nx-1)*hx] etc.; z is
+ * periodic with period (nz-1)*hz, x and y are not (out of range = EDOM). */
+typedef struct mspms_sf_grid {
+    int ntype;                 /* number of interp types                  */
+    int nx, ny, nz;
+    double x0, y0, z0;         /* Angstrom                                */
+    double hx, hy, hz;         /* Angstrom                                */
+    const float* nodes;        /* [ntype][8][nz][ny][nx] FP32             */
+} mspms_sf_grid;
+
+/* ---- run parameters ---------------------------------------------------------
+ * Field names follow src/vars.h:30-212.  Fill the "input" fields, then call
+ * mspms_derive_params() to compute the derived constants with the exact
+ * expressions of src/initvars.c:397-472 and :591-599, or fill them yourself
+ * (the drop-in copies the reference's globals verbatim). */
+typedef struct mspms_params {
+    /* inputs */
+    double sigma_base, epsilon_base;       /* src/readins.c:33-35          */
+    double boxlx, boxly, boxlz;            /* reduced                      */
+    double rcuton, rcutoff, rcutoffelec;   /* reduced                      */
+    int    isLJswitchOn;
+    double f0;                             /* 1-4 LJ scale                 */
+    int    iChargeType;                    /* MSPMS_ELEC_*                 */
+    double kappa;                          /* reduced (A^-1 * sigma_base)  */
+    int    kmaxx, kmaxy, kmaxz, ksqmax;
+    int    fEwald_BC;                      /* MSPMS_EWALD_BC_*             */
+    int    iSF_type;                       /* MSPMS_SF_*                   */
+    int    iExternal_FF_type;              /* must be 0                    */
+
+    /* derived */
+    double rcutoffsq, rcutonsq, rcutoffelecsq, roff2_minus_ron2_cube;
+    double shift1, shift4;
+    double coulomb_prefactor;
+    double twopi_lx, twopi_ly, twopi_lz;
+    double bfactor_ewald, vfactor_ewald, twopi_over_3v;
+    double wolfvcon1, wolfvcon2, wolfvcon3, wolffcon1, wolffcon2;
+
+    /* SF_HYPERGEO (src/sffrc.c:16-78): reduced tube data */
+    int    ntube;
+    const double* tube_x;                  /* [ntube] hgntc_xx             */
+    const double* tube_y;                  /* [ntube] hgntc_yy             */
+    const double* tube_r;                  /* [ntube] hgnt_radius          */
+    double const_3pisq_theta;
+    double solid_sigma, solid_epsilon;     /* reduced                      */
+
+    /* SF_ATOM_EXPLICIT (src/sffrc.c:81-160): reduced solid atoms */
+    int    solid_natom;
+    const double* solid_xx;
+    const double* solid_yy;
+    const double* solid_zz;
+
+    /* SF_TASOS */
+    const mspms_sf_grid* grid;
+} mspms_params;
+
+/* Outputs of frclong: the globals src/frclong.c writes (src/vars.h:173-210),
+ * plus pair statistics.  n_lj_pairs / n_elec_pairs count inter-molecular
+ * (i<j) site pairs inside rcutoff / rcutoffelec (src/frclong.c:123-125,
+ * :157); lj_pair_hash / elec_pair_hash are order-independent sums of
+ * splitmix64((uint64)i<<32 | j) over those pairs. */
+typedef struct mspms_long_terms {
+    double uvdw, ushift, ureal, uexcl, ufourier, uself, uvacuum, uelec;
+    double usflj, uotherff, uinter, virial_inter;
+    int64_t  n_lj_pairs, n_elec_pairs;
+    uint64_t lj_pair_hash, elec_pair_hash;
+} mspms_long_terms;
+
+/* Outputs of frcshort (src/frcshort.c:26-87). */
+typedef struct mspms_short_terms {
+    double ubond, uangle, udih, uimp, uintra, virial_intra;
+} mspms_short_terms;
+
+/* Parameter gradients of the nonbonded energy uinter (Wolf or Ewald real
+ * space, LJ) with respect to each site of each species' template:
+ * dE/d(epsilon), dE/d(sigma), dE/d(charge).  Arrays are indexed by global
+ * site id = sum of natom of earlier species + site.  No reference
+ * counterpart: the reference has no derivatives with respect to parameters. */
+typedef struct mspms_param_grad {
+    double* dE_depsilon;   /* [nsite] */
+    double* dE_dsigma;     /* [nsite] */
+    double* dE_dcharge;    /* [nsite] */
+} mspms_param_grad;
+
+/* ---- helpers ------------------------------------------------------------ */
+
+/* Computes the derived fields of *p from its inputs with the expressions of
+ * src/initvars.c:433-462 (Ewald, Wolf, Coulomb prefactor) and :591-599
+ * (cutoff squares, switch factor, shift).  Validates cutoff <= L/2
+ * (src/validation.c:613-626): returns MSPMS_ERANGE otherwise. */
+int mspms_derive_params(mspms_params* p);
+
+/* Fills unique[] with the isAngle_unique / isDih_unique rules of
+ * InitCheckUniques (src/initvars.c:13-146), including that routine's
+ * index quirks (src/initvars.c:66,86,134 mark entries by the wrong loop
+ * index).  agl_unique: [nangle], dih_unique: [ndih]. */
+int mspms_check_uniques(const mspms_species* sp, uint8_t* agl_unique,
+                        uint8_t* dih_unique);
+
+const char* mspms_last_error(void);
+
+/* ---- context API -------------------------------------------------------- */
+typedef struct mspms_ctx mspms_ctx;
+
+/* Creates a context on HIP device `device` with MSPMS_PREC_* `precision`. */
+int mspms_create(mspms_ctx** out, int device, int precision);
+int mspms_destroy(mspms_ctx* ctx);
+
+/* Topology + parameters of one system (copied; caller keeps ownership). */
+int mspms_set_system(mspms_ctx* ctx, int nspecie, const mspms_species* sp,
+                     const mspms_params* p);
+
+/* Replaces only the parameters (e.g. a new box after a volume move,
+ * src/nptnh.c:156-200 / src/volchg.c); the topology stays. */
+int mspms_set_params(mspms_ctx* ctx, const mspms_params* p);
+
+/* Batched replicas: `nrep` independent boxes sharing the species templates
+ * set by mspms_set_system.  nmole[r*nspecie + s] molecules of species s in
+ * replica r; params[r] gives each replica's box, cutoffs and Ewald/Wolf
+ * constants.  Positions and forces of all replicas are packed replica after
+ * replica, each replica's atoms in the reference order.  One call of
+ * mspms_frclong_* / mspms_frcshort_* then evaluates every replica in a
+ * single launch per kernel; terms[] receives one entry per replica. */
+int mspms_set_replicas(mspms_ctx* ctx, int nrep, const int* nmole,
+                       const mspms_params* params);
+
+int mspms_natom(const mspms_ctx* ctx);      /* total sites, all replicas */
+int mspms_nreplica(const mspms_ctx* ctx);
+
+/* Host-buffer evaluation: uploads x/y/z, evaluates, downloads forces and
+ * terms (one mspms_*_terms per replica).  Forces are overwritten, as the
+ * reference overwrites fxl / fxs (src/frclong.c:39-42, src/frcshort.c:33-36).
+ */
+int mspms_frclong_host(mspms_ctx* ctx, const double* xx, const double* yy,
+                       const double* zz, double* fxl, double* fyl, double* fzl,
+                       mspms_long_terms* terms);
+int mspms_frcshort_host(mspms_ctx* ctx, const double* xx, const double* yy,
+                        const double* zz, double* fxs, double* fys,
+                        double* fzs, mspms_short_terms* terms);
+
+/* Device-buffer evaluation on `stream` (a hipStream_t, NULL = the context's
+ * stream).  All pointers are device pointers; terms has one entry per
+ * replica.  Asynchronous: synchronize the stream before reading. */
+int mspms_frclong_device(mspms_ctx* ctx, const double* d_xx,
+                         const double* d_yy, const double* d_zz, double* d_fxl,
+                         double* d_fyl, double* d_fzl,
+                         mspms_long_terms* d_terms, void* stream);
+int mspms_frcshort_device(mspms_ctx* ctx, const double* d_xx,
+                          const double* d_yy, const double* d_zz,
+                          double* d_fxs, double* d_fys, double* d_fzs,
+                          mspms_short_terms* d_terms, void* stream);
+
+/* Adjoint of the nonbonded energy with respect to the per-site force-field
+ * parameters (Wolf or Ewald real space + LJ; replica 0 only).  Host
+ * buffers; grad arrays have one entry per global site id.  Also returns the
+ * forces (= -dE/dr) and the terms like mspms_frclong_host. */
+int mspms_energy_param_grad(mspms_ctx* ctx, const double* xx, const double* yy,
+                            const double* zz, double* fxl, double* fyl,
+                            double* fzl, mspms_long_terms* terms,
+                            mspms_param_grad* grad);
+
+/* Synchronises the context's stream and reports the status of the last
+ * device-buffer evaluation (list overflow, non-finite input, singular
+ * improper), which the host-buffer calls check by themselves. */
+int mspms_sync(mspms_ctx* ctx);
+
+/* The context's own HIP stream (hipStream_t). */
+void* mspms_stream(const mspms_ctx* ctx);
+
+/* Kernel-group timing with HIP events on the evaluation stream: switch on,
+ * evaluate, then read the mean milliseconds per call of each group
+ * (0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,
+ * 3 Ewald reciprocal, 4 finalize, 5 bonded).  Device-buffer calls are
+ * accumulated at mspms_sync(). */
+int mspms_set_profiling(mspms_ctx* ctx, int on);
+int mspms_last_timings(const mspms_ctx* ctx, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSPMS_GPU_H_ */

@@ -1,0 +1,308 @@
+"""Seeded synthetic workloads, written as mspms2 run directories.
+
+Each function returns a `RunDir` (in/cfg/xyz.mspms content) that the
+reference's own start-up path can read unchanged.  Positions come from
+numpy's PCG64 generator with a fixed seed, so the same call gives the same
+bytes on every machine with this numpy.
+
+  lj_fcc          LJ-864 argon on an FCC lattice, rho*=0.8442, rc=2.5 sigma
+  water_box       flexible 3-site water (SPC/Fw-like charges/LJ, harmonic
+                  O-H and H-O-H), Ewald or Wolf
+  butane_box      united-atom TraPPE n-butane
+  methane_tube    LJ methane inside a smooth cylindrical pore
+  reference_water the reference's Examples/Water inputs (TR water, Morse)
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .runio import RunDir, RunInput, Site, SpeciesSpec
+
+AVOGADRO = 6.0221415e23
+RGAS = 8.314472
+BOLTZMAN_CONSTANT = 1.3806505e-23
+
+
+def _time_base(mass_base, eps_base, sigma_base):
+    return math.sqrt(mass_base * 1.0e-27 / (eps_base * BOLTZMAN_CONSTANT)) * sigma_base * 1.0e5
+
+
+def _random_rotations(rng, n):
+    """Uniform random rotation matrices (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2),
+                  np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3),
+                  np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def _min_image_dist2(a, b, L):
+    d = a - b
+    d -= L * np.rint(d / L)
+    return np.sum(d * d, axis=-1)
+
+
+# --------------------------------------------------------------------------- #
+def lj_fcc(ncell: int = 6, rho: float = 0.8442, temp: float = 0.722,
+           rc: float = 2.5, jitter: float = 0.0, seed: int = 1,
+           ensemble: int = 1, nstep: int = 1000) -> RunDir:
+    """Config 1: N=4*ncell^3 LJ atoms on an FCC lattice (reduced units given
+    via argon bases sigma 3.405 A, epsilon 119.8 K, mass 1 reduced)."""
+    sb, eb = 3.405, 119.8
+    aw = 0.039948
+    mb = aw / AVOGADRO / 1.0e-27
+    n = 4 * ncell ** 3
+    L = (n / rho) ** (1.0 / 3.0)
+    a = L / ncell
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) + 0.25
+    cells = np.array([[i, j, k] for i in range(ncell) for j in range(ncell)
+                      for k in range(ncell)], dtype=np.float64)
+    pos = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    if jitter:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape) * a
+    tb = _time_base(mb, eb, sb)
+    run = RunInput(title="LJ-864 FCC argon", ij=1802, jk=9373, sigma_base=sb,
+                   epsilon_base=eb, mass_base=mb, treq=temp * eb, preq=1.0,
+                   boxlx=L * sb, boxly=L * sb, boxlz=L * sb, rcuton=rc * sb,
+                   rcutoff=rc * sb, rcutoffelec=rc * sb, isLJlrcOn=1, nstep=nstep,
+                   nstep_ave=100, nstep_print=100, delt=0.005 * tb, nstep_inner=1,
+                   f0=0.0, what_simulation=0, what_ensemble=ensemble, nconstraint=3,
+                   iChargeType=abi.ELEC_NONE, qq=100.0, qqs=100.0)
+    sp = SpeciesSpec(name="argon", nmole=n, sites=[
+        Site(type=0, name="Ar", xx=0.0, yy=0.0, zz=0.0, aw=aw, epsilon=eb, sigma=sb,
+             charge=0.0, ghost_type=0, interp_type=1)])
+    return RunDir(run=run, sysname="lj-fcc", species=[sp], xyz=pos * sb)
+
+
+# --------------------------------------------------------------------------- #
+WATER_Q_O, WATER_Q_H = -0.8476, 0.4238
+WATER_SIGMA_O, WATER_EPS_O = 3.166, 0.650e3 / RGAS        # Angstrom, K
+KCAL_TO_K = 4184.0 / RGAS
+WATER_KB = 1059.162 * KCAL_TO_K        # K/A^2, U = Kb/2 (r-r0)^2
+WATER_KTHETA = 75.90 * KCAL_TO_K       # K/rad^2, U = Kt/2 (th-th0)^2
+WATER_R0 = 1.0
+WATER_THETA0 = 109.47 * math.pi / 180.0
+
+
+def water_species(nmole: int) -> SpeciesSpec:
+    """Flexible three-site water with harmonic O-H bonds and H-O-H angle;
+    H sites carry charge only (GHOST_LJ, as the reference's Water example)."""
+    h = WATER_THETA0 / 2
+    tmpl = np.array([[0.0, 0.0, 0.0],
+                     [WATER_R0 * math.sin(h), 0.0, WATER_R0 * math.cos(h)],
+                     [-WATER_R0 * math.sin(h), 0.0, WATER_R0 * math.cos(h)]])
+    sites = [
+        Site(type=0, name="O", xx=tmpl[0, 0], yy=tmpl[0, 1], zz=tmpl[0, 2], aw=0.0159994,
+             epsilon=WATER_EPS_O, sigma=WATER_SIGMA_O, charge=WATER_Q_O, ghost_type=0,
+             interp_type=1),
+        Site(type=1, name="H", xx=tmpl[1, 0], yy=tmpl[1, 1], zz=tmpl[1, 2], aw=0.001008,
+             epsilon=0.0, sigma=0.0, charge=WATER_Q_H, ghost_type=1, interp_type=2),
+        Site(type=1, name="H", xx=tmpl[2, 0], yy=tmpl[2, 1], zz=tmpl[2, 2], aw=0.001008,
+             epsilon=0.0, sigma=0.0, charge=WATER_Q_H, ghost_type=1, interp_type=2),
+    ]
+    return SpeciesSpec(
+        name="water", nmole=nmole, sites=sites,
+        bonds=[(0, 1, abi.BOND_HARMONIC, WATER_KB, WATER_R0, 0.0),
+               (0, 2, abi.BOND_HARMONIC, WATER_KB, WATER_R0, 0.0)],
+        angles=[(1, 0, 2, abi.ANGLE_HARMONIC, WATER_KTHETA, WATER_THETA0, 0.0, 0.0, 0.0)])
+
+
+def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=200):
+    """Random orientation per molecule, rejecting contacts closer than
+    min_dist to molecules already placed (cell-list check)."""
+    nmol, nsite = centers.shape[0], tmpl.shape[0]
+    out = np.empty((nmol, nsite, 3))
+    ncell = max(1, int(L // max(min_dist, 1e-9)))
+    h = L / ncell
+    grid = {}
+    md2 = min_dist * min_dist
+    for m in range(nmol):
+        for _ in range(max_tries):
+            R = _random_rotations(rng, 1)[0]
+            xyz = centers[m] + tmpl @ R.T
+            ok = True
+            cells = set()
+            for s in range(nsite):
+                w = xyz[s] - L * np.floor(xyz[s] / L)
+                c = tuple((w // h).astype(int) % ncell)
+                for dx in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dz in (-1, 0, 1):
+                            cc = ((c[0] + dx) % ncell, (c[1] + dy) % ncell,
+                                  (c[2] + dz) % ncell)
+                            for p in grid.get(cc, ()):
+                                if _min_image_dist2(xyz[s], p, L) < md2:
+                                    ok = False
+                                    break
+                            if not ok:
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+                cells.add(c)
+            if ok:
+                break
+        out[m] = xyz
+        for s in range(nsite):
+            w = xyz[s] - L * np.floor(xyz[s] / L)
+            c = tuple((w // h).astype(int) % ncell)
+            grid.setdefault(c, []).append(xyz[s].copy())
+    return out
+
+
+def water_box(n_side: int = 16, L: float = 49.66, seed: int = 2,
+              elec: str = "ewald", kappa: float = 0.28, kmax: int = 8,
+              ksqmax: int = 64, rc: float = 10.0, jitter: float = 0.3,
+              nstep_inner: int = 8, delt_fs: float = 2.0, min_dist: float = 1.6,
+              ensemble: int = 0) -> RunDir:
+    """Config 2 (Ewald, n_side=16, 12,288 sites) and config 4 (Wolf): water on
+    a jittered cubic lattice with uniform random orientations."""
+    rng = np.random.default_rng(seed)
+    nmol = n_side ** 3
+    a = L / n_side
+    idx = np.array([[i, j, k] for i in range(n_side) for j in range(n_side)
+                    for k in range(n_side)], dtype=np.float64)
+    centers = (idx + 0.5) * a + rng.uniform(-jitter, jitter, size=(nmol, 3))
+    sp = water_species(nmol)
+    tmpl = np.array([[s.xx, s.yy, s.zz] for s in sp.sites])
+    tmpl = tmpl - (tmpl * np.array([15.9994, 1.008, 1.008])[:, None]).sum(0) / 18.0154
+    xyz = _place_rigid(rng, centers, tmpl, L, min_dist).reshape(-1, 3)
+    ct = {"ewald": abi.ELEC_EWALD, "wolf": abi.ELEC_WOLF, "none": abi.ELEC_NONE,
+          "coulomb": abi.ELEC_SIMPLE_COULOMB}[elec]
+    run = RunInput(title=f"water {nmol} {elec}", ij=1802, jk=9373, sigma_base=1.0,
+                   epsilon_base=1.0, mass_base=1.0, treq=298.0, preq=1.0, boxlx=L,
+                   boxly=L, boxlz=L, rcuton=rc, rcutoff=rc, rcutoffelec=rc,
+                   isLJlrcOn=0, delt=delt_fs, nstep_inner=nstep_inner, f0=0.0,
+                   what_simulation=0, what_ensemble=ensemble, nconstraint=3,
+                   iChargeType=ct, kappa=kappa, kmaxx=kmax, kmaxy=kmax, kmaxz=kmax,
+                   ksqmax=ksqmax, fEwald_BC=abi.EWALD_BC_TINFOIL, fEwald_Dim=3,
+                   qq=200.0, qqs=200.0)
+    return RunDir(run=run, sysname="water", species=[sp], xyz=xyz)
+
+
+# --------------------------------------------------------------------------- #
+BUTANE_DENSITY = 0.58 * AVOGADRO / 58.12 / 1.0e24   # molecules / A^3
+BUTANE_TMPL = np.array([[0.0, 0.0, 0.0], [-0.5133, -0.7260, 1.2574],
+                        [-2.0533, -0.7235, 1.2588], [-2.5667, -1.4500, 2.5160]])
+
+
+def butane_species(nmole: int) -> SpeciesSpec:
+    """TraPPE-UA n-butane: CH3 (3.75 A, 98 K), CH2 (3.95 A, 46 K), bonds
+    1.54 A (stiff harmonic), angle 114 deg, TraPPE cosine dihedral in the
+    reference's c1..c4 convention (root cfg.mspms:11-18)."""
+    s = []
+    for k, (nm, sg, ep, aw) in enumerate([("CH3", 3.75, 98.0, 0.015035),
+                                          ("CH2", 3.95, 46.0, 0.014027),
+                                          ("CH2", 3.95, 46.0, 0.014027),
+                                          ("CH3", 3.75, 98.0, 0.015035)]):
+        s.append(Site(type=0 if nm == "CH3" else 1, name=nm, xx=BUTANE_TMPL[k, 0],
+                      yy=BUTANE_TMPL[k, 1], zz=BUTANE_TMPL[k, 2], aw=aw, epsilon=ep,
+                      sigma=sg, charge=0.0, ghost_type=0, interp_type=1 if nm == "CH3" else 2))
+    kb, r0 = 905800.0, 1.54
+    kt, t0 = 62500.0, 114.0 * math.pi / 180.0
+    return SpeciesSpec(
+        name="n-butane", nmole=nmole, sites=s,
+        bonds=[(0, 1, abi.BOND_HARMONIC, kb, r0, 0.0), (1, 2, abi.BOND_HARMONIC, kb, r0, 0.0),
+               (2, 3, abi.BOND_HARMONIC, kb, r0, 0.0)],
+        angles=[(0, 1, 2, abi.ANGLE_HARMONIC, kt, t0, 0.0, 0.0, 0.0),
+                (1, 2, 3, abi.ANGLE_HARMONIC, kt, t0, 0.0, 0.0, 0.0)],
+        dihedrals=[(0, 1, 2, 3, abi.DIH_OPLS_COSIN, 0.0, 710.06, -136.38, 1582.64)])
+
+
+def butane_box(nmol: int = 128, seed: int = 3, rc: float = 14.0,
+               min_dist: float = 2.8, delt_fs: float = 1.0,
+               nstep_inner: int = 5) -> RunDir:
+    """Config 3 (one replica): n-butane at 0.58 g/cm^3, rc = min(rc, L/2)."""
+    rng = np.random.default_rng(seed)
+    L = (nmol / BUTANE_DENSITY) ** (1.0 / 3.0)
+    rcut = min(rc, 0.5 * L * (1 - 1e-12))
+    ns = int(math.ceil(nmol ** (1.0 / 3.0)))
+    a = L / ns
+    sites = np.array([[i, j, k] for i in range(ns) for j in range(ns)
+                      for k in range(ns)], dtype=np.float64)
+    pick = rng.permutation(len(sites))[:nmol]
+    centers = (sites[pick] + 0.5) * a + rng.uniform(-0.2, 0.2, size=(nmol, 3))
+    tmpl = BUTANE_TMPL - BUTANE_TMPL.mean(0)
+    xyz = _place_rigid(rng, centers, tmpl, L, min_dist).reshape(-1, 3)
+    run = RunInput(title=f"n-butane {nmol}", ij=1000 + seed % 30000, jk=2000 + seed % 28000,
+                   sigma_base=1.0, epsilon_base=1.0, mass_base=1.0, treq=300.0, preq=1.0,
+                   boxlx=L, boxly=L, boxlz=L, rcuton=rcut, rcutoff=rcut, rcutoffelec=rcut,
+                   isLJlrcOn=0, delt=delt_fs, nstep_inner=nstep_inner, f0=0.0,
+                   what_simulation=1, what_ensemble=0, nconstraint=3,
+                   iChargeType=abi.ELEC_NONE, nstep_md_per_hmc=10)
+    return RunDir(run=run, sysname="n-butane", species=[butane_species(nmol)], xyz=xyz)
+
+
+# --------------------------------------------------------------------------- #
+def methane_tube(natom: int = 2000, length: float = 400.0, box_xy: float = 100.0,
+                 tube_r: float = 50.0, seed: int = 5, rc: float = 12.0,
+                 sf: int = abi.SF_HYPERGEO, r_fill: Optional[float] = None,
+                 min_dist: float = 3.0) -> RunDir:
+    """Config 5: LJ methane (3.73 A, 148 K) in a smooth carbon tube of radius
+    tube_r centred on the z axis, axial box `length` (Angstrom)."""
+    rng = np.random.default_rng(seed)
+    r_fill = tube_r - 4.0 if r_fill is None else r_fill
+    pts = []
+    cell = min_dist
+    grid = {}
+    while len(pts) < natom:
+        m = 4 * (natom - len(pts))
+        r = r_fill * np.sqrt(rng.random(m))
+        th = 2 * np.pi * rng.random(m)
+        z = length * rng.random(m)
+        cand = np.stack([r * np.cos(th), r * np.sin(th), z], axis=1)
+        for c in cand:
+            key = (int(c[0] // cell), int(c[1] // cell), int((c[2] % length) // cell))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        kk = (key[0] + dx, key[1] + dy,
+                              (key[2] + dz) % max(1, int(length // cell)))
+                        for p in grid.get(kk, ()):
+                            d = c - p
+                            d[2] -= length * np.rint(d[2] / length)
+                            if d @ d < min_dist * min_dist:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pts.append(c)
+                grid.setdefault(key, []).append(c)
+                if len(pts) == natom:
+                    break
+    xyz = np.array(pts)
+    run = RunInput(title=f"methane tube {natom}", sigma_base=1.0, epsilon_base=1.0,
+                   mass_base=1.0, treq=300.0, boxlx=box_xy, boxly=box_xy, boxlz=length,
+                   rcuton=rc, rcutoff=rc, rcutoffelec=rc, isLJlrcOn=0, delt=2.0,
+                   nstep_inner=4, iChargeType=abi.ELEC_NONE, iSF_type=sf,
+                   tube_sigma=3.37, tube_epsilon=27.68, tubes=[(0.0, 0.0, tube_r)])
+    sp = SpeciesSpec(name="methane", nmole=natom, sites=[
+        Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.016043, epsilon=148.0,
+             sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
+    return RunDir(run=run, sysname="methane-tube", species=[sp], xyz=xyz)

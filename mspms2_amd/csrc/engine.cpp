@@ -1,0 +1,1122 @@
+// engine.cpp -- host side of the MI355X force engine: the C ABI of
+// include/mspms_gpu.h.  Builds the device tables (sites, molecules,
+// replicas, bonded terms, Ewald k rows and work lists), owns the device
+// buffers and enqueues the kernels of k_*.hip on one HIP stream.
+//
+// Reference boundary: src/funcs.h:225,233 (int frclong(), int frcshort())
+// and the globals they read and write (src/molevars.h, src/vars.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mspms_gpu.h"
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPC(expr)                                                                \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail(MSPMS_EIO, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+const double PI_REF = 3.141592653589793;  // src/defs.h:100
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count) {
+        if (count <= n && p) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+            return fail(MSPMS_ENOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        n = count;
+        return 0;
+    }
+    int upload(const std::vector<T>& v, hipStream_t s) {
+        int rc = ensure(v.size());
+        if (rc) return rc;
+        if (!v.empty()) HIPC(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct SpeciesHost {
+    int natom, nmole;
+    std::vector<int> ghost, interp;
+    std::vector<double> aw, sigma, epsilon, charge;
+    std::vector<int> bnd_idx, bnd_type;
+    std::vector<double> bnd_para;
+    std::vector<int> agl_idx, agl_type;
+    std::vector<double> agl_para;
+    std::vector<uint8_t> agl_unique;
+    std::vector<int> dih_idx, dih_type;
+    std::vector<double> dih_para;
+    std::vector<uint8_t> dih_unique;
+    std::vector<int> imp_idx, imp_type;
+    std::vector<double> imp_para;
+    int nbond() const { return (int)bnd_type.size(); }
+    int nangle() const { return (int)agl_type.size(); }
+    int ndih() const { return (int)dih_type.size(); }
+    int nimp() const { return (int)imp_type.size(); }
+};
+
+template <class T>
+std::vector<T> copy_arr(const T* p, size_t n) {
+    return p ? std::vector<T>(p, p + n) : std::vector<T>(n, T());
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------
+struct mspms_ctx {
+    int device = 0, precision = MSPMS_PREC_MIXED;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    std::vector<SpeciesHost> sp;
+    int nrep = 0;
+    std::vector<int> nmole;              // [nrep*nspecie]
+    std::vector<mspms_params> params;    // [nrep]
+    std::vector<std::vector<double>> tube_keep;
+    int natom = 0, nmol = 0, nsite = 0;
+    int charge_union = MSPMS_ELEC_NONE;  // charge type shared by the replicas
+    bool any_intra = false, any_bonded = false, any_recip = false, any_sf = false;
+    int want_hash = -1;                  // -1: FP64 only
+
+    // static device tables
+    DevBuf<SiteDev> d_sites;
+    DevBuf<SpeciesDev> d_species;
+    DevBuf<TermDev> d_terms;
+    DevBuf<MolDev> d_mols;
+    DevBuf<int> d_atom_mol, d_atom_site;
+    DevBuf<RepDev> d_reps;
+    std::vector<RepDev> reps;
+
+    // clustering
+    ClusterBufs cb{};
+    DevBuf<int> d_atom_cell, d_atom_slot, d_cell_count, d_cell_start, d_cell_members,
+        d_col_cell0, d_col_rep, d_col_cl0, d_cl_col, d_ncl_total, d_slot_orig, d_plist_n;
+    DevBuf<double> d_atom_wz;
+    DevBuf<int2> d_slot_info;
+    DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center;
+    DevBuf<float4> d_slot_off, d_cl_ext;
+    DevBuf<uint32_t> d_plist;
+    int cap_hint = 0;
+
+    // reciprocal
+    RecipBufs rb{};
+    DevBuf<KRow> d_rows, d_rows_s;
+    DevBuf<double> d_kvec, d_uk;
+    DevBuf<int> d_k_rep, d_rep_chunk0;
+    DevBuf<int4> d_work_s, d_work_f;
+    DevBuf<double2> d_s_part, d_ak;
+
+    // partial sums and outputs
+    DevBuf<double> d_ep_pair, d_ep_intra, d_ep_short, d_ep_sf;
+    DevBuf<unsigned long long> d_ip_pair;
+    DevBuf<mspms_long_terms> d_long;
+    DevBuf<mspms_short_terms> d_short;
+    DevBuf<Status> d_status;
+    Status* h_status = nullptr;  // pinned
+
+    // host-API staging
+    DevBuf<double> d_pos, d_fl, d_fs;
+
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> prof_ms;
+    int prof_calls = 0;
+};
+
+namespace {
+
+LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
+    LaunchCtx L;
+    L.stream = s;
+    L.natom = c->natom;
+    L.nmol = c->nmol;
+    L.nsite = c->nsite;
+    L.nrep = c->nrep;
+    L.sites = c->d_sites.p;
+    L.mols = c->d_mols.p;
+    L.atom_mol = c->d_atom_mol.p;
+    L.atom_site = c->d_atom_site.p;
+    L.reps = c->d_reps.p;
+    L.status = c->d_status.p;
+    return L;
+}
+
+// InitCheckUniques (src/initvars.c:13-146) including its index quirks:
+// the dihedral-vs-angle and dihedral-vs-bond tests mark isDih_unique by the
+// angle / bond index (:66, :86), the angle-vs-bond test marks isAngle_unique
+// by the bond index (:134).  Marks past the current counts are dropped.
+void check_uniques(const int* agl_idx, int nangle, const int* dih_idx, int ndih,
+                   const int* bnd_idx, int nbond, uint8_t* au, uint8_t* du) {
+    std::vector<uint8_t> dih(std::max({ndih, nangle, nbond, 1}), 1),
+        agl(std::max({nangle, nbond, 1}), 1);
+    for (int i = 0; i < ndih - 1; i++) {
+        if (!dih[i]) continue;
+        for (int j = i + 1; j < ndih; j++) {
+            if (!dih[j]) continue;
+            if ((dih_idx[4 * i] == dih_idx[4 * j] && dih_idx[4 * i + 3] == dih_idx[4 * j + 3]) ||
+                (dih_idx[4 * i] == dih_idx[4 * j + 3] && dih_idx[4 * i + 3] == dih_idx[4 * j]))
+                dih[j] = 0;
+        }
+    }
+    for (int i = 0; i < ndih; i++) {
+        if (!dih[i]) continue;
+        for (int j = 0; j < nangle; j++)
+            if ((dih_idx[4 * i] == agl_idx[3 * j] && dih_idx[4 * i + 3] == agl_idx[3 * j + 2]) ||
+                (dih_idx[4 * i] == agl_idx[3 * j + 2] && dih_idx[4 * i + 3] == agl_idx[3 * j]))
+                dih[j] = 0;
+    }
+    for (int i = 0; i < ndih; i++) {
+        if (!dih[i]) continue;
+        for (int j = 0; j < nbond; j++)
+            if ((dih_idx[4 * i] == bnd_idx[2 * j] && dih_idx[4 * i + 3] == bnd_idx[2 * j + 1]) ||
+                (dih_idx[4 * i] == bnd_idx[2 * j + 1] && dih_idx[4 * i + 3] == bnd_idx[2 * j]))
+                dih[j] = 0;
+    }
+    for (int i = 0; i < nangle - 1; i++) {
+        if (!agl[i]) continue;
+        for (int j = i + 1; j < nangle; j++) {
+            if (!agl[j]) continue;
+            if ((agl_idx[3 * i] == agl_idx[3 * j] && agl_idx[3 * i + 2] == agl_idx[3 * j + 2]) ||
+                (agl_idx[3 * i] == agl_idx[3 * j + 2] && agl_idx[3 * i + 2] == agl_idx[3 * j]))
+                agl[j] = 0;
+        }
+    }
+    for (int i = 0; i < nangle; i++) {
+        if (!agl[i]) continue;
+        for (int j = 0; j < nbond; j++)
+            if ((agl_idx[3 * i] == bnd_idx[2 * j] && agl_idx[3 * i + 2] == bnd_idx[2 * j + 1]) ||
+                (agl_idx[3 * i] == bnd_idx[2 * j + 1] && agl_idx[3 * i + 2] == bnd_idx[2 * j]))
+                agl[j] = 0;
+    }
+    for (int i = 0; i < nangle; i++) au[i] = agl[i];
+    for (int i = 0; i < ndih; i++) du[i] = dih[i];
+}
+
+int validate_species(const mspms_species& q) {
+    if (q.natom < 1 || q.natom > 256) return fail(MSPMS_EINVAL, "natom %d out of 1..256", q.natom);
+    if (q.nmole < 0) return fail(MSPMS_EINVAL, "negative nmole");
+    if (q.nnbp != 0)
+        return fail(MSPMS_EINVAL,
+                    "non-bonded pair lists are not supported: src/frclong.c:793 indexes "
+                    "nbp_idx with the dihedral counter");
+    for (int k = 0; k < q.nbond; k++) {
+        int t = q.bnd_type[k];
+        if (t < MSPMS_BOND_HARMONIC || t > MSPMS_BOND_FENE)
+            return fail(MSPMS_EINVAL, "bond type %d unsupported (reference: unknown or "
+                                      "uninitialised energy)", t);
+    }
+    for (int k = 0; k < q.nangle; k++) {
+        int t = q.agl_type[k];
+        if (t < MSPMS_ANGLE_HARMONIC || t > MSPMS_ANGLE_TR_WATER)
+            return fail(MSPMS_EINVAL, "angle type %d unsupported", t);
+    }
+    for (int k = 0; k < q.ndih; k++) {
+        int t = q.dih_type[k];
+        if (t < MSPMS_DIH_OPLS_COSIN || t > MSPMS_DIH_CHARMM)
+            return fail(MSPMS_EINVAL, "dihedral type %d unsupported", t);
+        if (t == MSPMS_DIH_CHARMM && !(std::fabs(q.dih_para[4 * k]) >= 1.0e-8))
+            return fail(MSPMS_EINVAL, "CHARMM dihedral with |kphi| < TOLERANCE leaves the "
+                                      "reference's energy uninitialised");
+    }
+    for (int k = 0; k < q.nimp; k++) {
+        if (q.imp_type[k] != MSPMS_IMP_CHARMM)
+            return fail(MSPMS_EINVAL, "improper type %d unsupported", q.imp_type[k]);
+        if (!(std::fabs(q.imp_para[2 * k]) >= 1e-5))
+            return fail(MSPMS_EINVAL, "CHARMM improper with |komega| < 1e-5 leaves the "
+                                      "reference's energy uninitialised");
+    }
+    if (q.ndih > 0 && q.nbond == 0 && q.nangle == 0)
+        return fail(MSPMS_EINVAL, "dihedrals without bonds or angles: the 1-4 section "
+                                  "would read an unset isSameMole");
+    auto chk_idx = [&](const int* idx, int n, int w) {
+        for (int k = 0; k < n * w; k++)
+            if (idx[k] < 0 || idx[k] >= q.natom) return false;
+        return true;
+    };
+    if (!chk_idx(q.bnd_idx, q.nbond, 2) || !chk_idx(q.agl_idx, q.nangle, 3) ||
+        !chk_idx(q.dih_idx, q.ndih, 4) || !chk_idx(q.imp_idx, q.nimp, 4))
+        return fail(MSPMS_EINVAL, "term index outside the molecule");
+    return 0;
+}
+
+int validate_params(const mspms_params& p) {
+    if (!(p.boxlx > 0 && p.boxly > 0 && p.boxlz > 0))
+        return fail(MSPMS_EINVAL, "invalid box %g %g %g", p.boxlx, p.boxly, p.boxlz);
+    double fmin = std::min(p.boxlx, std::min(p.boxly, p.boxlz)) / 2.0;
+    if (p.rcuton > fmin || p.rcutoff > fmin || p.rcutoffelec > fmin)
+        return fail(MSPMS_ERANGE, "cutoff larger than half the box (src/validation.c:617)");
+    if (p.iExternal_FF_type != 0)
+        return fail(MSPMS_EINVAL, "external force fields (metal cluster) are not supported");
+    if (p.iChargeType == MSPMS_ELEC_SIMPLE_COULOMB)
+        return fail(MSPMS_EINVAL, "simple Coulomb (iChargeType 3) is not implemented on the GPU");
+    if (p.iChargeType < 0 || p.iChargeType > 3) return fail(MSPMS_EINVAL, "bad iChargeType");
+    if (p.iChargeType == MSPMS_ELEC_EWALD) {
+        if (p.fEwald_BC == MSPMS_EWALD_BC_VACUUM)
+            return fail(MSPMS_EINVAL, "vacuum Ewald boundary is not implemented on the GPU");
+        if (p.kmaxx < 0 || p.kmaxy < 0 || p.kmaxz < 0 || p.kmaxx > 16 || p.kmaxy > 16 ||
+            p.kmaxz > 16)
+            return fail(MSPMS_EINVAL, "KMAX must be within 0..16");
+    }
+    if (p.iSF_type != MSPMS_SF_NONE)
+        return fail(MSPMS_EINVAL, "solid-fluid type %d is not implemented on the GPU", p.iSF_type);
+    return 0;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------
+extern "C" const char* mspms_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mspms_check_uniques(const mspms_species* sp, uint8_t* au, uint8_t* du) {
+    if (!sp) return fail(MSPMS_EINVAL, "null species");
+    check_uniques(sp->agl_idx, sp->nangle, sp->dih_idx, sp->ndih, sp->bnd_idx, sp->nbond, au, du);
+    return 0;
+}
+
+// src/initvars.c:591-599 and :433-462
+extern "C" int mspms_derive_params(mspms_params* p) {
+    if (!p) return fail(MSPMS_EINVAL, "null params");
+    volatile double rc = p->rcutoff;  // keep the reference's evaluation order
+    p->rcutoffsq = p->rcutoff * p->rcutoff;
+    p->rcutoffelecsq = p->rcutoffelec * p->rcutoffelec;
+    p->rcutonsq = p->rcuton * p->rcuton;
+    p->roff2_minus_ron2_cube = (p->rcutoffsq - p->rcutonsq) * (p->rcutoffsq - p->rcutonsq) *
+                               (p->rcutoffsq - p->rcutonsq);
+    p->shift1 = std::pow((1 / rc), 6.0);
+    p->shift4 = 4.0 * p->shift1;
+    if (p->iChargeType == MSPMS_ELEC_EWALD) {
+        p->bfactor_ewald = 1.0 / (4.0 * p->kappa * p->kappa);
+        p->vfactor_ewald = 2.0 * PI_REF / (p->boxlx * p->boxly * p->boxlz);
+        p->twopi_lx = 2.0 * PI_REF / p->boxlx;
+        p->twopi_ly = 2.0 * PI_REF / p->boxly;
+        p->twopi_lz = 2.0 * PI_REF / p->boxlz;
+        p->twopi_over_3v = 2.0 * PI_REF / 3.0 / p->boxlx / p->boxly / p->boxlz;
+    } else if (p->iChargeType == MSPMS_ELEC_WOLF) {
+        double k = p->kappa, rce = p->rcutoffelec;
+        p->wolfvcon1 = -std::erfc(k * rce) / rce;
+        p->wolfvcon2 = std::erfc(k * rce) / p->rcutoffelecsq +
+                       2.0 * k * std::exp(-(k * rce) * (k * rce)) / (std::sqrt(PI_REF) * rce);
+        p->wolfvcon3 = k / std::sqrt(PI_REF) + std::erfc(k * rce) / (2.0 * rce);
+        p->wolffcon1 = 2.0 * k / std::sqrt(PI_REF);
+        p->wolffcon2 = -p->wolfvcon2;
+    }
+    if (p->iChargeType != MSPMS_ELEC_NONE)
+        p->coulomb_prefactor = 167100.80840 / p->epsilon_base / p->sigma_base;
+    double fmin = std::min(p->boxlx, std::min(p->boxly, p->boxlz)) / 2.0;
+    if (p->rcuton > fmin || p->rcutoff > fmin || p->rcutoffelec > fmin)
+        return fail(MSPMS_ERANGE, "cutoff larger than half the box");
+    return 0;
+}
+
+extern "C" int mspms_create(mspms_ctx** out, int device, int precision) {
+    if (!out) return fail(MSPMS_EINVAL, "null out");
+    if (precision != MSPMS_PREC_FP64 && precision != MSPMS_PREC_MIXED)
+        return fail(MSPMS_EINVAL, "unknown precision %d", precision);
+    int ndev = 0;
+    HIPC(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MSPMS_EINVAL, "no HIP device %d", device);
+    HIPC(hipSetDevice(device));
+    mspms_ctx* c = new mspms_ctx();
+    c->device = device;
+    c->precision = precision;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MSPMS_EIO, "hipStreamCreate failed");
+    }
+    c->own_stream = true;
+    if (c->d_status.ensure(1) ||
+        hipHostMalloc((void**)&c->h_status, sizeof(Status), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return fail(MSPMS_ENOMEM, "status allocation failed");
+    }
+    memset(c->h_status, 0, sizeof(Status));
+    (void)hipMemcpy(c->d_status.p, c->h_status, sizeof(Status), hipMemcpyHostToDevice);
+    *out = c;
+    return 0;
+}
+
+extern "C" int mspms_destroy(mspms_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    c->d_sites.release();
+    c->d_species.release();
+    c->d_terms.release();
+    c->d_mols.release();
+    c->d_atom_mol.release();
+    c->d_atom_site.release();
+    c->d_reps.release();
+    c->d_atom_cell.release();
+    c->d_atom_slot.release();
+    c->d_cell_count.release();
+    c->d_cell_start.release();
+    c->d_cell_members.release();
+    c->d_col_cell0.release();
+    c->d_col_rep.release();
+    c->d_col_cl0.release();
+    c->d_cl_col.release();
+    c->d_ncl_total.release();
+    c->d_slot_orig.release();
+    c->d_plist_n.release();
+    c->d_atom_wz.release();
+    c->d_slot_info.release();
+    c->d_slot_x64.release();
+    c->d_slot_w64.release();
+    c->d_cl_center.release();
+    c->d_slot_off.release();
+    c->d_cl_ext.release();
+    c->d_plist.release();
+    c->d_rows.release();
+    c->d_rows_s.release();
+    c->d_kvec.release();
+    c->d_uk.release();
+    c->d_k_rep.release();
+    c->d_rep_chunk0.release();
+    c->d_work_s.release();
+    c->d_work_f.release();
+    c->d_s_part.release();
+    c->d_ak.release();
+    c->d_ep_pair.release();
+    c->d_ep_intra.release();
+    c->d_ep_short.release();
+    c->d_ep_sf.release();
+    c->d_ip_pair.release();
+    c->d_long.release();
+    c->d_short.release();
+    c->d_status.release();
+    c->d_pos.release();
+    c->d_fl.release();
+    c->d_fs.release();
+    if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+namespace {
+
+// Builds every device table from sp / nmole / params.
+int build(mspms_ctx* c) {
+    HIPC(hipSetDevice(c->device));
+    const int ns = (int)c->sp.size();
+    const hipStream_t s = c->stream;
+    // ---- sites, species, terms ----
+    std::vector<SiteDev> sites;
+    std::vector<SpeciesDev> spd;
+    std::vector<TermDev> terms;
+    c->any_intra = c->any_bonded = false;
+    for (int k = 0; k < ns; k++) {
+        const SpeciesHost& q = c->sp[k];
+        SpeciesDev d{};
+        d.natom = q.natom;
+        d.site0 = (int)sites.size();
+        for (int a = 0; a < q.natom; a++) {
+            SiteDev t{};
+            t.sigma = q.sigma[a];
+            t.epsilon = q.epsilon[a];
+            t.charge = q.charge[a];
+            t.aw = q.aw[a];
+            t.sqrt_eps = std::sqrt(q.epsilon[a]);
+            t.sigma_f = (float)q.sigma[a];
+            t.sqrt_eps_f = (float)std::sqrt(q.epsilon[a]);
+            t.charge_f = (float)q.charge[a];
+            t.half_sigma_f = (float)(0.5 * q.sigma[a]);
+            t.ghost = q.ghost[a];
+            t.species = k;
+            t.site = a;
+            t.interp = q.interp[a];
+            sites.push_back(t);
+        }
+        // frclong intramolecular pairs: bonds, unique angles, unique dihedrals
+        d.term0 = (int)terms.size();
+        for (int b = 0; b < q.nbond(); b++) {
+            TermDev t{};
+            t.kind = TERM_PAIR12;
+            t.idx[0] = q.bnd_idx[2 * b];
+            t.idx[1] = q.bnd_idx[2 * b + 1];
+            terms.push_back(t);
+        }
+        for (int a = 0; a < q.nangle(); a++) {
+            if (!q.agl_unique[a]) continue;
+            TermDev t{};
+            t.kind = TERM_PAIR13;
+            t.idx[0] = q.agl_idx[3 * a];
+            t.idx[1] = q.agl_idx[3 * a + 2];
+            terms.push_back(t);
+        }
+        for (int h = 0; h < q.ndih(); h++) {
+            if (!q.dih_unique[h]) continue;
+            TermDev t{};
+            t.kind = TERM_PAIR14;
+            t.idx[0] = q.dih_idx[4 * h];
+            t.idx[1] = q.dih_idx[4 * h + 3];
+            terms.push_back(t);
+        }
+        d.nintra = (int)terms.size() - d.term0;
+        // frcshort terms in src/frcshort.c:47-79 order
+        d.bterm0 = (int)terms.size();
+        for (int b = 0; b < q.nbond(); b++) {
+            TermDev t{};
+            t.kind = TERM_BOND;
+            t.type = q.bnd_type[b];
+            t.idx[0] = q.bnd_idx[2 * b];
+            t.idx[1] = q.bnd_idx[2 * b + 1];
+            for (int p = 0; p < 3; p++) t.para[p] = q.bnd_para[3 * b + p];
+            terms.push_back(t);
+        }
+        for (int a = 0; a < q.nangle(); a++) {
+            TermDev t{};
+            t.kind = TERM_ANGLE;
+            t.type = q.agl_type[a];
+            for (int p = 0; p < 3; p++) t.idx[p] = q.agl_idx[3 * a + p];
+            for (int p = 0; p < 5; p++) t.para[p] = q.agl_para[5 * a + p];
+            terms.push_back(t);
+        }
+        for (int h = 0; h < q.ndih(); h++) {
+            TermDev t{};
+            t.kind = TERM_DIH;
+            t.type = q.dih_type[h];
+            for (int p = 0; p < 4; p++) t.idx[p] = q.dih_idx[4 * h + p];
+            for (int p = 0; p < 4; p++) t.para[p] = q.dih_para[4 * h + p];
+            terms.push_back(t);
+        }
+        for (int m = 0; m < q.nimp(); m++) {
+            TermDev t{};
+            t.kind = TERM_IMP;
+            t.type = q.imp_type[m];
+            for (int p = 0; p < 4; p++) t.idx[p] = q.imp_idx[4 * m + p];
+            for (int p = 0; p < 2; p++) t.para[p] = q.imp_para[2 * m + p];
+            terms.push_back(t);
+        }
+        d.nbterm = (int)terms.size() - d.bterm0;
+        if (d.nintra > 0) c->any_intra = true;
+        if (d.nbterm > 0) c->any_bonded = true;
+        spd.push_back(d);
+    }
+    c->nsite = (int)sites.size();
+
+    // ---- atoms, molecules, replicas ----
+    std::vector<MolDev> mols;
+    std::vector<int> atom_mol, atom_site;
+    std::vector<RepDev> reps(c->nrep);
+    std::vector<int> col_cell0(1, 0), col_rep;
+    int cell0 = 0, col0 = 0;
+    c->charge_union = c->params[0].iChargeType;
+    c->any_recip = false;
+    c->any_sf = false;
+    std::vector<KRow> rows, rows_s;
+    std::vector<double> kvec;
+    std::vector<int> k_rep, rep_chunk0(1, 0);
+    std::vector<int4> work_s, work_f;
+    int ntab_max = 1, nkz_max = 1, nk_stride = 1;
+    size_t max_rep_cl = 1;
+    double max_est = 0.0;
+    for (int r = 0; r < c->nrep; r++) {
+        const mspms_params& P = c->params[r];
+        if (P.iChargeType != c->charge_union)
+            return fail(MSPMS_EINVAL, "all replicas must share one electrostatics type");
+        RepDev& R = reps[r];
+        memset(&R, 0, sizeof R);
+        R.atom0 = (int)atom_mol.size();
+        R.mol0 = (int)mols.size();
+        double q2 = 0.0;
+        int nreal = 0;
+        for (int k = 0; k < ns; k++) {
+            const SpeciesHost& q = c->sp[k];
+            int nm = c->nmole[(size_t)r * ns + k];
+            for (int m = 0; m < nm; m++) {
+                MolDev M;
+                M.atom0 = (int)atom_mol.size();
+                M.species = k;
+                M.rep = r;
+                M.natom = q.natom;
+                for (int a = 0; a < q.natom; a++) {
+                    atom_mol.push_back((int)mols.size());
+                    atom_site.push_back(spd[k].site0 + a);
+                    q2 += q.charge[a] * q.charge[a];  // src/frclong.c:1051-1057 order
+                    if (q.ghost[a] != MSPMS_GHOST_FULL) nreal++;
+                }
+                mols.push_back(M);
+            }
+        }
+        R.natom = (int)atom_mol.size() - R.atom0;
+        R.nmol = (int)mols.size() - R.mol0;
+        R.L[0] = P.boxlx;
+        R.L[1] = P.boxly;
+        R.L[2] = P.boxlz;
+        for (int d = 0; d < 3; d++) R.invL[d] = 1.0 / R.L[d];
+        R.rc_lj = P.rcutoff;
+        R.rc2_lj = P.rcutoffsq;
+        R.rc_el = P.rcutoffelec;
+        R.rc2_el = P.rcutoffelecsq;
+        double rmax = P.iChargeType == MSPMS_ELEC_NONE ? P.rcutoff
+                                                       : std::max(P.rcutoff, P.rcutoffelec);
+        R.rc_list = rmax * (1.0 + 1e-9) + 1e-9;
+        R.rcon2 = P.rcutonsq;
+        R.roff2_ron2_cube = P.roff2_minus_ron2_cube;
+        R.shift1 = P.shift1;
+        R.shift4 = P.shift4;
+        R.f0 = P.f0;
+        R.C = P.coulomb_prefactor;
+        R.kappa = P.kappa;
+        R.wvc1 = P.wolfvcon1;
+        R.wvc2 = P.wolfvcon2;
+        R.wvc3 = P.wolfvcon3;
+        R.wfc1 = P.wolffcon1;
+        R.wfc2 = P.wolffcon2;
+        R.vfactor = P.vfactor_ewald;
+        R.bfactor = P.bfactor_ewald;
+        R.twopi_l[0] = P.twopi_lx;
+        R.twopi_l[1] = P.twopi_ly;
+        R.twopi_l[2] = P.twopi_lz;
+        R.self_sum_q2 = q2;
+        R.charge_type = P.iChargeType;
+        R.switch_on = P.isLJswitchOn;
+        R.sf_type = P.iSF_type;
+        // cell grid: cubes holding ~8 real sites; columns along z
+        double vol = P.boxlx * P.boxly * P.boxlz;
+        double dens = std::max(nreal, 1) / vol;
+        double side = std::cbrt(8.0 / dens);
+        int nc[3];
+        for (int d = 0; d < 3; d++) {
+            nc[d] = std::max(1, (int)std::floor(R.L[d] / side));
+            nc[d] = std::min(nc[d], 1024);
+            R.h[d] = R.L[d] / nc[d];
+            R.inv_h[d] = 1.0 / R.h[d];
+        }
+        R.ncx = nc[0];
+        R.ncy = nc[1];
+        R.ncz = nc[2];
+        R.cell0 = cell0;
+        R.col0 = col0;
+        R.ncol = nc[0] * nc[1];
+        for (int g = 0; g < R.ncol; g++) {
+            col_cell0.push_back(col_cell0.back() + nc[2]);
+            col_rep.push_back(r);
+        }
+        cell0 += R.ncol * nc[2];
+        col0 += R.ncol;
+        max_rep_cl = std::max(max_rep_cl, (size_t)(nreal / MSPMS_CL + R.ncol + 1));
+        double reach = 2.0 * R.rc_list + 3.0 * side;
+        max_est = std::max(max_est, 1.5 * reach * reach * reach * dens / MSPMS_CL + 64.0);
+        // Ewald reciprocal rows in the reference's k order (src/frclong.c:996-1003)
+        R.kmax[0] = P.kmaxx;
+        R.kmax[1] = P.kmaxy;
+        R.kmax[2] = P.kmaxz;
+        R.ksqmax = P.ksqmax;
+        R.krow0 = (int)rows.size();
+        R.k0 = (int)kvec.size();
+        if (P.iChargeType == MSPMS_ELEC_EWALD) {
+            c->any_recip = true;
+            for (int kx = -P.kmaxx; kx <= P.kmaxx; kx++)
+                for (int ky = -P.kmaxy; ky <= P.kmaxy; ky++) {
+                    KRow row{};
+                    row.kx = kx;
+                    row.ky = ky;
+                    row.rep = r;
+                    int n = 0;
+                    for (int kz = -P.kmaxz; kz <= P.kmaxz; kz++) {
+                        int ksq = kx * kx + ky * ky + kz * kz;
+                        bool ok = ksq <= P.ksqmax && ksq != 0;
+                        if (ok) {
+                            if (n == 0) {
+                                row.kz_lo = kz;
+                                row.k0 = (int)kvec.size();
+                            }
+                            double rkx = P.twopi_lx * kx, rky = P.twopi_ly * ky,
+                                   rkz = P.twopi_lz * kz;
+                            volatile double rksq = rkx * rkx + rky * rky + rkz * rkz;
+                            kvec.push_back(std::exp(-P.bfactor_ewald * rksq) / rksq);
+                            k_rep.push_back(r);
+                            row.kz_hi = kz;
+                            n++;
+                        }
+                        // close the row at a gap (kz = 0 of the (0,0) row)
+                        bool next_ok = false;
+                        if (kz < P.kmaxz) {
+                            int k2 = kx * kx + ky * ky + (kz + 1) * (kz + 1);
+                            next_ok = k2 <= P.ksqmax && k2 != 0;
+                        }
+                        if (n > 0 && !next_ok) {
+                            row.nk = n;
+                            rows.push_back(row);
+                            n = 0;
+                        }
+                    }
+                }
+            R.nkrow = (int)rows.size() - R.krow0;
+            R.nk = (int)kvec.size() - R.k0;
+            nk_stride = std::max(nk_stride, R.nk);
+            ntab_max = std::max(ntab_max, P.kmaxx + P.kmaxy + std::max(P.kmaxz, 1) + 3);
+            nkz_max = std::max(nkz_max, 2 * P.kmaxz + 1);
+            // S pass: rows sorted by length; one workgroup per 64-atom chunk
+            std::vector<KRow> rs(rows.begin() + R.krow0, rows.end());
+            std::stable_sort(rs.begin(), rs.end(),
+                             [](const KRow& a, const KRow& b) { return a.nk > b.nk; });
+            rows_s.insert(rows_s.end(), rs.begin(), rs.end());
+            int nchunk = (R.natom + 63) / 64;
+            for (int ch = 0; ch < nchunk; ch++) {
+                int a0 = R.atom0 + 64 * ch;
+                work_s.push_back(make_int4(r, rep_chunk0.back() + ch, a0,
+                                           std::min(64, R.atom0 + R.natom - a0)));
+            }
+            rep_chunk0.push_back(rep_chunk0.back() + nchunk);
+        } else {
+            rep_chunk0.push_back(rep_chunk0.back());
+        }
+        if (P.iSF_type != MSPMS_SF_NONE) c->any_sf = true;
+    }
+    if (c->any_recip) {
+        // force pass: (atom block of 64) x (row group), ~2048 waves in total
+        long blocks = 0;
+        for (int r = 0; r < c->nrep; r++) blocks += (reps[r].natom + 63) / 64;
+        for (int r = 0; r < c->nrep; r++) {
+            const RepDev& R = reps[r];
+            int nb = (R.natom + 63) / 64;
+            int groups = (int)std::max(1L, std::min((long)R.nkrow, (2048L + blocks - 1) / std::max(blocks, 1L)));
+            // split rows into `groups` ranges of about equal k count
+            std::vector<int> cut(1, R.krow0);
+            double per = (double)R.nk / groups, acc = 0.0;
+            for (int i = R.krow0; i < R.krow0 + R.nkrow; i++) {
+                acc += rows[i].nk;
+                if (acc >= per * (double)cut.size() && (int)cut.size() < groups && i + 1 < R.krow0 + R.nkrow)
+                    cut.push_back(i + 1);
+            }
+            cut.push_back(R.krow0 + R.nkrow);
+            for (int b = 0; b < nb; b++)
+                for (size_t g = 0; g + 1 < cut.size(); g++)
+                    if (cut[g + 1] > cut[g])
+                        work_f.push_back(make_int4(r, R.atom0 + 64 * b, cut[g], cut[g + 1]));
+        }
+    }
+    c->natom = (int)atom_mol.size();
+    c->nmol = (int)mols.size();
+    c->reps = reps;
+
+    int rc = 0;
+    if ((rc = c->d_sites.upload(sites, s)) || (rc = c->d_species.upload(spd, s)) ||
+        (rc = c->d_terms.upload(terms, s)) || (rc = c->d_mols.upload(mols, s)) ||
+        (rc = c->d_atom_mol.upload(atom_mol, s)) || (rc = c->d_atom_site.upload(atom_site, s)) ||
+        (rc = c->d_reps.upload(reps, s)))
+        return rc;
+
+    // ---- clustering buffers ----
+    ClusterBufs& b = c->cb;
+    int ncell = cell0, ncol = col0;
+    size_t maxcl = (size_t)(c->natom / MSPMS_CL + ncol + 1);
+    int cap = (int)std::min<double>((double)max_rep_cl, std::max(max_est, (double)c->cap_hint));
+    cap = std::max(cap, 16);
+    if ((rc = c->d_atom_cell.ensure(c->natom)) || (rc = c->d_atom_slot.ensure(c->natom)) ||
+        (rc = c->d_atom_wz.ensure(c->natom)) || (rc = c->d_cell_count.ensure(ncell)) ||
+        (rc = c->d_cell_start.ensure(ncell + 1)) || (rc = c->d_cell_members.ensure(c->natom)) ||
+        (rc = c->d_col_cell0.upload(col_cell0, s)) || (rc = c->d_col_rep.upload(col_rep, s)) ||
+        (rc = c->d_col_cl0.ensure(ncol + 1)) || (rc = c->d_cl_col.ensure(maxcl)) ||
+        (rc = c->d_ncl_total.ensure(1)) || (rc = c->d_slot_orig.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_info.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_x64.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_w64.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
+        (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
+        (rc = c->d_plist_n.ensure(maxcl)))
+        return rc;
+    HIPC(hipMemsetAsync(c->d_cell_count.p, 0, sizeof(int) * (size_t)ncell, s));
+    b.atom_cell = c->d_atom_cell.p;
+    b.atom_slot = c->d_atom_slot.p;
+    b.atom_wz = c->d_atom_wz.p;
+    b.cell_count = c->d_cell_count.p;
+    b.cell_start = c->d_cell_start.p;
+    b.cell_members = c->d_cell_members.p;
+    b.col_cell0 = c->d_col_cell0.p;
+    b.col_rep = c->d_col_rep.p;
+    b.col_cl0 = c->d_col_cl0.p;
+    b.cl_col = c->d_cl_col.p;
+    b.ncl_total = c->d_ncl_total.p;
+    b.slot_orig = c->d_slot_orig.p;
+    b.slot_info = c->d_slot_info.p;
+    b.slot_x64 = c->d_slot_x64.p;
+    b.slot_w64 = c->d_slot_w64.p;
+    b.slot_off = c->d_slot_off.p;
+    b.cl_center = c->d_cl_center.p;
+    b.cl_ext = c->d_cl_ext.p;
+    b.plist = c->d_plist.p;
+    b.plist_n = c->d_plist_n.p;
+    b.maxcl = (int)maxcl;
+    b.cap = cap;
+    b.ncell = ncell;
+    b.ncol = ncol;
+
+    // ---- reciprocal buffers ----
+    RecipBufs& R = c->rb;
+    memset(&R, 0, sizeof R);
+    if (c->any_recip) {
+        int nchunk = rep_chunk0.back();
+        if ((rc = c->d_rows.upload(rows, s)) || (rc = c->d_rows_s.upload(rows_s, s)) ||
+            (rc = c->d_kvec.upload(kvec, s)) || (rc = c->d_k_rep.upload(k_rep, s)) ||
+            (rc = c->d_work_s.upload(work_s, s)) || (rc = c->d_work_f.upload(work_f, s)) ||
+            (rc = c->d_rep_chunk0.upload(rep_chunk0, s)) ||
+            (rc = c->d_s_part.ensure((size_t)nchunk * nk_stride)) ||
+            (rc = c->d_ak.ensure(kvec.size())) || (rc = c->d_uk.ensure(kvec.size())))
+            return rc;
+        R.rows = c->d_rows.p;
+        R.rows_s = c->d_rows_s.p;
+        R.kvec = c->d_kvec.p;
+        R.k_rep = c->d_k_rep.p;
+        R.work_s = c->d_work_s.p;
+        R.nwork_s = (int)work_s.size();
+        R.work_f = c->d_work_f.p;
+        R.nwork_f = (int)work_f.size();
+        R.rep_chunk0 = c->d_rep_chunk0.p;
+        R.s_part = c->d_s_part.p;
+        R.nk_stride = nk_stride;
+        R.ak = c->d_ak.p;
+        R.uk = c->d_uk.p;
+        R.nk_total = (int)kvec.size();
+        R.ntab_max = ntab_max;
+        R.nkz_max = nkz_max;
+    }
+    // ---- partials and outputs ----
+    if ((rc = c->d_ep_pair.ensure(maxcl * 4)) || (rc = c->d_ip_pair.ensure(maxcl * 4)) ||
+        (rc = c->d_ep_intra.ensure((size_t)c->nmol * 4)) ||
+        (rc = c->d_ep_short.ensure((size_t)c->nmol * 5)) ||
+        (rc = c->d_ep_sf.ensure((size_t)c->natom)) || (rc = c->d_long.ensure(c->nrep)) ||
+        (rc = c->d_short.ensure(c->nrep)) || (rc = c->d_pos.ensure((size_t)3 * c->natom)) ||
+        (rc = c->d_fl.ensure((size_t)3 * c->natom)) || (rc = c->d_fs.ensure((size_t)3 * c->natom)))
+        return rc;
+    HIPC(hipStreamSynchronize(s));
+    return 0;
+}
+
+int set_species(mspms_ctx* c, int nspecie, const mspms_species* sp) {
+    if (nspecie < 1 || !sp) return fail(MSPMS_EINVAL, "need at least one species");
+    std::vector<SpeciesHost> out;
+    for (int k = 0; k < nspecie; k++) {
+        const mspms_species& q = sp[k];
+        int rc = validate_species(q);
+        if (rc) return rc;
+        SpeciesHost h;
+        h.natom = q.natom;
+        h.nmole = q.nmole;
+        h.ghost = copy_arr(q.ghost_type, q.natom);
+        h.interp = copy_arr(q.interp_type, q.natom);
+        h.aw = copy_arr(q.aw, q.natom);
+        h.sigma = copy_arr(q.sigma, q.natom);
+        h.epsilon = copy_arr(q.epsilon, q.natom);
+        h.charge = copy_arr(q.charge, q.natom);
+        h.bnd_idx = copy_arr(q.bnd_idx, 2 * (size_t)q.nbond);
+        h.bnd_type = copy_arr(q.bnd_type, q.nbond);
+        h.bnd_para = copy_arr(q.bnd_para, 3 * (size_t)q.nbond);
+        h.agl_idx = copy_arr(q.agl_idx, 3 * (size_t)q.nangle);
+        h.agl_type = copy_arr(q.agl_type, q.nangle);
+        h.agl_para = copy_arr(q.agl_para, 5 * (size_t)q.nangle);
+        h.dih_idx = copy_arr(q.dih_idx, 4 * (size_t)q.ndih);
+        h.dih_type = copy_arr(q.dih_type, q.ndih);
+        h.dih_para = copy_arr(q.dih_para, 4 * (size_t)q.ndih);
+        h.imp_idx = copy_arr(q.imp_idx, 4 * (size_t)q.nimp);
+        h.imp_type = copy_arr(q.imp_type, q.nimp);
+        h.imp_para = copy_arr(q.imp_para, 2 * (size_t)q.nimp);
+        h.agl_unique.assign(std::max(q.nangle, 1), 1);
+        h.dih_unique.assign(std::max(q.ndih, 1), 1);
+        check_uniques(h.agl_idx.data(), q.nangle, h.dih_idx.data(), q.ndih, h.bnd_idx.data(),
+                      q.nbond, h.agl_unique.data(), h.dih_unique.data());
+        if (q.agl_unique) h.agl_unique.assign(q.agl_unique, q.agl_unique + q.nangle);
+        if (q.dih_unique) h.dih_unique.assign(q.dih_unique, q.dih_unique + q.ndih);
+        h.agl_unique.resize(std::max(q.nangle, 1), 1);
+        h.dih_unique.resize(std::max(q.ndih, 1), 1);
+        out.push_back(std::move(h));
+    }
+    c->sp = std::move(out);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mspms_set_system(mspms_ctx* c, int nspecie, const mspms_species* sp,
+                                const mspms_params* p) {
+    if (!c || !p) return fail(MSPMS_EINVAL, "null argument");
+    int rc = validate_params(*p);
+    if (rc) return rc;
+    if ((rc = set_species(c, nspecie, sp))) return rc;
+    c->nrep = 1;
+    c->nmole.resize(nspecie);
+    for (int k = 0; k < nspecie; k++) c->nmole[k] = sp[k].nmole;
+    c->params.assign(1, *p);
+    return build(c);
+}
+
+extern "C" int mspms_set_params(mspms_ctx* c, const mspms_params* p) {
+    if (!c || !p) return fail(MSPMS_EINVAL, "null argument");
+    if (c->sp.empty()) return fail(MSPMS_EINVAL, "mspms_set_system first");
+    int rc = validate_params(*p);
+    if (rc) return rc;
+    for (int r = 0; r < c->nrep; r++) c->params[r] = *p;
+    return build(c);
+}
+
+extern "C" int mspms_set_replicas(mspms_ctx* c, int nrep, const int* nmole,
+                                  const mspms_params* params) {
+    if (!c || nrep < 1 || !nmole || !params) return fail(MSPMS_EINVAL, "bad arguments");
+    if (c->sp.empty()) return fail(MSPMS_EINVAL, "mspms_set_system first");
+    const int ns = (int)c->sp.size();
+    for (int r = 0; r < nrep; r++) {
+        int rc = validate_params(params[r]);
+        if (rc) return rc;
+        for (int k = 0; k < ns; k++)
+            if (nmole[(size_t)r * ns + k] < 0) return fail(MSPMS_EINVAL, "negative nmole");
+    }
+    c->nrep = nrep;
+    c->nmole.assign(nmole, nmole + (size_t)nrep * ns);
+    c->params.assign(params, params + nrep);
+    return build(c);
+}
+
+extern "C" int mspms_natom(const mspms_ctx* c) { return c ? c->natom : 0; }
+extern "C" int mspms_nreplica(const mspms_ctx* c) { return c ? c->nrep : 0; }
+
+// ---- evaluation -----------------------------------------------------------
+namespace {
+
+enum { PROF_CLUSTER = 0, PROF_PAIR, PROF_INTRA, PROF_RECIP, PROF_FINAL, PROF_BONDED, PROF_N };
+
+struct ProfScope {
+    mspms_ctx* c;
+    hipStream_t s;
+    int slot;
+    ProfScope(mspms_ctx* c_, hipStream_t s_, int slot_) : c(c_), s(s_), slot(slot_) {
+        if (c->profiling) (void)hipEventRecord(c->ev[2 * slot], s);
+    }
+    ~ProfScope() {
+        if (c->profiling) (void)hipEventRecord(c->ev[2 * slot + 1], s);
+    }
+};
+
+__global__ void k_reset_status(Status* st) {
+    st->flags = 0;
+    st->max_list = 0;
+}
+
+int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
+                 double* fy, double* fz, mspms_long_terms* terms, hipStream_t s) {
+    LaunchCtx L = launch_ctx(c, s);
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
+    const int hash = c->want_hash < 0 ? (c->precision == MSPMS_PREC_FP64) : c->want_hash;
+    {
+        ProfScope p(c, s, PROF_CLUSTER);
+        launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, c->precision);
+    }
+    {
+        ProfScope p(c, s, PROF_PAIR);
+        launch_pair(L, c->cb, c->precision, c->charge_union, fx, fy, fz, c->d_ep_pair.p,
+                    c->d_ip_pair.p, hash);
+    }
+    if (c->any_intra) {
+        ProfScope p(c, s, PROF_INTRA);
+        launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p);
+    }
+    if (c->any_recip) {
+        ProfScope p(c, s, PROF_RECIP);
+        launch_recip(L, c->rb, x, y, z, fx, fy, fz);
+    }
+    {
+        ProfScope p(c, s, PROF_FINAL);
+        launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1,
+                             c->d_ep_intra.p, c->any_intra ? 1 : 0, c->rb.uk,
+                             c->any_recip ? 1 : 0, c->d_ep_sf.p, 0, terms);
+    }
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int enqueue_short(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
+                  double* fy, double* fz, mspms_short_terms* terms, hipStream_t s) {
+    LaunchCtx L = launch_ctx(c, s);
+    ProfScope p(c, s, PROF_BONDED);
+    launch_bonded(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_short.p);
+    launch_finalize_short(L, c->d_ep_short.p, terms);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int read_status(mspms_ctx* c, hipStream_t s) {
+    HIPC(hipMemcpyAsync(c->h_status, c->d_status.p, sizeof(Status), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return 0;
+}
+
+void accumulate_profile(mspms_ctx* c) {
+    if (!c->profiling) return;
+    for (int k = 0; k < PROF_N; k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess)
+            c->prof_ms[k] += ms;
+    }
+    c->prof_calls++;
+}
+
+}  // namespace
+
+extern "C" int mspms_frclong_device(mspms_ctx* c, const double* x, const double* y,
+                                    const double* z, double* fx, double* fy, double* fz,
+                                    mspms_long_terms* terms, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    return enqueue_long(c, x, y, z, fx, fy, fz, terms, s);
+}
+
+extern "C" int mspms_frcshort_device(mspms_ctx* c, const double* x, const double* y,
+                                     const double* z, double* fx, double* fy, double* fz,
+                                     mspms_short_terms* terms, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    return enqueue_short(c, x, y, z, fx, fy, fz, terms, s);
+}
+
+extern "C" int mspms_frclong_host(mspms_ctx* c, const double* xx, const double* yy,
+                                  const double* zz, double* fxl, double* fyl, double* fzl,
+                                  mspms_long_terms* terms) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!xx || !yy || !zz || !fxl || !fyl || !fzl || !terms)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const size_t n = (size_t)c->natom;
+    double* dx = c->d_pos.p;
+    double *dfx = c->d_fl.p, *dfy = dfx + n, *dfz = dfy + n;
+    HIPC(hipMemcpyAsync(dx, xx, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
+    for (int attempt = 0; attempt < 4; attempt++) {
+        int rc = enqueue_long(c, dx, dx + n, dx + 2 * n, dfx, dfy, dfz, c->d_long.p, s);
+        if (rc) return rc;
+        if ((rc = read_status(c, s))) return rc;
+        if (c->h_status->flags & ST_OVERFLOW) {
+            // pair-list row too short: grow to the largest row seen and redo
+            c->cap_hint = std::max(c->cap_hint * 2, c->h_status->max_list + 64);
+            if ((rc = build(c))) return rc;
+            continue;
+        }
+        accumulate_profile(c);
+        break;
+    }
+    HIPC(hipMemcpyAsync(fxl, dfx, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fyl, dfy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fzl, dfz, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(terms, c->d_long.p, sizeof(mspms_long_terms) * c->nrep,
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (c->h_status->flags & ST_OVERFLOW)
+        return fail(MSPMS_ENOMEM, "pair list overflow after growing");
+    if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
+    return 0;
+}
+
+extern "C" int mspms_frcshort_host(mspms_ctx* c, const double* xx, const double* yy,
+                                   const double* zz, double* fxs, double* fys, double* fzs,
+                                   mspms_short_terms* terms) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!xx || !yy || !zz || !fxs || !fys || !fzs || !terms)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const size_t n = (size_t)c->natom;
+    double* dx = c->d_pos.p;
+    double *dfx = c->d_fs.p, *dfy = dfx + n, *dfz = dfy + n;
+    HIPC(hipMemcpyAsync(dx, xx, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
+    int rc = enqueue_short(c, dx, dx + n, dx + 2 * n, dfx, dfy, dfz, c->d_short.p, s);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(fxs, dfx, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fys, dfy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fzs, dfz, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(terms, c->d_short.p, sizeof(mspms_short_terms) * c->nrep,
+                        hipMemcpyDeviceToHost, s));
+    if ((rc = read_status(c, s))) return rc;
+    if (c->h_status->flags & ST_SINGULAR)
+        return fail(MSPMS_EINVAL, "improper at a singular geometry (src/impfrc.c:96-134)");
+    return 0;
+}
+
+extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double*, const double*, const double*,
+                                       double*, double*, double*, mspms_long_terms*,
+                                       mspms_param_grad*) {
+    (void)c;
+    return fail(MSPMS_EINVAL, "parameter gradients are not built yet");
+}
+
+// ---- profiling ------------------------------------------------------------
+extern "C" int mspms_set_profiling(mspms_ctx* c, int on) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    if (on && c->ev.empty()) {
+        c->ev.resize(2 * PROF_N);
+        for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+    }
+    c->profiling = on != 0;
+    c->prof_ms.assign(PROF_N, 0.0);
+    c->prof_calls = 0;
+    return 0;
+}
+
+// ms[k] = mean milliseconds per call of kernel group k since profiling was
+// switched on: 0 clustering, 1 pair, 2 intramolecular, 3 reciprocal,
+// 4 finalize, 5 bonded.  Returns the number of groups written.
+extern "C" int mspms_last_timings(const mspms_ctx* c, double* ms, int n) {
+    if (!c || !ms) return 0;
+    int k = 0;
+    for (; k < n && k < PROF_N; k++)
+        ms[k] = c->prof_calls ? c->prof_ms[k] / c->prof_calls : 0.0;
+    return k;
+}
+
+// Synchronises the context stream, accumulates profiling and reports the
+// status of the last device-API evaluation.
+extern "C" int mspms_sync(mspms_ctx* c) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    int rc = read_status(c, c->stream);
+    if (rc) return rc;
+    accumulate_profile(c);
+    if (c->h_status->flags & ST_OVERFLOW) return fail(MSPMS_ENOMEM, "pair list overflow");
+    if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
+    if (c->h_status->flags & ST_SINGULAR) return fail(MSPMS_EINVAL, "singular improper");
+    return 0;
+}
+
+extern "C" void* mspms_stream(const mspms_ctx* c) { return c ? (void*)c->stream : nullptr; }

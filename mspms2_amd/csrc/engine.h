@@ -1,0 +1,164 @@
+// engine.h -- internal data layout shared by the host orchestration
+// (engine.cpp) and the gfx950 kernels (k_*.hip).  Not part of the C ABI.
+//
+// HBM layout (all replicas packed back to back):
+//   per atom      xx,yy,zz / fxl.. FP64 SoA (caller's arrays), atom_site,
+//                 atom_mol (int), scratch for binning
+//   per site      SiteDev table (a few hundred entries at most)
+//   per molecule  MolDev {atom0, species, rep, first site}
+//   per replica   RepDev: box, cutoffs, electrostatics constants, cell grid
+//   per cluster   8 consecutive slots of the spatially sorted arrays:
+//                 slot_orig (atom id), slot_info (mol, site),
+//                 slot_x64 (FP64 unwrapped position, exact mode),
+//                 slot_off (FP32 offset from the cluster centre + charge,
+//                 mixed mode); cl_center (FP64), cl_ext (FP32 half extent)
+//   pair list     per i-cluster a fixed-capacity row of packed j-cluster
+//                 entries (j << 7 | ambiguous << 6 | image shift bits)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "terms.h"
+
+#define MSPMS_CL 8            // atoms per cluster
+#define MSPMS_MAX_SITES 1024  // site table entries kept in LDS
+#define MSPMS_REDUCE_SLOTS 8  // per-block partial sums per kernel
+
+// energy partial-sum slots (FP64) of the frclong reduction
+enum {
+    ES_UVDW = 0, ES_USHIFT, ES_UREAL, ES_UEXCL, ES_VIR, ES_UFOURIER, ES_USF,
+    ES_NSLOT
+};
+// integer partial-sum slots (pair statistics)
+enum { IS_NLJ = 0, IS_NEL, IS_HLJ, IS_HEL, IS_NSLOT };
+
+struct SiteDev {
+    double sigma, epsilon, charge, aw;
+    double sqrt_eps;          // sqrt(epsilon), LB mixing
+    float sigma_f, sqrt_eps_f, charge_f, half_sigma_f;
+    int ghost, species, site, interp;
+};
+
+struct MolDev {
+    int atom0, species, rep, natom;
+};
+
+struct RepDev {
+    double L[3], invL[3];
+    double rc_lj, rc2_lj, rc_el, rc2_el, rc_list;
+    double rcon2, roff2_ron2_cube, shift1, shift4, f0;
+    double C;                 // coulomb_prefactor
+    double kappa;
+    double wvc1, wvc2, wvc3, wfc1, wfc2;
+    double vfactor, bfactor, twopi_l[3];
+    double self_sum_q2;       // sum of q^2 over the replica's atoms
+    int charge_type, switch_on, sf_type, pad0;
+    int atom0, natom, mol0, nmol;
+    // cell grid for clustering: columns along z, cells (cx,cy,cz)
+    int ncx, ncy, ncz, cell0;  // cell0: first global cell of the replica
+    int col0, ncol, pad1, pad2;
+    double h[3], inv_h[3];
+    // Ewald reciprocal work tables (per replica ranges)
+    int krow0, nkrow, k0, nk;
+    int kmax[3], ksqmax;
+};
+
+// Ewald k-vectors grouped in rows of fixed (kx, ky), kz contiguous
+struct KRow {
+    int kx, ky, kz_lo, kz_hi;  // kz range (inclusive); (0,0,0) excluded
+    int k0;                   // index of (kx,ky,kz_lo) in the k list
+    int rep;
+    int nk;                   // number of k in the row
+    int pad;
+};
+
+// the three groups of scratch used by one frclong evaluation
+struct ClusterBufs {
+    int* atom_cell;      // [natom]
+    int* atom_slot;      // [natom]
+    double* atom_wz;     // [natom] wrapped z
+    int* cell_count;     // [ncell]  zero between calls
+    int* cell_start;     // [ncell+1]
+    int* cell_members;   // [natom]
+    int* col_cell0;      // [ncol+1] first global cell of each column (static)
+    int* col_rep;        // [ncol]   replica of each column (static)
+    int* col_cl0;        // [ncol+1] first cluster of each column
+    int* cl_col;         // [maxcl]
+    int* ncl_total;      // [1]
+    int* slot_orig;      // [8*maxcl]
+    int2* slot_info;     // [8*maxcl] (mol, site)
+    double4* slot_x64;   // [8*maxcl] FP64 unwrapped (x,y,z,-)
+    double4* slot_w64;   // [8*maxcl] FP64 wrapped  (x,y,z,-)
+    float4* slot_off;    // [8*maxcl] FP32 offset from centre + charge
+    double4* cl_center;  // [maxcl]
+    float4* cl_ext;      // [maxcl]
+    uint32_t* plist;     // [maxcl*cap]
+    int* plist_n;        // [maxcl]
+    int maxcl, cap, ncell, ncol;
+};
+
+struct Status {
+    int flags;           // bit 0: non-finite position, bit 1: list overflow,
+                         // bit 2: grid out of range
+    int max_list;        // largest pair-list row seen
+    int pad[2];
+};
+
+#define ST_NONFINITE 1
+#define ST_OVERFLOW 2
+#define ST_GRID_EDOM 4
+
+// ---- kernel launchers (k_*.hip) ----------------------------------------
+struct LaunchCtx {
+    hipStream_t stream;
+    int natom, nmol, nsite, nrep;
+    const SiteDev* sites;
+    const MolDev* mols;
+    const int* atom_mol;
+    const int* atom_site;
+    const RepDev* reps;
+    Status* status;
+};
+
+struct RecipBufs {
+    KRow* rows;          // reference order, per replica ranges
+    KRow* rows_s;        // same rows sorted by length (S pass)
+    double* kvec;        // [nk_total] exp(-B k^2)/k^2 (host libm)
+    int* k_rep;          // [nk_total]
+    int4* work_s;        // (rep, chunk, first atom, atoms)
+    int nwork_s;
+    int4* work_f;        // (rep, first atom, first row, end row)
+    int nwork_f;
+    int* rep_chunk0;     // [nrep+1]
+    double2* s_part;     // [nchunk][nk_stride]
+    int nk_stride;
+    double2* ak;         // [nk_total] force coefficients
+    double* uk;          // [nk_total] kvec |S|^2
+    int nk_total, ntab_max, nkz_max;
+};
+
+void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x,
+                          const double* y, const double* z, double* fx,
+                          double* fy, double* fz, int precision);
+void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision,
+                 int charge_type, double* fx, double* fy, double* fz,
+                 double* epart, unsigned long long* ipart, int want_hash);
+void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
+                  const TermDev* terms, const double* x, const double* y,
+                  const double* z, double* fx, double* fy, double* fz,
+                  double* epart);
+void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
+                   const TermDev* terms, const double* x, const double* y,
+                   const double* z, double* fx, double* fy, double* fz,
+                   double* epart);
+int recip_nkz_supported(int kmaxz);
+void launch_recip(const LaunchCtx& c, const RecipBufs& rb, const double* x,
+                  const double* y, const double* z, double* fx, double* fy,
+                  double* fz);
+void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
+                          const double* ep_pair,
+                          const unsigned long long* ip_pair, int have_pair,
+                          const double* ep_intra, int have_intra,
+                          const double* uk, int have_recip,
+                          const double* ep_sf, int have_sf, void* terms);
+void launch_finalize_short(const LaunchCtx& c, const double* epart,
+                           void* terms);

@@ -1,0 +1,369 @@
+// k_cluster.hip -- spatial sort of atoms into 8-atom clusters and the
+// cluster-pair list (gfx950).
+//
+// Replaces the O(N^2) molecule-pair loop of src/frclong.c:52-220 as the way
+// candidate pairs are found; which pairs interact is still decided per pair
+// with the reference's cutoff test (k_pair.hip).
+//
+// Pipeline (one launch each, all replicas at once):
+//   k_bin      per atom: wrap into the box, integer cell (cx,cy,cz), count
+//   k_scan     one workgroup: exclusive scans over cells and columns
+//              (cells of a column are contiguous, ordered by cz)
+//   k_scatter  per atom: cell member lists
+//   k_place    per atom: exact rank inside its cell by (wrapped z, index)
+//              -> slot in its column; a column is cut into clusters of 8
+//   k_bbox     per cluster: bounding box, FP32 offsets from the centre
+//   k_plist    one wave per i-cluster: j-clusters whose boxes come within
+//              rc_list, compacted with a wave ballot (deterministic order)
+// The ranks make the final order independent of atomic arrival order.
+#include "engine.h"
+#include "../../include/mspms_gpu.h"
+
+namespace {
+
+__device__ inline double wrap(double x, double L, double invL) {
+    double w = x - L * floor(x * invL);
+    if (w >= L) w -= L;
+    if (w < 0.0) w += L;
+    return w;
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ void k_bin(int natom, const double* __restrict__ x, const double* __restrict__ y,
+                      const double* __restrict__ z, const int* __restrict__ atom_site,
+                      const int* __restrict__ atom_mol, const SiteDev* __restrict__ sites,
+                      const MolDev* __restrict__ mols, const RepDev* __restrict__ reps,
+                      int* __restrict__ atom_cell, int* __restrict__ atom_slot,
+                      double* __restrict__ atom_wz, int* __restrict__ cell_count,
+                      double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+                      Status* st) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    int s = atom_site[i];
+    if (sites[s].ghost == MSPMS_GHOST_FULL) {
+        // skipped by the pair loops (src/frclong.c:94,107); its force is
+        // built from the reciprocal and other per-atom terms only
+        atom_cell[i] = -1;
+        fx[i] = 0.0;
+        fy[i] = 0.0;
+        fz[i] = 0.0;
+        return;
+    }
+    const RepDev& R = reps[mols[atom_mol[i]].rep];
+    double xi = x[i], yi = y[i], zi = z[i];
+    if (!(isfinite(xi) && isfinite(yi) && isfinite(zi))) {
+        atomicOr(&st->flags, ST_NONFINITE);
+        xi = yi = zi = 0.0;
+    }
+    double wx = wrap(xi, R.L[0], R.invL[0]);
+    double wy = wrap(yi, R.L[1], R.invL[1]);
+    double wz = wrap(zi, R.L[2], R.invL[2]);
+    int cx = clampi((int)(wx * R.inv_h[0]), 0, R.ncx - 1);
+    int cy = clampi((int)(wy * R.inv_h[1]), 0, R.ncy - 1);
+    int cz = clampi((int)(wz * R.inv_h[2]), 0, R.ncz - 1);
+    int cell = R.cell0 + ((cy * R.ncx + cx) * R.ncz + cz);
+    atom_cell[i] = cell;
+    atom_wz[i] = wz;
+    atom_slot[i] = atomicAdd(&cell_count[cell], 1);
+}
+
+// Block-wide exclusive scan helper over 1024 threads.
+template <int NT>
+__device__ int block_exscan(int v, int* sh, int& total) {
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) sh[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        int w = (lane < NT / 64) ? sh[lane] : 0;
+        int wi = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(wi, o, 64);
+            if (lane >= o) wi += t;
+        }
+        if (lane < NT / 64) sh[lane] = wi - w;
+        if (lane == NT / 64 - 1) sh[NT / 64] = wi;
+    }
+    __syncthreads();
+    int ex = incl - v + sh[wid];
+    total = sh[NT / 64];
+    __syncthreads();
+    return ex;
+}
+
+// One workgroup: cell_start = exscan(cell_count), cell_count := 0;
+// per column the atom count, cluster count, cluster start and cl_col.
+__global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restrict__ cell_count,
+                                               int* __restrict__ cell_start,
+                                               const int* __restrict__ col_cell0,
+                                               int* __restrict__ col_cl0, int* __restrict__ cl_col,
+                                               int* __restrict__ ncl_total, int maxcl,
+                                               Status* st) {
+    __shared__ int sh[32];
+    int carry = 0;
+    for (int base = 0; base < ncell; base += 1024) {
+        int c = base + threadIdx.x;
+        int v = (c < ncell) ? cell_count[c] : 0;
+        int tot;
+        int ex = block_exscan<1024>(v, sh, tot);
+        if (c < ncell) {
+            cell_start[c] = carry + ex;
+            cell_count[c] = 0;
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) cell_start[ncell] = carry;
+    __syncthreads();
+    carry = 0;
+    for (int base = 0; base < ncol; base += 1024) {
+        int g = base + threadIdx.x;
+        int n = 0;
+        if (g < ncol) {
+            int cnt = cell_start[col_cell0[g + 1]] - cell_start[col_cell0[g]];
+            n = (cnt + MSPMS_CL - 1) / MSPMS_CL;
+        }
+        int tot;
+        int ex = block_exscan<1024>(n, sh, tot);
+        if (g < ncol) col_cl0[g] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        col_cl0[ncol] = carry;
+        *ncl_total = carry;
+        if (carry > maxcl) atomicOr(&st->flags, ST_OVERFLOW);
+    }
+    __syncthreads();
+    // cluster -> column map
+    for (int g = threadIdx.x; g < ncol; g += 1024) {
+        int a = col_cl0[g], b = col_cl0[g + 1];
+        for (int c = a; c < b && c < maxcl; c++) cl_col[c] = g;
+    }
+}
+
+__global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
+                          const int* __restrict__ atom_slot, const int* __restrict__ cell_start,
+                          int* __restrict__ cell_members) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    int c = atom_cell[i];
+    if (c < 0) return;
+    cell_members[cell_start[c] + atom_slot[i]] = i;
+}
+
+// Rank inside the cell by (wrapped z, atom index); place into the column's
+// slot range, which is 8 * (first cluster of the column) onwards.
+__global__ void k_place(int natom, const double* __restrict__ x, const double* __restrict__ y,
+                        const double* __restrict__ z, const int* __restrict__ atom_cell,
+                        const double* __restrict__ atom_wz, const int* __restrict__ cell_start,
+                        const int* __restrict__ cell_members, const int* __restrict__ atom_mol,
+                        const int* __restrict__ atom_site, const MolDev* __restrict__ mols,
+                        const RepDev* __restrict__ reps, const int* __restrict__ col_cell0,
+                        const int* __restrict__ col_cl0, int* __restrict__ slot_orig,
+                        int2* __restrict__ slot_info, double4* __restrict__ slot_x64,
+                        double4* __restrict__ slot_w64, int maxcl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    int c = atom_cell[i];
+    if (c < 0) return;
+    const RepDev& R = reps[mols[atom_mol[i]].rep];
+    double wz = atom_wz[i];
+    int a = cell_start[c], b = cell_start[c + 1];
+    int rank = 0;
+    for (int m = a; m < b; m++) {
+        int j = cell_members[m];
+        double wj = atom_wz[j];
+        rank += (wj < wz) || (wj == wz && j < i);
+    }
+    // column of this cell (cells of a replica: cell0 + col_local*ncz + cz)
+    int local = c - R.cell0;
+    int col = R.col0 + local / R.ncz;
+    int slot = MSPMS_CL * col_cl0[col] + (a - cell_start[col_cell0[col]]) + rank;
+    if (slot >= MSPMS_CL * maxcl) return;  // overflow flagged by k_scan
+    double xi = x[i], yi = y[i], zi = z[i];
+    if (!(isfinite(xi) && isfinite(yi) && isfinite(zi))) xi = yi = zi = 0.0;
+    slot_orig[slot] = i;
+    slot_info[slot] = make_int2(atom_mol[i], atom_site[i]);
+    slot_x64[slot] = make_double4(x[i], y[i], z[i], 0.0);
+    slot_w64[slot] = make_double4(wrap(xi, R.L[0], R.invL[0]), wrap(yi, R.L[1], R.invL[1]), wz,
+                                  0.0);
+}
+
+// 8 lanes per cluster: bounding box, centre, FP32 offsets; pads marked.
+__global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ cl_col,
+                       const int* __restrict__ col_cl0, const int* __restrict__ col_cell0,
+                       const int* __restrict__ cell_start, const int* __restrict__ col_rep,
+                       const SiteDev* __restrict__ sites, int* __restrict__ slot_orig,
+                       int2* __restrict__ slot_info, const double4* __restrict__ slot_w64,
+                       float4* __restrict__ slot_off, double4* __restrict__ cl_center,
+                       float4* __restrict__ cl_ext) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = t >> 3, k = t & 7;
+    int ncl = min(*ncl_total, maxcl);
+    bool live = c < ncl;
+    int col = live ? cl_col[c] : 0;
+    int cnt = live ? (cell_start[col_cell0[col + 1]] - cell_start[col_cell0[col]]) : 0;
+    int in_cl = live ? min(MSPMS_CL, cnt - MSPMS_CL * (c - col_cl0[col])) : 0;
+    bool valid = live && k < in_cl;
+    int slot = MSPMS_CL * c + k;
+    double4 w = valid ? slot_w64[slot] : make_double4(0, 0, 0, 0);
+    double lo[3], hi[3];
+    const double BIG = 1e300;
+    lo[0] = valid ? w.x : BIG;
+    lo[1] = valid ? w.y : BIG;
+    lo[2] = valid ? w.z : BIG;
+    hi[0] = valid ? w.x : -BIG;
+    hi[1] = valid ? w.y : -BIG;
+    hi[2] = valid ? w.z : -BIG;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    }
+    if (!live) return;
+    double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    if (k == 0) {
+        cl_center[c] = make_double4(cx, cy, cz, (double)col_rep[col]);
+        cl_ext[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.000001f + 1e-6f,
+                                (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
+                                (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
+    }
+    if (valid) {
+        int s = slot_info[slot].y;
+        slot_off[slot] = make_float4((float)(w.x - cx), (float)(w.y - cy), (float)(w.z - cz),
+                                     sites[s].charge_f);
+    } else {
+        slot_orig[slot] = -1;
+        slot_info[slot] = make_int2(-1, -1);
+        slot_off[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// One wave per i-cluster.  Candidate columns cover the x/y range of the
+// i-box widened by rc_list (periodic, each wrapped column at most once);
+// every cluster of a candidate column is tested box-to-box.  Entry layout:
+//   bits 0-1 x image + 1, 2-3 y image + 1, 4-5 z image + 1,
+//   bit 6 "ambiguous image: apply the minimum image per pair",
+//   bits 7-31 j-cluster index.
+__global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total, int maxcl,
+                                               const int* __restrict__ cl_col,
+                                               const int* __restrict__ col_cl0,
+                                               const int* __restrict__ col_rep,
+                                               const RepDev* __restrict__ reps,
+                                               const double4* __restrict__ cl_center,
+                                               const float4* __restrict__ cl_ext,
+                                               uint32_t* __restrict__ plist,
+                                               int* __restrict__ plist_n, int cap, Status* st) {
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int lane = threadIdx.x & 63;
+    int ncl = min(*ncl_total, maxcl);
+    if (wave >= ncl) return;
+    int ci = wave;
+    double4 Ci = cl_center[ci];
+    float4 Ei = cl_ext[ci];
+    const RepDev& R = reps[(int)Ci.w];
+    const double rl = R.rc_list;
+    int colx_lo = (int)floor((Ci.x - Ei.x - rl) * R.inv_h[0]);
+    int colx_hi = (int)floor((Ci.x + Ei.x + rl) * R.inv_h[0]);
+    int coly_lo = (int)floor((Ci.y - Ei.y - rl) * R.inv_h[1]);
+    int coly_hi = (int)floor((Ci.y + Ei.y + rl) * R.inv_h[1]);
+    // a range as wide as the box visits each wrapped column once; the image
+    // is then not fixed per column and the pair kernel takes it per pair
+    bool ambx = false, amby = false;
+    if (colx_hi - colx_lo + 1 >= R.ncx) {
+        colx_lo = 0;
+        colx_hi = R.ncx - 1;
+        ambx = true;
+    }
+    if (coly_hi - coly_lo + 1 >= R.ncy) {
+        coly_lo = 0;
+        coly_hi = R.ncy - 1;
+        amby = true;
+    }
+    const double rl2 = rl * rl;
+    int n = 0;
+    uint32_t* row = plist + (size_t)ci * cap;
+    for (int cyu = coly_lo; cyu <= coly_hi; cyu++) {
+        int cyw = ((cyu % R.ncy) + R.ncy) % R.ncy;
+        int sy = (cyu - cyw) / R.ncy;
+        for (int cxu = colx_lo; cxu <= colx_hi; cxu++) {
+            int cxw = ((cxu % R.ncx) + R.ncx) % R.ncx;
+            int sx = (cxu - cxw) / R.ncx;
+            int col = R.col0 + cyw * R.ncx + cxw;
+            int a = col_cl0[col], b = col_cl0[col + 1];
+            for (int base = a; base < b; base += 64) {
+                int cj = base + lane;
+                bool hit = false;
+                uint32_t entry = 0;
+                if (cj < b) {
+                    double4 Cj = cl_center[cj];
+                    float4 Ej = cl_ext[cj];
+                    double dxc = Ci.x - (Cj.x + sx * R.L[0]);
+                    double dyc = Ci.y - (Cj.y + sy * R.L[1]);
+                    double dzc = Ci.z - Cj.z;
+                    double nz = rint(dzc * R.invL[2]);
+                    dzc -= nz * R.L[2];
+                    bool ambz = (double)Ei.z + (double)Ej.z + rl >= 0.5 * R.L[2];
+                    double gx = ambx ? 0.0 : fmax(0.0, fabs(dxc) - ((double)Ei.x + (double)Ej.x));
+                    double gy = amby ? 0.0 : fmax(0.0, fabs(dyc) - ((double)Ei.y + (double)Ej.y));
+                    double gz = ambz ? 0.0 : fmax(0.0, fabs(dzc) - ((double)Ei.z + (double)Ej.z));
+                    hit = gx * gx + gy * gy + gz * gz < rl2;
+                    int szi = (int)nz;  // j image: Cj + (sx, sy, sz) * L
+                    bool a2 = ambx || amby || ambz || sx < -1 || sx > 1 || sy < -1 || sy > 1 ||
+                              szi < -1 || szi > 1;
+                    if (a2) szi = 0;
+                    entry = ((uint32_t)cj << 7) | ((a2 ? 1u : 0u) << 6) |
+                            ((uint32_t)(szi + 1) << 4) |
+                            ((uint32_t)((a2 ? 0 : sy) + 1) << 2) | (uint32_t)((a2 ? 0 : sx) + 1);
+                }
+                uint64_t mask = __ballot(hit);
+                int pre = __popcll(mask & ((1ull << lane) - 1ull));
+                if (hit && n + pre < cap) row[n + pre] = entry;
+                n += __popcll(mask);
+            }
+        }
+    }
+    if (lane == 0) {
+        plist_n[ci] = min(n, cap);
+        if (n > cap) atomicOr(&st->flags, ST_OVERFLOW);
+        atomicMax(&st->max_list, n);
+    }
+}
+
+}  // namespace
+
+void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, const double* y,
+                          const double* z, double* fx, double* fy, double* fz, int precision) {
+    (void)precision;
+    const int T = 256;
+    int gA = (c.natom + T - 1) / T;
+    hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
+                       c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
+                       b.cell_count, fx, fy, fz, c.status);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, b.ncell, b.ncol, b.cell_count,
+                       b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total, b.maxcl,
+                       c.status);
+    hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
+                       b.atom_slot, b.cell_start, b.cell_members);
+    hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,
+                       b.atom_wz, b.cell_start, b.cell_members, c.atom_mol, c.atom_site, c.mols,
+                       c.reps, b.col_cell0, b.col_cl0, b.slot_orig, b.slot_info, b.slot_x64,
+                       b.slot_w64, b.maxcl);
+    int gB = (b.maxcl * MSPMS_CL + T - 1) / T;
+    hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
+                       b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,
+                       b.slot_info, b.slot_w64, b.slot_off, b.cl_center, b.cl_ext);
+    int gP = (b.maxcl * 64 + T - 1) / T;
+    hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
+                       b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,
+                       b.cap, c.status);
+}

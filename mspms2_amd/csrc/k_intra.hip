@@ -1,0 +1,363 @@
+// k_intra.hip -- per-molecule kernels (gfx950), FP64 in every precision
+// mode:
+//   k_intra          intramolecular corrections of frclong: bonded 1-2 and
+//                    unique 1-3 ending pairs (Ewald exclusion + real space,
+//                    src/frclong.c:234-593), unique 1-4 pairs (f0-scaled LJ +
+//                    real space, :597-787)
+//   k_bonded         frcshort: bonds, angles, dihedrals, impropers
+//                    (src/frcshort.c:20-97 and the term files)
+//   k_finalize_long  per-replica deterministic reduction of every partial
+//   k_finalize_short   sum and the reference's energy bookkeeping
+//                    (src/frclong.c:985, :1048-1061, :1128, :1142, :1223)
+// One thread owns one molecule and therefore its atoms: forces are
+// read-modify-written without atomics, in the reference's term order.
+// These terms are O(N_bonded) and carry the largest cancellations of the
+// electrostatics (exclusion vs real space), so they stay FP64.
+#include "engine.h"
+#include "physics.h"
+#include "terms.h"
+#include "../../include/mspms_gpu.h"
+
+namespace {
+
+struct IntraAcc {
+    double uvdw, ureal, uexcl, vir;
+};
+
+__device__ inline double mic(double d, double L) {
+#pragma clang fp contract(off)
+    return d - L * rint(d / L);
+}
+
+__device__ inline void add_pair(double* fx, double* fy, double* fz, int ii, int jj, double f,
+                                double rx, double ry, double rz) {
+#pragma clang fp contract(off)
+    double ax = f * rx, ay = f * ry, az = f * rz;
+    fx[ii] += ax;
+    fy[ii] += ay;
+    fz[ii] += az;
+    fx[jj] -= ax;
+    fy[jj] -= ay;
+    fz[jj] -= az;
+}
+
+__device__ inline void intra_lj(const RepDev& R, const SiteDev& a, const SiteDev& b, int ii,
+                                int jj, double rx, double ry, double rz, double escale,
+                                IntraAcc& acc, double* fx, double* fy, double* fz) {
+#pragma clang fp contract(off)
+    double rsq = rx * rx + ry * ry + rz * rz;
+    if (!(rsq < R.rc2_lj)) return;
+    double u, f, ush;
+    lj64(rsq, 0.5 * (a.sigma + b.sigma), escale * sqrt(a.epsilon * b.epsilon), R.switch_on,
+         R.rc2_lj, R.rcon2, R.roff2_ron2_cube, R.shift1, &u, &f, &ush);
+    acc.uvdw += u;
+    acc.vir += f * rsq;
+    add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+}
+
+// electrostatics of an intramolecular pair (src/frclong.c:316-408 with
+// with_excl, :708-786 without)
+__device__ inline void intra_elec(const RepDev& R, double qq, int ii, int jj, double rx,
+                                  double ry, double rz, bool with_excl, IntraAcc& acc,
+                                  double* fx, double* fy, double* fz) {
+#pragma clang fp contract(off)
+    const int ct = R.charge_type;
+    if (ct != MSPMS_ELEC_EWALD && ct != MSPMS_ELEC_WOLF) return;
+    double rsq, u, f;
+    if (with_excl) {
+        rsq = rx * rx + ry * ry + rz * rz;
+        if (rsq < R.rc2_el) {
+            ewald_excl64(rsq, qq, R.C, &u, &f);
+            acc.uexcl += u;
+            if (ct == MSPMS_ELEC_WOLF) acc.vir += f * rsq;
+            add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+        }
+    }
+    rx = mic(rx, R.L[0]);
+    ry = mic(ry, R.L[1]);
+    rz = mic(rz, R.L[2]);
+    rsq = rx * rx + ry * ry + rz * rz;
+    if (rsq < R.rc2_el) {
+        if (ct == MSPMS_ELEC_EWALD) {
+            ewald_real64(rsq, qq, R.C, R.kappa, &u, &f);
+        } else {
+            wolf_real64(rsq, qq, R.C, R.kappa, R.rc_el, R.wvc1, R.wvc2, R.wfc1, R.wfc2, &u, &f);
+            acc.vir += f * rsq;
+        }
+        acc.ureal += u;
+        add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+    }
+}
+
+__device__ inline bool is_long(const RepDev& R, double rx, double ry, double rz) {
+    return fabs(rx) > R.L[0] * 0.5 || fabs(ry) > R.L[1] * 0.5 || fabs(rz) > R.L[2] * 0.5;
+}
+
+__global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restrict__ mols,
+                                               const RepDev* __restrict__ reps,
+                                               const SiteDev* __restrict__ sites,
+                                               const SpeciesDev* __restrict__ species,
+                                               const TermDev* __restrict__ terms,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z, double* fx,
+                                               double* fy, double* fz, double* epart) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const MolDev M = mols[m];
+    const SpeciesDev S = species[M.species];
+    const RepDev& R = reps[M.rep];
+    IntraAcc acc = {0.0, 0.0, 0.0, 0.0};
+    bool same = true;  // isSameMole carried into the 1-4 section
+    const int base = M.atom0;
+    for (int t = S.term0; t < S.term0 + S.nintra; t++) {
+        const TermDev T = terms[t];
+        const int mm = T.idx[0], nn = T.idx[1];
+        const SiteDev a = sites[S.site0 + mm], b = sites[S.site0 + nn];
+        const int ii = base + mm, jj = base + nn;
+        double rx, ry, rz;
+        {
+#pragma clang fp contract(off)
+            rx = x[ii] - x[jj];
+            ry = y[ii] - y[jj];
+            rz = z[ii] - z[jj];
+        }
+        const double qq = a.charge * b.charge;
+        const bool lng = is_long(R, rx, ry, rz);
+        if (T.kind != TERM_PAIR14) {
+            // 1-2 / 1-3 ending pair (src/frclong.c:254-409, :438-593)
+            same = !lng;
+            if (lng) {
+                if (a.ghost == MSPMS_GHOST_FULL || b.ghost == MSPMS_GHOST_FULL) continue;
+                double mx = mic(rx, R.L[0]), my = mic(ry, R.L[1]), mz = mic(rz, R.L[2]);
+                if (a.ghost != MSPMS_GHOST_LJ && b.ghost != MSPMS_GHOST_LJ)
+                    intra_lj(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz);
+                // restored through int variables (src/frclong.c:223): truncation
+                rx = (double)(int)rx;
+                ry = (double)(int)ry;
+                rz = (double)(int)rz;
+            }
+            intra_elec(R, qq, ii, jj, rx, ry, rz, true, acc, fx, fy, fz);
+        } else {
+            // 1-4 ending pair (src/frclong.c:603-787); `same` is whatever the
+            // bond / angle sections left (:622, :632 compare, not assign)
+            if (lng && (a.ghost == MSPMS_GHOST_FULL || b.ghost == MSPMS_GHOST_FULL)) continue;
+            if (!same) {
+                double mx = mic(rx, R.L[0]), my = mic(ry, R.L[1]), mz = mic(rz, R.L[2]);
+                if (a.ghost != MSPMS_GHOST_LJ && b.ghost != MSPMS_GHOST_LJ)
+                    intra_lj(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz);
+                rx = (double)(int)rx;
+                ry = (double)(int)ry;
+                rz = (double)(int)rz;
+            } else {
+                intra_lj(R, a, b, ii, jj, rx, ry, rz, R.f0, acc, fx, fy, fz);
+            }
+            intra_elec(R, qq, ii, jj, rx, ry, rz, false, acc, fx, fy, fz);
+        }
+    }
+    double* ep = epart + (size_t)m * 4;
+    ep[0] = acc.uvdw;
+    ep[1] = acc.ureal;
+    ep[2] = acc.uexcl;
+    ep[3] = acc.vir;
+}
+
+__global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restrict__ mols,
+                                                const SpeciesDev* __restrict__ species,
+                                                const TermDev* __restrict__ terms,
+                                                const double* __restrict__ x,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ z, double* fx,
+                                                double* fy, double* fz, double* epart,
+                                                Status* st) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const MolDev M = mols[m];
+    const SpeciesDev S = species[M.species];
+    const int base = M.atom0;
+    for (int a = 0; a < M.natom; a++) {
+        fx[base + a] = 0.0;
+        fy[base + a] = 0.0;
+        fz[base + a] = 0.0;
+    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
+        const TermDev T = terms[t];
+        double u = 0.0, v = 0.0;
+        int cls = bonded_term(T, base, x, y, z, fx, fy, fz, &u, &v, &st->flags);
+        e[cls] += u;
+        vir += v;
+    }
+    double* ep = epart + (size_t)m * 5;
+    ep[0] = e[0];
+    ep[1] = e[1];
+    ep[2] = e[2];
+    ep[3] = e[3];
+    ep[4] = vir;
+}
+
+// ---- deterministic block reductions ------------------------------------
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = v[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sh[wid * NV + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += sh[w * NV + k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <int NV>
+__device__ void block_sum_u64(unsigned long long (&v)[NV], unsigned long long* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        unsigned long long s = v[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sh[wid * NV + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; k++) {
+            unsigned long long s = 0;
+            for (int w = 0; w < nw; w++) s += sh[w * NV + k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup per replica.
+__global__ void __launch_bounds__(256) k_finalize_long(
+    const RepDev* __restrict__ reps, const int* __restrict__ col_cl0,
+    const double* __restrict__ ep_pair, const unsigned long long* __restrict__ ip_pair,
+    int have_pair, const double* __restrict__ ep_intra, int have_intra,
+    const double* __restrict__ uk, int have_recip, const double* __restrict__ ep_sf, int have_sf,
+    mspms_long_terms* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double sh[4 * 8];
+    __shared__ unsigned long long shu[4 * 8];
+    const RepDev& R = reps[blockIdx.x];
+    double p[4] = {0, 0, 0, 0};
+    unsigned long long q[4] = {0, 0, 0, 0};
+    if (have_pair) {
+        int c0 = col_cl0[R.col0], c1 = col_cl0[R.col0 + R.ncol];
+        for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+            for (int k = 0; k < 4; k++) {
+                p[k] += ep_pair[(size_t)c * 4 + k];
+                q[k] += ip_pair[(size_t)c * 4 + k];
+            }
+        }
+    }
+    block_sum<4>(p, sh);
+    block_sum_u64<4>(q, shu);
+    double w[4] = {0, 0, 0, 0};
+    if (have_intra) {
+        for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x)
+            for (int k = 0; k < 4; k++) w[k] += ep_intra[(size_t)m * 4 + k];
+    }
+    block_sum<4>(w, sh);
+    double f[1] = {0.0};
+    if (have_recip)
+        for (int k = R.k0 + threadIdx.x; k < R.k0 + R.nk; k += blockDim.x) f[0] += uk[k];
+    block_sum<1>(f, sh);
+    double sf[1] = {0.0};
+    if (have_sf)
+        for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x)
+            sf[0] += ep_sf[a];
+    block_sum<1>(sf, sh);
+    if (threadIdx.x != 0) return;
+    mspms_long_terms t;
+    // full lists visit every inter-molecular pair twice
+    t.uvdw = 0.5 * p[0] + w[0];
+    t.ushift = (0.5 * p[1]) * R.shift4;   // src/frclong.c:985
+    t.ureal = 0.5 * p[2] + w[1];
+    t.uexcl = w[2];
+    double vir = 0.5 * p[3] + w[3];
+    t.ufourier = 0.0;
+    t.uself = 0.0;
+    t.uvacuum = 0.0;
+    t.uelec = 0.0;
+    if (R.charge_type == MSPMS_ELEC_EWALD) {
+        t.ufourier = f[0] * (R.vfactor * R.C);                        // :1048
+        t.uself = R.self_sum_q2 * (R.C * sqrt(R.kappa * R.kappa / MSPMS_PI));  // :1058
+        t.uelec = t.ureal + t.ufourier - t.uself - t.uexcl;           // :1061
+        vir += t.uelec;                                               // :1128
+    } else if (R.charge_type == MSPMS_ELEC_WOLF) {
+        t.uself = R.self_sum_q2 * (R.C * R.wvc3);                     // :1140
+        t.uelec = t.ureal - t.uself - t.uexcl;                        // :1142
+    }
+    t.usflj = sf[0];
+    t.uotherff = 0.0;
+    t.uinter = t.uvdw + t.uelec + t.usflj + t.uotherff;               // :1223
+    t.virial_inter = vir;
+    t.n_lj_pairs = (int64_t)q[0];
+    t.n_elec_pairs = (int64_t)q[1];
+    t.lj_pair_hash = q[2];
+    t.elec_pair_hash = q[3];
+    out[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_finalize_short(const RepDev* __restrict__ reps,
+                                                        const double* __restrict__ ep,
+                                                        mspms_short_terms* __restrict__ out) {
+    __shared__ double sh[5 * 8];
+    const RepDev& R = reps[blockIdx.x];
+    double p[5] = {0, 0, 0, 0, 0};
+    for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x)
+        for (int k = 0; k < 5; k++) p[k] += ep[(size_t)m * 5 + k];
+    block_sum<5>(p, sh);
+    if (threadIdx.x != 0) return;
+    mspms_short_terms t;
+    t.ubond = p[0];
+    t.uangle = p[1];
+    t.udih = p[2];
+    t.uimp = p[3];
+    t.uintra = t.ubond + t.uangle + t.udih + t.uimp;  // src/frcshort.c:87
+    t.virial_intra = p[4];
+    out[blockIdx.x] = t;
+}
+
+}  // namespace
+
+void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
+                  const double* x, const double* y, const double* z, double* fx, double* fy,
+                  double* fz, double* epart) {
+    int g = (c.nmol + 255) / 256;
+    hipLaunchKernelGGL(k_intra, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps, c.sites,
+                       species, terms, x, y, z, fx, fy, fz, epart);
+}
+
+void launch_bonded(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
+                   const double* x, const double* y, const double* z, double* fx, double* fy,
+                   double* fz, double* epart) {
+    int g = (c.nmol + 255) / 256;
+    hipLaunchKernelGGL(k_bonded, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, species, terms,
+                       x, y, z, fx, fy, fz, epart, c.status);
+}
+
+void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* ep_pair,
+                          const unsigned long long* ip_pair, int have_pair,
+                          const double* ep_intra, int have_intra, const double* uk,
+                          int have_recip, const double* ep_sf, int have_sf, void* terms) {
+    hipLaunchKernelGGL(k_finalize_long, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, col_cl0,
+                       ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip, ep_sf,
+                       have_sf, (mspms_long_terms*)terms);
+}
+
+void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms) {
+    hipLaunchKernelGGL(k_finalize_short, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, epart,
+                       (mspms_short_terms*)terms);
+}

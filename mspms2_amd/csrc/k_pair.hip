@@ -1,0 +1,337 @@
+// k_pair.hip -- inter-molecular nonbonded pairs on gfx950: LJ + Ewald real
+// space or Wolf, per cluster pair.  Reference: the atom-pair body of
+// src/frclong.c:105-196 (ljfrc, ewald_real_frc, wolf_real_frc).
+//
+// One wave64 per i-cluster walks its row of the cluster-pair list.  Lane
+// (i = lane>>3, j = lane&7) evaluates one site pair per j-cluster; each i's
+// force is reduced over its 8 lanes with wave shuffles at the end, and the
+// energy / virial partials over the whole wave.  The lists are full (both
+// i->j and j->i), so every atom's force is written once, without atomics,
+// in a fixed order: the result is deterministic.  Energies and virial are
+// summed twice and halved in the finalize kernel; pair counts and hashes are
+// taken on i<j only.
+//
+// Precision modes:
+//   FP64   separation from the unwrapped FP64 positions with the reference's
+//          minimum image (d - L*rint(d/L), true division, no contraction),
+//          reference-order FP64 terms: the (i<j, r<rc) pair sets are those of
+//          src/frclong.c:114-125 bit for bit.
+//   MIXED  separation = (FP32 offset_i - offset_j) + (cluster-centre delta
+//          and image shift, FP64 rounded once to FP32); FP32 terms with the
+//          polynomial erfc; FP64 accumulation of forces, energies, virial.
+#include "engine.h"
+#include "physics.h"
+#include "../../include/mspms_gpu.h"
+
+namespace {
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double sum8(double v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+// XCD-aware block order: consecutive i-clusters (spatial neighbours that
+// share j-clusters) land on the same XCD and its L2.
+__device__ inline int xcd_block(int b, int nb) {
+    const int NX = 8;
+    int per = (nb + NX - 1) / NX;
+    int xcd = b % NX, k = b / NX;
+    int lb = xcd * per + k;
+    return lb;
+}
+
+template <int ELEC, bool FP64>
+__global__ void __launch_bounds__(256) k_pair(
+    const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
+    const SiteDev* __restrict__ sites, const double4* __restrict__ cl_center,
+    const uint32_t* __restrict__ plist, const int* __restrict__ plist_n, int cap,
+    const int* __restrict__ slot_orig, const int2* __restrict__ slot_info,
+    const double4* __restrict__ slot_x64, const float4* __restrict__ slot_off,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash) {
+    const int nb = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, nb);
+    const int ci = lb * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ncl = min(*ncl_total, maxcl);
+    if (lb >= nb || ci >= ncl) return;
+    const int isub = lane >> 3, jsub = lane & 7;
+
+    const double4 Ci = cl_center[ci];
+    const RepDev& R = reps[(int)Ci.w];
+    const int islot = ci * MSPMS_CL + isub;
+    const int2 infoi = slot_info[islot];
+    const bool vi = infoi.y >= 0;
+    const int oi = slot_orig[islot];
+    SiteDev si;
+    if (vi) si = sites[infoi.y];
+    const bool lji = vi && si.ghost != MSPMS_GHOST_LJ;
+
+    double afx = 0.0, afy = 0.0, afz = 0.0;
+    double e_lj = 0.0, e_sh = 0.0, e_re = 0.0, vir = 0.0;
+    unsigned long long n_lj = 0, n_el = 0, h_lj = 0, h_el = 0;
+
+    const int nrow = plist_n[ci];
+    const uint32_t* row = plist + (size_t)ci * cap;
+
+    if (FP64) {
+        const double4 pi4 = slot_x64[islot];
+        for (int e = 0; e < nrow; e++) {
+            const uint32_t ent = row[e];
+            const int cj = (int)(ent >> 7);
+            const int jslot = cj * MSPMS_CL + jsub;
+            const int2 infoj = slot_info[jslot];
+            if (!(vi && infoj.y >= 0 && infoj.x != infoi.x)) continue;
+            const double4 pj4 = slot_x64[jslot];
+            double rx, ry, rz, rsq;
+            {
+#pragma clang fp contract(off)
+                rx = pi4.x - pj4.x;
+                ry = pi4.y - pj4.y;
+                rz = pi4.z - pj4.z;
+                rx = rx - R.L[0] * rint(rx / R.L[0]);
+                ry = ry - R.L[1] * rint(ry / R.L[1]);
+                rz = rz - R.L[2] * rint(rz / R.L[2]);
+                rsq = rx * rx + ry * ry + rz * rz;
+            }
+            const SiteDev sj = sites[infoj.y];
+            const int oj = slot_orig[jslot];
+            const bool count = oi < oj;
+            double fij_t = 0.0;
+            if (lji && sj.ghost != MSPMS_GHOST_LJ && rsq < R.rc2_lj) {
+                double u, f, ush;
+                double sij, eij;
+                {
+#pragma clang fp contract(off)
+                    sij = 0.5 * (si.sigma + sj.sigma);
+                    eij = sqrt(si.epsilon * sj.epsilon);
+                }
+                lj64(rsq, sij, eij, R.switch_on, R.rc2_lj, R.rcon2, R.roff2_ron2_cube, R.shift1,
+                     &u, &f, &ush);
+                e_lj += u;
+                e_sh += ush;
+                vir += f * rsq;
+                fij_t += 0.0;
+                afx += f * rx;
+                afy += f * ry;
+                afz += f * rz;
+                if (count) {
+                    n_lj++;
+                    if (want_hash) h_lj += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                }
+            }
+            if (ELEC != 0 && rsq < R.rc2_el) {
+                double u, f;
+                double qq;
+                {
+#pragma clang fp contract(off)
+                    qq = si.charge * sj.charge;
+                }
+                if (ELEC == MSPMS_ELEC_EWALD) {
+                    ewald_real64(rsq, qq, R.C, R.kappa, &u, &f);
+                } else {
+                    wolf_real64(rsq, qq, R.C, R.kappa, R.rc_el, R.wvc1, R.wvc2, R.wfc1, R.wfc2, &u,
+                                &f);
+                    vir += f * rsq;
+                }
+                e_re += u;
+                afx += f * rx;
+                afy += f * ry;
+                afz += f * rz;
+                if (count) {
+                    n_el++;
+                    if (want_hash) h_el += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                }
+            }
+        }
+    } else {
+        const float4 pi4 = slot_off[islot];
+        const float sig_i = vi ? si.sigma_f : 0.f;
+        const float se_i = vi ? si.sqrt_eps_f : 0.f;
+        const float q_i = pi4.w;
+        const float rc2lj = (float)R.rc2_lj, rc2el = (float)R.rc2_el;
+        const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
+        const float iLx = (float)R.invL[0], iLy = (float)R.invL[1], iLz = (float)R.invL[2];
+        const float kappa = (float)R.kappa, Cf = (float)R.C, shift1 = (float)R.shift1;
+        const float rcel = (float)R.rc_el, v1 = (float)R.wvc1, v2 = (float)R.wvc2,
+                    f1 = (float)R.wfc1, f2 = (float)R.wfc2;
+        for (int e = 0; e < nrow; e++) {
+            const uint32_t ent = row[e];
+            const int cj = (int)(ent >> 7);
+            const int jslot = cj * MSPMS_CL + jsub;
+            const int2 infoj = slot_info[jslot];
+            if (!(vi && infoj.y >= 0 && infoj.x != infoi.x)) continue;
+            const double4 Cj = cl_center[cj];
+            const float4 pj4 = slot_off[jslot];
+            float rx, ry, rz;
+            if (ent & 64u) {
+                rx = (float)(Ci.x - Cj.x) + (pi4.x - pj4.x);
+                ry = (float)(Ci.y - Cj.y) + (pi4.y - pj4.y);
+                rz = (float)(Ci.z - Cj.z) + (pi4.z - pj4.z);
+                rx -= Lx * rintf(rx * iLx);
+                ry -= Ly * rintf(ry * iLy);
+                rz -= Lz * rintf(rz * iLz);
+            } else {
+                const int sx = (int)(ent & 3u) - 1, sy = (int)((ent >> 2) & 3u) - 1,
+                          sz = (int)((ent >> 4) & 3u) - 1;
+                const float Dx = (float)(Ci.x - (Cj.x + sx * R.L[0]));
+                const float Dy = (float)(Ci.y - (Cj.y + sy * R.L[1]));
+                const float Dz = (float)(Ci.z - (Cj.z + sz * R.L[2]));
+                rx = (pi4.x - pj4.x) + Dx;
+                ry = (pi4.y - pj4.y) + Dy;
+                rz = (pi4.z - pj4.z) + Dz;
+            }
+            const float rsq = rx * rx + ry * ry + rz * rz;
+            const bool in_lj = rsq < rc2lj;
+            const bool in_el = ELEC != 0 && rsq < rc2el;
+            if (!(in_lj || in_el)) continue;
+            const SiteDev sj = sites[infoj.y];
+            const int oj = slot_orig[jslot];
+            const bool count = oi < oj;
+            const float rinv2 = __frcp_rn(rsq);
+            float fsum = 0.f;
+            if (in_lj && lji && sj.ghost != MSPMS_GHOST_LJ) {
+                const float sij = 0.5f * (sig_i + sj.sigma_f);
+                const float eij = se_i * sj.sqrt_eps_f;
+                const float s2 = sij * sij;
+                float s6 = s2 * rinv2;
+                s6 = s6 * s6 * s6;
+                const float s12 = s6 * s6;
+                float u = 4.0f * eij * (s12 - s6);
+                float f = 24.0f * eij * (2.0f * s12 - s6) * rinv2;
+                float ush = 0.f;
+                if (R.switch_on) {
+                    if (rsq >= (float)R.rcon2) {
+                        const float a = rc2lj - rsq;
+                        const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * (float)R.rcon2) /
+                                         (float)R.roff2_ron2_cube;
+                        f = f * sw - u * 12.0f * a * ((float)R.rcon2 - rsq) /
+                                         (float)R.roff2_ron2_cube;
+                        u *= sw;
+                    }
+                } else {
+                    const float sg6 = s2 * s2 * s2;
+                    ush = eij * sg6 * (sg6 * shift1 - 1.0f);
+                }
+                e_lj += (double)u;
+                e_sh += (double)ush;
+                vir += (double)(f * rsq);
+                fsum += f;
+                if (count) {
+                    n_lj++;
+                    if (want_hash) h_lj += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                }
+            }
+            if (in_el) {
+                const float rinv = rsqrtf(rsq);
+                const float r = rsq * rinv;
+                const float x = kappa * r;
+                const float ex = __expf(-x * x);
+                const float ec = ex * erfc_scaled_poly(x);
+                const float cq = Cf * q_i * pj4.w;
+                float u, f;
+                if (ELEC == MSPMS_ELEC_EWALD) {
+                    u = cq * rinv * ec;
+                    f = (u + cq * rinv * MSPMS_TWO_OVER_SQRTPI_F * x * ex) * rinv2;
+                } else {
+                    u = cq * (ec * rinv + v1 + v2 * (r - rcel));
+                    f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
+                    vir += (double)(f * rsq);
+                }
+                e_re += (double)u;
+                fsum += f;
+                if (count) {
+                    n_el++;
+                    if (want_hash) h_el += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                }
+            }
+            afx += (double)(fsum * rx);
+            afy += (double)(fsum * ry);
+            afz += (double)(fsum * rz);
+        }
+    }
+
+    // per-atom force: 8 j-lanes of each i
+    afx = sum8(afx);
+    afy = sum8(afy);
+    afz = sum8(afz);
+    if (jsub == 0 && vi) {
+        fx[oi] = afx;
+        fy[oi] = afy;
+        fz[oi] = afz;
+    }
+    e_lj = wave_sum(e_lj);
+    e_sh = wave_sum(e_sh);
+    e_re = wave_sum(e_re);
+    vir = wave_sum(vir);
+    n_lj = wave_sum_u64(n_lj);
+    n_el = wave_sum_u64(n_el);
+    if (want_hash) {
+        h_lj = wave_sum_u64(h_lj);
+        h_el = wave_sum_u64(h_el);
+    }
+    if (lane == 0) {
+        double* ep = epart + (size_t)ci * 4;
+        ep[0] = e_lj;
+        ep[1] = e_sh;
+        ep[2] = e_re;
+        ep[3] = vir;
+        unsigned long long* ip = ipart + (size_t)ci * 4;
+        ip[0] = n_lj;
+        ip[1] = n_el;
+        ip[2] = h_lj;
+        ip[3] = h_el;
+    }
+}
+
+template <int ELEC>
+void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* fx, double* fy,
+              double* fz, double* epart, unsigned long long* ipart, int want_hash) {
+    int nb = (b.maxcl + 3) / 4;
+    // pad the grid to a multiple of 8 so the XCD remap stays a bijection
+    int nb8 = (nb + 7) / 8 * 8;
+    if (precision == MSPMS_PREC_FP64)
+        hipLaunchKernelGGL((k_pair<ELEC, true>), dim3(nb8), dim3(256), 0, c.stream, b.ncl_total,
+                           b.maxcl, c.reps, c.sites, b.cl_center, b.plist, b.plist_n, b.cap,
+                           b.slot_orig, b.slot_info, b.slot_x64, b.slot_off, fx, fy, fz, epart,
+                           ipart, want_hash);
+    else
+        hipLaunchKernelGGL((k_pair<ELEC, false>), dim3(nb8), dim3(256), 0, c.stream,
+                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
+                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
+                           fx, fy, fz, epart, ipart, want_hash);
+}
+
+}  // namespace
+
+int pair_partial_count(const ClusterBufs& b) { return b.maxcl; }
+
+void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision, int charge_type,
+                 double* fx, double* fy, double* fz, double* epart, unsigned long long* ipart,
+                 int want_hash) {
+    if (charge_type == MSPMS_ELEC_EWALD)
+        launch_t<MSPMS_ELEC_EWALD>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+    else if (charge_type == MSPMS_ELEC_WOLF)
+        launch_t<MSPMS_ELEC_WOLF>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+    else
+        launch_t<0>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+}

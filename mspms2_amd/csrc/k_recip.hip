@@ -1,0 +1,245 @@
+// k_recip.hip -- Ewald reciprocal space (FP64) on gfx950.
+// Reference: src/frclong.c:988-1048.  For every k = 2*pi*(kx/Lx, ky/Ly,
+// kz/Lz) with 0 < kx^2+ky^2+kz^2 <= KSQMAX:
+//   S(k) = sum_i q_i exp(i k.r_i),  U_k = kvec |S(k)|^2,
+//   F_i += 2 q_i (S_r sin(k.r_i) - S_i cos(k.r_i)) Vf kvec C k.
+// The reference evaluates cos/sin of k.r per (k, atom), twice.  Here
+// exp(i k.r) = ex^kx ey^ky ez^kz with e = exp(i 2 pi x / L): three sincos
+// per atom, then complex multiplies (|k| <= KMAX steps, error ~KMAX ulp).
+//
+//   k_sk     one workgroup per (replica, 64-atom chunk); a lane owns one
+//            (kx, ky) row and accumulates S for all its kz in registers
+//            over the chunk; per-atom power tables live in LDS.  Partial
+//            S per chunk is written out (no atomics).
+//   k_sred   per k: deterministic sum over chunks, U_k, and the force
+//            coefficients A_k, B_k = 2 Vf C kvec (S_r, S_i).
+//   k_kforce one lane per atom over a range of rows; exp(i k.r) by
+//            recurrence in registers; forces added with FP64 atomics
+//            (one per atom per row range).
+#include "engine.h"
+#include "../../include/mspms_gpu.h"
+
+namespace {
+
+struct cplx {
+    double re, im;
+};
+__device__ inline cplx cmul(cplx a, cplx b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ inline cplx conjc(cplx a) { return {a.re, -a.im}; }
+__device__ inline cplx cpow(cplx b, int n) {  // b^n, n >= 0
+    cplx r = {1.0, 0.0};
+    while (n > 0) {
+        if (n & 1) r = cmul(r, b);
+        b = cmul(b, b);
+        n >>= 1;
+    }
+    return r;
+}
+__device__ inline cplx cpow_signed(cplx b, int n) {
+    cplx r = cpow(b, n < 0 ? -n : n);
+    return n < 0 ? conjc(r) : r;
+}
+
+// tab layout: [atom][n], n over x powers 0..kmx, y 0..kmy, z 0..kmz
+template <int NKZ>
+__global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
+                                            const RepDev* __restrict__ reps,
+                                            const KRow* __restrict__ rows_s,
+                                            const double* __restrict__ x,
+                                            const double* __restrict__ y,
+                                            const double* __restrict__ z,
+                                            const int* __restrict__ atom_site,
+                                            const SiteDev* __restrict__ sites,
+                                            double2* __restrict__ s_part, int nk_stride) {
+    extern __shared__ double2 tab[];
+    __shared__ double qsh[64];
+    const int4 w = work[blockIdx.x];
+    const RepDev& R = reps[w.x];
+    const int chunk = w.y, a0 = w.z, na = w.w;
+    const int kmx = R.kmax[0], kmy = R.kmax[1], kmz = R.kmax[2];
+    const int ntab = kmx + kmy + (kmz > 0 ? kmz : 1) + 3;  // z table keeps exp(i t)
+    // tables: thread t -> (atom t&63, dim t>>6)
+    for (int t = threadIdx.x; t < 3 * 64; t += blockDim.x) {
+        const int a = t & 63, d = t >> 6;
+        const int km = d == 0 ? kmx : (d == 1 ? kmy : (kmz > 0 ? kmz : 1));
+        const int off = d == 0 ? 0 : (d == 1 ? kmx + 1 : kmx + kmy + 2);
+        double2* row = tab + a * ntab + off;
+        if (a < na) {
+            const double* p = d == 0 ? x : (d == 1 ? y : z);
+            double s, c;
+            sincos(R.twopi_l[d] * p[a0 + a], &s, &c);
+            cplx e = {c, s}, v = {1.0, 0.0};
+            for (int n = 0; n <= km; n++) {
+                row[n] = make_double2(v.re, v.im);
+                v = cmul(v, e);
+            }
+            if (d == 0) qsh[a] = sites[atom_site[a0 + a]].charge;
+        } else {
+            for (int n = 0; n <= km; n++) row[n] = make_double2(0.0, 0.0);
+            if (d == 0) qsh[a] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < R.nkrow; r += blockDim.x) {
+        const KRow K = rows_s[R.krow0 + r];
+        const int nk = K.nk;
+        double sr[NKZ], si[NKZ];
+#pragma unroll
+        for (int t = 0; t < NKZ; t++) sr[t] = si[t] = 0.0;
+        const int ax = K.kx < 0 ? -K.kx : K.kx, ay = K.ky < 0 ? -K.ky : K.ky;
+        const int alo = K.kz_lo < 0 ? -K.kz_lo : K.kz_lo;
+        for (int a = 0; a < na; a++) {
+            const double2* ta = tab + a * ntab;
+            double2 tx = ta[ax], ty = ta[kmx + 1 + ay], tz = ta[kmx + kmy + 2 + alo],
+                    t1 = ta[kmx + kmy + 2 + 1];
+            cplx cx = {tx.x, K.kx < 0 ? -tx.y : tx.y};
+            cplx cy = {ty.x, K.ky < 0 ? -ty.y : ty.y};
+            cplx cz = {tz.x, K.kz_lo < 0 ? -tz.y : tz.y};
+            const cplx step = {t1.x, t1.y};
+            cplx c = cmul(cmul(cx, cy), cz);
+            const double q = qsh[a];
+#pragma unroll
+            for (int t = 0; t < NKZ; t++) {
+                if (t < nk) {
+                    sr[t] += q * c.re;
+                    si[t] += q * c.im;
+                    c = cmul(c, step);
+                }
+            }
+        }
+        double2* out = s_part + (size_t)chunk * nk_stride + (K.k0 - R.k0);
+#pragma unroll
+        for (int t = 0; t < NKZ; t++)
+            if (t < nk) out[t] = make_double2(sr[t], si[t]);
+    }
+}
+
+__global__ void k_sred(int nk_total, const int* __restrict__ k_rep,
+                       const RepDev* __restrict__ reps, const int* __restrict__ rep_chunk0,
+                       const double2* __restrict__ s_part, int nk_stride,
+                       const double* __restrict__ kvec, double2* __restrict__ ak,
+                       double* __restrict__ uk) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk_total) return;
+    const int r = k_rep[k];
+    const RepDev& R = reps[r];
+    const int kl = k - R.k0;
+    double sr = 0.0, si = 0.0;
+    for (int c = rep_chunk0[r]; c < rep_chunk0[r + 1]; c++) {
+        double2 v = s_part[(size_t)c * nk_stride + kl];
+        sr += v.x;
+        si += v.y;
+    }
+    const double kv = kvec[k];
+    uk[k] = kv * (sr * sr + si * si);
+    const double s = 2.0 * R.vfactor * R.C * kv;
+    ak[k] = make_double2(s * sr, s * si);
+}
+
+// work: (rep, first atom, first row, end row); 64 atoms per wave
+__global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
+                                               const RepDev* __restrict__ reps,
+                                               const KRow* __restrict__ rows,
+                                               const double2* __restrict__ ak,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z,
+                                               const int* __restrict__ atom_site,
+                                               const SiteDev* __restrict__ sites, double* fx,
+                                               double* fy, double* fz) {
+    const int4 w = work[blockIdx.x];
+    const RepDev& R = reps[w.x];
+    const int a = w.y + threadIdx.x;
+    const bool live = a < R.atom0 + R.natom;
+    const int ai = live ? a : R.atom0;
+    cplx e[3];
+    {
+        double s, c;
+        sincos(R.twopi_l[0] * x[ai], &s, &c);
+        e[0] = {c, s};
+        sincos(R.twopi_l[1] * y[ai], &s, &c);
+        e[1] = {c, s};
+        sincos(R.twopi_l[2] * z[ai], &s, &c);
+        e[2] = {c, s};
+    }
+    const cplx ezc = conjc(e[2]);
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    int pkx = 1 << 30, pky = 1 << 30;
+    cplx cx = {1.0, 0.0}, cy = {1.0, 0.0};
+    for (int r = w.z; r < w.w; r++) {
+        const KRow K = rows[r];
+        if (K.kx != pkx) {
+            cx = cpow_signed(e[0], K.kx);
+            pkx = K.kx;
+            pky = 1 << 30;
+        }
+        if (K.ky == pky + 1)
+            cy = cmul(cy, e[1]);
+        else if (K.ky != pky)
+            cy = cpow_signed(e[1], K.ky);
+        pky = K.ky;
+        const cplx cxy = cmul(cx, cy);
+        double gsum = 0.0, gzr = 0.0;
+        const double2* A = ak + K.k0 - K.kz_lo;  // A[kz] for kz in [lo, hi]
+        if (K.kz_lo <= 0 && 0 <= K.kz_hi) {
+            const double2 ab = A[0];
+            const double g = ab.x * cxy.im - ab.y * cxy.re;
+            gsum += g;
+        }
+        cplx c = cxy;
+        for (int kz = 1; kz <= K.kz_hi; kz++) {
+            c = cmul(c, e[2]);
+            if (kz >= K.kz_lo) {
+                const double2 ab = A[kz];
+                const double g = ab.x * c.im - ab.y * c.re;
+                gsum += g;
+                gzr += g * kz;
+            }
+        }
+        c = cxy;
+        for (int kz = -1; kz >= K.kz_lo; kz--) {
+            c = cmul(c, ezc);
+            if (kz <= K.kz_hi) {
+                const double2 ab = A[kz];
+                const double g = ab.x * c.im - ab.y * c.re;
+                gsum += g;
+                gzr += g * kz;
+            }
+        }
+        gx += gsum * K.kx;
+        gy += gsum * K.ky;
+        gz += gzr;
+    }
+    if (!live) return;
+    const double q = sites[atom_site[a]].charge;
+    atomicAdd(&fx[a], q * gx * R.twopi_l[0]);
+    atomicAdd(&fy[a], q * gy * R.twopi_l[1]);
+    atomicAdd(&fz[a], q * gz * R.twopi_l[2]);
+}
+
+}  // namespace
+
+int recip_nkz_supported(int kmaxz) { return kmaxz <= 16; }
+
+void launch_recip(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
+                  const double* z, double* fx, double* fy, double* fz) {
+    size_t lds = (size_t)64 * rb.ntab_max * sizeof(double2);
+    if (rb.nkz_max <= 9)
+        hipLaunchKernelGGL(k_sk<9>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s, c.reps,
+                           rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part, rb.nk_stride);
+    else if (rb.nkz_max <= 17)
+        hipLaunchKernelGGL(k_sk<17>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s,
+                           c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
+                           rb.nk_stride);
+    else
+        hipLaunchKernelGGL(k_sk<33>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s,
+                           c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
+                           rb.nk_stride);
+    hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 255) / 256), dim3(256), 0, c.stream,
+                       rb.nk_total, rb.k_rep, c.reps, rb.rep_chunk0, rb.s_part, rb.nk_stride,
+                       rb.kvec, rb.ak, rb.uk);
+    hipLaunchKernelGGL(k_kforce, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f, c.reps,
+                       rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz);
+}

@@ -1,0 +1,112 @@
+/*
+ * ref_harness.c -- driver around the reference's own force routines.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Linked against objects compiled from the
+ * reference's sources (see build_ref.sh); never part of the product.
+ *
+ * In a run directory holding in.mspms / cfg.mspms / xyz.mspms it runs the
+ * reference's start-up path (src/mspms2.c:39-51: opening, readins,
+ * fnValidateInput, init_vars, fnValidateInit), then frclong() and
+ * frcshort(), and writes every output global to a binary dump:
+ *
+ *   "MSPMSREF" | int32 version=1 | int32 natom | int32 nscalar |
+ *   double scalar[nscalar] | double xx,yy,zz,fxl,fyl,fzl,fxs,fys,fzs [natom]
+ *
+ * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K]
+ *   --repeat K times K further frclong() calls and prints seconds/call.
+ *
+ * Configurations that would reach code absent from this build (the
+ * Fortran grid library, iSF_type 3; the GSL metal-cluster force field,
+ * iExternal_FF_type != 0) are refused before init_vars.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "mspms2.h"
+
+static void put(FILE* f, const double* v, int n) { fwrite(v, sizeof(double), (size_t)n, f); }
+
+int main(int argc, char** argv)
+{
+    int repeat = 0, k;
+    char outpath[4096];
+    FILE* fo;
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s RUNDIR OUTFILE [--repeat K]\n", argv[0]);
+        return 2;
+    }
+    for (k = 3; k < argc; k++)
+        if (!strcmp(argv[k], "--repeat") && k + 1 < argc) repeat = atoi(argv[++k]);
+    if (argv[2][0] == '/') {
+        snprintf(outpath, sizeof outpath, "%s", argv[2]);
+    } else {
+        char cwd[2048];
+        if (!getcwd(cwd, sizeof cwd)) return 2;
+        snprintf(outpath, sizeof outpath, "%s/%s", cwd, argv[2]);
+    }
+    if (chdir(argv[1]) != 0) {
+        perror("chdir");
+        return 2;
+    }
+    opening();
+    readins();
+    if (iSF_type == SF_NANOTUBE_TASOS || iSF_type == SF_NANOTUBE_MY_INTERP ||
+        iExternal_FF_type != FF_NONE) {
+        fprintf(stderr, "ref_harness: iSF_type=%d / iExternal_FF_type=%d need "
+                        "libraries absent from this build\n",
+                iSF_type, iExternal_FF_type);
+        return 3;
+    }
+    fnValidateInput();
+    init_vars();
+    fnValidateInit();
+    frclong();
+    frcshort();
+    if (repeat > 0) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        for (k = 0; k < repeat; k++) frclong();
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        printf("frclong_seconds_per_call %.9f\n",
+               ((t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec)) / repeat);
+    }
+    fo = fopen(outpath, "wb");
+    if (!fo) {
+        perror("fopen");
+        return 2;
+    }
+    {
+        const double sc[] = {
+            uvdw, ushift, ureal, uexcl, ufourier, uself, uvacuum, uelec,
+            usflj, uotherff, uinter, virial_inter,
+            ubond, uangle, udih, uimp, uintra, virial_intra,
+            boxlx, boxly, boxlz, rcutoff, rcutoffsq, rcutoffelec, rcutoffelecsq,
+            rcuton, rcutonsq, roff2_minus_ron2_cube, shift1, shift4,
+            kappa, coulomb_prefactor, TWOPI_LX, TWOPI_LY, TWOPI_LZ,
+            Bfactor_ewald, Vfactor_ewald, twopi_over_3v,
+            wolfvcon1, wolfvcon2, wolfvcon3, wolffcon1, wolffcon2,
+            f0, sigma_base, epsilon_base, uljlrc, pljlrc, time_base, delt,
+            delts, const_3pisq_theta};
+        int nsc = (int)(sizeof sc / sizeof sc[0]), ver = 1;
+        fwrite("MSPMSREF", 1, 8, fo);
+        fwrite(&ver, sizeof ver, 1, fo);
+        fwrite(&natom, sizeof natom, 1, fo);
+        fwrite(&nsc, sizeof nsc, 1, fo);
+        put(fo, sc, nsc);
+    }
+    put(fo, xx, natom);
+    put(fo, yy, natom);
+    put(fo, zz, natom);
+    put(fo, fxl, natom);
+    put(fo, fyl, natom);
+    put(fo, fzl, natom);
+    put(fo, fxs, natom);
+    put(fo, fys, natom);
+    put(fo, fzs, natom);
+    fclose(fo);
+    return 0;
+}

@@ -1,0 +1,67 @@
+"""Shared helpers of the test suite: system builders and comparisons."""
+import os
+import tempfile
+
+import numpy as np
+
+from mspms2_amd import runio
+from mspms2_amd.system import System
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+ENERGY_KEYS = ("uvdw", "ushift", "ureal", "uexcl", "ufourier", "uself", "uelec", "usflj",
+               "uinter", "virial_inter")
+SHORT_KEYS = ("ubond", "uangle", "udih", "uimp", "uintra", "virial_intra")
+
+
+def system_of(rundir_obj):
+    """RunDir -> System through a write/read round trip (the reference's
+    own path reads the same text)."""
+    with tempfile.TemporaryDirectory(prefix="mspms_t_") as tmp:
+        rundir_obj.write(tmp)
+        return System.from_rundir(runio.RunDir.read(tmp))
+
+
+def rel_rms(f, g):
+    """||f - g|| / ||g|| over all atoms and components."""
+    g = np.asarray(g)
+    d = np.linalg.norm(np.asarray(f) - g)
+    n = np.linalg.norm(g)
+    return d / n if n > 0 else d
+
+
+def force_close(f, g, t_ref, natom, rc, rtol):
+    """||f - g|| <= rtol * max(||g||, force scale).  The floor is the
+    per-atom force implied by the virial and LJ energy (|W| + |U|)/(N rc):
+    on a perfect lattice the reference forces are rounding noise (~1e-14)
+    and a relative error means nothing."""
+    g = np.asarray(g)
+    d = np.linalg.norm(np.asarray(f) - g)
+    fs = (abs(t_ref["virial_inter"]) + abs(t_ref["uvdw"])) / (natom * rc) * np.sqrt(g.size)
+    return d <= rtol * max(np.linalg.norm(g), fs), d / max(np.linalg.norm(g), fs)
+
+
+def energy_scale(t):
+    """Scale of the electrostatic cancellation uelec = ureal + ufourier -
+    uself - uexcl (+ uvacuum)."""
+    return sum(abs(t.get(k, 0.0)) for k in ("ureal", "ufourier", "uself", "uexcl", "uvacuum"))
+
+
+def assert_long_close(t, ref, rtol, what=""):
+    """Each energy within rtol of the reference, relative to its own size
+    (uelec / uinter / virial relative to the cancelling components)."""
+    scale_el = energy_scale(ref)
+    for k in ENERGY_KEYS:
+        a, b = t[k], ref[k]
+        s = abs(b)
+        if k in ("uelec", "uinter", "virial_inter"):
+            s = max(s, scale_el, abs(ref["uvdw"]))
+        assert abs(a - b) <= rtol * max(s, 1e-300) + 1e-300, \
+            f"{what} {k}: {a!r} vs {b!r} (rel {abs(a - b) / max(s, 1e-300):.3e}, tol {rtol})"
+
+
+def assert_short_close(t, ref, rtol, what=""):
+    for k in SHORT_KEYS:
+        a, b = t[k], ref[k]
+        s = max(abs(b), abs(ref["uintra"]), 1e-300)
+        assert abs(a - b) <= rtol * s, f"{what} {k}: {a!r} vs {b!r}"
