@@ -1,0 +1,214 @@
+"""Benchmark: RESPA MD of 12,288-site flexible water with Ewald on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Workload (config 2 of the benchmark definition): 4096 three-site waters on
+a jittered 16^3 lattice in a 49.66 A box, Ewald alpha 0.28 / A, kmax 8,
+KSQMAX 64 (2,108 k-vectors), rc 10 A; RESPA with the bonded terms in 8
+inner steps of 0.25 fs and the nonbonded (frclong) in the 2 fs outer step.
+A step is one device-resident RESPA velocity-Verlet outer step
+(mspms_vver_step_device): 1 frclong + 8 frcshort + kicks/drifts, inputs
+resident in HBM.  Pair arithmetic FP32 with FP64 accumulation (FP64
+reciprocal, bonded and integrator); the plain-FP64 engine is timed beside it.
+
+Prints one JSON line (the benchmark contract) with `roofline` for the
+dominant kernel (live HIP-event timing on the launch stream) and
+`cpu_baseline`: the reference's own frclong/frcshort (oracle/_ref, built
+from the reference sources) timed on the host, one thread.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DT_FS = 2.0
+NSTEP_INNER = 8
+PEAK_FP32_TFLOPS = 157.3   # MI355X published vector/matrix FP32
+PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64
+PEAK_HBM_GBS = 8000.0
+
+
+def make_workload():
+    from mspms2_amd import configs
+    from tests.util import system_of
+    rd = configs.water_box(n_side=16, L=49.66, kappa=0.28, kmax=8, ksqmax=64, rc=10.0,
+                           seed=2, nstep_inner=NSTEP_INNER, delt_fs=DT_FS)
+    return rd, system_of(rd)
+
+
+def maxwell_velocities(system, seed=7):
+    """Maxwell-Boltzmann velocities at treq, zero total momentum (the
+    reference's velinit, src/initvars.c:151-232, with numpy's generator)."""
+    rng = np.random.default_rng(seed)
+    m = system.masses()
+    v = rng.standard_normal((3, system.natom)) * np.sqrt(system.treq / m)
+    p = (v * m).sum(axis=1, keepdims=True) / m.sum()
+    return v - p
+
+
+def run_gpu(system, precision, steps, warmup, profile=True):
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from mspms2_amd import abi
+    import ctypes as C
+    dev = torch.device("cuda:0")
+    eng = Engine(system, precision=precision)
+    n = system.natom
+    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), dtype=torch.float64,
+                       device=dev).contiguous()
+    vel = torch.tensor(maxwell_velocities(system), dtype=torch.float64, device=dev).contiguous()
+    fl = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    fs = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    lterms = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    sterms = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    ukin = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    # forces of the starting positions (src/mspms2.c:50-51); the host call
+    # also sizes the pair list
+    f0, t0 = eng.frclong()
+    fl.copy_(torch.tensor(f0))
+    fs0, _ = eng.frcshort()
+    fs.copy_(torch.tensor(fs0))
+    delt = system.delt
+    for _ in range(warmup):
+        eng.vver_step_device(pos, vel, fl, fs, delt, NSTEP_INNER, lterms, sterms, ukin, stream)
+    torch.cuda.synchronize()
+    eng.sync()
+    if profile:
+        eng.set_profiling(True)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        eng.vver_step_device(pos, vel, fl, fs, delt, NSTEP_INNER, lterms, sterms, ukin, stream)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    eng.sync()
+    ms = start.elapsed_time(stop) / steps
+    groups = eng.timings() if profile else {}
+    raw = lterms.cpu().numpy().view(np.uint8)
+    lt = abi.LongTerms.from_buffer_copy(raw.tobytes()).as_dict()
+    out = dict(ms_per_step=ms, wall_ms_per_step=1e3 * wall / steps, groups=groups, terms=lt,
+               finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0)
+    eng.close()
+    return out
+
+
+def run_reference_cpu(rd, repeat=2):
+    """The reference's own frclong timed on one host core (oracle/_ref)."""
+    from oracle import oracle as O
+    with tempfile.TemporaryDirectory(prefix="mspms_bench_") as tmp:
+        rd.write(tmp)
+        if O.reference_available():
+            d = O.run_reference(tmp, repeat=repeat, timeout=600)
+            return dict(kind="reference", seconds_per_frclong=d["frclong_seconds_per_call"],
+                        sample=f"{repeat} frclong calls of the 12,288-site system "
+                               f"(oracle/_ref/mspms_ref_harness, reference sources)")
+    # restatement fallback (same loop order, FP64)
+    from mspms2_amd import runio
+    from mspms2_amd.system import System
+    s = System.from_rundir(rd)
+    o = O.Oracle(s)
+    t = time.perf_counter()
+    for _ in range(repeat):
+        o.frclong()
+    return dict(kind="port", seconds_per_frclong=(time.perf_counter() - t) / repeat,
+                sample=f"{repeat} frclong calls of the oracle restatement")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--fp64-steps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU per process; --gpus {args.gpus} is not "
+              f"supported (replica batches shard across GPUs by replica)", file=sys.stderr)
+        sys.exit(2)
+    rd, system = make_workload()
+    mixed = run_gpu(system, "mixed", args.steps, args.warmup)
+    fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False)
+
+    ms = mixed["ms_per_step"]
+    ns_day = DT_FS * 1e-6 * 86400e3 / ms
+    t = mixed["terms"]
+    n_el, n_lj = t["n_elec_pairs"], t["n_lj_pairs"]
+    pairs_per_s = n_el / (ms * 1e-3)
+    # force error of the mixed path against the FP64 path (same positions)
+    fm, ff = mixed["f_initial"], fp64["f_initial"]
+    force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
+
+    g = mixed["groups"]
+    natom = system.natom
+    nk = 2108
+    # algorithmic work per launch (pair: 20 geometry + 24 LJ + 25 Ewald real
+    # FLOP-eq per i<j pair in range; reciprocal: 29 FP64 FLOP-eq per (k, atom))
+    pair_flop = 20.0 * n_el + 24.0 * n_lj + 25.0 * n_el
+    recip_flop = 29.0 * nk * natom
+    cand = {
+        "k_pair": (g.get("pair", 0.0), pair_flop, PEAK_FP32_TFLOPS),
+        "k_sk+k_kforce": (g.get("recip_s", 0.0) + g.get("recip_f", 0.0), recip_flop,
+                          PEAK_FP64_TFLOPS),
+    }
+    name, (kms, flop, peak) = max(cand.items(), key=lambda kv: kv[1][0])
+    achieved = flop / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    roofline = {"bound": "valu", "kernel": name, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "kernel_ms": kms, "groups_ms": g}
+
+    cpu = None
+    if not args.no_cpu:
+        c = run_reference_cpu(rd)
+        cpu_step = c["seconds_per_frclong"]
+        cpu = {"value": DT_FS * 1e-6 * 86400.0 / cpu_step, "unit": "ns/day", "cores": 1,
+               "kind": c["kind"], "sample": c["sample"],
+               "seconds_per_frclong": cpu_step}
+
+    line = {
+        "metric": "ns/day + pair-interactions/s, 12k-site water; batched/FP32-acc vs plain FP64",
+        "value": ns_day,
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pairs / f64 accumulate",
+        "data": "synthetic (seeded jittered-lattice water, random orientations)",
+        "config": {"workload": "water-12288 Ewald RESPA 2 fs / 8 x 0.25 fs", "natom": natom,
+                   "kvectors": nk, "rc_A": 10.0, "kappa_per_A": 0.28, "box_A": 49.66,
+                   "precision": "mixed"},
+        "pair_interactions_per_s": pairs_per_s,
+        "pairs_in_cutoff": n_el,
+        "fp64_ns_per_day": DT_FS * 1e-6 * 86400e3 / fp64["ms_per_step"],
+        "fp64_ms_per_step": fp64["ms_per_step"],
+        "mixed_over_fp64": fp64["ms_per_step"] / ms,
+        "force_rms_rel_err_vs_fp64": force_err,
+        "finite": mixed["finite"] and fp64["finite"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -293,6 +293,21 @@ int mspms_energy_param_grad(mspms_ctx* ctx, const double* xx, const double* yy,
                             double* fzl, mspms_long_terms* terms,
                             mspms_param_grad* grad);
 
+/* One RESPA velocity-Verlet outer step in the NVE ensemble, device
+ * resident: replaces vver (src/md.c:22-83, the integrator of
+ * src/mspms2.c:99-132 and md_move src/dispmd.c:42-46) with every buffer on
+ * the device.  pos/vel/fl/fs are (3, N) FP64 SoA device arrays; fl and fs
+ * must hold the long / short forces of the current positions (one
+ * mspms_frclong_device + mspms_frcshort_device call before the first step,
+ * as src/mspms2.c:50-51 does) and are updated to those of the new
+ * positions.  lterms / sterms / ukin receive one value per replica: the
+ * frclong terms, the frcshort terms of the last inner substep and the
+ * kinetic energy (src/md.c:76-78). */
+int mspms_vver_step_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
+                           double* fs, double delt, int nstep_inner,
+                           mspms_long_terms* lterms, mspms_short_terms* sterms,
+                           double* ukin, void* stream);
+
 /* Synchronises the context's stream and reports the status of the last
  * device-buffer evaluation (list overflow, non-finite input, singular
  * improper), which the host-buffer calls check by themselves. */
@@ -304,8 +319,8 @@ void* mspms_stream(const mspms_ctx* ctx);
 /* Kernel-group timing with HIP events on the evaluation stream: switch on,
  * evaluate, then read the mean milliseconds per call of each group
  * (0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,
- * 3 Ewald reciprocal, 4 finalize, 5 bonded).  Device-buffer calls are
- * accumulated at mspms_sync(). */
+ * 3 Ewald S(k) + reduction, 4 Ewald forces, 5 finalize, 6 bonded / RESPA
+ * inner loop).  Device-buffer calls are accumulated at mspms_sync(). */
 int mspms_set_profiling(mspms_ctx* ctx, int on);
 int mspms_last_timings(const mspms_ctx* ctx, double* ms, int n);
 

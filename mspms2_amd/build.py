@@ -20,7 +20,8 @@ LIB = os.path.join(HERE, "libmspms_gpu.so")
 DROPIN = os.path.join(HERE, "libmspms_dropin.a")
 ARCH = os.environ.get("MSPMS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["k_cluster.hip", "k_pair.hip", "k_intra.hip", "k_recip.hip", "engine.cpp"]
+HIP_SOURCES = ["k_cluster.hip", "k_pair.hip", "k_intra.hip", "k_recip.hip", "k_integrate.hip",
+               "engine.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
     # packed FP32 instructions are kept out of the generated code

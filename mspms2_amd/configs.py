@@ -124,9 +124,13 @@ def water_species(nmole: int) -> SpeciesSpec:
         angles=[(1, 0, 2, abi.ANGLE_HARMONIC, WATER_KTHETA, WATER_THETA0, 0.0, 0.0, 0.0)])
 
 
-def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=200):
+def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=400, recenter=False,
+                 jitter=None):
     """Random orientation per molecule, rejecting contacts closer than
-    min_dist to molecules already placed (cell-list check)."""
+    min_dist to molecules already placed (cell-list check).  Every 20
+    failed tries the centre is redrawn: uniformly in the box (recenter) or
+    within +-jitter of its lattice site (jitter); running out of tries is an
+    error."""
     nmol, nsite = centers.shape[0], tmpl.shape[0]
     out = np.empty((nmol, nsite, 3))
     ncell = max(1, int(L // max(min_dist, 1e-9)))
@@ -134,9 +138,18 @@ def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=200):
     grid = {}
     md2 = min_dist * min_dist
     for m in range(nmol):
-        for _ in range(max_tries):
+        c0 = centers[m].copy()
+        if jitter:
+            c0 = centers[m] + rng.uniform(-jitter, jitter, 3)
+        ok = False
+        for attempt in range(max_tries):
+            if attempt and attempt % 20 == 0:
+                if recenter:
+                    c0 = rng.uniform(0, L, 3)
+                elif jitter:
+                    c0 = centers[m] + rng.uniform(-jitter, jitter, 3)
             R = _random_rotations(rng, 1)[0]
-            xyz = centers[m] + tmpl @ R.T
+            xyz = c0 + tmpl @ R.T
             ok = True
             cells = set()
             for s in range(nsite):
@@ -162,6 +175,8 @@ def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=200):
                 cells.add(c)
             if ok:
                 break
+        if not ok:
+            raise RuntimeError(f"could not place molecule {m} without contacts < {min_dist} A")
         out[m] = xyz
         for s in range(nsite):
             w = xyz[s] - L * np.floor(xyz[s] / L)
@@ -182,11 +197,11 @@ def water_box(n_side: int = 16, L: float = 49.66, seed: int = 2,
     a = L / n_side
     idx = np.array([[i, j, k] for i in range(n_side) for j in range(n_side)
                     for k in range(n_side)], dtype=np.float64)
-    centers = (idx + 0.5) * a + rng.uniform(-jitter, jitter, size=(nmol, 3))
+    centers = (idx + 0.5) * a
     sp = water_species(nmol)
     tmpl = np.array([[s.xx, s.yy, s.zz] for s in sp.sites])
     tmpl = tmpl - (tmpl * np.array([15.9994, 1.008, 1.008])[:, None]).sum(0) / 18.0154
-    xyz = _place_rigid(rng, centers, tmpl, L, min_dist).reshape(-1, 3)
+    xyz = _place_rigid(rng, centers, tmpl, L, min_dist, jitter=jitter).reshape(-1, 3)
     ct = {"ewald": abi.ELEC_EWALD, "wolf": abi.ELEC_WOLF, "none": abi.ELEC_NONE,
           "coulomb": abi.ELEC_SIMPLE_COULOMB}[elec]
     run = RunInput(title=f"water {nmol} {elec}", ij=1802, jk=9373, sigma_base=1.0,
@@ -198,6 +213,116 @@ def water_box(n_side: int = 16, L: float = 49.66, seed: int = 2,
                    ksqmax=ksqmax, fEwald_BC=abi.EWALD_BC_TINFOIL, fEwald_Dim=3,
                    qq=200.0, qqs=200.0)
     return RunDir(run=run, sysname="water", species=[sp], xyz=xyz)
+
+
+def water4_species(nmole: int) -> SpeciesSpec:
+    """Four-site water variant: the negative charge sits on a GHOST_FULL site
+    M 0.15 A along the bisector (skipped by every pair loop of the
+    reference, kept in the Ewald sums), O carries LJ only."""
+    sp = water_species(nmole)
+    o, h1, h2 = (np.array([s.xx, s.yy, s.zz]) for s in sp.sites)
+    bis = (h1 + h2) / 2 - o
+    mpos = o + 0.15 * bis / np.linalg.norm(bis)
+    sp.sites[0].charge = 0.0
+    sp.sites.append(Site(type=2, name="M", xx=mpos[0], yy=mpos[1], zz=mpos[2], aw=0.0,
+                         epsilon=0.0, sigma=0.0, charge=WATER_Q_O, ghost_type=2, interp_type=0))
+    return sp
+
+
+def co_species(nmole: int) -> SpeciesSpec:
+    """Charged diatomic (CO-like), Morse bond; a second species for the
+    cross-species loops of src/frclong.c:74-80."""
+    sites = [Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.012011, epsilon=35.0,
+                  sigma=3.55, charge=0.2, ghost_type=0, interp_type=1),
+             Site(type=1, name="O", xx=0.0, yy=0.0, zz=1.128, aw=0.0159994, epsilon=61.6,
+                  sigma=2.95, charge=-0.2, ghost_type=0, interp_type=2)]
+    return SpeciesSpec(name="co", nmole=nmole, sites=sites,
+                       bonds=[(0, 1, abi.BOND_MORSE, 1.3e5, 1.128, 2.3)])
+
+
+def mixture_box(n_water: int = 150, n_co: int = 60, L: float = 22.0, seed: int = 8,
+                elec: str = "ewald", kappa: float = 0.3, kmax: int = 5, ksqmax: int = 27,
+                rc: float = 9.0, water4: bool = False, bc: int = abi.EWALD_BC_TINFOIL,
+                switch: bool = False) -> RunDir:
+    """Two species (water, CO) at random positions and orientations."""
+    rng = np.random.default_rng(seed)
+    wsp = water4_species(n_water) if water4 else water_species(n_water)
+    csp = co_species(n_co)
+    out = []
+    for sp, n in ((wsp, n_water), (csp, n_co)):
+        tmpl = np.array([[s.xx, s.yy, s.zz] for s in sp.sites])
+        tmpl = tmpl - tmpl.mean(0)
+        cen = rng.uniform(0, L, size=(n, 3))
+        out.append((sp, cen, tmpl))
+    xyz = []
+    placed = np.zeros((0, 3))
+    for sp, cen, tmpl in out:
+        for c in cen:
+            for _ in range(500):
+                R = _random_rotations(rng, 1)[0]
+                cand = c + tmpl @ R.T
+                if placed.shape[0] == 0:
+                    ok = True
+                else:
+                    d2 = np.min(_min_image_dist2(cand[:, None, :], placed[None, :, :], L))
+                    ok = d2 > 2.0 ** 2
+                if ok:
+                    break
+                c = rng.uniform(0, L, 3)
+            xyz.append(cand)
+            placed = np.vstack([placed, cand])
+    ct = {"ewald": abi.ELEC_EWALD, "wolf": abi.ELEC_WOLF, "none": abi.ELEC_NONE,
+          "coulomb": abi.ELEC_SIMPLE_COULOMB}[elec]
+    run = RunInput(title="water/CO mixture", sigma_base=1.0, epsilon_base=1.0, mass_base=1.0,
+                   treq=300.0, boxlx=L, boxly=L, boxlz=L,
+                   rcuton=(0.8 * rc if switch else rc), rcutoff=rc, rcutoffelec=rc,
+                   isLJswitchOn=1 if switch else 0, iChargeType=ct, kappa=kappa, kmaxx=kmax,
+                   kmaxy=kmax, kmaxz=kmax, ksqmax=ksqmax, fEwald_BC=bc, delt=1.0,
+                   nstep_inner=4)
+    return RunDir(run=run, sysname="mixture", species=[wsp, csp], xyz=np.vstack(xyz))
+
+
+def bonded_zoo(nmole: int = 40, L: float = 30.0, seed: int = 9, f0: float = 0.5) -> RunDir:
+    """A 6-site synthetic molecule with every bonded type the reference has:
+    harmonic / Morse / FENE bonds, harmonic angles, OPLS and CHARMM
+    dihedrals (one pair sharing ending atoms, for InitCheckUniques), a
+    CHARMM improper; charged sites, 1-4 pairs with f0 scaling, Ewald."""
+    rng = np.random.default_rng(seed)
+    tmpl = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [2.0, 1.4, 0.0], [3.5, 1.5, 0.3],
+                     [1.9, -0.9, 1.1], [0.2, 1.2, -0.8]])
+    sites = []
+    for k in range(6):
+        sites.append(Site(type=k, name=f"X{k}", xx=tmpl[k, 0], yy=tmpl[k, 1], zz=tmpl[k, 2],
+                          aw=0.014, epsilon=40.0 + 10 * k, sigma=3.0 + 0.1 * k,
+                          charge=[0.3, -0.2, 0.1, -0.25, 0.15, -0.1][k], ghost_type=0,
+                          interp_type=1))
+    sp = SpeciesSpec(
+        name="zoo", nmole=nmole, sites=sites,
+        bonds=[(0, 1, abi.BOND_HARMONIC, 9.0e5, 1.5, 0.0), (1, 2, abi.BOND_MORSE, 4.0e4, 1.45, 2.0),
+               (2, 3, abi.BOND_FENE, 1.5e4, 2.2, 0.0), (1, 4, abi.BOND_HARMONIC, 8.0e5, 1.45, 0.0),
+               (0, 5, abi.BOND_HARMONIC, 8.5e5, 1.5, 0.0)],
+        angles=[(0, 1, 2, abi.ANGLE_HARMONIC, 6.0e4, 1.95, 0, 0, 0),
+                (1, 2, 3, abi.ANGLE_HARMONIC, 6.2e4, 1.9, 0, 0, 0),
+                (0, 1, 4, abi.ANGLE_HARMONIC, 5.0e4, 1.9, 0, 0, 0),
+                (2, 1, 4, abi.ANGLE_HARMONIC, 5.5e4, 1.95, 0, 0, 0),
+                (1, 0, 5, abi.ANGLE_HARMONIC, 5.8e4, 1.92, 0, 0, 0)],
+        dihedrals=[(0, 1, 2, 3, abi.DIH_OPLS_COSIN, 0.0, 710.06, -136.38, 1582.64),
+                   (4, 1, 2, 3, abi.DIH_CHARMM, 300.0, 3.0, 0.0, 0.0),
+                   (5, 0, 1, 2, abi.DIH_OPLS_COSIN, 10.0, 500.0, 100.0, 800.0),
+                   (5, 0, 1, 4, abi.DIH_CHARMM, 200.0, 2.0, 180.0, 0.0),
+                   (3, 2, 1, 0, abi.DIH_OPLS_COSIN, 0.0, 100.0, 50.0, 20.0),
+                   # ending pair (0,2) = angle 0's: InitCheckUniques then marks
+                   # isDih_unique by the angle index (src/initvars.c:66)
+                   (0, 5, 1, 2, abi.DIH_OPLS_COSIN, 0.0, 300.0, 0.0, 0.0)],
+        impropers=[(1, 0, 2, 4, abi.IMP_CHARMM, 4.0e3, 0.0)])
+    cen = rng.uniform(0, L, size=(nmole, 3))
+    xyz = _place_rigid(rng, cen, tmpl - tmpl.mean(0), L, 2.6, recenter=True).reshape(-1, 3)
+    xyz = xyz + rng.uniform(-0.05, 0.05, size=xyz.shape)
+    run = RunInput(title="bonded zoo", sigma_base=1.0, epsilon_base=1.0, mass_base=1.0,
+                   treq=300.0, boxlx=L, boxly=L, boxlz=L, rcuton=11.0, rcutoff=11.0,
+                   rcutoffelec=11.0, iChargeType=abi.ELEC_EWALD, kappa=0.3, kmaxx=5, kmaxy=5,
+                   kmaxz=5, ksqmax=27, f0=f0, delt=1.0, nstep_inner=4)
+    return RunDir(run=run, sysname="zoo", species=[sp], xyz=xyz)
 
 
 # --------------------------------------------------------------------------- #
@@ -241,9 +366,9 @@ def butane_box(nmol: int = 128, seed: int = 3, rc: float = 14.0,
     sites = np.array([[i, j, k] for i in range(ns) for j in range(ns)
                       for k in range(ns)], dtype=np.float64)
     pick = rng.permutation(len(sites))[:nmol]
-    centers = (sites[pick] + 0.5) * a + rng.uniform(-0.2, 0.2, size=(nmol, 3))
+    centers = (sites[pick] + 0.5) * a
     tmpl = BUTANE_TMPL - BUTANE_TMPL.mean(0)
-    xyz = _place_rigid(rng, centers, tmpl, L, min_dist).reshape(-1, 3)
+    xyz = _place_rigid(rng, centers, tmpl, L, min_dist, jitter=0.2).reshape(-1, 3)
     run = RunInput(title=f"n-butane {nmol}", ij=1000 + seed % 30000, jk=2000 + seed % 28000,
                    sigma_base=1.0, epsilon_base=1.0, mass_base=1.0, treq=300.0, preq=1.0,
                    boxlx=L, boxly=L, boxlz=L, rcuton=rcut, rcutoff=rcut, rcutoffelec=rcut,

@@ -1,0 +1,261 @@
+/*
+ * dropin.c -- frclong() / frcshort() over the reference program's globals,
+ * evaluated by the GPU engine (libmspms_gpu.so).  See mspms_dropin.h.
+ *
+ * The reference defines its state as global variables in headers
+ * (src/molevars.h:75-104, src/vars.h:30-212) and builds with -fcommon; this
+ * file declares the ones the force routines use as `extern`, with the
+ * reference's exact types.  SAMPLE_MOLECULE's layout (src/molevars.h:40-73,
+ * limits :18-23) is restated in ref_sample_molecule; a test compares its
+ * field offsets with the reference's compiled layout.
+ */
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/mspms_dropin.h"
+#include "../../include/mspms_gpu.h"
+
+#include "ref_layout.h"
+
+/* ---- the reference's globals (src/molevars.h, src/vars.h) ---------------- */
+extern ref_sample_molecule sample_mole[];
+extern int natom, nspecie;
+extern int nmole_per_specie[];
+extern double xx[], yy[], zz[];
+extern double fxl[], fyl[], fzl[], fxs[], fys[], fzs[];
+extern double sigma_base, epsilon_base;
+extern double boxlx, boxly, boxlz;
+extern double rcuton, rcutoff, rcutoffelec, rcutoffsq, rcutonsq, rcutoffelecsq;
+extern double roff2_minus_ron2_cube, shift1, shift4, f0;
+extern int isLJswitchOn, iChargeType, iSF_type, iExternal_FF_type;
+extern double kappa;
+extern int KMAXX, KMAXY, KMAXZ, KSQMAX, fEwald_BC;
+extern double TWOPI_LX, TWOPI_LY, TWOPI_LZ, Bfactor_ewald, Vfactor_ewald, twopi_over_3v;
+extern double wolfvcon1, wolfvcon2, wolfvcon3, wolffcon1, wolffcon2;
+extern double coulomb_prefactor;
+extern double uvdw, ushift, ureal, uexcl, ufourier, uself, uvacuum, uelec, usflj, uotherff;
+extern double uinter, virial_inter;
+extern double ubond, uangle, udih, uimp, uintra, virial_intra;
+extern int ntube;
+extern double *hgntc_xx, *hgntc_yy, *hgnt_radius;
+extern double const_3pisq_theta;
+extern double *solid_sigma, *solid_epsilon;
+extern int solid_natom;
+extern double *solid_xx, *solid_yy, *solid_zz;
+
+/* ---- cached device context ------------------------------------------------ */
+#define MAXSP 3 /* NSPECIE_MAX, src/molevars.h:15 */
+static mspms_ctx* g_ctx;
+static int g_nspecie = -1, g_natom = -1, g_nmole[MAXSP];
+static mspms_params g_params;
+/* interleaved parameter copies the C ABI takes */
+static double g_bnd[MAXSP][3 * REF_SAMPLE_NTERM_MAX], g_agl[MAXSP][5 * REF_SAMPLE_NTERM_MAX],
+    g_dih[MAXSP][4 * REF_SAMPLE_NTERM_MAX], g_imp[MAXSP][2 * REF_SAMPLE_NTERM_MAX];
+static unsigned char g_au[MAXSP][REF_SAMPLE_NTERM_MAX], g_du[MAXSP][REF_SAMPLE_NTERM_MAX];
+
+static void die(const char* what, int rc)
+{
+    fprintf(stderr, "mspms gpu %s failed (%d): %s\n", what, rc, mspms_last_error());
+    exit(1);
+}
+
+static void read_params(mspms_params* p)
+{
+    memset(p, 0, sizeof *p);
+    p->sigma_base = sigma_base;
+    p->epsilon_base = epsilon_base;
+    p->boxlx = boxlx;
+    p->boxly = boxly;
+    p->boxlz = boxlz;
+    p->rcuton = rcuton;
+    p->rcutoff = rcutoff;
+    p->rcutoffelec = rcutoffelec;
+    p->isLJswitchOn = isLJswitchOn;
+    p->f0 = f0;
+    p->iChargeType = iChargeType;
+    p->kappa = kappa;
+    p->kmaxx = KMAXX;
+    p->kmaxy = KMAXY;
+    p->kmaxz = KMAXZ;
+    p->ksqmax = KSQMAX;
+    p->fEwald_BC = fEwald_BC;
+    p->iSF_type = iSF_type;
+    p->iExternal_FF_type = iExternal_FF_type;
+    p->rcutoffsq = rcutoffsq;
+    p->rcutonsq = rcutonsq;
+    p->rcutoffelecsq = rcutoffelecsq;
+    p->roff2_minus_ron2_cube = roff2_minus_ron2_cube;
+    p->shift1 = shift1;
+    p->shift4 = shift4;
+    p->coulomb_prefactor = coulomb_prefactor;
+    p->twopi_lx = TWOPI_LX;
+    p->twopi_ly = TWOPI_LY;
+    p->twopi_lz = TWOPI_LZ;
+    p->bfactor_ewald = Bfactor_ewald;
+    p->vfactor_ewald = Vfactor_ewald;
+    p->twopi_over_3v = twopi_over_3v;
+    p->wolfvcon1 = wolfvcon1;
+    p->wolfvcon2 = wolfvcon2;
+    p->wolfvcon3 = wolfvcon3;
+    p->wolffcon1 = wolffcon1;
+    p->wolffcon2 = wolffcon2;
+    if (iSF_type == MSPMS_SF_HYPERGEO) {
+        p->ntube = ntube;
+        p->tube_x = hgntc_xx;
+        p->tube_y = hgntc_yy;
+        p->tube_r = hgnt_radius;
+        p->const_3pisq_theta = const_3pisq_theta;
+        p->solid_sigma = solid_sigma ? *solid_sigma : 0.0;
+        p->solid_epsilon = solid_epsilon ? *solid_epsilon : 0.0;
+    } else if (iSF_type == MSPMS_SF_ATOM_EXPLICIT) {
+        p->solid_natom = solid_natom;
+        p->solid_xx = solid_xx;
+        p->solid_yy = solid_yy;
+        p->solid_zz = solid_zz;
+        p->solid_sigma = solid_sigma ? *solid_sigma : 0.0;
+        p->solid_epsilon = solid_epsilon ? *solid_epsilon : 0.0;
+    }
+}
+
+static void build_species(mspms_species* out)
+{
+    int s, k;
+    for (s = 0; s < nspecie; s++) {
+        const ref_sample_molecule* m = sample_mole + s;
+        mspms_species* q = out + s;
+        memset(q, 0, sizeof *q);
+        q->natom = m->natom;
+        q->nmole = nmole_per_specie[s];
+        q->ghost_type = m->ghost_type;
+        q->interp_type = m->interp_type;
+        q->aw = m->aw;
+        q->sigma = m->sigma;
+        q->epsilon = m->epsilon;
+        q->charge = m->charge;
+        q->nbond = m->nbond;
+        q->bnd_idx = &m->bnd_idx[0][0];
+        q->bnd_type = m->bnd_type;
+        for (k = 0; k < m->nbond; k++) {
+            g_bnd[s][3 * k] = m->bnd_para_0[k];
+            g_bnd[s][3 * k + 1] = m->bnd_para_1[k];
+            g_bnd[s][3 * k + 2] = m->bnd_para_2[k];
+        }
+        q->bnd_para = g_bnd[s];
+        q->nangle = m->nangle;
+        q->agl_idx = &m->agl_idx[0][0];
+        q->agl_type = m->agl_type;
+        for (k = 0; k < m->nangle; k++) {
+            g_agl[s][5 * k] = m->agl_para_1[k];
+            g_agl[s][5 * k + 1] = m->agl_para_2[k];
+            g_agl[s][5 * k + 2] = m->agl_para_3[k];
+            g_agl[s][5 * k + 3] = m->agl_para_4[k];
+            g_agl[s][5 * k + 4] = m->agl_para_5[k];
+            g_au[s][k] = m->isAngle_unique[k] ? 1 : 0;
+        }
+        q->agl_para = g_agl[s];
+        q->agl_unique = g_au[s];
+        q->ndih = m->ndih;
+        q->dih_idx = &m->dih_idx[0][0];
+        q->dih_type = m->dih_type;
+        for (k = 0; k < m->ndih; k++) {
+            g_dih[s][4 * k] = m->dih_para_1[k];
+            g_dih[s][4 * k + 1] = m->dih_para_2[k];
+            g_dih[s][4 * k + 2] = m->dih_para_3[k];
+            g_dih[s][4 * k + 3] = m->dih_para_4[k];
+            g_du[s][k] = m->isDih_unique[k] ? 1 : 0;
+        }
+        q->dih_para = g_dih[s];
+        q->dih_unique = g_du[s];
+        q->nimp = m->nimp;
+        q->imp_idx = &m->imp_idx[0][0];
+        q->imp_type = m->imp_type;
+        for (k = 0; k < m->nimp; k++) {
+            g_imp[s][2 * k] = m->imp_para_0[k];
+            g_imp[s][2 * k + 1] = m->imp_para_1[k];
+        }
+        q->imp_para = g_imp[s];
+        q->nnbp = m->nnbp;
+    }
+}
+
+static void ensure_context(void)
+{
+    int s, rc, same = (g_ctx != NULL && g_nspecie == nspecie && g_natom == natom);
+    mspms_params p;
+    mspms_species sp[MAXSP];
+    if (nspecie < 1 || nspecie > MAXSP) die("setup (nspecie)", MSPMS_EINVAL);
+    for (s = 0; same && s < nspecie; s++) same = g_nmole[s] == nmole_per_specie[s];
+    read_params(&p);
+    if (!g_ctx) {
+        const char* prec = getenv("MSPMS_GPU_PRECISION");
+        const char* dev = getenv("MSPMS_GPU_DEVICE");
+        int precision = (prec && !strcmp(prec, "fp64")) ? MSPMS_PREC_FP64 : MSPMS_PREC_MIXED;
+        rc = mspms_create(&g_ctx, dev ? atoi(dev) : 0, precision);
+        if (rc) die("mspms_create", rc);
+    }
+    if (!same) {
+        build_species(sp);
+        rc = mspms_set_system(g_ctx, nspecie, sp, &p);
+        if (rc) die("mspms_set_system", rc);
+        g_nspecie = nspecie;
+        g_natom = natom;
+        for (s = 0; s < nspecie; s++) g_nmole[s] = nmole_per_specie[s];
+        g_params = p;
+    } else if (memcmp(&p, &g_params, sizeof p) != 0) {
+        rc = mspms_set_params(g_ctx, &p);
+        if (rc) die("mspms_set_params", rc);
+        g_params = p;
+    }
+}
+
+int frclong(void)
+{
+    mspms_long_terms t;
+    int rc;
+    ensure_context();
+    rc = mspms_frclong_host(g_ctx, xx, yy, zz, fxl, fyl, fzl, &t);
+    if (rc) die("frclong", rc);
+    uvdw = t.uvdw;
+    ushift = t.ushift;
+    ureal = t.ureal;
+    uexcl = t.uexcl;
+    uelec = t.uelec;
+    if (iChargeType == MSPMS_ELEC_EWALD) {
+        /* the reference leaves these untouched for other charge types */
+        ufourier = t.ufourier;
+        uself = t.uself;
+        if (fEwald_BC == MSPMS_EWALD_BC_VACUUM) uvacuum = t.uvacuum;
+    } else if (iChargeType == MSPMS_ELEC_WOLF) {
+        uself = t.uself;
+    }
+    usflj = t.usflj;
+    uotherff = t.uotherff;
+    uinter = t.uinter;
+    virial_inter = t.virial_inter;
+    return 0;
+}
+
+int frcshort(void)
+{
+    mspms_short_terms t;
+    int rc;
+    ensure_context();
+    rc = mspms_frcshort_host(g_ctx, xx, yy, zz, fxs, fys, fzs, &t);
+    if (rc) die("frcshort", rc);
+    ubond = t.ubond;
+    uangle = t.uangle;
+    udih = t.udih;
+    uimp = t.uimp;
+    uintra = t.uintra;
+    virial_intra = t.virial_intra;
+    return 0;
+}
+
+void mspms_dropin_release(void)
+{
+    if (g_ctx) mspms_destroy(g_ctx);
+    g_ctx = NULL;
+    g_nspecie = g_natom = -1;
+}

@@ -148,14 +148,17 @@ struct mspms_ctx {
     DevBuf<Status> d_status;
     Status* h_status = nullptr;  // pinned
 
-    // host-API staging
-    DevBuf<double> d_pos, d_fl, d_fs;
+    // host-API staging and integrator scratch
+    DevBuf<double> d_pos, d_fl, d_fs, d_kin;
 
-    // profiling
+    // profiling: a pool of events, one (start, stop) pair per recorded
+    // kernel group, read back at the next synchronisation
     bool profiling = false;
     std::vector<hipEvent_t> ev;
-    std::vector<double> prof_ms;
-    int prof_calls = 0;
+    size_t ev_used = 0;
+    std::vector<int> pending;          // group of each recorded pair
+    std::vector<double> prof_ms;       // summed milliseconds per group
+    std::vector<long> prof_n;          // recorded launches per group
 };
 
 namespace {
@@ -423,6 +426,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_pos.release();
     c->d_fl.release();
     c->d_fs.release();
+    c->d_kin.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -816,7 +820,8 @@ int build(mspms_ctx* c) {
         (rc = c->d_ep_short.ensure((size_t)c->nmol * 5)) ||
         (rc = c->d_ep_sf.ensure((size_t)c->natom)) || (rc = c->d_long.ensure(c->nrep)) ||
         (rc = c->d_short.ensure(c->nrep)) || (rc = c->d_pos.ensure((size_t)3 * c->natom)) ||
-        (rc = c->d_fl.ensure((size_t)3 * c->natom)) || (rc = c->d_fs.ensure((size_t)3 * c->natom)))
+        (rc = c->d_fl.ensure((size_t)3 * c->natom)) || (rc = c->d_fs.ensure((size_t)3 * c->natom)) ||
+        (rc = c->d_kin.ensure((size_t)c->natom)))
         return rc;
     HIPC(hipStreamSynchronize(s));
     return 0;
@@ -911,17 +916,38 @@ extern "C" int mspms_nreplica(const mspms_ctx* c) { return c ? c->nrep : 0; }
 // ---- evaluation -----------------------------------------------------------
 namespace {
 
-enum { PROF_CLUSTER = 0, PROF_PAIR, PROF_INTRA, PROF_RECIP, PROF_FINAL, PROF_BONDED, PROF_N };
+enum {
+    PROF_CLUSTER = 0, PROF_PAIR, PROF_INTRA, PROF_RECIP_S, PROF_RECIP_F, PROF_FINAL, PROF_BONDED,
+    PROF_N
+};
 
 struct ProfScope {
     mspms_ctx* c;
     hipStream_t s;
     int slot;
+    size_t idx = 0;
+    bool on = false;
     ProfScope(mspms_ctx* c_, hipStream_t s_, int slot_) : c(c_), s(s_), slot(slot_) {
-        if (c->profiling) (void)hipEventRecord(c->ev[2 * slot], s);
+        if (!c->profiling) return;
+        if (c->ev_used + 2 > c->ev.size()) {
+            size_t old = c->ev.size(), grow = std::max<size_t>(64, old);
+            c->ev.resize(old + grow);
+            for (size_t k = old; k < c->ev.size(); k++)
+                if (hipEventCreate(&c->ev[k]) != hipSuccess) {
+                    c->ev.resize(k);
+                    return;
+                }
+            if (c->ev_used + 2 > c->ev.size()) return;
+        }
+        idx = c->ev_used;
+        c->ev_used += 2;
+        on = true;
+        (void)hipEventRecord(c->ev[idx], s);
     }
     ~ProfScope() {
-        if (c->profiling) (void)hipEventRecord(c->ev[2 * slot + 1], s);
+        if (!on) return;
+        (void)hipEventRecord(c->ev[idx + 1], s);
+        c->pending.push_back(slot);
     }
 };
 
@@ -949,8 +975,12 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p);
     }
     if (c->any_recip) {
-        ProfScope p(c, s, PROF_RECIP);
-        launch_recip(L, c->rb, x, y, z, fx, fy, fz);
+        {
+            ProfScope p(c, s, PROF_RECIP_S);
+            launch_recip_s(L, c->rb, x, y, z);
+        }
+        ProfScope p(c, s, PROF_RECIP_F);
+        launch_recip_f(L, c->rb, x, y, z, fx, fy, fz);
     }
     {
         ProfScope p(c, s, PROF_FINAL);
@@ -979,13 +1009,17 @@ int read_status(mspms_ctx* c, hipStream_t s) {
 }
 
 void accumulate_profile(mspms_ctx* c) {
-    if (!c->profiling) return;
-    for (int k = 0; k < PROF_N; k++) {
+    if (!c->profiling || c->pending.empty()) return;
+    (void)hipEventSynchronize(c->ev[2 * c->pending.size() - 1]);
+    for (size_t p = 0; p < c->pending.size(); p++) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess)
-            c->prof_ms[k] += ms;
+        if (hipEventElapsedTime(&ms, c->ev[2 * p], c->ev[2 * p + 1]) == hipSuccess) {
+            c->prof_ms[c->pending[p]] += ms;
+            c->prof_n[c->pending[p]]++;
+        }
     }
-    c->prof_calls++;
+    c->pending.clear();
+    c->ev_used = 0;
 }
 
 }  // namespace
@@ -1082,27 +1116,58 @@ extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double*, const double
     return fail(MSPMS_EINVAL, "parameter gradients are not built yet");
 }
 
+// One RESPA velocity-Verlet outer step, NVE (src/md.c:22-83), all buffers
+// on the device, (3, N) SoA: pos, vel, long forces fl (in: forces of the
+// current positions, out: of the new ones), short forces fs (same).
+extern "C" int mspms_vver_step_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                      double* fs, double delt, int nstep_inner,
+                                      mspms_long_terms* lterms, mspms_short_terms* sterms,
+                                      double* ukin, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (nstep_inner < 1) return fail(MSPMS_EINVAL, "nstep_inner must be >= 1");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const size_t n = (size_t)c->natom;
+    // src/initvars.c:605-607
+    const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
+    LaunchCtx L = launch_ctx(c, s);
+    {
+        ProfScope p(c, s, PROF_BONDED);
+        launch_respa_inner(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, vel,
+                           vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
+                           deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
+        launch_finalize_short(L, c->d_ep_short.p, sterms);
+    }
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s);
+    if (rc) return rc;
+    launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
+                       c->d_kin.p, ukin);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
 // ---- profiling ------------------------------------------------------------
 extern "C" int mspms_set_profiling(mspms_ctx* c, int on) {
     if (!c) return fail(MSPMS_EINVAL, "null ctx");
-    if (on && c->ev.empty()) {
-        c->ev.resize(2 * PROF_N);
-        for (auto& e : c->ev) HIPC(hipEventCreate(&e));
-    }
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipDeviceSynchronize());
     c->profiling = on != 0;
+    c->pending.clear();
+    c->ev_used = 0;
     c->prof_ms.assign(PROF_N, 0.0);
-    c->prof_calls = 0;
+    c->prof_n.assign(PROF_N, 0);
     return 0;
 }
 
 // ms[k] = mean milliseconds per call of kernel group k since profiling was
-// switched on: 0 clustering, 1 pair, 2 intramolecular, 3 reciprocal,
-// 4 finalize, 5 bonded.  Returns the number of groups written.
+// switched on: 0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,
+// 3 reciprocal S(k) + reduction, 4 reciprocal forces, 5 finalize,
+// 6 bonded / RESPA inner loop.  Returns the number of groups written.
 extern "C" int mspms_last_timings(const mspms_ctx* c, double* ms, int n) {
     if (!c || !ms) return 0;
     int k = 0;
     for (; k < n && k < PROF_N; k++)
-        ms[k] = c->prof_calls ? c->prof_ms[k] / c->prof_calls : 0.0;
+        ms[k] = (k < (int)c->prof_n.size() && c->prof_n[k]) ? c->prof_ms[k] / c->prof_n[k] : 0.0;
     return k;
 }
 
@@ -1110,6 +1175,9 @@ extern "C" int mspms_last_timings(const mspms_ctx* c, double* ms, int n) {
 // status of the last device-API evaluation.
 extern "C" int mspms_sync(mspms_ctx* c) {
     if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    HIPC(hipSetDevice(c->device));
+    // device-buffer calls may have run on a caller's stream
+    HIPC(hipDeviceSynchronize());
     int rc = read_status(c, c->stream);
     if (rc) return rc;
     accumulate_profile(c);

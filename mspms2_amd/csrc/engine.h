@@ -151,9 +151,11 @@ void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
                    const double* z, double* fx, double* fy, double* fz,
                    double* epart);
 int recip_nkz_supported(int kmaxz);
-void launch_recip(const LaunchCtx& c, const RecipBufs& rb, const double* x,
-                  const double* y, const double* z, double* fx, double* fy,
-                  double* fz);
+void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x,
+                    const double* y, const double* z);
+void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x,
+                    const double* y, const double* z, double* fx, double* fy,
+                    double* fz);
 void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_pair,
                           const unsigned long long* ip_pair, int have_pair,
@@ -162,3 +164,13 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_sf, int have_sf, void* terms);
 void launch_finalize_short(const LaunchCtx& c, const double* epart,
                            void* terms);
+void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species,
+                        const TermDev* terms, double* x, double* y, double* z,
+                        double* vx, double* vy, double* vz, const double* fxl,
+                        const double* fyl, const double* fzl, double* fxs,
+                        double* fys, double* fzs, double deltby2, double delts,
+                        double deltsby2, int nstep_inner, double* epart);
+void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
+                        const double* fxl, const double* fyl,
+                        const double* fzl, double deltby2, double* kin,
+                        double* ukin);

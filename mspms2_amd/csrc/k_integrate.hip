@@ -1,0 +1,134 @@
+// k_integrate.hip -- device-resident RESPA velocity Verlet (NVE) on gfx950.
+// Reference: vver, src/md.c:22-83 (called from src/mspms2.c:99-132 and
+// md_move, src/dispmd.c:42-46).
+//
+// One outer step:
+//   k_respa_inner   per molecule (one thread): half kick with the long
+//                   forces, then nstep_inner times {half kick + drift with
+//                   the short forces, bonded forces of this molecule
+//                   (frcshort's terms), half kick}.  The inner loop touches
+//                   only the molecule's own atoms, so all nstep_inner
+//                   substeps run in one launch with no grid-wide sync.
+//   frclong         (the engine's long-range kernels)
+//   k_respa_final   per atom: half kick with the new long forces and the
+//                   kinetic energy partial (src/md.c:67-80)
+// Per-atom arithmetic follows md.c expression for expression.
+#include "engine.h"
+#include "terms.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_respa_inner(
+    int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
+    const TermDev* __restrict__ terms, const SiteDev* __restrict__ sites, double* x, double* y,
+    double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
+    const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
+    double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
+    Status* st) {
+#pragma clang fp contract(off)
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const MolDev M = mols[m];
+    const SpeciesDev S = species[M.species];
+    const int a0 = M.atom0, a1 = M.atom0 + M.natom;
+    for (int i = a0; i < a1; i++) {
+        double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
+        vx[i] += (deltby2 * fxl[i] * r_mass);
+        vy[i] += (deltby2 * fyl[i] * r_mass);
+        vz[i] += (deltby2 * fzl[i] * r_mass);
+    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    for (int ll = 0; ll < nstep_inner; ll++) {
+        for (int i = a0; i < a1; i++) {
+            double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
+            vx[i] += (deltsby2 * fxs[i] * r_mass);
+            vy[i] += (deltsby2 * fys[i] * r_mass);
+            vz[i] += (deltsby2 * fzs[i] * r_mass);
+            x[i] = x[i] + delts * vx[i];
+            y[i] = y[i] + delts * vy[i];
+            z[i] = z[i] + delts * vz[i];
+        }
+        // frcshort of this molecule (src/frcshort.c:33-87)
+        for (int i = a0; i < a1; i++) fxs[i] = fys[i] = fzs[i] = 0.0;
+        e[0] = e[1] = e[2] = e[3] = 0.0;
+        vir = 0.0;
+        for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
+            double u = 0.0, v = 0.0;
+            int cls = bonded_term(terms[t], a0, x, y, z, fxs, fys, fzs, &u, &v, &st->flags);
+            e[cls] += u;
+            vir += v;
+        }
+        for (int i = a0; i < a1; i++) {
+            double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
+            vx[i] += (deltsby2 * fxs[i] * r_mass);
+            vy[i] += (deltsby2 * fys[i] * r_mass);
+            vz[i] += (deltsby2 * fzs[i] * r_mass);
+        }
+    }
+    double* ep = epart + (size_t)m * 5;
+    ep[0] = e[0];
+    ep[1] = e[1];
+    ep[2] = e[2];
+    ep[3] = e[3];
+    ep[4] = vir;
+}
+
+__global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __restrict__ atom_site,
+                                                     const SiteDev* __restrict__ sites, double* vx,
+                                                     double* vy, double* vz,
+                                                     const double* __restrict__ fxl,
+                                                     const double* __restrict__ fyl,
+                                                     const double* __restrict__ fzl,
+                                                     double deltby2, double* kin) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    double r_mass = 1.0 / sites[atom_site[i]].aw;
+    double a = vx[i] + (deltby2 * fxl[i] * r_mass);
+    double b = vy[i] + (deltby2 * fyl[i] * r_mass);
+    double c = vz[i] + (deltby2 * fzl[i] * r_mass);
+    vx[i] = a;
+    vy[i] = b;
+    vz[i] = c;
+    kin[i] = (a * a + b * b + c * c) / r_mass;
+}
+
+// per replica: ukin = 0.5 * sum (v.v)/r_mass in atom order blocks
+__global__ void __launch_bounds__(256) k_kinetic(const RepDev* __restrict__ reps,
+                                                 const double* __restrict__ kin,
+                                                 double* __restrict__ ukin) {
+    __shared__ double sh[8];
+    const RepDev& R = reps[blockIdx.x];
+    double s = 0.0;
+    for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x) s += kin[a];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+        ukin[blockIdx.x] = t * 0.5;
+    }
+}
+
+}  // namespace
+
+void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
+                        double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                        const double* fxl, const double* fyl, const double* fzl, double* fxs,
+                        double* fys, double* fzs, double deltby2, double delts,
+                        double deltsby2, int nstep_inner, double* epart) {
+    int g = (c.nmol + 255) / 256;
+    hipLaunchKernelGGL(k_respa_inner, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, species,
+                       terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl, fxs, fys, fzs, deltby2,
+                       delts, deltsby2, nstep_inner, epart, c.status);
+}
+
+void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
+                        const double* fxl, const double* fyl, const double* fzl,
+                        double deltby2, double* kin, double* ukin) {
+    int g = (c.natom + 255) / 256;
+    hipLaunchKernelGGL(k_respa_final, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site,
+                       c.sites, vx, vy, vz, fxl, fyl, fzl, deltby2, kin);
+    hipLaunchKernelGGL(k_kinetic, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, kin, ukin);
+}

@@ -223,8 +223,8 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
 
 int recip_nkz_supported(int kmaxz) { return kmaxz <= 16; }
 
-void launch_recip(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
-                  const double* z, double* fx, double* fy, double* fz) {
+void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
+                    const double* z) {
     size_t lds = (size_t)64 * rb.ntab_max * sizeof(double2);
     if (rb.nkz_max <= 9)
         hipLaunchKernelGGL(k_sk<9>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s, c.reps,
@@ -240,6 +240,10 @@ void launch_recip(const LaunchCtx& c, const RecipBufs& rb, const double* x, cons
     hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 255) / 256), dim3(256), 0, c.stream,
                        rb.nk_total, rb.k_rep, c.reps, rb.rep_chunk0, rb.s_part, rb.nk_stride,
                        rb.kvec, rb.ak, rb.uk);
+}
+
+void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
+                    const double* z, double* fx, double* fy, double* fz) {
     hipLaunchKernelGGL(k_kforce, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f, c.reps,
                        rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz);
 }

@@ -25,6 +25,7 @@ EXPORTS = [
     "mspms_natom", "mspms_nreplica", "mspms_frclong_host", "mspms_frcshort_host",
     "mspms_frclong_device", "mspms_frcshort_device", "mspms_energy_param_grad",
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
+    "mspms_vver_step_device",
 ]
 
 
@@ -57,6 +58,8 @@ def lib():
     L.mspms_frcshort_host.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.POINTER(abi.ShortTerms)]
     L.mspms_frclong_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.mspms_frcshort_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mspms_vver_step_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp,
+                                         vp]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -77,7 +80,7 @@ def derive_params(p: abi.Params) -> abi.Params:
     return p
 
 
-PROFILE_GROUPS = ("cluster", "pair", "intra", "recip", "finalize", "bonded")
+PROFILE_GROUPS = ("cluster", "pair", "intra", "recip_s", "recip_f", "finalize", "bonded")
 
 
 class Engine:
@@ -172,6 +175,16 @@ class Engine:
         _check(lib().mspms_frcshort_device(self._h, px, px + 8 * n, px + 16 * n, fx, fx + 8 * n,
                                            fx + 16 * n, self._ptr(terms), stream),
                "mspms_frcshort_device")
+
+    def vver_step_device(self, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin,
+                         stream=None):
+        """One RESPA velocity-Verlet outer step on device tensors (src/md.c
+        vver); delt in reduced time units."""
+        _check(lib().mspms_vver_step_device(self._h, self._ptr(pos), self._ptr(vel),
+                                            self._ptr(fl), self._ptr(fs), float(delt),
+                                            int(nstep_inner), self._ptr(lterms),
+                                            self._ptr(sterms), self._ptr(ukin), stream),
+               "mspms_vver_step_device")
 
     def sync(self):
         _check(lib().mspms_sync(self._h), "mspms_sync")

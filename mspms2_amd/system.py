@@ -160,13 +160,18 @@ class System:
         xx = np.ascontiguousarray(xyz[:, 0] / sb)
         yy = np.ascontiguousarray(xyz[:, 1] / sb)
         zz = np.ascontiguousarray(xyz[:, 2] / sb)
+        # the ELECTROSTATIC section is read only for Ewald (kappa, KMAX*,
+        # KSQMAX, boundary) and Wolf (kappa) (src/initvars.c:421-459)
+        ewald = r.iChargeType == abi.ELEC_EWALD
         p = dict(
             sigma_base=sb, epsilon_base=eb,
             boxlx=r.boxlx / sb, boxly=r.boxly / sb, boxlz=r.boxlz / sb,
             rcuton=r.rcuton / sb, rcutoff=r.rcutoff / sb, rcutoffelec=r.rcutoffelec / sb,
             isLJswitchOn=r.isLJswitchOn, f0=r.f0, iChargeType=r.iChargeType,
-            kappa=r.kappa * sb, kmaxx=r.kmaxx, kmaxy=r.kmaxy, kmaxz=r.kmaxz,
-            ksqmax=r.ksqmax, fEwald_BC=r.fEwald_BC, iSF_type=r.iSF_type,
+            kappa=r.kappa * sb if r.iChargeType in (abi.ELEC_EWALD, abi.ELEC_WOLF) else 0.0,
+            kmaxx=r.kmaxx if ewald else 0, kmaxy=r.kmaxy if ewald else 0,
+            kmaxz=r.kmaxz if ewald else 0, ksqmax=r.ksqmax if ewald else 0,
+            fEwald_BC=r.fEwald_BC if ewald else 0, iSF_type=r.iSF_type,
             iExternal_FF_type=r.iExternal_FF_type)
         if r.iSF_type == abi.SF_HYPERGEO:
             p.update(ntube=len(r.tubes),

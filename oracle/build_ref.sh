@@ -89,3 +89,15 @@ EOF
     ./layout > "$OUT/sample_molecule_layout.txt"
 )
 echo "build_ref.sh: built $OUT/mspms_ref_harness"
+
+# Integration harness: the reference's start-up path and callers with the
+# engine's drop-in frclong/frcshort (libmspms_dropin.a + libmspms_gpu.so)
+# linked in place of src/frclong.c and src/frcshort.c.
+PKG="$HERE/../mspms2_amd"
+if [ -f "$PKG/libmspms_dropin.a" ] && [ -f "$PKG/libmspms_gpu.so" ]; then
+    gcc -fcommon -o "$OUT/mspms_dropin_harness" "$OUT/ref_harness.o" "$PKG/libmspms_dropin.a" \
+        "$OUT/libmspms_ref_callers.a" -L"$PKG" -lmspms_gpu -lm \
+        -Wl,-rpath,'$ORIGIN/../../mspms2_amd' -Wl,-rpath-link,/opt/rocm/lib \
+        -Wl,--unresolved-symbols=ignore-in-object-files
+    echo "build_ref.sh: built $OUT/mspms_dropin_harness"
+fi

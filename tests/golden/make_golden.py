@@ -1,0 +1,104 @@
+"""Writes the golden fixtures of tests/golden from the reference build.
+
+    python tests/golden/make_golden.py      (needs oracle/_ref, i.e. the
+                                             reference checkout; run here)
+
+For each case: the run directory (in/cfg/xyz.mspms, the reference's own
+input format) and ref.npz with the reference's outputs after its start-up
+path + frclong() + frcshort() (oracle/ref_harness.c): every energy /
+virial global, the derived constants, the long and short forces.  The
+12,288-site water case keeps only the seed, the SHA-256 of its run files,
+all scalars and the forces of every 12th atom.
+
+examples_water holds the input files of the reference's Examples/Water
+(flexible Toukan-Rahman water with Morse bonds), unchanged.
+"""
+import hashlib
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from mspms2_amd import abi, configs  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+REF = os.environ.get("MSPMS_REF_DIR", "/root/reference")
+
+CASES = {
+    "lj864_lattice": lambda: configs.lj_fcc(),
+    "lj864_jitter": lambda: configs.lj_fcc(jitter=0.05, seed=7),
+    "water216_ewald": lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
+                                                kmax=5, ksqmax=27, seed=3),
+    "water216_wolf": lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.2,
+                                               elec="wolf", seed=4),
+    "water216_coulomb": lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0,
+                                                  elec="coulomb", seed=4),
+    "butane64": lambda: configs.butane_box(nmol=64, seed=5),
+    "tube300_hypergeo": lambda: configs.methane_tube(natom=300, length=60.0, box_xy=40.0,
+                                                     tube_r=15.0, rc=10.0),
+    "mixture_ewald": lambda: configs.mixture_box(),
+    "mixture_water4_vacuum": lambda: configs.mixture_box(water4=True, bc=abi.EWALD_BC_VACUUM,
+                                                         seed=12),
+    "mixture_wolf_switch": lambda: configs.mixture_box(elec="wolf", kappa=0.2, switch=True,
+                                                       seed=13),
+    "bonded_zoo": lambda: configs.bonded_zoo(),
+}
+
+BIG = {
+    "water12288_ewald": lambda: configs.water_box(n_side=16, L=49.66, kappa=0.28, kmax=8,
+                                                  ksqmax=64, rc=10.0, seed=2),
+}
+
+
+def _sha(path):
+    h = hashlib.sha256()
+    for fn in ("in.mspms", "cfg.mspms", "xyz.mspms"):
+        with open(os.path.join(path, fn), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def save(path, d, keep_every=1):
+    sc = {k: np.float64(v) for k, v in d.items() if isinstance(v, float)}
+    idx = np.arange(0, d["natom"], keep_every)
+    np.savez_compressed(os.path.join(path, "ref.npz"), fl=d["fl"][:, idx], fs=d["fs"][:, idx],
+                        idx=idx, natom=d["natom"], **{f"s_{k}": v for k, v in sc.items()})
+
+
+def main():
+    if not O.reference_available():
+        raise SystemExit("oracle/_ref is not built (needs the reference checkout)")
+    for name, fn in CASES.items():
+        path = os.path.join(HERE, name)
+        os.makedirs(path, exist_ok=True)
+        fn().write(path)
+        d = O.run_reference(path)
+        save(path, d)
+        print(name, d["natom"], d["uinter"], d["uintra"])
+    ew = os.path.join(HERE, "examples_water")
+    os.makedirs(ew, exist_ok=True)
+    for fnm in ("in.mspms", "cfg.mspms", "xyz.mspms"):
+        shutil.copy(os.path.join(REF, "Examples", "Water", fnm), os.path.join(ew, fnm))
+    d = O.run_reference(ew)
+    save(ew, d)
+    print("examples_water", d["natom"], d["uinter"], d["uintra"])
+    for name, fn in BIG.items():
+        path = os.path.join(HERE, name)
+        os.makedirs(path, exist_ok=True)
+        with tempfile.TemporaryDirectory() as tmp:
+            fn().write(tmp)
+            d = O.run_reference(tmp)
+            with open(os.path.join(path, "runfiles.sha256"), "w") as f:
+                f.write(_sha(tmp) + "\n")
+        save(path, d, keep_every=12)
+        print(name, d["natom"], d["uinter"], d["uintra"])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,60 @@
+"""Drop-in integration on the GPU: the reference's own start-up path
+(readins, init_vars: src/mspms2.c:39-49) linked with the engine's frclong /
+frcshort (libmspms_dropin.a + libmspms_gpu.so) in place of src/frclong.c
+and src/frcshort.c, on the golden run directories.  The dump must equal the
+reference's (tests/golden/*/ref.npz) to 1e-10 in FP64 mode.
+
+The harness binary is built from the reference sources by
+oracle/build_ref.sh (here); it travels to the GPU box prebuilt.
+"""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.util import GOLDEN, assert_long_close, assert_short_close
+
+pytestmark = pytest.mark.gpu
+
+HARNESS = os.path.join(os.path.dirname(O.__file__), "_ref", "mspms_dropin_harness")
+
+CASES = ["lj864_jitter", "water216_ewald", "water216_wolf", "butane64", "mixture_ewald",
+         "mixture_wolf_switch", "bonded_zoo", "examples_water"]
+
+
+def _golden(name):
+    z = np.load(os.path.join(GOLDEN, name, "ref.npz"))
+    sc = {k[2:]: float(z[k]) for k in z.files if k.startswith("s_")}
+    return sc, z["fl"], z["fs"]
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dropin(name, precision):
+    if not os.path.exists(HARNESS):
+        pytest.fail(f"{HARNESS} missing: run __graft_entry__.build() where the reference is")
+    print(f"dropin {name} {precision}", flush=True)
+    sc, fl, fs = _golden(name)
+    with tempfile.TemporaryDirectory(prefix="mspms_dropin_") as tmp:
+        for fn in ("in.mspms", "cfg.mspms", "xyz.mspms"):
+            shutil.copy(os.path.join(GOLDEN, name, fn), tmp)
+        env = dict(os.environ, MSPMS_GPU_PRECISION=precision)
+        out = os.path.join(tmp, "dump.bin")
+        r = subprocess.run([HARNESS, tmp, out], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        d = O.read_ref_dump(out)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert_long_close(d, sc, tol, name)
+    assert_short_close(d, sc, 1e-10, name)
+    fscale = max(np.linalg.norm(fl), (abs(sc["virial_inter"]) + abs(sc["uvdw"])) /
+                 (d["natom"] * sc["rcutoff"]) * np.sqrt(fl.size))
+    err = np.linalg.norm(d["fl"] - fl) / fscale
+    print(f"  force rel {err:.2e}")
+    assert err <= tol
+    if np.linalg.norm(fs) > 0:
+        assert np.linalg.norm(d["fs"] - fs) <= 1e-10 * np.linalg.norm(fs) + 1e-12
