@@ -142,6 +142,13 @@ struct mspms_ctx {
 
     // partial sums and outputs
     DevBuf<double> d_ep_pair, d_ep_intra, d_ep_short, d_ep_sf;
+
+    // solid-fluid tables
+    SFDev sf{};
+    DevBuf<double> d_sf_tube, d_sf_solid, d_sf_minpe;
+    DevBuf<float> d_sf_nodes;
+    size_t sf_nodes_uploaded = 0;
+    const float* sf_nodes_src = nullptr;
     DevBuf<unsigned long long> d_ip_pair;
     DevBuf<mspms_long_terms> d_long;
     DevBuf<mspms_short_terms> d_short;
@@ -295,8 +302,19 @@ int validate_params(const mspms_params& p) {
             p.kmaxz > 16)
             return fail(MSPMS_EINVAL, "KMAX must be within 0..16");
     }
-    if (p.iSF_type != MSPMS_SF_NONE)
-        return fail(MSPMS_EINVAL, "solid-fluid type %d is not implemented on the GPU", p.iSF_type);
+    if (p.iSF_type < MSPMS_SF_NONE || p.iSF_type > MSPMS_SF_TASOS)
+        return fail(MSPMS_EINVAL,
+                    "solid-fluid type %d unsupported (4, my-interp, has no grid generator)",
+                    p.iSF_type);
+    if (p.iSF_type == MSPMS_SF_HYPERGEO && (p.ntube < 1 || !p.tube_x || !p.tube_y || !p.tube_r))
+        return fail(MSPMS_EINVAL, "hypergeometric tube needs ntube >= 1 and tube arrays");
+    if (p.iSF_type == MSPMS_SF_ATOM_EXPLICIT && p.solid_natom > 0 &&
+        (!p.solid_xx || !p.solid_yy || !p.solid_zz))
+        return fail(MSPMS_EINVAL, "explicit solid needs solid_xx/yy/zz");
+    if (p.iSF_type == MSPMS_SF_TASOS &&
+        (!p.grid || !p.grid->nodes || p.grid->ntype < 1 || p.grid->nx < 2 || p.grid->ny < 2 ||
+         p.grid->nz < 2 || !(p.grid->hx > 0) || !(p.grid->hy > 0) || !(p.grid->hz > 0)))
+        return fail(MSPMS_EINVAL, "SF grid missing or malformed");
     return 0;
 }
 
@@ -814,6 +832,71 @@ int build(mspms_ctx* c) {
         R.ntab_max = ntab_max;
         R.nkz_max = nkz_max;
     }
+    // ---- solid-fluid tables (replica 0's model, shared) ----
+    memset(&c->sf, 0, sizeof c->sf);
+    if (c->any_sf) {
+        const mspms_params& P = c->params[0];
+        for (int r = 1; r < c->nrep; r++)
+            if (c->params[r].iSF_type != P.iSF_type)
+                return fail(MSPMS_EINVAL, "replicas must share one solid-fluid model");
+        SFDev& F = c->sf;
+        F.type = P.iSF_type;
+        F.solid_sigma = P.solid_sigma;
+        F.solid_epsilon = P.solid_epsilon;
+        F.c3pisq = P.const_3pisq_theta;
+        F.sigma_base = P.sigma_base;
+        F.epsilon_base = P.epsilon_base;
+        if (F.type == MSPMS_SF_HYPERGEO) {
+            std::vector<double> t;
+            for (int k = 0; k < P.ntube; k++) {
+                t.push_back(P.tube_x[k]);
+                t.push_back(P.tube_y[k]);
+                t.push_back(P.tube_r[k]);
+            }
+            if ((rc = c->d_sf_tube.upload(t, s))) return rc;
+            F.ntube = P.ntube;
+            F.tube = c->d_sf_tube.p;
+        } else if (F.type == MSPMS_SF_ATOM_EXPLICIT) {
+            std::vector<double> t;
+            for (int k = 0; k < P.solid_natom; k++) {
+                t.push_back(P.solid_xx[k]);
+                t.push_back(P.solid_yy[k]);
+                t.push_back(P.solid_zz[k]);
+            }
+            if ((rc = c->d_sf_solid.upload(t, s))) return rc;
+            F.nsolid = P.solid_natom;
+            F.solid = c->d_sf_solid.p;
+        } else {
+            const mspms_sf_grid& G = *P.grid;
+            size_t nn = (size_t)G.nx * G.ny * G.nz, tot = nn * 8 * (size_t)G.ntype;
+            // the node table is large (up to GiB): upload once per source
+            if (c->sf_nodes_src != G.nodes || c->sf_nodes_uploaded != tot) {
+                if ((rc = c->d_sf_nodes.ensure(tot))) return rc;
+                HIPC(hipMemcpyAsync(c->d_sf_nodes.p, G.nodes, tot * sizeof(float),
+                                    hipMemcpyHostToDevice, s));
+                c->sf_nodes_src = G.nodes;
+                c->sf_nodes_uploaded = tot;
+            }
+            std::vector<double> mn(G.ntype, 10e10);  // grid.f90:61
+            for (int t = 0; t < G.ntype; t++)
+                for (size_t k = 0; k < nn; k++)
+                    mn[t] = std::min(mn[t], (double)G.nodes[(size_t)t * 8 * nn + k]);
+            if ((rc = c->d_sf_minpe.upload(mn, s))) return rc;
+            F.ntype = G.ntype;
+            F.nx = G.nx;
+            F.ny = G.ny;
+            F.nz = G.nz;
+            F.x0 = G.x0;
+            F.y0 = G.y0;
+            F.z0 = G.z0;
+            F.hx = G.hx;
+            F.hy = G.hy;
+            F.hz = G.hz;
+            F.nodes = c->d_sf_nodes.p;
+            F.min_pe = c->d_sf_minpe.p;
+        }
+    }
+
     // ---- partials and outputs ----
     if ((rc = c->d_ep_pair.ensure(maxcl * 4)) || (rc = c->d_ip_pair.ensure(maxcl * 4)) ||
         (rc = c->d_ep_intra.ensure((size_t)c->nmol * 4)) ||
@@ -918,6 +1001,7 @@ namespace {
 
 enum {
     PROF_CLUSTER = 0, PROF_PAIR, PROF_INTRA, PROF_RECIP_S, PROF_RECIP_F, PROF_FINAL, PROF_BONDED,
+    PROF_SF,
     PROF_N
 };
 
@@ -982,11 +1066,15 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         ProfScope p(c, s, PROF_RECIP_F);
         launch_recip_f(L, c->rb, x, y, z, fx, fy, fz);
     }
+    if (c->any_sf) {
+        ProfScope p(c, s, PROF_SF);
+        launch_sf(L, c->sf, x, y, z, fx, fy, fz, c->d_ep_sf.p);
+    }
     {
         ProfScope p(c, s, PROF_FINAL);
         launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1,
                              c->d_ep_intra.p, c->any_intra ? 1 : 0, c->rb.uk,
-                             c->any_recip ? 1 : 0, c->d_ep_sf.p, 0, terms);
+                             c->any_recip ? 1 : 0, c->d_ep_sf.p, c->any_sf ? 1 : 0, terms);
     }
     HIPC(hipGetLastError());
     return 0;

@@ -136,6 +136,23 @@ struct RecipBufs {
     int nk_total, ntab_max, nkz_max;
 };
 
+// solid-fluid model (shared by all replicas), passed by value to k_sf
+struct SFDev {
+    int type, ntube, nsolid, ntype;
+    const double* tube;      // [3*ntube] x, y, radius (reduced)
+    const double* solid;     // [3*nsolid] x, y, z (reduced)
+    double solid_sigma, solid_epsilon, c3pisq;
+    int nx, ny, nz, pad;     // Hermite grid (Angstrom, K)
+    double x0, y0, z0, hx, hy, hz;
+    double sigma_base, epsilon_base;
+    const float* nodes;      // [ntype][8][nz][ny][nx]
+    const double* min_pe;    // [ntype] min node F (mylibtasos/grid.f90:102)
+};
+
+void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
+               const double* y, const double* z, double* fx, double* fy,
+               double* fz, double* usf);
+
 void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x,
                           const double* y, const double* z, double* fx,
                           double* fy, double* fz, int precision);

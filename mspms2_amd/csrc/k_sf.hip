@@ -1,0 +1,219 @@
+// k_sf.hip -- solid-fluid terms of frclong on gfx950 (src/frclong.c:1145-1209).
+// One thread per site; only GHOST_NONE sites interact with the solid.
+//   SF_HYPERGEO       smooth-cylinder LJ tube, hypergeometric series
+//                     (cal_sf_hypergeo, src/sffrc.c:1092-1209), FP64
+//   SF_ATOM_EXPLICIT  LJ against explicit wall atoms
+//                     (cal_sf_atom_explicit, src/sffrc.c:1213-1314), FP64
+//   SF_TASOS          tricubic Hermite interpolation of an FP32 node grid
+//                     (call_tasos_forces, src/sffrc.c:1322-1337 ->
+//                     CForce_atom, mylibtasos/forces.f90:6-173): a gather of
+//                     8 corners x 8 fields = 256 B per site, FP64 arithmetic
+// Forces are added to the long-range forces (the site's own thread is the
+// only writer); the energy goes to a per-site partial for the finalize.
+#include "engine.h"
+#include "physics.h"
+
+namespace {
+
+__device__ inline double mic(double d, double L) {
+#pragma clang fp contract(off)
+    return d - L * rint(d / L);
+}
+
+// hypergeo (src/sffrc.c:1092-1116)
+__device__ inline void hypergeo(double a, double b, double c, double z, double rr, double* sumV,
+                                double* sumF) {
+#pragma clang fp contract(off)
+    double zn = 1.0, an1 = 1.0;
+    double sv = 1.0, sf = 0.0;
+    for (int i = 0; i <= 10; i++) {
+        double an = (a + i) * (b + i) / ((c + i) * (1 + i));
+        an1 = an1 * an;
+        zn *= z;
+        sv = sv + an1 * zn;
+        sf = sf - 2.0 * (i + 1.0) * an1 * zn / rr;
+    }
+    *sumV = sv;
+    *sumF = sf;
+}
+
+// 1-D cubic Hermite basis on t in [0,1], spacing h: value weights of the two
+// nodes, slope weights (times h), and their t-derivatives divided by h
+// (mylibtasos/cubicauxiliary.f90:1-60: g, -f, h w(t), -h w(1-t)).
+__device__ inline void hermite_1d(double t, double h, double b[4], double db[4]) {
+    double t2 = t * t, t3 = t2 * t;
+    b[0] = 2.0 * t3 - 3.0 * t2 + 1.0;
+    b[1] = 3.0 * t2 - 2.0 * t3;
+    b[2] = h * (t3 - 2.0 * t2 + t);
+    b[3] = h * (t3 - t2);
+    db[0] = (6.0 * t2 - 6.0 * t) / h;
+    db[1] = (6.0 * t - 6.0 * t2) / h;
+    db[2] = 3.0 * t2 - 4.0 * t + 1.0;
+    db[3] = 3.0 * t2 - 2.0 * t;
+}
+
+__global__ void __launch_bounds__(256) k_sf(int natom, const int* __restrict__ atom_site,
+                                            const int* __restrict__ atom_mol,
+                                            const MolDev* __restrict__ mols,
+                                            const SiteDev* __restrict__ sites,
+                                            const RepDev* __restrict__ reps, SFDev sf,
+                                            const double* __restrict__ x,
+                                            const double* __restrict__ y,
+                                            const double* __restrict__ z, double* fx,
+                                            double* fy, double* fz, double* __restrict__ usf,
+                                            Status* st) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    const SiteDev S = sites[atom_site[i]];
+    if (S.ghost != MSPMS_GHOST_NONE) {
+        usf[i] = 0.0;
+        return;
+    }
+    const RepDev& R = reps[mols[atom_mol[i]].rep];
+    double u = 0.0;
+    if (sf.type == MSPMS_SF_HYPERGEO) {
+        double sij = 0.5 * (S.sigma + sf.solid_sigma);
+        double eij = sqrt(S.epsilon * sf.solid_epsilon);
+        double fxi = fx[i], fyi = fy[i];
+        for (int t = 0; t < sf.ntube; t++) {
+            double rx = mic(x[i] - sf.tube[3 * t], R.L[0]);
+            double ry = mic(y[i] - sf.tube[3 * t + 1], R.L[1]);
+            double rsq = rx * rx + ry * ry, r = sqrt(rsq);
+            double TR = sf.tube[3 * t + 2], TRsq = TR * TR;
+            double ljc, k, zf, hr, hfr, ha, hfa, k2, k4, k10, r1, r2, a1, a2, fij;
+            if (r < TR) {
+                ljc = sf.c3pisq * eij * sij * sij;
+                k = sij * TR / (TRsq - rsq);
+                zf = rsq / TRsq;
+                hypergeo(-4.5, -4.5, 1.0, zf, r, &hr, &hfr);
+                k2 = k * k;
+                k4 = k2 * k2;
+                k10 = k4 * k4 * k2;
+                r1 = 0.65625 * k10;
+                r2 = -hfr + hr * (20.0 * r / (TRsq - rsq));
+                hypergeo(-1.5, -1.5, 1.0, zf, r, &ha, &hfa);
+                a1 = -k4;
+                a2 = -hfa + ha * (8.0 * r / (TRsq - rsq));
+                u = ljc * (r1 * hr + a1 * ha);
+                fij = -ljc * (r1 * r2 + a1 * a2);
+            } else {
+                ljc = sf.c3pisq * eij * sij * sij * TR;
+                k = sij * r / (rsq - TRsq);
+                zf = TRsq / rsq;
+                hypergeo(-4.5, -4.5, 1.0, zf, r, &hr, &hfr);
+                k2 = k * k;
+                k4 = k2 * k2;
+                k10 = k4 * k4 * k2;
+                r1 = 0.65625 * k10;
+                r2 = hfr / r + hr * (-20.0 / (rsq - TRsq) + 9.0 / rsq);
+                hypergeo(-1.5, -1.5, 1.0, zf, r, &ha, &hfa);
+                a1 = -k4;
+                a2 = hfa / r + ha * (-8.0 / (rsq - TRsq) + 3.0 / rsq);
+                u = ljc * (r1 * hr + a1 * ha) / r;
+                fij = -ljc * (r1 * r2 + a1 * a2);
+            }
+            fxi += fij * rx / r;
+            fyi += fij * ry / r;
+        }
+        fx[i] = fxi;
+        fy[i] = fyi;
+    } else if (sf.type == MSPMS_SF_ATOM_EXPLICIT) {
+        double sij = 0.5 * (S.sigma + sf.solid_sigma);
+        double eij = sqrt(S.epsilon * sf.solid_epsilon);
+        double fxi = fx[i], fyi = fy[i], fzi = fz[i], us = 0.0;
+        for (int j = 0; j < sf.nsolid; j++) {
+            double rx = mic(x[i] - sf.solid[3 * j], R.L[0]);
+            double ry = mic(y[i] - sf.solid[3 * j + 1], R.L[1]);
+            double rz = mic(z[i] - sf.solid[3 * j + 2], R.L[2]);
+            double rsq = rx * rx + ry * ry + rz * rz;
+            if (!(rsq < R.rc2_lj)) continue;
+            double sw = 0.0;
+            if (R.switch_on) {
+                if (rsq < R.rcon2)
+                    sw = 1.0;
+                else
+                    sw = (R.rc2_lj - rsq) * (R.rc2_lj - rsq) * (R.rc2_lj + 2.0 * rsq - 3.0 * R.rcon2) /
+                         R.roff2_ron2_cube;
+            }
+            double rr = sij * sij / rsq, r6 = rr * rr * rr, r12 = r6 * r6, d = r12 - r6;
+            double ut = eij * d, fij;
+            us += R.switch_on ? ut * sw : ut;
+            if (R.switch_on) {
+                if (rsq < R.rcon2)
+                    fij = 24.0 * eij * (d + r12) / rsq;
+                else
+                    fij = 24.0 * eij * (d + r12) / rsq * sw -
+                          4.0 * ut * 12.0 * (R.rc2_lj - rsq) * (R.rcon2 - rsq) / R.roff2_ron2_cube;
+            } else {
+                fij = 24.0 * eij * (d + r12) / rsq;
+            }
+            fxi += fij * rx;
+            fyi += fij * ry;
+            fzi += fij * rz;
+        }
+        fx[i] = fxi;
+        fy[i] = fyi;
+        fz[i] = fzi;
+        u = us * 4.0;
+    } else if (sf.type == MSPMS_SF_TASOS) {
+        // call_tasos_forces: Angstrom in, K and K/A out, reduced back
+        const int ty = S.interp - 1;
+        if (ty < 0 || ty >= sf.ntype) {
+            atomicOr(&st->flags, ST_GRID_EDOM);
+            usf[i] = 0.0;
+            return;
+        }
+        double xp = x[i] * sf.sigma_base - sf.x0, yp = y[i] * sf.sigma_base - sf.y0,
+               zp = z[i] * sf.sigma_base - sf.z0;
+        const double pz = (sf.nz - 1) * sf.hz;
+        zp = zp - floor(zp / pz) * pz;  // mylibtasos/pbc.f90:82-86
+        int ix = (int)floor(xp / sf.hx), iy = (int)floor(yp / sf.hy), iz = (int)floor(zp / sf.hz);
+        if (!(ix >= 0 && iy >= 0 && iz >= 0 && ix <= sf.nx - 2 && iy <= sf.ny - 2 &&
+              iz <= sf.nz - 2)) {
+            atomicOr(&st->flags, ST_GRID_EDOM);
+            usf[i] = 0.0;
+            return;
+        }
+        double bx[4], by[4], bz[4], dbx[4], dby[4], dbz[4];
+        hermite_1d((xp - ix * sf.hx) / sf.hx, sf.hx, bx, dbx);
+        hermite_1d((yp - iy * sf.hy) / sf.hy, sf.hy, by, dby);
+        hermite_1d((zp - iz * sf.hz) / sf.hz, sf.hz, bz, dbz);
+        const size_t nn = (size_t)sf.nx * sf.ny * sf.nz;
+        const float* base = sf.nodes + (size_t)ty * 8 * nn;
+        double fval = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+        // field order F, Fx, Fy, Fz, Fxy, Fxz, Fyz, Fxyz; derivative bits (x,y,z)
+        const int field_of[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+            const size_t node = ((size_t)(iz + cz) * sf.ny + (iy + cy)) * sf.nx + (ix + cx);
+#pragma unroll
+            for (int o = 0; o < 8; o++) {
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+                const double v = (double)base[(size_t)field_of[o] * nn + node];
+                const double wx = bx[cx + 2 * ox], wy = by[cy + 2 * oy], wz = bz[cz + 2 * oz];
+                fval += v * wx * wy * wz;
+                gx += v * dbx[cx + 2 * ox] * wy * wz;
+                gy += v * wx * dby[cy + 2 * oy] * wz;
+                gz += v * wx * wy * dbz[cz + 2 * oz];
+            }
+        }
+        double pe = (fval < 1.5 * sf.min_pe[ty]) ? -fval : fval;  // forces.f90:166-171
+        const double fconv = sf.sigma_base / sf.epsilon_base;
+        u = pe / sf.epsilon_base;
+        fx[i] += -gx * fconv;
+        fy[i] += -gy * fconv;
+        fz[i] += -gz * fconv;
+    }
+    usf[i] = u;
+}
+
+}  // namespace
+
+void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x, const double* y,
+               const double* z, double* fx, double* fy, double* fz, double* usf) {
+    int g = (c.natom + 255) / 256;
+    hipLaunchKernelGGL(k_sf, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site, c.atom_mol,
+                       c.mols, c.sites, c.reps, sf, x, y, z, fx, fy, fz, usf, c.status);
+}

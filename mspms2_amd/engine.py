@@ -80,7 +80,7 @@ def derive_params(p: abi.Params) -> abi.Params:
     return p
 
 
-PROFILE_GROUPS = ("cluster", "pair", "intra", "recip_s", "recip_f", "finalize", "bonded")
+PROFILE_GROUPS = ("cluster", "pair", "intra", "recip_s", "recip_f", "finalize", "bonded", "sf")
 
 
 class Engine:

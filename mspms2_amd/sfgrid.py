@@ -1,0 +1,102 @@
+"""Solid-fluid potential grids for the tricubic Hermite lookup (SF_TASOS).
+
+`tube_grid` tabulates the smooth-cylinder (hypergeometric) LJ potential of
+cal_sf_hypergeo (src/sffrc.c:1092-1209) on a node lattice, with the eight
+fields the Hermite interpolant needs per node (mylibtasos/grid.f90:81-99
+reads the same eight per node): F, dF/dx, dF/dy, dF/dz, d2F/dxdy,
+d2F/dxdz, d2F/dydz, d3F/dxdydz, in K and Angstrom.  The tube is
+z-invariant, so every z derivative is zero.  F and dF/dr come from the
+closed form (energy and its force expression); d2F/dr2 is a central
+difference of the closed-form dF/dr.
+
+Layout: nodes[type][field][iz][iy][ix] float32 (mspms_sf_grid).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+C3_PI_SQ_THETA = 11.3105666436484  # src/defs.h:99
+CLAMP = 1.0e30                      # keeps nodes on the wall finite in FP32
+
+
+def _hypergeo(a, b, c, z, rr):
+    """the 11-term series of src/sffrc.c:1092-1116 (vectorized)"""
+    zn = np.ones_like(z)
+    an1 = np.ones_like(z)
+    sv = np.ones_like(z)
+    sf = np.zeros_like(z)
+    for i in range(11):
+        an = (a + i) * (b + i) / ((c + i) * (1 + i))
+        an1 = an1 * an
+        zn = zn * z
+        sv = sv + an1 * zn
+        sf = sf - 2.0 * (i + 1.0) * an1 * zn / rr
+    return sv, sf
+
+
+def tube_u_du(r, R, sigma_f, eps_f, sigma_s, eps_s, c3pisq=C3_PI_SQ_THETA):
+    """Energy u(r) and du/dr of a fluid site at distance r from the axis of a
+    smooth tube of radius R (cal_sf_hypergeo, Lorentz-Berthelot mixing)."""
+    r = np.asarray(r, dtype=np.float64)
+    sij = 0.5 * (sigma_f + sigma_s)
+    eij = np.sqrt(eps_f * eps_s)
+    rsq = r * r
+    Rsq = R * R
+    u = np.empty_like(r)
+    f = np.empty_like(r)
+    ins = r < R
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ri, rsi = r[ins], rsq[ins]
+        ljc = c3pisq * eij * sij * sij
+        k = sij * R / (Rsq - rsi)
+        z = rsi / Rsq
+        hr, hfr = _hypergeo(-4.5, -4.5, 1.0, z, ri)
+        k4 = k ** 4
+        r1 = 0.65625 * k ** 10
+        r2 = -hfr + hr * (20.0 * ri / (Rsq - rsi))
+        ha, hfa = _hypergeo(-1.5, -1.5, 1.0, z, ri)
+        a2 = -hfa + ha * (8.0 * ri / (Rsq - rsi))
+        u[ins] = ljc * (r1 * hr - k4 * ha)
+        f[ins] = -ljc * (r1 * r2 - k4 * a2)
+        out = ~ins
+        ro, rso = r[out], rsq[out]
+        ljc = c3pisq * eij * sij * sij * R
+        k = sij * ro / (rso - Rsq)
+        z = Rsq / rso
+        hr, hfr = _hypergeo(-4.5, -4.5, 1.0, z, ro)
+        k4 = k ** 4
+        r1 = 0.65625 * k ** 10
+        r2 = hfr / ro + hr * (-20.0 / (rso - Rsq) + 9.0 / rso)
+        ha, hfa = _hypergeo(-1.5, -1.5, 1.0, z, ro)
+        a2 = hfa / ro + ha * (-8.0 / (rso - Rsq) + 3.0 / rso)
+        u[out] = ljc * (r1 * hr - k4 * ha) / ro
+        f[out] = -ljc * (r1 * r2 - k4 * a2)
+    # the reference's radial force is -du/dr
+    return u, -f
+
+
+def tube_grid(site_params, R, x0, y0, nx, ny, hx, hy, nz, hz, sigma_s, eps_s,
+              center=(0.0, 0.0), c3pisq=C3_PI_SQ_THETA, dr=1e-4):
+    """Node table for each (sigma_f, eps_f) in site_params (one grid type
+    each, interp_type = index + 1).  Returns the mspms_sf_grid fields."""
+    xs = x0 + hx * np.arange(nx)
+    ys = y0 + hy * np.arange(ny)
+    X, Y = np.meshgrid(xs - center[0], ys - center[1], indexing="xy")   # [ny][nx]
+    r = np.sqrt(X * X + Y * Y)
+    r = np.where(r == 0.0, 1e-12, r)
+    nodes = np.zeros((len(site_params), 8, nz, ny, nx), dtype=np.float32)
+    for t, (sf, ef) in enumerate(site_params):
+        u, du = tube_u_du(r, R, sf, ef, sigma_s, eps_s, c3pisq)
+        _, dup = tube_u_du(r + dr, R, sf, ef, sigma_s, eps_s, c3pisq)
+        _, dum = tube_u_du(r - dr, R, sf, ef, sigma_s, eps_s, c3pisq)
+        d2u = (dup - dum) / (2 * dr)
+        fx = du * X / r
+        fy = du * Y / r
+        fxy = d2u * X * Y / (r * r) - du * X * Y / (r ** 3)
+        planes = [u, fx, fy, np.zeros_like(u), fxy, np.zeros_like(u), np.zeros_like(u),
+                  np.zeros_like(u)]
+        for k, pl in enumerate(planes):
+            pl = np.nan_to_num(pl, nan=CLAMP, posinf=CLAMP, neginf=-CLAMP)
+            nodes[t, k, :, :, :] = np.clip(pl, -CLAMP, CLAMP)[None, :, :]
+    return dict(ntype=len(site_params), nx=nx, ny=ny, nz=nz, x0=x0, y0=y0, z0=0.0, hx=hx,
+                hy=hy, hz=hz, nodes=nodes)

@@ -245,3 +245,20 @@ def test_check_uniques_quirks():
     du2 = np.zeros(6, dtype=np.uint8)
     assert lib().mspms_check_uniques(C.byref(sp[0]), abi.u8ptr(au2), abi.u8ptr(du2)) == 0
     assert list(au2) == list(au) and list(du2) == list(du)
+
+
+def test_grid_reproduces_tube_closed_form():
+    """The Hermite grid tabulated from the hypergeometric tube (0.25 A
+    spacing; the interpolation error falls as h^4) reproduces the
+    closed-form energy and forces of cal_sf_hypergeo (src/sffrc.c:1120-1209)
+    inside the pore."""
+    from tests.util import tube_grid_system
+    sg = tube_grid_system(hx=0.25)
+    sh = _golden_system("tube300_hypergeo")
+    assert np.array_equal(sg.xx, sh.xx)
+    fg, tg = O.Oracle(sg).frclong()
+    fh, th = O.Oracle(sh).frclong()
+    assert tg["usflj"] == pytest.approx(th["usflj"], rel=1e-4)
+    # pair forces are identical; the solid-fluid part differs by the
+    # interpolation error only
+    assert np.linalg.norm(fg - fh) <= 1e-3 * np.linalg.norm(fh)

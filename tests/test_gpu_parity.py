@@ -9,13 +9,17 @@ Tolerances:
               to the same scales; bonded / intramolecular / reciprocal terms
               stay FP64 and are held to 1e-10.
 """
+import os
+
 import numpy as np
 import pytest
 
-from mspms2_amd import configs
+from mspms2_amd import configs, runio
 from mspms2_amd.engine import Engine
+from mspms2_amd.system import System
 from oracle.oracle import Oracle
-from tests.util import assert_long_close, assert_short_close, force_close, rel_rms, system_of
+from tests.util import (GOLDEN, assert_long_close, assert_short_close, force_close, rel_rms,
+                        system_of, tube_grid_system)
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +33,14 @@ CASES = {
     "butane64": lambda: configs.butane_box(nmol=64, seed=5),
     "water512_ewald_k8": lambda: configs.water_box(n_side=8, L=8 * 3.104, rc=10.0, kappa=0.28,
                                                    kmax=8, ksqmax=64, seed=11),
+    "mixture_ewald_ghost": lambda: configs.mixture_box(water4=True, seed=12),
+    "mixture_wolf_switch": lambda: configs.mixture_box(elec="wolf", kappa=0.2, switch=True,
+                                                       seed=13),
+    "bonded_zoo": lambda: configs.bonded_zoo(),
+    "tube300_hypergeo": lambda: configs.methane_tube(natom=300, length=60.0, box_xy=40.0,
+                                                     tube_r=15.0, rc=10.0),
+    "tube300_grid": "grid",
+    "examples_water": "golden",
 }
 
 _cache = {}
@@ -36,7 +48,12 @@ _cache = {}
 
 def _case(name):
     if name not in _cache:
-        s = system_of(CASES[name]())
+        if CASES[name] == "grid":
+            s = tube_grid_system()
+        elif CASES[name] == "golden":
+            s = System.from_rundir(runio.RunDir.read(os.path.join(GOLDEN, name)))
+        else:
+            s = system_of(CASES[name]())
         o = Oracle(s)
         fl, tl = o.frclong()
         fs, ts = o.frcshort()

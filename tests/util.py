@@ -22,6 +22,19 @@ def system_of(rundir_obj):
         return System.from_rundir(runio.RunDir.read(tmp))
 
 
+def tube_grid_system(hx=0.5, natom=300):
+    """The 300-site methane-in-tube system with the tube given as a
+    tricubic Hermite grid (SF_TASOS) tabulated from the closed form."""
+    from mspms2_amd import configs, sfgrid
+    rd = configs.methane_tube(natom=natom, length=60.0, box_xy=40.0, tube_r=15.0, rc=10.0)
+    s = system_of(rd)
+    s.params["iSF_type"] = 3
+    n = int(round(40.0 / hx)) + 1
+    s.grid = sfgrid.tube_grid([(3.73, 148.0)], R=15.0, x0=-20.0, y0=-20.0, nx=n, ny=n, hx=hx,
+                              hy=hx, nz=5, hz=15.0, sigma_s=3.37, eps_s=27.68)
+    return s
+
+
 def rel_rms(f, g):
     """||f - g|| / ||g|| over all atoms and components."""
     g = np.asarray(g)
