@@ -308,9 +308,27 @@ int mspms_vver_step_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
                            mspms_long_terms* lterms, mspms_short_terms* sterms,
                            double* ukin, void* stream);
 
-/* Synchronises the context's stream and reports the status of the last
- * device-buffer evaluation (list overflow, non-finite input, singular
- * improper), which the host-buffer calls check by themselves. */
+/* nsteps calls of mspms_vver_step_device with the same arguments.  With
+ * use_graph != 0 one outer step (all kernels of every replica) is captured
+ * once as a HIP graph on the context stream and replayed on `stream`; the
+ * graph is kept for the same argument set until the system is rebuilt.
+ * lhist / shist / ukin_hist (device, optional) receive every step's terms
+ * and kinetic energies, nsteps * nreplica entries, step-major.  No
+ * counterpart in the reference: its outer loop (src/mspms2.c:99-132) calls
+ * the integrator once per step from the host. */
+int mspms_vver_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
+                          double* fs, double delt, int nstep_inner, int nsteps,
+                          mspms_long_terms* lterms, mspms_short_terms* sterms,
+                          double* ukin, mspms_long_terms* lhist,
+                          mspms_short_terms* shist, double* ukin_hist,
+                          int use_graph, void* stream);
+
+/* Synchronises the device and reports the status accumulated by the
+ * device-buffer evaluations since the previous mspms_sync (list overflow,
+ * non-finite input, grid out of range, singular improper), which the
+ * host-buffer calls check by themselves; the status is cleared.  On a
+ * pair-list overflow the lists are regrown before MSPMS_ENOMEM is returned,
+ * so that the caller can repeat the steps since the previous sync. */
 int mspms_sync(mspms_ctx* ctx);
 
 /* The context's own HIP stream (hipStream_t). */
@@ -320,7 +338,8 @@ void* mspms_stream(const mspms_ctx* ctx);
  * evaluate, then read the mean milliseconds per call of each group
  * (0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,
  * 3 Ewald S(k) + reduction, 4 Ewald forces, 5 finalize, 6 bonded / RESPA
- * inner loop).  Device-buffer calls are accumulated at mspms_sync(). */
+ * inner loop, 7 solid-fluid).  Device-buffer calls are accumulated at
+ * mspms_sync(); steps replayed from a graph are not timed. */
 int mspms_set_profiling(mspms_ctx* ctx, int on);
 int mspms_last_timings(const mspms_ctx* ctx, double* ms, int n);
 

@@ -124,13 +124,25 @@ def water_species(nmole: int) -> SpeciesSpec:
         angles=[(1, 0, 2, abi.ANGLE_HARMONIC, WATER_KTHETA, WATER_THETA0, 0.0, 0.0, 0.0)])
 
 
+def _spin_tilt_rotation(rng, max_tilt):
+    """Rotation by a uniform angle about z after a tilt of at most max_tilt
+    radians about a random horizontal axis."""
+    phi, psi = 2 * np.pi * rng.random(), 2 * np.pi * rng.random()
+    th = max_tilt * rng.random()
+    a = np.array([np.cos(psi), np.sin(psi), 0.0])
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    T = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * (K @ K)
+    Rz = np.array([[np.cos(phi), -np.sin(phi), 0], [np.sin(phi), np.cos(phi), 0], [0, 0, 1]])
+    return Rz @ T
+
+
 def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=400, recenter=False,
-                 jitter=None):
-    """Random orientation per molecule, rejecting contacts closer than
-    min_dist to molecules already placed (cell-list check).  Every 20
-    failed tries the centre is redrawn: uniformly in the box (recenter) or
-    within +-jitter of its lattice site (jitter); running out of tries is an
-    error."""
+                 jitter=None, rotation=None):
+    """Random orientation per molecule (uniform, or `rotation(rng)`),
+    rejecting contacts closer than min_dist to molecules already placed
+    (cell-list check).  Every 20 failed tries the centre is redrawn:
+    uniformly in the box (recenter) or within +-jitter of its lattice site
+    (jitter); running out of tries is an error."""
     nmol, nsite = centers.shape[0], tmpl.shape[0]
     out = np.empty((nmol, nsite, 3))
     ncell = max(1, int(L // max(min_dist, 1e-9)))
@@ -148,7 +160,7 @@ def _place_rigid(rng, centers, tmpl, L, min_dist, max_tries=400, recenter=False,
                     c0 = rng.uniform(0, L, 3)
                 elif jitter:
                     c0 = centers[m] + rng.uniform(-jitter, jitter, 3)
-            R = _random_rotations(rng, 1)[0]
+            R = _random_rotations(rng, 1)[0] if rotation is None else rotation(rng)
             xyz = c0 + tmpl @ R.T
             ok = True
             cells = set()
@@ -361,14 +373,21 @@ def butane_box(nmol: int = 128, seed: int = 3, rc: float = 14.0,
     rng = np.random.default_rng(seed)
     L = (nmol / BUTANE_DENSITY) ** (1.0 / 3.0)
     rcut = min(rc, 0.5 * L * (1 - 1e-12))
-    ns = int(math.ceil(nmol ** (1.0 / 3.0)))
-    a = L / ns
-    sites = np.array([[i, j, k] for i in range(ns) for j in range(ns)
-                      for k in range(ns)], dtype=np.float64)
+    # chains roughly along z on a lattice of columns (>= 6.9 A apart along
+    # z, about 4.4 A apart across): a random isotropic packing of rigid
+    # chains jams at this density
+    nz = max(1, int(L / 6.9))
+    nxy = int(math.ceil(math.sqrt(nmol / nz)))
+    sites = np.array([[i, j, k] for k in range(nz) for i in range(nxy)
+                      for j in range(nxy)], dtype=np.float64)
     pick = rng.permutation(len(sites))[:nmol]
-    centers = (sites[pick] + 0.5) * a
+    centers = (sites[pick] + 0.5) * np.array([L / nxy, L / nxy, L / nz])
     tmpl = BUTANE_TMPL - BUTANE_TMPL.mean(0)
-    xyz = _place_rigid(rng, centers, tmpl, L, min_dist, jitter=0.2).reshape(-1, 3)
+    # principal (longest) axis of the chain along z
+    w, V = np.linalg.eigh(tmpl.T @ tmpl)
+    tmpl = tmpl @ V[:, ::-1][:, [1, 2, 0]]
+    xyz = _place_rigid(rng, centers, tmpl, L, min_dist, jitter=0.2,
+                       rotation=lambda g: _spin_tilt_rotation(g, 0.15)).reshape(-1, 3)
     run = RunInput(title=f"n-butane {nmol}", ij=1000 + seed % 30000, jk=2000 + seed % 28000,
                    sigma_base=1.0, epsilon_base=1.0, mass_base=1.0, treq=300.0, preq=1.0,
                    boxlx=L, boxly=L, boxlz=L, rcuton=rcut, rcutoff=rcut, rcutoffelec=rcut,

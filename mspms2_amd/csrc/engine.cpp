@@ -166,9 +166,31 @@ struct mspms_ctx {
     std::vector<int> pending;          // group of each recorded pair
     std::vector<double> prof_ms;       // summed milliseconds per group
     std::vector<long> prof_n;          // recorded launches per group
+    bool capturing = false;            // no event records inside a capture
+
+    // one RESPA outer step captured as a graph (mspms_vver_run_device),
+    // valid for the argument set in gkey until the next build
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    struct GraphKey {
+        void *pos, *vel, *fl, *fs, *lt, *st, *uk;
+        double delt;
+        int nstep_inner;
+        bool operator==(const GraphKey& o) const {
+            return pos == o.pos && vel == o.vel && fl == o.fl && fs == o.fs && lt == o.lt &&
+                   st == o.st && uk == o.uk && delt == o.delt && nstep_inner == o.nstep_inner;
+        }
+    } gkey{};
 };
 
 namespace {
+
+void drop_graph(mspms_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
 
 LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     LaunchCtx L;
@@ -395,6 +417,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     for (auto e : c->ev) (void)hipEventDestroy(e);
     c->d_sites.release();
     c->d_species.release();
@@ -456,6 +479,7 @@ namespace {
 // Builds every device table from sp / nmole / params.
 int build(mspms_ctx* c) {
     HIPC(hipSetDevice(c->device));
+    drop_graph(c);  // buffers may move
     const int ns = (int)c->sp.size();
     const hipStream_t s = c->stream;
     // ---- sites, species, terms ----
@@ -1012,7 +1036,7 @@ struct ProfScope {
     size_t idx = 0;
     bool on = false;
     ProfScope(mspms_ctx* c_, hipStream_t s_, int slot_) : c(c_), s(s_), slot(slot_) {
-        if (!c->profiling) return;
+        if (!c->profiling || c->capturing) return;
         if (c->ev_used + 2 > c->ev.size()) {
             size_t old = c->ev.size(), grow = std::max<size_t>(64, old);
             c->ev.resize(old + grow);
@@ -1042,9 +1066,10 @@ __global__ void k_reset_status(Status* st) {
 
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
                  double* fy, double* fz, mspms_long_terms* terms, hipStream_t s) {
+    // the status word is sticky: the host API clears it before each call,
+    // device-API runs accumulate it until mspms_sync reads it
     LaunchCtx L = launch_ctx(c, s);
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
-    const int hash = c->want_hash < 0 ? (c->precision == MSPMS_PREC_FP64) : c->want_hash;
+    const int hash =c->want_hash < 0 ? (c->precision == MSPMS_PREC_FP64) : c->want_hash;
     {
         ProfScope p(c, s, PROF_CLUSTER);
         launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, c->precision);
@@ -1086,6 +1111,30 @@ int enqueue_short(mspms_ctx* c, const double* x, const double* y, const double* 
     ProfScope p(c, s, PROF_BONDED);
     launch_bonded(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_short.p);
     launch_finalize_short(L, c->d_ep_short.p, terms);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// One RESPA velocity-Verlet outer step, NVE (src/md.c:22-83) on device
+// (3, N) SoA buffers; kernels only, so that it can be captured.
+int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs, double delt,
+                 int nstep_inner, mspms_long_terms* lterms, mspms_short_terms* sterms,
+                 double* ukin, hipStream_t s) {
+    const size_t n = (size_t)c->natom;
+    // src/initvars.c:605-607
+    const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
+    LaunchCtx L = launch_ctx(c, s);
+    {
+        ProfScope p(c, s, PROF_BONDED);
+        launch_respa_inner(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, vel,
+                           vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
+                           deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
+        launch_finalize_short(L, c->d_ep_short.p, sterms);
+    }
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s);
+    if (rc) return rc;
+    launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
+                       c->d_kin.p, ukin);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1145,6 +1194,7 @@ extern "C" int mspms_frclong_host(mspms_ctx* c, const double* xx, const double* 
     HIPC(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
     for (int attempt = 0; attempt < 4; attempt++) {
+        hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
         int rc = enqueue_long(c, dx, dx + n, dx + 2 * n, dfx, dfy, dfz, c->d_long.p, s);
         if (rc) return rc;
         if ((rc = read_status(c, s))) return rc;
@@ -1166,6 +1216,8 @@ extern "C" int mspms_frclong_host(mspms_ctx* c, const double* xx, const double* 
     if (c->h_status->flags & ST_OVERFLOW)
         return fail(MSPMS_ENOMEM, "pair list overflow after growing");
     if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
+    if (c->h_status->flags & ST_GRID_EDOM)
+        return fail(MSPMS_EDOM, "site outside the solid-fluid grid or without a grid type");
     return 0;
 }
 
@@ -1215,22 +1267,71 @@ extern "C" int mspms_vver_step_device(mspms_ctx* c, double* pos, double* vel, do
     if (nstep_inner < 1) return fail(MSPMS_EINVAL, "nstep_inner must be >= 1");
     HIPC(hipSetDevice(c->device));
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    const size_t n = (size_t)c->natom;
-    // src/initvars.c:605-607
-    const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
-    LaunchCtx L = launch_ctx(c, s);
-    {
-        ProfScope p(c, s, PROF_BONDED);
-        launch_respa_inner(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, vel,
-                           vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
-                           deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
-        launch_finalize_short(L, c->d_ep_short.p, sterms);
+    return enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
+}
+
+// nsteps RESPA outer steps; with use_graph the step is captured once as a
+// HIP graph (on the context stream) and replayed on `stream`.  Optional
+// histories receive each step's terms (nsteps * nrep entries) and kinetic
+// energies (nsteps * nrep doubles).
+extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                     double* fs, double delt, int nstep_inner, int nsteps,
+                                     mspms_long_terms* lterms, mspms_short_terms* sterms,
+                                     double* ukin, mspms_long_terms* lhist,
+                                     mspms_short_terms* shist, double* ukin_hist, int use_graph,
+                                     void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (nstep_inner < 1 || nsteps < 0) return fail(MSPMS_EINVAL, "bad step counts");
+    if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const int nr = c->nrep;
+    if (use_graph) {
+        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
+        if (!c->graph_exec || !(k == c->gkey)) {
+            drop_graph(c);
+            // the capture stream must not have work in flight from the caller
+            HIPC(hipStreamSynchronize(s));
+            HIPC(hipStreamSynchronize(c->stream));
+            c->capturing = true;
+            HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin,
+                                  c->stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->capturing = false;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(MSPMS_EIO, "graph capture: %s", hipGetErrorString(e));
+            c->graph = g;
+            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                drop_graph(c);
+                return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
+            }
+            c->gkey = k;
+        }
     }
-    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s);
-    if (rc) return rc;
-    launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
-                       c->d_kin.p, ukin);
-    HIPC(hipGetLastError());
+    for (int t = 0; t < nsteps; t++) {
+        if (use_graph) {
+            HIPC(hipGraphLaunch(c->graph_exec, s));
+        } else {
+            int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
+            if (rc) return rc;
+        }
+        if (lhist)
+            HIPC(hipMemcpyAsync(lhist + (size_t)t * nr, lterms, sizeof(mspms_long_terms) * nr,
+                                hipMemcpyDeviceToDevice, s));
+        if (shist)
+            HIPC(hipMemcpyAsync(shist + (size_t)t * nr, sterms, sizeof(mspms_short_terms) * nr,
+                                hipMemcpyDeviceToDevice, s));
+        if (ukin_hist)
+            HIPC(hipMemcpyAsync(ukin_hist + (size_t)t * nr, ukin, sizeof(double) * nr,
+                                hipMemcpyDeviceToDevice, s));
+    }
     return 0;
 }
 
@@ -1269,8 +1370,19 @@ extern "C" int mspms_sync(mspms_ctx* c) {
     int rc = read_status(c, c->stream);
     if (rc) return rc;
     accumulate_profile(c);
-    if (c->h_status->flags & ST_OVERFLOW) return fail(MSPMS_ENOMEM, "pair list overflow");
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, c->stream, c->d_status.p);
+    HIPC(hipStreamSynchronize(c->stream));
+    if (c->h_status->flags & ST_OVERFLOW) {
+        // the evaluations since the last sync dropped pairs; size the lists
+        // for the largest row seen so that a repeat of them fits
+        c->cap_hint = std::max(c->cap_hint * 2, c->h_status->max_list + 64);
+        drop_graph(c);
+        if ((rc = build(c))) return rc;
+        return fail(MSPMS_ENOMEM, "pair list overflow (lists regrown; repeat the steps)");
+    }
     if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
+    if (c->h_status->flags & ST_GRID_EDOM)
+        return fail(MSPMS_EDOM, "site outside the solid-fluid grid or without a grid type");
     if (c->h_status->flags & ST_SINGULAR) return fail(MSPMS_EINVAL, "singular improper");
     return 0;
 }

@@ -25,7 +25,7 @@ EXPORTS = [
     "mspms_natom", "mspms_nreplica", "mspms_frclong_host", "mspms_frcshort_host",
     "mspms_frclong_device", "mspms_frcshort_device", "mspms_energy_param_grad",
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
-    "mspms_vver_step_device",
+    "mspms_vver_step_device", "mspms_vver_run_device",
 ]
 
 
@@ -60,6 +60,8 @@ def lib():
     L.mspms_frcshort_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.mspms_vver_step_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp,
                                          vp]
+    L.mspms_vver_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp,
+                                        vp, vp, vp, vp, vp, C.c_int, vp]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -126,12 +128,48 @@ class Engine:
                "mspms_set_replicas")
         self.nrep = n
 
+    @classmethod
+    def replicas(cls, systems, precision: str = "mixed", device: int = 0) -> "Engine":
+        """One engine over a packed table of independent replicas (same
+        species templates, any molecule counts, boxes and cutoffs).  Atoms
+        are packed replica after replica; `rep_atom0[r]` is the first atom
+        of replica r.  Positions of the packed system are the replicas'
+        positions concatenated (`pack`)."""
+        s0 = systems[0]
+        for s in systems[1:]:
+            if len(s.species) != len(s0.species) or any(
+                    a.natom != b.natom for a, b in zip(s.species, s0.species)):
+                raise EngineError("replicas must share the species templates")
+        eng = cls(s0, precision, device)
+        eng.set_replicas([[sp.nmole for sp in s.species] for s in systems],
+                         [s.c_params() for s in systems])
+        eng.replica_systems = list(systems)
+        eng.rep_atom0 = np.cumsum([0] + [s.natom for s in systems])
+        return eng
+
+    def pack(self, arrays):
+        """Concatenates per-replica (3, n_r) arrays into the packed (3, N)."""
+        return np.concatenate([np.asarray(a, dtype=np.float64) for a in arrays], axis=1)
+
+    def unpack(self, f):
+        """Splits a packed (3, N) array into per-replica views."""
+        a = self.rep_atom0
+        return [f[:, a[r]:a[r + 1]] for r in range(len(a) - 1)]
+
+    def packed_positions(self):
+        return self.pack([np.stack([s.xx, s.yy, s.zz]) for s in self.replica_systems])
+
     # ---- host-buffer evaluation --------------------------------------------
+    def _xyz(self, xx, yy, zz):
+        if xx is None:
+            if getattr(self, "replica_systems", None):
+                xx, yy, zz = self.packed_positions()
+            else:
+                xx, yy, zz = self.system.xx, self.system.yy, self.system.zz
+        return tuple(np.ascontiguousarray(v, dtype=np.float64) for v in (xx, yy, zz))
+
     def frclong(self, xx=None, yy=None, zz=None):
-        s = self.system
-        xx = np.ascontiguousarray(s.xx if xx is None else xx, dtype=np.float64)
-        yy = np.ascontiguousarray(s.yy if yy is None else yy, dtype=np.float64)
-        zz = np.ascontiguousarray(s.zz if zz is None else zz, dtype=np.float64)
+        xx, yy, zz = self._xyz(xx, yy, zz)
         n = self.natom
         f = np.empty((3, n))
         t = (abi.LongTerms * self.nrep)()
@@ -142,10 +180,7 @@ class Engine:
         return f, (terms[0] if self.nrep == 1 else terms)
 
     def frcshort(self, xx=None, yy=None, zz=None):
-        s = self.system
-        xx = np.ascontiguousarray(s.xx if xx is None else xx, dtype=np.float64)
-        yy = np.ascontiguousarray(s.yy if yy is None else yy, dtype=np.float64)
-        zz = np.ascontiguousarray(s.zz if zz is None else zz, dtype=np.float64)
+        xx, yy, zz = self._xyz(xx, yy, zz)
         n = self.natom
         f = np.empty((3, n))
         t = (abi.ShortTerms * self.nrep)()
@@ -185,6 +220,16 @@ class Engine:
                                             int(nstep_inner), self._ptr(lterms),
                                             self._ptr(sterms), self._ptr(ukin), stream),
                "mspms_vver_step_device")
+
+    def vver_run_device(self, pos, vel, fl, fs, delt, nstep_inner, nsteps, lterms, sterms, ukin,
+                        lhist=None, shist=None, ukin_hist=None, graph=True, stream=None):
+        """nsteps RESPA outer steps; graph=True replays one captured step."""
+        p = self._ptr
+        _check(lib().mspms_vver_run_device(
+            self._h, p(pos), p(vel), p(fl), p(fs), float(delt), int(nstep_inner), int(nsteps),
+            p(lterms), p(sterms), p(ukin), None if lhist is None else p(lhist),
+            None if shist is None else p(shist), None if ukin_hist is None else p(ukin_hist),
+            1 if graph else 0, stream), "mspms_vver_run_device")
 
     def sync(self):
         _check(lib().mspms_sync(self._h), "mspms_sync")
