@@ -192,7 +192,8 @@ typedef struct mspms_params {
  * plus pair statistics.  n_lj_pairs / n_elec_pairs count inter-molecular
  * (i<j) site pairs inside rcutoff / rcutoffelec (src/frclong.c:123-125,
  * :157); lj_pair_hash / elec_pair_hash are order-independent sums of
- * splitmix64((uint64)i<<32 | j) over those pairs. */
+ * splitmix64((uint64)i<<32 | j) over those pairs, i < j numbered within the
+ * replica (0 = its first atom). */
 typedef struct mspms_long_terms {
     double uvdw, ushift, ureal, uexcl, ufourier, uself, uvacuum, uelec;
     double usflj, uotherff, uinter, virial_inter;
@@ -205,11 +206,16 @@ typedef struct mspms_short_terms {
     double ubond, uangle, udih, uimp, uintra, virial_intra;
 } mspms_short_terms;
 
-/* Parameter gradients of the nonbonded energy uinter (Wolf or Ewald real
- * space, LJ) with respect to each site of each species' template:
- * dE/d(epsilon), dE/d(sigma), dE/d(charge).  Arrays are indexed by global
- * site id = sum of natom of earlier species + site.  No reference
- * counterpart: the reference has no derivatives with respect to parameters. */
+/* Parameter gradients of the nonbonded energy uinter (LJ + Ewald or Wolf,
+ * with the intramolecular corrections, self and reciprocal terms) with
+ * respect to the per-site template values epsilon, sigma, charge (the
+ * reduced values of mspms_species), one row per replica:
+ * [replica][global site id], global site id = sum of natom of earlier
+ * species + site.  A parameter shared by several sites (an atom type) has
+ * the sum of their entries as its derivative.  With epsilon_i = 0 < epsilon_j
+ * for an interacting pair, dE/d(epsilon_i) is +-inf (the mixing rule
+ * sqrt(eps_i eps_j) has no derivative there).  No reference counterpart:
+ * the reference has no derivatives with respect to parameters. */
 typedef struct mspms_param_grad {
     double* dE_depsilon;   /* [nsite] */
     double* dE_dsigma;     /* [nsite] */
@@ -285,9 +291,10 @@ int mspms_frcshort_device(mspms_ctx* ctx, const double* d_xx,
                           mspms_short_terms* d_terms, void* stream);
 
 /* Adjoint of the nonbonded energy with respect to the per-site force-field
- * parameters (Wolf or Ewald real space + LJ; replica 0 only).  Host
- * buffers; grad arrays have one entry per global site id.  Also returns the
- * forces (= -dE/dr) and the terms like mspms_frclong_host. */
+ * parameters, in FP64 whatever the context precision.  Host buffers; grad
+ * arrays hold nreplica * nsite entries.  Also returns the FP64 frclong
+ * forces (= -dE/dr) and terms like mspms_frclong_host.  Solid-fluid terms
+ * are not differentiated (MSPMS_EINVAL when a replica has one). */
 int mspms_energy_param_grad(mspms_ctx* ctx, const double* xx, const double* yy,
                             const double* zz, double* fxl, double* fyl,
                             double* fzl, mspms_long_terms* terms,

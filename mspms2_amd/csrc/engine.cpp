@@ -157,6 +157,9 @@ struct mspms_ctx {
 
     // host-API staging and integrator scratch
     DevBuf<double> d_pos, d_fl, d_fs, d_kin;
+    // parameter adjoint: per-atom derivatives, per-(replica, site) sums
+    DevBuf<double> d_grad, d_grad_out;
+    DevBuf<int4> d_grad_work;
 
     // profiling: a pool of events, one (start, stop) pair per recorded
     // kernel group, read back at the next synchronisation
@@ -199,6 +202,8 @@ LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     L.nmol = c->nmol;
     L.nsite = c->nsite;
     L.nrep = c->nrep;
+    L.max_mol_atoms = 0;
+    for (const SpeciesHost& q : c->sp) L.max_mol_atoms = std::max(L.max_mol_atoms, q.natom);
     L.sites = c->d_sites.p;
     L.mols = c->d_mols.p;
     L.atom_mol = c->d_atom_mol.p;
@@ -468,6 +473,9 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_fl.release();
     c->d_fs.release();
     c->d_kin.release();
+    c->d_grad.release();
+    c->d_grad_out.release();
+    c->d_grad_work.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1064,8 +1072,15 @@ __global__ void k_reset_status(Status* st) {
     st->max_list = 0;
 }
 
+// grad (optional): per-atom parameter derivatives gq | ge | gs, 3 * natom;
+// with it the pair path is the FP64 one whatever the context precision
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
-                 double* fy, double* fz, mspms_long_terms* terms, hipStream_t s) {
+                 double* fy, double* fz, mspms_long_terms* terms, hipStream_t s,
+                 double* grad = nullptr) {
+    const size_t n = (size_t)c->natom;
+    double* gq = grad;
+    double* ge = grad ? grad + n : nullptr;
+    double* gs = grad ? grad + 2 * n : nullptr;
     // the status word is sticky: the host API clears it before each call,
     // device-API runs accumulate it until mspms_sync reads it
     LaunchCtx L = launch_ctx(c, s);
@@ -1076,12 +1091,13 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     }
     {
         ProfScope p(c, s, PROF_PAIR);
-        launch_pair(L, c->cb, c->precision, c->charge_union, fx, fy, fz, c->d_ep_pair.p,
-                    c->d_ip_pair.p, hash);
+        launch_pair(L, c->cb, grad ? MSPMS_PREC_FP64 : c->precision, c->charge_union, fx, fy,
+                    fz, c->d_ep_pair.p, c->d_ip_pair.p, grad ? 1 : hash, gq, ge, gs);
     }
     if (c->any_intra) {
         ProfScope p(c, s, PROF_INTRA);
-        launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p);
+        launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p, gq,
+                     ge, gs);
     }
     if (c->any_recip) {
         {
@@ -1089,7 +1105,7 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
             launch_recip_s(L, c->rb, x, y, z);
         }
         ProfScope p(c, s, PROF_RECIP_F);
-        launch_recip_f(L, c->rb, x, y, z, fx, fy, fz);
+        launch_recip_f(L, c->rb, x, y, z, fx, fy, fz, gq);
     }
     if (c->any_sf) {
         ProfScope p(c, s, PROF_SF);
@@ -1249,11 +1265,96 @@ extern "C" int mspms_frcshort_host(mspms_ctx* c, const double* xx, const double*
     return 0;
 }
 
-extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double*, const double*, const double*,
-                                       double*, double*, double*, mspms_long_terms*,
-                                       mspms_param_grad*) {
-    (void)c;
-    return fail(MSPMS_EINVAL, "parameter gradients are not built yet");
+// Parameter adjoint of uinter (FP64): the frclong evaluation of
+// mspms_frclong_host plus, per replica and global site id, dE/d(epsilon),
+// dE/d(sigma), dE/d(charge).  The pair, intramolecular and reciprocal
+// kernels write each atom's derivative with respect to its own parameters
+// (every inter-molecular pair is visited from both atoms, so each visit
+// differentiates only the visiting atom's side); k_grad_sites sums them
+// over the atoms that share a template site.
+extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double* xx, const double* yy,
+                                       const double* zz, double* fxl, double* fyl, double* fzl,
+                                       mspms_long_terms* terms, mspms_param_grad* grad) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!xx || !yy || !zz || !fxl || !fyl || !fzl || !terms || !grad || !grad->dE_depsilon ||
+        !grad->dE_dsigma || !grad->dE_dcharge)
+        return fail(MSPMS_EINVAL, "null argument");
+    if (c->any_sf)
+        return fail(MSPMS_EINVAL, "parameter gradients of the solid-fluid term are not provided");
+    HIPC(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const size_t n = (size_t)c->natom;
+    const int ns = (int)c->sp.size();
+    // work items (rep, global site, first atom of the site, molecule count)
+    std::vector<int4> work;
+    {
+        int atom = 0;
+        for (int r = 0; r < c->nrep; r++) {
+            int site0 = 0;
+            for (int k = 0; k < ns; k++) {
+                const int nm = c->nmole[(size_t)r * ns + k], na = c->sp[k].natom;
+                for (int a = 0; a < na; a++) work.push_back(make_int4(r, site0 + a, atom + a, nm));
+                atom += nm * na;
+                site0 += na;
+            }
+        }
+    }
+    const size_t nout = (size_t)c->nrep * c->nsite;
+    int rc = 0;
+    if ((rc = c->d_grad.ensure(3 * n)) || (rc = c->d_grad_out.ensure(3 * nout)) ||
+        (rc = c->d_grad_work.upload(work, s)))
+        return rc;
+    double* dx = c->d_pos.p;
+    double *dfx = c->d_fl.p, *dfy = dfx + n, *dfz = dfy + n;
+    HIPC(hipMemcpyAsync(dx, xx, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
+    for (int attempt = 0; attempt < 4; attempt++) {
+        hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
+        rc = enqueue_long(c, dx, dx + n, dx + 2 * n, dfx, dfy, dfz, c->d_long.p, s, c->d_grad.p);
+        if (rc) return rc;
+        if ((rc = read_status(c, s))) return rc;
+        if (c->h_status->flags & ST_OVERFLOW) {
+            c->cap_hint = std::max(c->cap_hint * 2, c->h_status->max_list + 64);
+            if ((rc = build(c))) return rc;
+            // build() may move every buffer, the uploaded positions included
+            if ((rc = c->d_grad.ensure(3 * n)) || (rc = c->d_grad_out.ensure(3 * nout)) ||
+                (rc = c->d_grad_work.upload(work, s)))
+                return rc;
+            dx = c->d_pos.p;
+            dfx = c->d_fl.p;
+            dfy = dfx + n;
+            dfz = dfy + n;
+            HIPC(hipMemcpyAsync(dx, xx, n * sizeof(double), hipMemcpyHostToDevice, s));
+            HIPC(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s));
+            HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
+            continue;
+        }
+        break;
+    }
+    double* go = c->d_grad_out.p;
+    {
+        LaunchCtx L = launch_ctx(c, s);
+        const double* g = c->d_grad.p;
+        launch_grad_sites(L, c->d_species.p, c->d_grad_work.p, (int)work.size(), g, g + n,
+                          g + 2 * n, go, go + nout, go + 2 * nout);
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipMemcpyAsync(fxl, dfx, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fyl, dfy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(fzl, dfz, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(terms, c->d_long.p, sizeof(mspms_long_terms) * c->nrep,
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(grad->dE_dcharge, go, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(grad->dE_depsilon, go + nout, nout * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(grad->dE_dsigma, go + 2 * nout, nout * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (c->h_status->flags & ST_OVERFLOW)
+        return fail(MSPMS_ENOMEM, "pair list overflow after growing");
+    if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
+    return 0;
 }
 
 // One RESPA velocity-Verlet outer step, NVE (src/md.c:22-83), all buffers

@@ -111,6 +111,7 @@ struct Status {
 struct LaunchCtx {
     hipStream_t stream;
     int natom, nmol, nsite, nrep;
+    int max_mol_atoms;        // largest molecule (atoms) of any species
     const SiteDev* sites;
     const MolDev* mols;
     const int* atom_mol;
@@ -156,13 +157,18 @@ void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
 void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x,
                           const double* y, const double* z, double* fx,
                           double* fy, double* fz, int precision);
+// gq/ge/gs: per-atom derivatives of the energy with respect to the atom's
+// own charge, epsilon and sigma (parameter adjoint, FP64); null: none
 void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision,
                  int charge_type, double* fx, double* fy, double* fz,
-                 double* epart, unsigned long long* ipart, int want_hash);
+                 double* epart, unsigned long long* ipart, int want_hash,
+                 double* gq = nullptr, double* ge = nullptr,
+                 double* gs = nullptr);
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const TermDev* terms, const double* x, const double* y,
                   const double* z, double* fx, double* fy, double* fz,
-                  double* epart);
+                  double* epart, double* gq = nullptr, double* ge = nullptr,
+                  double* gs = nullptr);
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
                    const TermDev* terms, const double* x, const double* y,
                    const double* z, double* fx, double* fy, double* fz,
@@ -172,7 +178,15 @@ void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x,
                     const double* y, const double* z);
 void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x,
                     const double* y, const double* z, double* fx, double* fy,
-                    double* fz);
+                    double* fz, double* gq = nullptr);
+// per (replica, site): sum of the per-atom derivatives over the replica's
+// atoms of that site, plus the self-energy term -2 q C coef (Ewald / Wolf)
+// (work items: rep, global site, first atom of the site, molecule count;
+// outputs [nrep][nsite])
+void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species,
+                       const int4* work, int nwork, const double* gq,
+                       const double* ge, const double* gs, double* out_q,
+                       double* out_e, double* out_s);
 void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_pair,
                           const unsigned long long* ip_pair, int have_pair,

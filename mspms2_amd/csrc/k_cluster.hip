@@ -292,44 +292,73 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
     const double rl2 = rl * rl;
     int n = 0;
     uint32_t* row = plist + (size_t)ci * cap;
-    for (int cyu = coly_lo; cyu <= coly_hi; cyu++) {
-        int cyw = ((cyu % R.ncy) + R.ncy) % R.ncy;
-        int sy = (cyu - cyw) / R.ncy;
-        for (int cxu = colx_lo; cxu <= colx_hi; cxu++) {
-            int cxw = ((cxu % R.ncx) + R.ncx) % R.ncx;
-            int sx = (cxu - cxw) / R.ncx;
-            int col = R.col0 + cyw * R.ncx + cxw;
-            int a = col_cl0[col], b = col_cl0[col + 1];
-            for (int base = a; base < b; base += 64) {
-                int cj = base + lane;
-                bool hit = false;
-                uint32_t entry = 0;
-                if (cj < b) {
-                    double4 Cj = cl_center[cj];
-                    float4 Ej = cl_ext[cj];
-                    double dxc = Ci.x - (Cj.x + sx * R.L[0]);
-                    double dyc = Ci.y - (Cj.y + sy * R.L[1]);
-                    double dzc = Ci.z - Cj.z;
-                    double nz = rint(dzc * R.invL[2]);
-                    dzc -= nz * R.L[2];
-                    bool ambz = (double)Ei.z + (double)Ej.z + rl >= 0.5 * R.L[2];
-                    double gx = ambx ? 0.0 : fmax(0.0, fabs(dxc) - ((double)Ei.x + (double)Ej.x));
-                    double gy = amby ? 0.0 : fmax(0.0, fabs(dyc) - ((double)Ei.y + (double)Ej.y));
-                    double gz = ambz ? 0.0 : fmax(0.0, fabs(dzc) - ((double)Ei.z + (double)Ej.z));
-                    hit = gx * gx + gy * gy + gz * gz < rl2;
-                    int szi = (int)nz;  // j image: Cj + (sx, sy, sz) * L
-                    bool a2 = ambx || amby || ambz || sx < -1 || sx > 1 || sy < -1 || sy > 1 ||
-                              szi < -1 || szi > 1;
-                    if (a2) szi = 0;
-                    entry = ((uint32_t)cj << 7) | ((a2 ? 1u : 0u) << 6) |
-                            ((uint32_t)(szi + 1) << 4) |
-                            ((uint32_t)((a2 ? 0 : sy) + 1) << 2) | (uint32_t)((a2 ? 0 : sx) + 1);
-                }
-                uint64_t mask = __ballot(hit);
-                int pre = __popcll(mask & ((1ull << lane) - 1ull));
-                if (hit && n + pre < cap) row[n + pre] = entry;
-                n += __popcll(mask);
+    // candidate columns in (y, x) order, 64 per batch; their clusters are
+    // flattened so that each pass of the wave tests 64 clusters whatever the
+    // column heights (a column holds ~ncz clusters, far fewer than 64)
+    const int ncx_c = colx_hi - colx_lo + 1, ncand = ncx_c * (coly_hi - coly_lo + 1);
+    for (int cb = 0; cb < ncand; cb += 64) {
+        const int nb = min(64, ncand - cb);
+        int a = 0, cnt = 0, sx = 0, sy = 0;
+        if (lane < nb) {
+            const int q = cb + lane;
+            const int cxu = colx_lo + q % ncx_c, cyu = coly_lo + q / ncx_c;
+            const int cxw = ((cxu % R.ncx) + R.ncx) % R.ncx;
+            const int cyw = ((cyu % R.ncy) + R.ncy) % R.ncy;
+            sx = (cxu - cxw) / R.ncx;
+            sy = (cyu - cyw) / R.ncy;
+            const int col = R.col0 + cyw * R.ncx + cxw;
+            a = col_cl0[col];
+            cnt = col_cl0[col + 1] - a;
+        }
+        // inclusive prefix of the column sizes over the lanes
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        for (int base = 0; base < total; base += 64) {
+            const int f = base + lane;
+            // column of flattened index f: number of columns with incl <= f
+            int qc = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int probe = qc + step - 1;
+                const int v = __shfl(incl, probe & 63, 64);
+                if (probe < nb && v <= f) qc += step;
             }
+            const int qa = __shfl(a, qc & 63, 64), qcnt = __shfl(cnt, qc & 63, 64);
+            const int qx = __shfl(sx, qc & 63, 64), qy = __shfl(sy, qc & 63, 64);
+            const int qincl = __shfl(incl, qc & 63, 64);
+            bool hit = false;
+            uint32_t entry = 0;
+            if (f < total) {
+                const int cj = qa + (f - (qincl - qcnt));
+                double4 Cj = cl_center[cj];
+                float4 Ej = cl_ext[cj];
+                double dxc = Ci.x - (Cj.x + qx * R.L[0]);
+                double dyc = Ci.y - (Cj.y + qy * R.L[1]);
+                double dzc = Ci.z - Cj.z;
+                double nz = rint(dzc * R.invL[2]);
+                dzc -= nz * R.L[2];
+                bool ambz = (double)Ei.z + (double)Ej.z + rl >= 0.5 * R.L[2];
+                double gx = ambx ? 0.0 : fmax(0.0, fabs(dxc) - ((double)Ei.x + (double)Ej.x));
+                double gy = amby ? 0.0 : fmax(0.0, fabs(dyc) - ((double)Ei.y + (double)Ej.y));
+                double gz = ambz ? 0.0 : fmax(0.0, fabs(dzc) - ((double)Ei.z + (double)Ej.z));
+                hit = gx * gx + gy * gy + gz * gz < rl2;
+                int szi = (int)nz;  // j image: Cj + (sx, sy, sz) * L
+                bool a2 = ambx || amby || ambz || qx < -1 || qx > 1 || qy < -1 || qy > 1 ||
+                          szi < -1 || szi > 1;
+                if (a2) szi = 0;
+                entry = ((uint32_t)cj << 7) | ((a2 ? 1u : 0u) << 6) |
+                        ((uint32_t)(szi + 1) << 4) |
+                        ((uint32_t)((a2 ? 0 : qy) + 1) << 2) | (uint32_t)((a2 ? 0 : qx) + 1);
+            }
+            uint64_t mask = __ballot(hit);
+            int pre = __popcll(mask & ((1ull << lane) - 1ull));
+            if (hit && n + pre < cap) row[n + pre] = entry;
+            n += __popcll(mask);
         }
     }
     if (lane == 0) {

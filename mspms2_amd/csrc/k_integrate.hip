@@ -73,6 +73,91 @@ __global__ void __launch_bounds__(256) k_respa_inner(
     ep[4] = vir;
 }
 
+// The same step with the molecule's atoms held in LDS for all substeps:
+// one wave per workgroup, lane l owns molecule blockIdx.x * 64 + l and the
+// slots [atom k][lane l] (stride 64, conflict-free across the wave).  No
+// barriers: every lane touches only its own slots.  NA bounds the atoms per
+// molecule.
+template <int NA>
+__global__ void __launch_bounds__(64) k_respa_inner_lds(
+    int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
+    const TermDev* __restrict__ terms, const SiteDev* __restrict__ sites, double* x, double* y,
+    double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
+    const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
+    double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
+    Status* st) {
+#pragma clang fp contract(off)
+    __shared__ double lx[NA * 64], ly[NA * 64], lz[NA * 64], lvx[NA * 64], lvy[NA * 64],
+        lvz[NA * 64], lfx[NA * 64], lfy[NA * 64], lfz[NA * 64], lrm[NA * 64];
+    const int l = threadIdx.x;
+    const int m = blockIdx.x * 64 + l;
+    if (m >= nmol) return;
+    const MolDev M = mols[m];
+    const SpeciesDev S = species[M.species];
+    const int na = M.natom, a0 = M.atom0;
+    for (int k = 0; k < na; k++) {
+        const int i = a0 + k, s = k * 64 + l;
+        const double r_mass = 1.0 / sites[S.site0 + k].aw;
+        lrm[s] = r_mass;
+        lx[s] = x[i];
+        ly[s] = y[i];
+        lz[s] = z[i];
+        lvx[s] = vx[i] + (deltby2 * fxl[i] * r_mass);
+        lvy[s] = vy[i] + (deltby2 * fyl[i] * r_mass);
+        lvz[s] = vz[i] + (deltby2 * fzl[i] * r_mass);
+        lfx[s] = fxs[i];
+        lfy[s] = fys[i];
+        lfz[s] = fzs[i];
+    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    for (int ll = 0; ll < nstep_inner; ll++) {
+        for (int k = 0; k < na; k++) {
+            const int s = k * 64 + l;
+            const double r_mass = lrm[s];
+            lvx[s] += (deltsby2 * lfx[s] * r_mass);
+            lvy[s] += (deltsby2 * lfy[s] * r_mass);
+            lvz[s] += (deltsby2 * lfz[s] * r_mass);
+            lx[s] = lx[s] + delts * lvx[s];
+            ly[s] = ly[s] + delts * lvy[s];
+            lz[s] = lz[s] + delts * lvz[s];
+            lfx[s] = lfy[s] = lfz[s] = 0.0;
+        }
+        e[0] = e[1] = e[2] = e[3] = 0.0;
+        vir = 0.0;
+        for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
+            double u = 0.0, v = 0.0;
+            int cls = bonded_term(terms[t], l, lx, ly, lz, lfx, lfy, lfz, &u, &v, &st->flags, 64);
+            e[cls] += u;
+            vir += v;
+        }
+        for (int k = 0; k < na; k++) {
+            const int s = k * 64 + l;
+            const double r_mass = lrm[s];
+            lvx[s] += (deltsby2 * lfx[s] * r_mass);
+            lvy[s] += (deltsby2 * lfy[s] * r_mass);
+            lvz[s] += (deltsby2 * lfz[s] * r_mass);
+        }
+    }
+    for (int k = 0; k < na; k++) {
+        const int i = a0 + k, s = k * 64 + l;
+        x[i] = lx[s];
+        y[i] = ly[s];
+        z[i] = lz[s];
+        vx[i] = lvx[s];
+        vy[i] = lvy[s];
+        vz[i] = lvz[s];
+        fxs[i] = lfx[s];
+        fys[i] = lfy[s];
+        fzs[i] = lfz[s];
+    }
+    double* ep = epart + (size_t)m * 5;
+    ep[0] = e[0];
+    ep[1] = e[1];
+    ep[2] = e[2];
+    ep[3] = e[3];
+    ep[4] = vir;
+}
+
 __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __restrict__ atom_site,
                                                      const SiteDev* __restrict__ sites, double* vx,
                                                      double* vy, double* vz,
@@ -118,10 +203,24 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
                         const double* fxl, const double* fyl, const double* fzl, double* fxs,
                         double* fys, double* fzs, double deltby2, double delts,
                         double deltsby2, int nstep_inner, double* epart) {
-    int g = (c.nmol + 255) / 256;
-    hipLaunchKernelGGL(k_respa_inner, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, species,
-                       terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl, fxs, fys, fzs, deltby2,
-                       delts, deltsby2, nstep_inner, epart, c.status);
+#define RESPA_LDS(NA)                                                                        \
+    hipLaunchKernelGGL(k_respa_inner_lds<NA>, dim3((c.nmol + 63) / 64), dim3(64), 0, c.stream,   \
+                       c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl,   \
+                       fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart,        \
+                       c.status)
+    if (c.max_mol_atoms <= 4) {
+        RESPA_LDS(4);
+    } else if (c.max_mol_atoms <= 8) {
+        RESPA_LDS(8);
+    } else if (c.max_mol_atoms <= 16) {
+        RESPA_LDS(16);
+    } else {
+        int g = (c.nmol + 255) / 256;
+        hipLaunchKernelGGL(k_respa_inner, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
+                           species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl, fxs, fys,
+                           fzs, deltby2, delts, deltsby2, nstep_inner, epart, c.status);
+    }
+#undef RESPA_LDS
 }
 
 void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,

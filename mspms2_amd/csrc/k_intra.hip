@@ -41,25 +41,44 @@ __device__ inline void add_pair(double* fx, double* fy, double* fz, int ii, int 
     fz[jj] -= az;
 }
 
+// per-atom parameter-derivative outputs of the adjoint (null: not wanted)
+struct GradOut {
+    double *gq, *ge, *gs;
+};
+
+template <bool GRAD>
 __device__ inline void intra_lj(const RepDev& R, const SiteDev& a, const SiteDev& b, int ii,
                                 int jj, double rx, double ry, double rz, double escale,
-                                IntraAcc& acc, double* fx, double* fy, double* fz) {
+                                IntraAcc& acc, double* fx, double* fy, double* fz,
+                                const GradOut& G) {
 #pragma clang fp contract(off)
     double rsq = rx * rx + ry * ry + rz * rz;
     if (!(rsq < R.rc2_lj)) return;
     double u, f, ush;
-    lj64(rsq, 0.5 * (a.sigma + b.sigma), escale * sqrt(a.epsilon * b.epsilon), R.switch_on,
-         R.rc2_lj, R.rcon2, R.roff2_ron2_cube, R.shift1, &u, &f, &ush);
+    const double e0 = sqrt(a.epsilon * b.epsilon), sij = 0.5 * (a.sigma + b.sigma);
+    lj64(rsq, sij, escale * e0, R.switch_on, R.rc2_lj, R.rcon2, R.roff2_ron2_cube, R.shift1, &u,
+         &f, &ush);
     acc.uvdw += u;
     acc.vir += f * rsq;
     add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+    if (GRAD) {
+        double dds, dde;
+        lj64_dparam(rsq, sij, escale * e0, R.switch_on, R.rc2_lj, R.rcon2, R.roff2_ron2_cube,
+                    &dds, &dde);
+        G.gs[ii] += 0.5 * dds;
+        G.gs[jj] += 0.5 * dds;
+        G.ge[ii] += dde * escale * deps_mix(e0, b.epsilon);
+        G.ge[jj] += dde * escale * deps_mix(e0, a.epsilon);
+    }
 }
 
 // electrostatics of an intramolecular pair (src/frclong.c:316-408 with
 // with_excl, :708-786 without)
-__device__ inline void intra_elec(const RepDev& R, double qq, int ii, int jj, double rx,
-                                  double ry, double rz, bool with_excl, IntraAcc& acc,
-                                  double* fx, double* fy, double* fz) {
+template <bool GRAD>
+__device__ inline void intra_elec(const RepDev& R, double qq, double qa, double qb, int ii,
+                                  int jj, double rx, double ry, double rz, bool with_excl,
+                                  IntraAcc& acc, double* fx, double* fy, double* fz,
+                                  const GradOut& G) {
 #pragma clang fp contract(off)
     const int ct = R.charge_type;
     if (ct != MSPMS_ELEC_EWALD && ct != MSPMS_ELEC_WOLF) return;
@@ -71,6 +90,14 @@ __device__ inline void intra_elec(const RepDev& R, double qq, int ii, int jj, do
             acc.uexcl += u;
             if (ct == MSPMS_ELEC_WOLF) acc.vir += f * rsq;
             add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+            if (GRAD) {
+                // uexcl enters uelec with a minus sign (src/frclong.c:1061, :1142)
+                double ug, fg;
+                ewald_excl64(rsq, qb, R.C, &ug, &fg);
+                G.gq[ii] -= ug;
+                ewald_excl64(rsq, qa, R.C, &ug, &fg);
+                G.gq[jj] -= ug;
+            }
         }
     }
     rx = mic(rx, R.L[0]);
@@ -86,6 +113,18 @@ __device__ inline void intra_elec(const RepDev& R, double qq, int ii, int jj, do
         }
         acc.ureal += u;
         add_pair(fx, fy, fz, ii, jj, f, rx, ry, rz);
+        if (GRAD) {
+            double ug, fg;
+            for (int side = 0; side < 2; side++) {
+                const double qo = side ? qa : qb;
+                if (ct == MSPMS_ELEC_EWALD)
+                    ewald_real64(rsq, qo, R.C, R.kappa, &ug, &fg);
+                else
+                    wolf_real64(rsq, qo, R.C, R.kappa, R.rc_el, R.wvc1, R.wvc2, R.wfc1, R.wfc2,
+                                &ug, &fg);
+                G.gq[side ? jj : ii] += ug;
+            }
+        }
     }
 }
 
@@ -93,6 +132,7 @@ __device__ inline bool is_long(const RepDev& R, double rx, double ry, double rz)
     return fabs(rx) > R.L[0] * 0.5 || fabs(ry) > R.L[1] * 0.5 || fabs(rz) > R.L[2] * 0.5;
 }
 
+template <bool GRAD>
 __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restrict__ mols,
                                                const RepDev* __restrict__ reps,
                                                const SiteDev* __restrict__ sites,
@@ -101,7 +141,8 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, double* fx,
-                                               double* fy, double* fz, double* epart) {
+                                               double* fy, double* fz, double* epart,
+                                               GradOut G) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
@@ -131,13 +172,14 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
                 if (a.ghost == MSPMS_GHOST_FULL || b.ghost == MSPMS_GHOST_FULL) continue;
                 double mx = mic(rx, R.L[0]), my = mic(ry, R.L[1]), mz = mic(rz, R.L[2]);
                 if (a.ghost != MSPMS_GHOST_LJ && b.ghost != MSPMS_GHOST_LJ)
-                    intra_lj(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz);
+                    intra_lj<GRAD>(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz, G);
                 // restored through int variables (src/frclong.c:223): truncation
                 rx = (double)(int)rx;
                 ry = (double)(int)ry;
                 rz = (double)(int)rz;
             }
-            intra_elec(R, qq, ii, jj, rx, ry, rz, true, acc, fx, fy, fz);
+            intra_elec<GRAD>(R, qq, a.charge, b.charge, ii, jj, rx, ry, rz, true, acc, fx, fy, fz,
+                             G);
         } else {
             // 1-4 ending pair (src/frclong.c:603-787); `same` is whatever the
             // bond / angle sections left (:622, :632 compare, not assign)
@@ -145,14 +187,15 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
             if (!same) {
                 double mx = mic(rx, R.L[0]), my = mic(ry, R.L[1]), mz = mic(rz, R.L[2]);
                 if (a.ghost != MSPMS_GHOST_LJ && b.ghost != MSPMS_GHOST_LJ)
-                    intra_lj(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz);
+                    intra_lj<GRAD>(R, a, b, ii, jj, mx, my, mz, 1.0, acc, fx, fy, fz, G);
                 rx = (double)(int)rx;
                 ry = (double)(int)ry;
                 rz = (double)(int)rz;
             } else {
-                intra_lj(R, a, b, ii, jj, rx, ry, rz, R.f0, acc, fx, fy, fz);
+                intra_lj<GRAD>(R, a, b, ii, jj, rx, ry, rz, R.f0, acc, fx, fy, fz, G);
             }
-            intra_elec(R, qq, ii, jj, rx, ry, rz, false, acc, fx, fy, fz);
+            intra_elec<GRAD>(R, qq, a.charge, b.charge, ii, jj, rx, ry, rz, false, acc, fx, fy,
+                             fz, G);
         }
     }
     double* ep = epart + (size_t)m * 4;
@@ -330,14 +373,63 @@ __global__ void __launch_bounds__(256) k_finalize_short(const RepDev* __restrict
     out[blockIdx.x] = t;
 }
 
+// Parameter adjoint: one workgroup per (replica, site) work item
+// (rep, global site, first atom of that site, molecule count).  Sums the
+// per-atom derivatives over the replica's atoms of the site and adds the
+// self-energy term: uelec carries -uself = -C coef sum q^2
+// (src/frclong.c:1058, :1140), so d/dq_i = -2 C coef q_i.
+__global__ void __launch_bounds__(256) k_grad_sites(
+    const int4* __restrict__ work, const RepDev* __restrict__ reps,
+    const SiteDev* __restrict__ sites, const SpeciesDev* __restrict__ species,
+    const double* __restrict__ gq, const double* __restrict__ ge, const double* __restrict__ gs,
+    int nsite, double* __restrict__ oq, double* __restrict__ oe, double* __restrict__ os) {
+    __shared__ double sh[3 * 8];
+    const int4 w = work[blockIdx.x];
+    const RepDev& R = reps[w.x];
+    const SiteDev S = sites[w.y];
+    const int stride = species[S.species].natom;
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int m = threadIdx.x; m < w.w; m += blockDim.x) {
+        const int a = w.z + m * stride;
+        p[0] += gq[a];
+        p[1] += ge[a];
+        p[2] += gs[a];
+    }
+    block_sum<3>(p, sh);
+    if (threadIdx.x != 0) return;
+    double coef = 0.0;
+    if (R.charge_type == MSPMS_ELEC_EWALD)
+        coef = R.C * sqrt(R.kappa * R.kappa / MSPMS_PI);
+    else if (R.charge_type == MSPMS_ELEC_WOLF)
+        coef = R.C * R.wvc3;
+    const size_t o = (size_t)w.x * nsite + w.y;
+    oq[o] = p[0] - 2.0 * coef * S.charge * (double)w.w;
+    oe[o] = p[1];
+    os[o] = p[2];
+}
+
 }  // namespace
+
+void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species, const int4* work,
+                       int nwork, const double* gq, const double* ge, const double* gs,
+                       double* out_q, double* out_e, double* out_s) {
+    if (nwork <= 0) return;
+    hipLaunchKernelGGL(k_grad_sites, dim3(nwork), dim3(256), 0, c.stream, work, c.reps, c.sites,
+                       species, gq, ge, gs, c.nsite, out_q, out_e, out_s);
+}
 
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
                   const double* x, const double* y, const double* z, double* fx, double* fy,
-                  double* fz, double* epart) {
+                  double* fz, double* epart, double* gq, double* ge, double* gs) {
     int g = (c.nmol + 255) / 256;
-    hipLaunchKernelGGL(k_intra, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps, c.sites,
-                       species, terms, x, y, z, fx, fy, fz, epart);
+    if (gq)
+        hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps,
+                           c.sites, species, terms, x, y, z, fx, fy, fz, epart,
+                           GradOut{gq, ge, gs});
+    else
+        hipLaunchKernelGGL(k_intra<false>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
+                           c.reps, c.sites, species, terms, x, y, z, fx, fy, fz, epart,
+                           GradOut{nullptr, nullptr, nullptr});
 }
 
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,

@@ -49,6 +49,12 @@ __device__ inline double sum8(double v) {
     return v;
 }
 
+// pair-set hash key of (i, j), replica-local atom ids (the numbering a
+// replica has as a system of its own)
+__device__ inline uint64_t pair_key(int i, int j) {
+    return splitmix64(((uint64_t)(uint32_t)i << 32) | (uint32_t)j);
+}
+
 // XCD-aware block order: consecutive i-clusters (spatial neighbours that
 // share j-clusters) land on the same XCD and its L2.
 __device__ inline int xcd_block(int b, int nb) {
@@ -59,7 +65,7 @@ __device__ inline int xcd_block(int b, int nb) {
     return lb;
 }
 
-template <int ELEC, bool FP64>
+template <int ELEC, bool FP64, bool GRAD>
 __global__ void __launch_bounds__(256) k_pair(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const SiteDev* __restrict__ sites, const double4* __restrict__ cl_center,
@@ -67,7 +73,9 @@ __global__ void __launch_bounds__(256) k_pair(
     const int* __restrict__ slot_orig, const int2* __restrict__ slot_info,
     const double4* __restrict__ slot_x64, const float4* __restrict__ slot_off,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash) {
+    double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash,
+    double* __restrict__ gq, double* __restrict__ ge, double* __restrict__ gs) {
+    static_assert(!GRAD || FP64, "parameter gradients are FP64 only");
     const int nb = gridDim.x;
     const int lb = xcd_block(blockIdx.x, nb);
     const int ci = lb * 4 + (threadIdx.x >> 6);
@@ -87,6 +95,7 @@ __global__ void __launch_bounds__(256) k_pair(
     const bool lji = vi && si.ghost != MSPMS_GHOST_LJ;
 
     double afx = 0.0, afy = 0.0, afz = 0.0;
+    double g_q = 0.0, g_e = 0.0, g_s = 0.0;  // d(pair energy)/d(own q, eps, sigma)
     double e_lj = 0.0, e_sh = 0.0, e_re = 0.0, vir = 0.0;
     unsigned long long n_lj = 0, n_el = 0, h_lj = 0, h_el = 0;
 
@@ -116,7 +125,6 @@ __global__ void __launch_bounds__(256) k_pair(
             const SiteDev sj = sites[infoj.y];
             const int oj = slot_orig[jslot];
             const bool count = oi < oj;
-            double fij_t = 0.0;
             if (lji && sj.ghost != MSPMS_GHOST_LJ && rsq < R.rc2_lj) {
                 double u, f, ush;
                 double sij, eij;
@@ -130,13 +138,19 @@ __global__ void __launch_bounds__(256) k_pair(
                 e_lj += u;
                 e_sh += ush;
                 vir += f * rsq;
-                fij_t += 0.0;
+                if (GRAD) {
+                    double dds, dde;
+                    lj64_dparam(rsq, sij, eij, R.switch_on, R.rc2_lj, R.rcon2, R.roff2_ron2_cube,
+                                &dds, &dde);
+                    g_s += 0.5 * dds;  // d sigma_ij / d sigma_i = 1/2
+                    g_e += dde * deps_mix(eij, sj.epsilon);
+                }
                 afx += f * rx;
                 afy += f * ry;
                 afz += f * rz;
                 if (count) {
                     n_lj++;
-                    if (want_hash) h_lj += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                    if (want_hash) h_lj += pair_key(oi - R.atom0, oj - R.atom0);
                 }
             }
             if (ELEC != 0 && rsq < R.rc2_el) {
@@ -154,12 +168,22 @@ __global__ void __launch_bounds__(256) k_pair(
                     vir += f * rsq;
                 }
                 e_re += u;
+                if (GRAD) {
+                    // the pair energy is linear in q_i: du/dq_i = u(q_i = 1)
+                    double ug, fg;
+                    if (ELEC == MSPMS_ELEC_EWALD)
+                        ewald_real64(rsq, sj.charge, R.C, R.kappa, &ug, &fg);
+                    else
+                        wolf_real64(rsq, sj.charge, R.C, R.kappa, R.rc_el, R.wvc1, R.wvc2, R.wfc1,
+                                    R.wfc2, &ug, &fg);
+                    g_q += ug;
+                }
                 afx += f * rx;
                 afy += f * ry;
                 afz += f * rz;
                 if (count) {
                     n_el++;
-                    if (want_hash) h_el += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                    if (want_hash) h_el += pair_key(oi - R.atom0, oj - R.atom0);
                 }
             }
         }
@@ -238,7 +262,7 @@ __global__ void __launch_bounds__(256) k_pair(
                 fsum += f;
                 if (count) {
                     n_lj++;
-                    if (want_hash) h_lj += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                    if (want_hash) h_lj += pair_key(oi - R.atom0, oj - R.atom0);
                 }
             }
             if (in_el) {
@@ -261,7 +285,7 @@ __global__ void __launch_bounds__(256) k_pair(
                 fsum += f;
                 if (count) {
                     n_el++;
-                    if (want_hash) h_el += splitmix64(((uint64_t)(uint32_t)oi << 32) | (uint32_t)oj);
+                    if (want_hash) h_el += pair_key(oi - R.atom0, oj - R.atom0);
                 }
             }
             afx += (double)(fsum * rx);
@@ -278,6 +302,16 @@ __global__ void __launch_bounds__(256) k_pair(
         fx[oi] = afx;
         fy[oi] = afy;
         fz[oi] = afz;
+    }
+    if (GRAD) {
+        g_q = sum8(g_q);
+        g_e = sum8(g_e);
+        g_s = sum8(g_s);
+        if (jsub == 0 && vi) {
+            gq[oi] = g_q;
+            ge[oi] = g_e;
+            gs[oi] = g_s;
+        }
     }
     e_lj = wave_sum(e_lj);
     e_sh = wave_sum(e_sh);
@@ -305,33 +339,44 @@ __global__ void __launch_bounds__(256) k_pair(
 
 template <int ELEC>
 void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* fx, double* fy,
-              double* fz, double* epart, unsigned long long* ipart, int want_hash) {
+              double* fz, double* epart, unsigned long long* ipart, int want_hash, double* gq,
+              double* ge, double* gs) {
     int nb = (b.maxcl + 3) / 4;
     // pad the grid to a multiple of 8 so the XCD remap stays a bijection
     int nb8 = (nb + 7) / 8 * 8;
-    if (precision == MSPMS_PREC_FP64)
-        hipLaunchKernelGGL((k_pair<ELEC, true>), dim3(nb8), dim3(256), 0, c.stream, b.ncl_total,
-                           b.maxcl, c.reps, c.sites, b.cl_center, b.plist, b.plist_n, b.cap,
-                           b.slot_orig, b.slot_info, b.slot_x64, b.slot_off, fx, fy, fz, epart,
-                           ipart, want_hash);
-    else
-        hipLaunchKernelGGL((k_pair<ELEC, false>), dim3(nb8), dim3(256), 0, c.stream,
+    if (gq)
+        hipLaunchKernelGGL((k_pair<ELEC, true, true>), dim3(nb8), dim3(256), 0, c.stream,
                            b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
                            b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
-                           fx, fy, fz, epart, ipart, want_hash);
+                           fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+    else if (precision == MSPMS_PREC_FP64)
+        hipLaunchKernelGGL((k_pair<ELEC, true, false>), dim3(nb8), dim3(256), 0, c.stream,
+                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
+                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
+                           fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_pair<ELEC, false, false>), dim3(nb8), dim3(256), 0, c.stream,
+                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
+                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
+                           fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
 }
 
 }  // namespace
 
 int pair_partial_count(const ClusterBufs& b) { return b.maxcl; }
 
+// gq/ge/gs non-null: FP64 evaluation that also writes each atom's
+// derivative of the pair energy with respect to its own charge, epsilon and
+// sigma (the parameter adjoint)
 void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision, int charge_type,
                  double* fx, double* fy, double* fz, double* epart, unsigned long long* ipart,
-                 int want_hash) {
+                 int want_hash, double* gq, double* ge, double* gs) {
     if (charge_type == MSPMS_ELEC_EWALD)
-        launch_t<MSPMS_ELEC_EWALD>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+        launch_t<MSPMS_ELEC_EWALD>(c, b, precision, fx, fy, fz, epart, ipart, want_hash, gq, ge,
+                                   gs);
     else if (charge_type == MSPMS_ELEC_WOLF)
-        launch_t<MSPMS_ELEC_WOLF>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+        launch_t<MSPMS_ELEC_WOLF>(c, b, precision, fx, fy, fz, epart, ipart, want_hash, gq, ge,
+                                  gs);
     else
-        launch_t<0>(c, b, precision, fx, fy, fz, epart, ipart, want_hash);
+        launch_t<0>(c, b, precision, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
 }

@@ -116,29 +116,44 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
     }
 }
 
-__global__ void k_sred(int nk_total, const int* __restrict__ k_rep,
-                       const RepDev* __restrict__ reps, const int* __restrict__ rep_chunk0,
-                       const double2* __restrict__ s_part, int nk_stride,
-                       const double* __restrict__ kvec, double2* __restrict__ ak,
-                       double* __restrict__ uk) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nk_total) return;
-    const int r = k_rep[k];
+// 32 lanes per k: lane j sums chunks j, j+32, ... and a fixed shuffle tree
+// combines the 32 partials (deterministic); 8 k per 256-thread workgroup.
+__global__ void __launch_bounds__(256) k_sred(int nk_total, const int* __restrict__ k_rep,
+                                              const RepDev* __restrict__ reps,
+                                              const int* __restrict__ rep_chunk0,
+                                              const double2* __restrict__ s_part, int nk_stride,
+                                              const double* __restrict__ kvec,
+                                              double2* __restrict__ ak, double* __restrict__ uk) {
+    const int k = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int j = threadIdx.x & 31;
+    const bool live = k < nk_total;
+    const int kk = live ? k : nk_total - 1;
+    const int r = k_rep[kk];
     const RepDev& R = reps[r];
-    const int kl = k - R.k0;
+    const int kl = kk - R.k0;
     double sr = 0.0, si = 0.0;
-    for (int c = rep_chunk0[r]; c < rep_chunk0[r + 1]; c++) {
-        double2 v = s_part[(size_t)c * nk_stride + kl];
-        sr += v.x;
-        si += v.y;
+    if (live)
+        for (int c = rep_chunk0[r] + j; c < rep_chunk0[r + 1]; c += 32) {
+            double2 v = s_part[(size_t)c * nk_stride + kl];
+            sr += v.x;
+            si += v.y;
+        }
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        sr += __shfl_xor(sr, o, 64);
+        si += __shfl_xor(si, o, 64);
     }
+    if (!live || j != 0) return;
     const double kv = kvec[k];
     uk[k] = kv * (sr * sr + si * si);
     const double s = 2.0 * R.vfactor * R.C * kv;
     ak[k] = make_double2(s * sr, s * si);
 }
 
-// work: (rep, first atom, first row, end row); 64 atoms per wave
+// work: (rep, first atom, first row, end row); 64 atoms per wave.
+// GRAD: also dU_fourier/dq_i = Vf C sum_k kvec 2 (S_r cos + S_i sin)
+//     = sum_k (A_k cos + B_k sin), added into gq.
+template <bool GRAD>
 __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                                                const RepDev* __restrict__ reps,
                                                const KRow* __restrict__ rows,
@@ -148,7 +163,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                                                const double* __restrict__ z,
                                                const int* __restrict__ atom_site,
                                                const SiteDev* __restrict__ sites, double* fx,
-                                               double* fy, double* fz) {
+                                               double* fy, double* fz, double* gq) {
     const int4 w = work[blockIdx.x];
     const RepDev& R = reps[w.x];
     const int a = w.y + threadIdx.x;
@@ -165,7 +180,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
         e[2] = {c, s};
     }
     const cplx ezc = conjc(e[2]);
-    double gx = 0.0, gy = 0.0, gz = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, dq = 0.0;
     int pkx = 1 << 30, pky = 1 << 30;
     cplx cx = {1.0, 0.0}, cy = {1.0, 0.0};
     for (int r = w.z; r < w.w; r++) {
@@ -187,6 +202,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
             const double2 ab = A[0];
             const double g = ab.x * cxy.im - ab.y * cxy.re;
             gsum += g;
+            if (GRAD) dq += ab.x * cxy.re + ab.y * cxy.im;
         }
         cplx c = cxy;
         for (int kz = 1; kz <= K.kz_hi; kz++) {
@@ -196,6 +212,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                 const double g = ab.x * c.im - ab.y * c.re;
                 gsum += g;
                 gzr += g * kz;
+                if (GRAD) dq += ab.x * c.re + ab.y * c.im;
             }
         }
         c = cxy;
@@ -206,6 +223,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                 const double g = ab.x * c.im - ab.y * c.re;
                 gsum += g;
                 gzr += g * kz;
+                if (GRAD) dq += ab.x * c.re + ab.y * c.im;
             }
         }
         gx += gsum * K.kx;
@@ -217,6 +235,7 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
     atomicAdd(&fx[a], q * gx * R.twopi_l[0]);
     atomicAdd(&fy[a], q * gy * R.twopi_l[1]);
     atomicAdd(&fz[a], q * gz * R.twopi_l[2]);
+    if (GRAD) atomicAdd(&gq[a], dq);
 }
 
 }  // namespace
@@ -237,13 +256,18 @@ void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x, co
         hipLaunchKernelGGL(k_sk<33>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s,
                            c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
                            rb.nk_stride);
-    hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 255) / 256), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 7) / 8), dim3(256), 0, c.stream,
                        rb.nk_total, rb.k_rep, c.reps, rb.rep_chunk0, rb.s_part, rb.nk_stride,
                        rb.kvec, rb.ak, rb.uk);
 }
 
 void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
-                    const double* z, double* fx, double* fy, double* fz) {
-    hipLaunchKernelGGL(k_kforce, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f, c.reps,
-                       rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz);
+                    const double* z, double* fx, double* fy, double* fz, double* gq) {
+    if (gq)
+        hipLaunchKernelGGL(k_kforce<true>, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,
+                           c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz, gq);
+    else
+        hipLaunchKernelGGL(k_kforce<false>, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,
+                           c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz,
+                           nullptr);
 }

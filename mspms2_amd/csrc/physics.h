@@ -46,6 +46,30 @@ __device__ inline void lj64(double rijsq, double sigmaij, double epsilonij, int 
     *uij = u;
 }
 
+// Parameter derivatives of the ljfrc energy: du/dsigma_ij and du/deps_ij of
+// u = 4 eps (s^12 - s^6) [* sw(r)], s = sigma / r; the switch factor does
+// not depend on the parameters.
+__device__ inline void lj64_dparam(double rijsq, double sigmaij, double epsilonij, int switch_on,
+                                   double rcutoffsq, double rcutonsq, double roff2_ron2_cube,
+                                   double* du_dsigma, double* du_deps) {
+    double sw = 1.0;
+    if (switch_on && !(rijsq < rcutonsq))
+        sw = (rcutoffsq - rijsq) * (rcutoffsq - rijsq) * (rcutoffsq + 2.0 * rijsq - 3.0 * rcutonsq) /
+             roff2_ron2_cube;
+    double r6 = sigmaij * sigmaij / rijsq;
+    r6 = r6 * r6 * r6;
+    const double r12 = r6 * r6;
+    *du_deps = 4.0 * (r12 - r6) * sw;
+    *du_dsigma = 24.0 * epsilonij * (2.0 * r12 - r6) / sigmaij * sw;
+}
+
+// d eps_ij / d eps_i of the mixing rule eps_ij = sqrt(eps_i eps_j): infinite
+// at eps_i = 0 < eps_j, where the square root has no derivative
+__device__ inline double deps_mix(double eij, double eps_j) {
+    if (eij > 0.0) return 0.5 * eps_j / eij;
+    return eps_j == 0.0 ? 0.0 : __builtin_huge_val();
+}
+
 // ewald_real_frc (src/elecfrc.c:17-33)
 __device__ inline void ewald_real64(double rijsq, double qq, double C, double kappa, double* uij,
                                     double* fij) {

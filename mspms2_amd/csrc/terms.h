@@ -87,13 +87,15 @@ __device__ inline void charmm_forces(int i1, int i2, int i3, int i4, const doubl
 }
 
 // Returns the energy class (0 bond, 1 angle, 2 dihedral, 3 improper).
+// Atom k of the term sits at index base + idx[k] * stride of the arrays
+// (stride 1: the global SoA arrays; a per-molecule LDS slice otherwise).
 __device__ inline int bonded_term(const TermDev& T, int base, const double* xx, const double* yy,
                                   const double* zz, double* fxs, double* fys, double* fzs,
-                                  double* u, double* vir, int* flags) {
+                                  double* u, double* vir, int* flags, int stride = 1) {
 #pragma clang fp contract(off)
     const double* pr = T.para;
     if (T.kind == TERM_BOND) {
-        int i1 = base + T.idx[0], i2 = base + T.idx[1];
+        int i1 = base + T.idx[0] * stride, i2 = base + T.idx[1] * stride;
         double rx = xx[i1] - xx[i2], ry = yy[i1] - yy[i2], rz = zz[i1] - zz[i2];
         double rsq = rx * rx + ry * ry + rz * rz, r = sqrt(rsq), fij = 0.0;
         if (T.type == MSPMS_BOND_HARMONIC) {
@@ -119,7 +121,8 @@ __device__ inline int bonded_term(const TermDev& T, int base, const double* xx, 
         return 0;
     }
     if (T.kind == TERM_ANGLE) {
-        int ia = base + T.idx[0], ib = base + T.idx[1], ic = base + T.idx[2];
+        int ia = base + T.idx[0] * stride, ib = base + T.idx[1] * stride,
+            ic = base + T.idx[2] * stride;
         if (T.type == MSPMS_ANGLE_HARMONIC) {
             double xab = xx[ia] - xx[ib], yab = yy[ia] - yy[ib], zab = zz[ia] - zz[ib];
             double xbc = xx[ic] - xx[ib], ybc = yy[ic] - yy[ib], zbc = zz[ic] - zz[ib];
@@ -194,7 +197,8 @@ __device__ inline int bonded_term(const TermDev& T, int base, const double* xx, 
         return 1;
     }
     if (T.kind == TERM_DIH) {
-        int ia = base + T.idx[0], ib = base + T.idx[1], ic = base + T.idx[2], id = base + T.idx[3];
+        int ia = base + T.idx[0] * stride, ib = base + T.idx[1] * stride,
+            ic = base + T.idx[2] * stride, id = base + T.idx[3] * stride;
         if (T.type == MSPMS_DIH_OPLS_COSIN) {
             double xab = xx[ia] - xx[ib], yab = yy[ia] - yy[ib], zab = zz[ia] - zz[ib];
             double xbc = xx[ib] - xx[ic], ybc = yy[ib] - yy[ic], zbc = zz[ib] - zz[ic];
@@ -261,7 +265,8 @@ __device__ inline int bonded_term(const TermDev& T, int base, const double* xx, 
     }
     // TERM_IMP, IMP_CHARMM
     {
-        int ia = base + T.idx[0], ib = base + T.idx[1], ic = base + T.idx[2], id = base + T.idx[3];
+        int ia = base + T.idx[0] * stride, ib = base + T.idx[1] * stride,
+            ic = base + T.idx[2] * stride, id = base + T.idx[3] * stride;
         double kom = pr[0], om0 = pr[1];
         double g[18];
         charmm_geom(ia, ib, ic, id, xx, yy, zz, g);

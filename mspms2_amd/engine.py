@@ -40,6 +40,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise EngineError(f"{LIB_PATH} is missing: build it with `python -m mspms2_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.
+    # Loaded first, it satisfies this library's dependency of that name; the
+    # other order loads a second runtime that leaves torch without a GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p
@@ -62,6 +69,8 @@ def lib():
                                          vp]
     L.mspms_vver_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp,
                                         vp, vp, vp, vp, vp, C.c_int, vp]
+    L.mspms_energy_param_grad.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.POINTER(abi.LongTerms),
+                                          C.POINTER(abi.ParamGrad)]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -189,6 +198,43 @@ class Engine:
                "mspms_frcshort_host")
         terms = [x.as_dict() for x in t]
         return f, (terms[0] if self.nrep == 1 else terms)
+
+    def param_grad(self, xx=None, yy=None, zz=None):
+        """FP64 frclong plus dE/d(epsilon, sigma, charge) of uinter per
+        replica and global site id: returns (forces, terms, grads) with
+        grads[name] of shape (nrep, nsite)."""
+        xx, yy, zz = self._xyz(xx, yy, zz)
+        n = self.natom
+        nsite = sum(s.natom for s in self.system.species)
+        f = np.empty((3, n))
+        t = (abi.LongTerms * self.nrep)()
+        g = {k: np.zeros((self.nrep, nsite)) for k in ("epsilon", "sigma", "charge")}
+        pg = abi.ParamGrad(abi.dptr(g["epsilon"]), abi.dptr(g["sigma"]), abi.dptr(g["charge"]))
+        _check(lib().mspms_energy_param_grad(self._h, abi.dptr(xx), abi.dptr(yy), abi.dptr(zz),
+                                             abi.dptr(f[0]), abi.dptr(f[1]), abi.dptr(f[2]), t,
+                                             C.byref(pg)), "mspms_energy_param_grad")
+        terms = [x.as_dict() for x in t]
+        return f, (terms[0] if self.nrep == 1 else terms), g
+
+    def set_site_params(self, epsilon=None, sigma=None, charge=None):
+        """Replaces the per-site template values (arrays over global site
+        ids, reduced units) and rebuilds the device tables."""
+        off = 0
+        for sp in self.system.species:
+            sl = slice(off, off + sp.natom)
+            for name, v in (("epsilon", epsilon), ("sigma", sigma), ("charge", charge)):
+                if v is not None:
+                    getattr(sp, name)[:] = np.asarray(v, dtype=np.float64)[sl]
+            off += sp.natom
+        reps = getattr(self, "replica_systems", None)
+        self._sp, self._keep = self.system.c_species()
+        _check(lib().mspms_set_system(self._h, len(self.system.species), self._sp,
+                                      C.byref(self.params)), "mspms_set_system")
+        if reps:
+            arr, nm, params = self._rep_keep
+            _check(lib().mspms_set_replicas(self._h, len(params),
+                                            nm.ctypes.data_as(C.POINTER(C.c_int)), arr),
+                   "mspms_set_replicas")
 
     # ---- device-buffer evaluation (torch tensors or raw pointers) -----------
     @staticmethod
