@@ -112,6 +112,76 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     return out
 
 
+def make_replicas(nrep=256, seed=2024):
+    """Batched-replica workload: n-butane boxes of U{64..512} molecules at
+    0.58 g/cm^3, rc = min(14 A, L/2), one seed per box."""
+    from mspms2_amd import configs
+    from tests.util import system_of
+    rng = np.random.default_rng(seed)
+    n = rng.integers(64, 513, size=nrep)
+    return [system_of(configs.butane_box(nmol=int(k), seed=1000 + r)) for r, k in enumerate(n)]
+
+
+def _replica_run(systems, steps, warmup, nstep_inner, dev):
+    """RESPA steps of one engine over `systems` (one replica table), replayed
+    from a captured graph; returns (ms per step, i<j pairs in range per step)."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from mspms2_amd import abi
+    eng = Engine.replicas(systems, precision="mixed")
+    nr = len(systems)
+    pos = torch.tensor(eng.packed_positions(), device=dev)
+    vel = torch.tensor(np.concatenate([maxwell_velocities(s, seed=11) for s in systems], axis=1),
+                       device=dev)
+    f0, _ = eng.frclong()
+    fs0, _ = eng.frcshort()
+    fl, fs = torch.tensor(f0, device=dev), torch.tensor(fs0, device=dev)
+    lt = torch.zeros(nr * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    stt = torch.zeros(nr * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(nr, dtype=torch.float64, device=dev)
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    delt = systems[0].delt
+    eng.vver_run_device(pos, vel, fl, fs, delt, nstep_inner, warmup, lt, stt, uk, graph=True,
+                        stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    eng.sync()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    eng.vver_run_device(pos, vel, fl, fs, delt, nstep_inner, steps, lt, stt, uk, graph=True,
+                        stream=st.cuda_stream)
+    b.record(st)
+    torch.cuda.synchronize()
+    eng.sync()
+    raw = lt.cpu().numpy().view(np.uint8).tobytes()
+    sz = LONG_TERMS_BYTES
+    pairs = sum(abi.LongTerms.from_buffer_copy(raw[r * sz:(r + 1) * sz]).n_lj_pairs
+                for r in range(nr))
+    finite = bool(torch.isfinite(pos).all().item())
+    eng.close()
+    return a.elapsed_time(b) / steps, pairs, finite
+
+
+def run_replica_batch(systems, steps=20, warmup=3, nstep_inner=5):
+    """All replicas in one table (one launch set per kernel) against the
+    same replicas run one after another, each its own engine and graph."""
+    import torch
+    dev = torch.device("cuda:0")
+    ms_b, pairs, fin = _replica_run(systems, steps, warmup, nstep_inner, dev)
+    ms_seq, fin_s = 0.0, True
+    for s in systems:
+        ms1, _, f1 = _replica_run([s], max(2, steps // 4), 1, nstep_inner, dev)
+        ms_seq += ms1
+        fin_s = fin_s and f1
+    return {"workload": f"{len(systems)} n-butane boxes, 64-512 molecules each, "
+                        f"{sum(s.natom for s in systems)} sites; RESPA 1 fs / {nstep_inner}",
+            "batched_ms_per_step": ms_b, "sequential_ms_per_step": ms_seq,
+            "pairs_per_step": int(pairs),
+            "batched_pair_interactions_per_s": pairs / (ms_b * 1e-3),
+            "sequential_pair_interactions_per_s": pairs / (ms_seq * 1e-3),
+            "batched_over_sequential": ms_seq / ms_b, "finite": fin and fin_s}
+
+
 def run_reference_cpu(rd, repeat=2):
     """The reference's own frclong timed on one host core (oracle/_ref)."""
     from oracle import oracle as O
@@ -141,6 +211,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--fp64-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replicas", type=int, default=256,
+                    help="boxes in the batched-replica leg (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU per process; --gpus {args.gpus} is not "
@@ -214,6 +286,8 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
+    if args.replicas > 0:
+        line["replicas"] = run_replica_batch(make_replicas(args.replicas))
     print(json.dumps(line))
 
 

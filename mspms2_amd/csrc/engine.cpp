@@ -110,6 +110,7 @@ struct mspms_ctx {
     int natom = 0, nmol = 0, nsite = 0;
     int charge_union = MSPMS_ELEC_NONE;  // charge type shared by the replicas
     bool any_intra = false, any_bonded = false, any_recip = false, any_sf = false;
+    bool any_vacuum = false, any_coul = false;
     int want_hash = -1;                  // -1: FP64 only
 
     // static device tables
@@ -128,7 +129,7 @@ struct mspms_ctx {
     DevBuf<double> d_atom_wz;
     DevBuf<int2> d_slot_info;
     DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center;
-    DevBuf<float4> d_slot_off, d_cl_ext;
+    DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext;
     DevBuf<uint32_t> d_plist;
     int cap_hint = 0;
 
@@ -141,7 +142,8 @@ struct mspms_ctx {
     DevBuf<double2> d_s_part, d_ak;
 
     // partial sums and outputs
-    DevBuf<double> d_ep_pair, d_ep_intra, d_ep_short, d_ep_sf;
+    DevBuf<double> d_ep_pair, d_ep_intra, d_ep_short, d_ep_sf, d_uvac, d_ep_coul;
+    DevBuf<double4> d_mcom;
 
     // solid-fluid tables
     SFDev sf{};
@@ -319,12 +321,8 @@ int validate_params(const mspms_params& p) {
         return fail(MSPMS_ERANGE, "cutoff larger than half the box (src/validation.c:617)");
     if (p.iExternal_FF_type != 0)
         return fail(MSPMS_EINVAL, "external force fields (metal cluster) are not supported");
-    if (p.iChargeType == MSPMS_ELEC_SIMPLE_COULOMB)
-        return fail(MSPMS_EINVAL, "simple Coulomb (iChargeType 3) is not implemented on the GPU");
     if (p.iChargeType < 0 || p.iChargeType > 3) return fail(MSPMS_EINVAL, "bad iChargeType");
     if (p.iChargeType == MSPMS_ELEC_EWALD) {
-        if (p.fEwald_BC == MSPMS_EWALD_BC_VACUUM)
-            return fail(MSPMS_EINVAL, "vacuum Ewald boundary is not implemented on the GPU");
         if (p.kmaxx < 0 || p.kmaxy < 0 || p.kmaxz < 0 || p.kmaxx > 16 || p.kmaxy > 16 ||
             p.kmaxz > 16)
             return fail(MSPMS_EINVAL, "KMAX must be within 0..16");
@@ -449,6 +447,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_slot_w64.release();
     c->d_cl_center.release();
     c->d_slot_off.release();
+    c->d_slot_par.release();
     c->d_cl_ext.release();
     c->d_plist.release();
     c->d_rows.release();
@@ -465,6 +464,9 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_ep_intra.release();
     c->d_ep_short.release();
     c->d_ep_sf.release();
+    c->d_uvac.release();
+    c->d_ep_coul.release();
+    c->d_mcom.release();
     c->d_ip_pair.release();
     c->d_long.release();
     c->d_short.release();
@@ -500,6 +502,8 @@ int build(mspms_ctx* c) {
         SpeciesDev d{};
         d.natom = q.natom;
         d.site0 = (int)sites.size();
+        d.mw = 0.0;
+        for (int a = 0; a < q.natom; a++) d.mw += q.aw[a];  // src/readins.c:115,138
         for (int a = 0; a < q.natom; a++) {
             SiteDev t{};
             t.sigma = q.sigma[a];
@@ -593,6 +597,14 @@ int build(mspms_ctx* c) {
     int cell0 = 0, col0 = 0;
     c->charge_union = c->params[0].iChargeType;
     c->any_recip = false;
+    c->any_vacuum = false;
+    c->any_coul = c->charge_union == MSPMS_ELEC_SIMPLE_COULOMB;
+    if (c->any_coul) {
+        int mx = 0;
+        for (const SpeciesHost& q : c->sp) mx = std::max(mx, q.natom);
+        if (!coulomb_mol_supported(mx))
+            return fail(MSPMS_EINVAL, "simple Coulomb supports molecules of at most 16 sites");
+    }
     c->any_sf = false;
     std::vector<KRow> rows, rows_s;
     std::vector<double> kvec;
@@ -663,6 +675,9 @@ int build(mspms_ctx* c) {
         R.charge_type = P.iChargeType;
         R.switch_on = P.isLJswitchOn;
         R.sf_type = P.iSF_type;
+        R.twopi_over_3v = P.twopi_over_3v;
+        R.ewald_bc = P.iChargeType == MSPMS_ELEC_EWALD ? P.fEwald_BC : 0;
+        if (R.ewald_bc == MSPMS_EWALD_BC_VACUUM) c->any_vacuum = true;
         // cell grid: cubes holding ~8 real sites; columns along z
         double vol = P.boxlx * P.boxly * P.boxlz;
         double dens = std::max(nreal, 1) / vol;
@@ -805,7 +820,8 @@ int build(mspms_ctx* c) {
         (rc = c->d_slot_info.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_x64.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_w64.ensure(maxcl * MSPMS_CL)) ||
-        (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
+        (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_par.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
         (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
         (rc = c->d_plist_n.ensure(maxcl)))
         return rc;
@@ -826,6 +842,7 @@ int build(mspms_ctx* c) {
     b.slot_x64 = c->d_slot_x64.p;
     b.slot_w64 = c->d_slot_w64.p;
     b.slot_off = c->d_slot_off.p;
+    b.slot_par = c->d_slot_par.p;
     b.cl_center = c->d_cl_center.p;
     b.cl_ext = c->d_cl_ext.p;
     b.plist = c->d_plist.p;
@@ -930,7 +947,8 @@ int build(mspms_ctx* c) {
     }
 
     // ---- partials and outputs ----
-    if ((rc = c->d_ep_pair.ensure(maxcl * 4)) || (rc = c->d_ip_pair.ensure(maxcl * 4)) ||
+    if ((rc = c->d_ep_pair.ensure(maxcl * 4)) || (rc = c->d_uvac.ensure(c->nrep)) ||
+        (rc = c->d_ep_coul.ensure(c->nmol)) || (rc = c->d_mcom.ensure(c->nmol)) || (rc = c->d_ip_pair.ensure(maxcl * 4)) ||
         (rc = c->d_ep_intra.ensure((size_t)c->nmol * 4)) ||
         (rc = c->d_ep_short.ensure((size_t)c->nmol * 5)) ||
         (rc = c->d_ep_sf.ensure((size_t)c->natom)) || (rc = c->d_long.ensure(c->nrep)) ||
@@ -1106,6 +1124,12 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         }
         ProfScope p(c, s, PROF_RECIP_F);
         launch_recip_f(L, c->rb, x, y, z, fx, fy, fz, gq);
+        if (c->any_vacuum)
+            launch_vacuum(L, c->d_species.p, x, y, z, fx, fy, fz, c->d_uvac.p, gq);
+    }
+    if (c->any_coul) {
+        ProfScope p(c, s, PROF_INTRA);
+        launch_coulomb_mol(L, c->d_species.p, x, y, z, c->d_mcom.p, fx, fy, fz, c->d_ep_coul.p);
     }
     if (c->any_sf) {
         ProfScope p(c, s, PROF_SF);
@@ -1115,7 +1139,9 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         ProfScope p(c, s, PROF_FINAL);
         launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1,
                              c->d_ep_intra.p, c->any_intra ? 1 : 0, c->rb.uk,
-                             c->any_recip ? 1 : 0, c->d_ep_sf.p, c->any_sf ? 1 : 0, terms);
+                             c->any_recip ? 1 : 0, c->d_ep_sf.p, c->any_sf ? 1 : 0,
+                             c->any_vacuum ? c->d_uvac.p : nullptr,
+                             c->any_coul ? c->d_ep_coul.p : nullptr, terms);
     }
     HIPC(hipGetLastError());
     return 0;
@@ -1281,6 +1307,8 @@ extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double* xx, const dou
         return fail(MSPMS_EINVAL, "null argument");
     if (c->any_sf)
         return fail(MSPMS_EINVAL, "parameter gradients of the solid-fluid term are not provided");
+    if (c->any_coul)
+        return fail(MSPMS_EINVAL, "parameter gradients of simple Coulomb are not provided");
     HIPC(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
     const size_t n = (size_t)c->natom;

@@ -60,6 +60,8 @@ struct RepDev {
     // Ewald reciprocal work tables (per replica ranges)
     int krow0, nkrow, k0, nk;
     int kmax[3], ksqmax;
+    double twopi_over_3v;     // vacuum boundary term, src/initvars.c:441
+    int ewald_bc, pad3;
 };
 
 // Ewald k-vectors grouped in rows of fixed (kx, ky), kz contiguous
@@ -89,6 +91,8 @@ struct ClusterBufs {
     double4* slot_x64;   // [8*maxcl] FP64 unwrapped (x,y,z,-)
     double4* slot_w64;   // [8*maxcl] FP64 wrapped  (x,y,z,-)
     float4* slot_off;    // [8*maxcl] FP32 offset from centre + charge
+    float4* slot_par;    // [8*maxcl] FP32 sigma, sqrt(eps) (0: no LJ),
+                         //   molecule id and flags as int bits (SLOT_*)
     double4* cl_center;  // [maxcl]
     float4* cl_ext;      // [maxcl]
     uint32_t* plist;     // [maxcl*cap]
@@ -102,6 +106,10 @@ struct Status {
     int max_list;        // largest pair-list row seen
     int pad[2];
 };
+
+// slot_par.w flags
+#define SLOT_VALID 1   // a site (not padding)
+#define SLOT_LJ 2      // takes part in LJ (not GHOST_LJ)
 
 #define ST_NONFINITE 1
 #define ST_OVERFLOW 2
@@ -192,7 +200,20 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const unsigned long long* ip_pair, int have_pair,
                           const double* ep_intra, int have_intra,
                           const double* uk, int have_recip,
-                          const double* ep_sf, int have_sf, void* terms);
+                          const double* ep_sf, int have_sf, const double* uvac,
+                          const double* ep_coul, void* terms);
+// simple Coulomb between molecules (iChargeType 3): COMs into com[nmol],
+// per-molecule energy partials into ep[nmol]; molecules of <= 16 sites
+int coulomb_mol_supported(int max_mol_atoms);
+void launch_coulomb_mol(const LaunchCtx& c, const SpeciesDev* species, const double* x,
+                        const double* y, const double* z, double4* com, double* fx,
+                        double* fy, double* fz, double* ep);
+// Ewald vacuum boundary (src/frclong.c:1065-1126): per replica the dipole
+// of the COM-wrapped molecules, uvac[rep], and its force; gq (adjoint,
+// optional) receives d uvacuum / d q_i
+void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* x,
+                   const double* y, const double* z, double* fx, double* fy, double* fz,
+                   double* uvac, double* gq);
 void launch_finalize_short(const LaunchCtx& c, const double* epart,
                            void* terms);
 void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species,

@@ -201,7 +201,8 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
                        const int* __restrict__ cell_start, const int* __restrict__ col_rep,
                        const SiteDev* __restrict__ sites, int* __restrict__ slot_orig,
                        int2* __restrict__ slot_info, const double4* __restrict__ slot_w64,
-                       float4* __restrict__ slot_off, double4* __restrict__ cl_center,
+                       float4* __restrict__ slot_off, float4* __restrict__ slot_par,
+                       double4* __restrict__ cl_center,
                        float4* __restrict__ cl_ext) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int c = t >> 3, k = t & 7;
@@ -239,12 +240,18 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
     }
     if (valid) {
         int s = slot_info[slot].y;
+        const int mol = slot_info[slot].x;
+        const SiteDev& S = sites[s];
+        const bool lj = S.ghost != MSPMS_GHOST_LJ;
         slot_off[slot] = make_float4((float)(w.x - cx), (float)(w.y - cy), (float)(w.z - cz),
-                                     sites[s].charge_f);
+                                     S.charge_f);
+        slot_par[slot] = make_float4(S.sigma_f, lj ? S.sqrt_eps_f : 0.f, __int_as_float(mol),
+                                     __int_as_float(SLOT_VALID | (lj ? SLOT_LJ : 0)));
     } else {
         slot_orig[slot] = -1;
         slot_info[slot] = make_int2(-1, -1);
         slot_off[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        slot_par[slot] = make_float4(0.f, 0.f, __int_as_float(-1), __int_as_float(0));
     }
 }
 
@@ -390,7 +397,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     int gB = (b.maxcl * MSPMS_CL + T - 1) / T;
     hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,
-                       b.slot_info, b.slot_w64, b.slot_off, b.cl_center, b.cl_ext);
+                       b.slot_info, b.slot_w64, b.slot_off, b.slot_par, b.cl_center, b.cl_ext);
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,

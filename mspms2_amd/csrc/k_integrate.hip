@@ -179,10 +179,10 @@ __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __res
 }
 
 // per replica: ukin = 0.5 * sum (v.v)/r_mass in atom order blocks
-__global__ void __launch_bounds__(256) k_kinetic(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(1024) k_kinetic(const RepDev* __restrict__ reps,
                                                  const double* __restrict__ kin,
                                                  double* __restrict__ ukin) {
-    __shared__ double sh[8];
+    __shared__ double sh[16];
     const RepDev& R = reps[blockIdx.x];
     double s = 0.0;
     for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x) s += kin[a];
@@ -229,5 +229,5 @@ void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
     int g = (c.natom + 255) / 256;
     hipLaunchKernelGGL(k_respa_final, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site,
                        c.sites, vx, vy, vz, fxl, fyl, fzl, deltby2, kin);
-    hipLaunchKernelGGL(k_kinetic, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, kin, ukin);
+    hipLaunchKernelGGL(k_kinetic, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, kin, ukin);
 }

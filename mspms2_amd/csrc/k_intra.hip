@@ -283,15 +283,16 @@ __device__ void block_sum_u64(unsigned long long (&v)[NV], unsigned long long* s
 }
 
 // One workgroup per replica.
-__global__ void __launch_bounds__(256) k_finalize_long(
+__global__ void __launch_bounds__(1024) k_finalize_long(
     const RepDev* __restrict__ reps, const int* __restrict__ col_cl0,
     const double* __restrict__ ep_pair, const unsigned long long* __restrict__ ip_pair,
     int have_pair, const double* __restrict__ ep_intra, int have_intra,
     const double* __restrict__ uk, int have_recip, const double* __restrict__ ep_sf, int have_sf,
+    const double* __restrict__ uvac, const double* __restrict__ ep_coul,
     mspms_long_terms* __restrict__ out) {
 #pragma clang fp contract(off)
-    __shared__ double sh[4 * 8];
-    __shared__ unsigned long long shu[4 * 8];
+    __shared__ double sh[4 * 16];
+    __shared__ unsigned long long shu[4 * 16];
     const RepDev& R = reps[blockIdx.x];
     double p[4] = {0, 0, 0, 0};
     unsigned long long q[4] = {0, 0, 0, 0};
@@ -321,6 +322,10 @@ __global__ void __launch_bounds__(256) k_finalize_long(
         for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x)
             sf[0] += ep_sf[a];
     block_sum<1>(sf, sh);
+    double uc[1] = {0.0};
+    if (ep_coul && R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB)
+        for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x) uc[0] += ep_coul[m];
+    block_sum<1>(uc, sh);
     if (threadIdx.x != 0) return;
     mspms_long_terms t;
     // full lists visit every inter-molecular pair twice
@@ -337,10 +342,19 @@ __global__ void __launch_bounds__(256) k_finalize_long(
         t.ufourier = f[0] * (R.vfactor * R.C);                        // :1048
         t.uself = R.self_sum_q2 * (R.C * sqrt(R.kappa * R.kappa / MSPMS_PI));  // :1058
         t.uelec = t.ureal + t.ufourier - t.uself - t.uexcl;           // :1061
+        if (uvac && R.ewald_bc == MSPMS_EWALD_BC_VACUUM) {
+            t.uvacuum = uvac[blockIdx.x];
+            t.uelec += t.uvacuum;                                     // :1125
+        }
         vir += t.uelec;                                               // :1128
     } else if (R.charge_type == MSPMS_ELEC_WOLF) {
         t.uself = R.self_sum_q2 * (R.C * R.wvc3);                     // :1140
         t.uelec = t.ureal - t.uself - t.uexcl;                        // :1142
+    } else if (R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB) {
+        // every molecule pair twice; the Coulomb virial is its energy
+        // (src/elecfrc.c:134-135)
+        t.uelec = 0.5 * uc[0];
+        vir += t.uelec;
     }
     t.usflj = sf[0];
     t.uotherff = 0.0;
@@ -353,10 +367,10 @@ __global__ void __launch_bounds__(256) k_finalize_long(
     out[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(256) k_finalize_short(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(1024) k_finalize_short(const RepDev* __restrict__ reps,
                                                         const double* __restrict__ ep,
                                                         mspms_short_terms* __restrict__ out) {
-    __shared__ double sh[5 * 8];
+    __shared__ double sh[5 * 16];
     const RepDev& R = reps[blockIdx.x];
     double p[5] = {0, 0, 0, 0, 0};
     for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x)
@@ -408,6 +422,187 @@ __global__ void __launch_bounds__(256) k_grad_sites(
     os[o] = p[2];
 }
 
+// Site k of molecule M rebuilt around the molecule's centre of mass wrapped
+// into the primal box: cal_com_and_efg_one + reconstruct_from_com_one
+// (src/misc.c:71-122) with the wrap of src/frclong.c:1084-1087.
+__device__ inline void com_wrapped(const RepDev& R, const MolDev& M, const SpeciesDev& S,
+                                   const SiteDev* sites, const double* x, const double* y,
+                                   const double* z, double com[3], double comw[3]) {
+#pragma clang fp contract(off)
+    com[0] = com[1] = com[2] = 0.0;
+    for (int k = 0; k < M.natom; k++) {
+        const double aw = sites[S.site0 + k].aw;
+        com[0] += x[M.atom0 + k] * aw;
+        com[1] += y[M.atom0 + k] * aw;
+        com[2] += z[M.atom0 + k] * aw;
+    }
+    for (int d = 0; d < 3; d++) {
+        com[d] /= S.mw;
+        comw[d] = com[d] - R.L[d] * rint(com[d] / R.L[d]);
+    }
+}
+
+// Molecular centres of mass (unwrapped), cal_com_and_efg_one order.
+__global__ void __launch_bounds__(256) k_mol_com(int nmol, const MolDev* __restrict__ mols,
+                                                 const SpeciesDev* __restrict__ species,
+                                                 const SiteDev* __restrict__ sites,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z,
+                                                 double4* __restrict__ com) {
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const MolDev M = mols[m];
+    const SpeciesDev S = species[M.species];
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < M.natom; k++) {
+        const double aw = sites[S.site0 + k].aw;
+        c[0] += x[M.atom0 + k] * aw;
+        c[1] += y[M.atom0 + k] * aw;
+        c[2] += z[M.atom0 + k] * aw;
+    }
+    com[m] = make_double4(c[0] / S.mw, c[1] / S.mw, c[2] / S.mw, 0.0);
+}
+
+// Simple Coulomb between molecules (xmole_coulomb_frc, src/elecfrc.c:67-157):
+// molecule pairs whose minimum-image COM distance is inside rcutoffelec
+// interact through all their non-GHOST_FULL site pairs, the site separation
+// built on the image of the COM separation.  One wave per i-molecule, the
+// lanes over the replica's other molecules (both orders of every pair are
+// visited: forces on i only, energy halved in the finalize).  FP64.
+template <int NA>
+__global__ void __launch_bounds__(64) k_coulomb_mol(
+    const RepDev* __restrict__ reps, const MolDev* __restrict__ mols,
+    const SpeciesDev* __restrict__ species, const SiteDev* __restrict__ sites,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double4* __restrict__ com, double* fx, double* fy, double* fz,
+    double* __restrict__ ep) {
+#pragma clang fp contract(off)
+    const int mi = blockIdx.x, lane = threadIdx.x;
+    const MolDev Mi = mols[mi];
+    const RepDev& R = reps[Mi.rep];
+    const SpeciesDev Si = species[Mi.species];
+    const double4 ci = com[mi];
+    double ax[NA], ay[NA], az[NA];
+#pragma unroll
+    for (int k = 0; k < NA; k++) ax[k] = ay[k] = az[k] = 0.0;
+    double eu = 0.0;
+    for (int mj = R.mol0 + lane; mj < R.mol0 + R.nmol; mj += 64) {
+        if (mj == mi) continue;
+        const MolDev Mj = mols[mj];
+        const SpeciesDev Sj = species[Mj.species];
+        const double4 cj = com[mj];
+        double rxcm = ci.x - cj.x, rycm = ci.y - cj.y, rzcm = ci.z - cj.z;
+        rxcm = rxcm - R.L[0] * rint(rxcm / R.L[0]);
+        rycm = rycm - R.L[1] * rint(rycm / R.L[1]);
+        rzcm = rzcm - R.L[2] * rint(rzcm / R.L[2]);
+        const double rcmsq = rxcm * rxcm + rycm * rycm + rzcm * rzcm;
+        if (!(rcmsq < R.rc2_el)) continue;
+#pragma unroll
+        for (int mm = 0; mm < NA; mm++) {
+            if (mm >= Mi.natom) break;
+            const SiteDev& A = sites[Si.site0 + mm];
+            if (A.ghost == MSPMS_GHOST_FULL) continue;
+            const int ii = Mi.atom0 + mm;
+            const double exi = x[ii] - ci.x, eyi = y[ii] - ci.y, ezi = z[ii] - ci.z;
+            for (int nn = 0; nn < Mj.natom; nn++) {
+                const SiteDev& B = sites[Sj.site0 + nn];
+                if (B.ghost == MSPMS_GHOST_FULL) continue;
+                const int jj = Mj.atom0 + nn;
+                const double rxij = exi + rxcm - (x[jj] - cj.x);
+                const double ryij = eyi + rycm - (y[jj] - cj.y);
+                const double rzij = ezi + rzcm - (z[jj] - cj.z);
+                const double rijsq = rxij * rxij + ryij * ryij + rzij * rzij;
+                const double rij = sqrt(rijsq);
+                const double uij = R.C * A.charge * B.charge / rij;
+                eu += uij;
+                const double fij = uij / rijsq;
+                ax[mm] += fij * rxij;
+                ay[mm] += fij * ryij;
+                az[mm] += fij * rzij;
+            }
+        }
+    }
+#pragma unroll
+    for (int mm = 0; mm < NA; mm++) {
+        if (mm >= Mi.natom) break;
+        double gx = ax[mm], gy = ay[mm], gz = az[mm];
+        for (int o = 32; o >= 1; o >>= 1) {
+            gx += __shfl_xor(gx, o, 64);
+            gy += __shfl_xor(gy, o, 64);
+            gz += __shfl_xor(gz, o, 64);
+        }
+        if (lane == 0) {
+            const int ii = Mi.atom0 + mm;
+            fx[ii] += gx;
+            fy[ii] += gy;
+            fz[ii] += gz;
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) eu += __shfl_xor(eu, o, 64);
+    if (lane == 0) ep[mi] = eu;
+}
+
+// Ewald vacuum boundary term (src/frclong.c:1065-1126), one workgroup per
+// replica: M = sum_i q_i r_i over the rebuilt sites, uvacuum = |M|^2 *
+// (2 pi / 3V) C, F_i += -2 (2 pi / 3V) q_i C M.  The adjoint adds
+// d uvacuum / d q_i = 2 (2 pi / 3V) C (M . r_i) to gq.
+__global__ void __launch_bounds__(1024) k_vacuum(
+    const RepDev* __restrict__ reps, const MolDev* __restrict__ mols,
+    const SpeciesDev* __restrict__ species, const SiteDev* __restrict__ sites,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* fx, double* fy, double* fz, double* __restrict__ uvac, double* gq) {
+#pragma clang fp contract(off)
+    __shared__ double sh[3 * 16];
+    __shared__ double dip[3];
+    const RepDev& R = reps[blockIdx.x];
+    if (R.ewald_bc != MSPMS_EWALD_BC_VACUUM) return;  // whole workgroup
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x) {
+        const MolDev M = mols[m];
+        const SpeciesDev S = species[M.species];
+        double com[3], comw[3];
+        com_wrapped(R, M, S, sites, x, y, z, com, comw);
+        for (int k = 0; k < M.natom; k++) {
+            const int i = M.atom0 + k;
+            const double q = sites[S.site0 + k].charge;
+            p[0] = p[0] + q * ((x[i] - com[0]) + comw[0]);
+            p[1] = p[1] + q * ((y[i] - com[1]) + comw[1]);
+            p[2] = p[2] + q * ((z[i] - com[2]) + comw[2]);
+        }
+    }
+    block_sum<3>(p, sh);
+    if (threadIdx.x == 0) {
+        dip[0] = p[0];
+        dip[1] = p[1];
+        dip[2] = p[2];
+        double u = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        uvac[blockIdx.x] = u * (R.twopi_over_3v * R.C);
+    }
+    __syncthreads();
+    const double qrx = dip[0], qry = dip[1], qrz = dip[2];
+    for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x) {
+        const MolDev M = mols[m];
+        const SpeciesDev S = species[M.species];
+        double com[3], comw[3];
+        if (gq) com_wrapped(R, M, S, sites, x, y, z, com, comw);
+        for (int k = 0; k < M.natom; k++) {
+            const int i = M.atom0 + k;
+            const double q = sites[S.site0 + k].charge;
+            const double fij = -2.0 * R.twopi_over_3v * q * R.C;
+            fx[i] += fij * qrx;
+            fy[i] += fij * qry;
+            fz[i] += fij * qrz;
+            if (gq) {
+                const double rx = (x[i] - com[0]) + comw[0], ry = (y[i] - com[1]) + comw[1],
+                             rz = (z[i] - com[2]) + comw[2];
+                gq[i] += 2.0 * R.twopi_over_3v * R.C * (qrx * rx + qry * ry + qrz * rz);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species, const int4* work,
@@ -443,13 +638,36 @@ void launch_bonded(const LaunchCtx& c, const SpeciesDev* species, const TermDev*
 void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* ep_pair,
                           const unsigned long long* ip_pair, int have_pair,
                           const double* ep_intra, int have_intra, const double* uk,
-                          int have_recip, const double* ep_sf, int have_sf, void* terms) {
-    hipLaunchKernelGGL(k_finalize_long, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, col_cl0,
+                          int have_recip, const double* ep_sf, int have_sf,
+                          const double* uvac, const double* ep_coul, void* terms) {
+    hipLaunchKernelGGL(k_finalize_long, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, col_cl0,
                        ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip, ep_sf,
-                       have_sf, (mspms_long_terms*)terms);
+                       have_sf, uvac, ep_coul, (mspms_long_terms*)terms);
+}
+
+int coulomb_mol_supported(int max_mol_atoms) { return max_mol_atoms <= 16; }
+
+void launch_coulomb_mol(const LaunchCtx& c, const SpeciesDev* species, const double* x,
+                        const double* y, const double* z, double4* com, double* fx, double* fy,
+                        double* fz, double* ep) {
+    hipLaunchKernelGGL(k_mol_com, dim3((c.nmol + 255) / 256), dim3(256), 0, c.stream, c.nmol,
+                       c.mols, species, c.sites, x, y, z, com);
+    if (c.max_mol_atoms <= 4)
+        hipLaunchKernelGGL(k_coulomb_mol<4>, dim3(c.nmol), dim3(64), 0, c.stream, c.reps, c.mols,
+                           species, c.sites, x, y, z, com, fx, fy, fz, ep);
+    else
+        hipLaunchKernelGGL(k_coulomb_mol<16>, dim3(c.nmol), dim3(64), 0, c.stream, c.reps,
+                           c.mols, species, c.sites, x, y, z, com, fx, fy, fz, ep);
+}
+
+void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* x,
+                   const double* y, const double* z, double* fx, double* fy, double* fz,
+                   double* uvac, double* gq) {
+    hipLaunchKernelGGL(k_vacuum, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, c.mols, species,
+                       c.sites, x, y, z, fx, fy, fz, uvac, gq);
 }
 
 void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms) {
-    hipLaunchKernelGGL(k_finalize_short, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, epart,
+    hipLaunchKernelGGL(k_finalize_short, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, epart,
                        (mspms_short_terms*)terms);
 }

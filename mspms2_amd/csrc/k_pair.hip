@@ -2,14 +2,17 @@
 // space or Wolf, per cluster pair.  Reference: the atom-pair body of
 // src/frclong.c:105-196 (ljfrc, ewald_real_frc, wolf_real_frc).
 //
-// One wave64 per i-cluster walks its row of the cluster-pair list.  Lane
-// (i = lane>>3, j = lane&7) evaluates one site pair per j-cluster; each i's
-// force is reduced over its 8 lanes with wave shuffles at the end, and the
-// energy / virial partials over the whole wave.  The lists are full (both
-// i->j and j->i), so every atom's force is written once, without atomics,
-// in a fixed order: the result is deterministic.  Energies and virial are
-// summed twice and halved in the finalize kernel; pair counts and hashes are
-// taken on i<j only.
+// A wave64 walks (part of) an i-cluster's row of the cluster-pair list.
+// Lane (i = lane>>3, j = lane&7) evaluates one site pair per j-cluster;
+// each i's force is reduced over its 8 lanes with wave shuffles at the end,
+// and the energy / virial partials over the whole wave.  The lists are full
+// (both i->j and j->i), so every atom's force is written once, without
+// atomics, in a fixed order: the result is deterministic.  Energies and
+// virial are summed twice and halved in the finalize kernel; pair counts and
+// hashes are taken on i<j only.
+//   k_pair     exact mode: one wave per i-cluster
+//   k_pair_mx  mixed mode: WPC waves per i-cluster sharing its row, the
+//              partials combined through LDS in wave order
 //
 // Precision modes:
 //   FP64   separation from the unwrapped FP64 positions with the reference's
@@ -65,6 +68,8 @@ __device__ inline int xcd_block(int b, int nb) {
     return lb;
 }
 
+// Exact mode (FP64, the reference's operation order), one wave per
+// i-cluster; GRAD adds the per-atom parameter derivatives.
 template <int ELEC, bool FP64, bool GRAD>
 __global__ void __launch_bounds__(256) k_pair(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
@@ -75,7 +80,7 @@ __global__ void __launch_bounds__(256) k_pair(
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash,
     double* __restrict__ gq, double* __restrict__ ge, double* __restrict__ gs) {
-    static_assert(!GRAD || FP64, "parameter gradients are FP64 only");
+    static_assert(FP64, "the mixed-precision path is k_pair_mx");
     const int nb = gridDim.x;
     const int lb = xcd_block(blockIdx.x, nb);
     const int ci = lb * 4 + (threadIdx.x >> 6);
@@ -102,7 +107,7 @@ __global__ void __launch_bounds__(256) k_pair(
     const int nrow = plist_n[ci];
     const uint32_t* row = plist + (size_t)ci * cap;
 
-    if (FP64) {
+    {
         const double4 pi4 = slot_x64[islot];
         for (int e = 0; e < nrow; e++) {
             const uint32_t ent = row[e];
@@ -187,111 +192,6 @@ __global__ void __launch_bounds__(256) k_pair(
                 }
             }
         }
-    } else {
-        const float4 pi4 = slot_off[islot];
-        const float sig_i = vi ? si.sigma_f : 0.f;
-        const float se_i = vi ? si.sqrt_eps_f : 0.f;
-        const float q_i = pi4.w;
-        const float rc2lj = (float)R.rc2_lj, rc2el = (float)R.rc2_el;
-        const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
-        const float iLx = (float)R.invL[0], iLy = (float)R.invL[1], iLz = (float)R.invL[2];
-        const float kappa = (float)R.kappa, Cf = (float)R.C, shift1 = (float)R.shift1;
-        const float rcel = (float)R.rc_el, v1 = (float)R.wvc1, v2 = (float)R.wvc2,
-                    f1 = (float)R.wfc1, f2 = (float)R.wfc2;
-        for (int e = 0; e < nrow; e++) {
-            const uint32_t ent = row[e];
-            const int cj = (int)(ent >> 7);
-            const int jslot = cj * MSPMS_CL + jsub;
-            const int2 infoj = slot_info[jslot];
-            if (!(vi && infoj.y >= 0 && infoj.x != infoi.x)) continue;
-            const double4 Cj = cl_center[cj];
-            const float4 pj4 = slot_off[jslot];
-            float rx, ry, rz;
-            if (ent & 64u) {
-                rx = (float)(Ci.x - Cj.x) + (pi4.x - pj4.x);
-                ry = (float)(Ci.y - Cj.y) + (pi4.y - pj4.y);
-                rz = (float)(Ci.z - Cj.z) + (pi4.z - pj4.z);
-                rx -= Lx * rintf(rx * iLx);
-                ry -= Ly * rintf(ry * iLy);
-                rz -= Lz * rintf(rz * iLz);
-            } else {
-                const int sx = (int)(ent & 3u) - 1, sy = (int)((ent >> 2) & 3u) - 1,
-                          sz = (int)((ent >> 4) & 3u) - 1;
-                const float Dx = (float)(Ci.x - (Cj.x + sx * R.L[0]));
-                const float Dy = (float)(Ci.y - (Cj.y + sy * R.L[1]));
-                const float Dz = (float)(Ci.z - (Cj.z + sz * R.L[2]));
-                rx = (pi4.x - pj4.x) + Dx;
-                ry = (pi4.y - pj4.y) + Dy;
-                rz = (pi4.z - pj4.z) + Dz;
-            }
-            const float rsq = rx * rx + ry * ry + rz * rz;
-            const bool in_lj = rsq < rc2lj;
-            const bool in_el = ELEC != 0 && rsq < rc2el;
-            if (!(in_lj || in_el)) continue;
-            const SiteDev sj = sites[infoj.y];
-            const int oj = slot_orig[jslot];
-            const bool count = oi < oj;
-            const float rinv2 = __frcp_rn(rsq);
-            float fsum = 0.f;
-            if (in_lj && lji && sj.ghost != MSPMS_GHOST_LJ) {
-                const float sij = 0.5f * (sig_i + sj.sigma_f);
-                const float eij = se_i * sj.sqrt_eps_f;
-                const float s2 = sij * sij;
-                float s6 = s2 * rinv2;
-                s6 = s6 * s6 * s6;
-                const float s12 = s6 * s6;
-                float u = 4.0f * eij * (s12 - s6);
-                float f = 24.0f * eij * (2.0f * s12 - s6) * rinv2;
-                float ush = 0.f;
-                if (R.switch_on) {
-                    if (rsq >= (float)R.rcon2) {
-                        const float a = rc2lj - rsq;
-                        const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * (float)R.rcon2) /
-                                         (float)R.roff2_ron2_cube;
-                        f = f * sw - u * 12.0f * a * ((float)R.rcon2 - rsq) /
-                                         (float)R.roff2_ron2_cube;
-                        u *= sw;
-                    }
-                } else {
-                    const float sg6 = s2 * s2 * s2;
-                    ush = eij * sg6 * (sg6 * shift1 - 1.0f);
-                }
-                e_lj += (double)u;
-                e_sh += (double)ush;
-                vir += (double)(f * rsq);
-                fsum += f;
-                if (count) {
-                    n_lj++;
-                    if (want_hash) h_lj += pair_key(oi - R.atom0, oj - R.atom0);
-                }
-            }
-            if (in_el) {
-                const float rinv = rsqrtf(rsq);
-                const float r = rsq * rinv;
-                const float x = kappa * r;
-                const float ex = __expf(-x * x);
-                const float ec = ex * erfc_scaled_poly(x);
-                const float cq = Cf * q_i * pj4.w;
-                float u, f;
-                if (ELEC == MSPMS_ELEC_EWALD) {
-                    u = cq * rinv * ec;
-                    f = (u + cq * rinv * MSPMS_TWO_OVER_SQRTPI_F * x * ex) * rinv2;
-                } else {
-                    u = cq * (ec * rinv + v1 + v2 * (r - rcel));
-                    f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
-                    vir += (double)(f * rsq);
-                }
-                e_re += (double)u;
-                fsum += f;
-                if (count) {
-                    n_el++;
-                    if (want_hash) h_el += pair_key(oi - R.atom0, oj - R.atom0);
-                }
-            }
-            afx += (double)(fsum * rx);
-            afy += (double)(fsum * ry);
-            afz += (double)(fsum * rz);
-        }
     }
 
     // per-atom force: 8 j-lanes of each i
@@ -337,6 +237,200 @@ __global__ void __launch_bounds__(256) k_pair(
     }
 }
 
+// Mixed precision, WPC waves per i-cluster: wave w of the workgroup takes
+// row entries w, w + WPC, ...; the WPC partial forces and energies of each
+// i are combined through LDS in wave order (deterministic).  Per entry a
+// lane loads two float4 (offset + charge, sigma / sqrt(eps) / molecule /
+// flags) and the wave one FP64 cluster centre.
+constexpr int WPC = 4;
+
+template <int ELEC>
+__global__ void __launch_bounds__(64 * WPC) k_pair_mx(
+    const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
+    const double4* __restrict__ cl_center, const uint32_t* __restrict__ plist,
+    const int* __restrict__ plist_n, int cap, const int* __restrict__ slot_orig,
+    const float4* __restrict__ slot_off, const float4* __restrict__ slot_par,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash) {
+    __shared__ double sh_f[WPC][MSPMS_CL][3];
+    __shared__ double sh_e[WPC][4];
+    __shared__ unsigned long long sh_n[WPC][4];
+    const int ci = xcd_block(blockIdx.x, gridDim.x);
+    const int ncl = min(*ncl_total, maxcl);
+    if (ci >= ncl) return;  // whole workgroup
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int isub = lane >> 3, jsub = lane & 7;
+
+    const double4 Ci = cl_center[ci];
+    const RepDev& R = reps[(int)Ci.w];
+    const int islot = ci * MSPMS_CL + isub;
+    const float4 pi4 = slot_off[islot];
+    const float4 qi4 = slot_par[islot];
+    const int moli = __float_as_int(qi4.z), fli = __float_as_int(qi4.w);
+    const bool vi = (fli & SLOT_VALID) != 0;
+    const int oi = want_hash && vi ? slot_orig[islot] - R.atom0 : 0;
+    const float q_i = pi4.w;
+    const float rc2lj = (float)R.rc2_lj, rc2el = (float)R.rc2_el;
+    const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
+    const float iLx = (float)R.invL[0], iLy = (float)R.invL[1], iLz = (float)R.invL[2];
+    const float kappa = (float)R.kappa, Cf = (float)R.C, shift1 = (float)R.shift1;
+    const float rcel = (float)R.rc_el, v1 = (float)R.wvc1, v2 = (float)R.wvc2,
+                f1 = (float)R.wfc1, f2 = (float)R.wfc2;
+    const float rcon2 = (float)R.rcon2, sw_den = (float)R.roff2_ron2_cube;
+
+    double afx = 0.0, afy = 0.0, afz = 0.0;
+    double e_lj = 0.0, e_sh = 0.0, e_re = 0.0, vir = 0.0;
+    unsigned long long n_lj = 0, n_el = 0, h_lj = 0, h_el = 0;
+
+    const int nrow = plist_n[ci];
+    const uint32_t* row = plist + (size_t)ci * cap;
+    for (int e = w; e < nrow; e += WPC) {
+        const uint32_t ent = row[e];
+        const int cj = (int)(ent >> 7);
+        const int jslot = cj * MSPMS_CL + jsub;
+        const float4 pj4 = slot_off[jslot];
+        const float4 qj4 = slot_par[jslot];
+        const int flj = __float_as_int(qj4.w);
+        if (!(vi && (flj & SLOT_VALID) && __float_as_int(qj4.z) != moli)) continue;
+        const double4 Cj = cl_center[cj];
+        float rx, ry, rz;
+        if (ent & 64u) {
+            rx = (float)(Ci.x - Cj.x) + (pi4.x - pj4.x);
+            ry = (float)(Ci.y - Cj.y) + (pi4.y - pj4.y);
+            rz = (float)(Ci.z - Cj.z) + (pi4.z - pj4.z);
+            rx -= Lx * rintf(rx * iLx);
+            ry -= Ly * rintf(ry * iLy);
+            rz -= Lz * rintf(rz * iLz);
+        } else {
+            const int sx = (int)(ent & 3u) - 1, sy = (int)((ent >> 2) & 3u) - 1,
+                      sz = (int)((ent >> 4) & 3u) - 1;
+            const float Dx = (float)(Ci.x - (Cj.x + sx * R.L[0]));
+            const float Dy = (float)(Ci.y - (Cj.y + sy * R.L[1]));
+            const float Dz = (float)(Ci.z - (Cj.z + sz * R.L[2]));
+            rx = (pi4.x - pj4.x) + Dx;
+            ry = (pi4.y - pj4.y) + Dy;
+            rz = (pi4.z - pj4.z) + Dz;
+        }
+        const float rsq = rx * rx + ry * ry + rz * rz;
+        const bool in_lj = rsq < rc2lj && (fli & flj & SLOT_LJ);
+        const bool in_el = ELEC != 0 && rsq < rc2el;
+        if (!(in_lj || in_el)) continue;
+        const bool count = islot < jslot;
+        const int oj = want_hash ? slot_orig[jslot] - R.atom0 : 0;
+        const float rinv2 = __frcp_rn(rsq);
+        float fsum = 0.f;
+        if (in_lj) {
+            const float sij = 0.5f * (qi4.x + qj4.x);
+            const float eij = qi4.y * qj4.y;
+            const float s2 = sij * sij;
+            float s6 = s2 * rinv2;
+            s6 = s6 * s6 * s6;
+            const float s12 = s6 * s6;
+            float u = 4.0f * eij * (s12 - s6);
+            float f = 24.0f * eij * (2.0f * s12 - s6) * rinv2;
+            float ush = 0.f;
+            if (R.switch_on) {
+                if (rsq >= rcon2) {
+                    const float a = rc2lj - rsq;
+                    const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * rcon2) / sw_den;
+                    f = f * sw - u * 12.0f * a * (rcon2 - rsq) / sw_den;
+                    u *= sw;
+                }
+            } else {
+                const float sg6 = s2 * s2 * s2;
+                ush = eij * sg6 * (sg6 * shift1 - 1.0f);
+            }
+            e_lj += (double)u;
+            e_sh += (double)ush;
+            vir += (double)(f * rsq);
+            fsum += f;
+            if (count) {
+                n_lj++;
+                if (want_hash) h_lj += pair_key(oi, oj);
+            }
+        }
+        if (in_el) {
+            const float rinv = rsqrtf(rsq);
+            const float r = rsq * rinv;
+            const float x = kappa * r;
+            const float ex = __expf(-x * x);
+            const float ec = ex * erfc_scaled_poly(x);
+            const float cq = Cf * q_i * pj4.w;
+            float u, f;
+            if (ELEC == MSPMS_ELEC_EWALD) {
+                u = cq * rinv * ec;
+                f = (u + cq * rinv * MSPMS_TWO_OVER_SQRTPI_F * x * ex) * rinv2;
+            } else {
+                u = cq * (ec * rinv + v1 + v2 * (r - rcel));
+                f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
+                vir += (double)(f * rsq);
+            }
+            e_re += (double)u;
+            fsum += f;
+            if (count) {
+                n_el++;
+                if (want_hash) h_el += pair_key(oi, oj);
+            }
+        }
+        afx += (double)(fsum * rx);
+        afy += (double)(fsum * ry);
+        afz += (double)(fsum * rz);
+    }
+
+    afx = sum8(afx);
+    afy = sum8(afy);
+    afz = sum8(afz);
+    if (jsub == 0) {
+        sh_f[w][isub][0] = afx;
+        sh_f[w][isub][1] = afy;
+        sh_f[w][isub][2] = afz;
+    }
+    e_lj = wave_sum(e_lj);
+    e_sh = wave_sum(e_sh);
+    e_re = wave_sum(e_re);
+    vir = wave_sum(vir);
+    n_lj = wave_sum_u64(n_lj);
+    n_el = wave_sum_u64(n_el);
+    if (want_hash) {
+        h_lj = wave_sum_u64(h_lj);
+        h_el = wave_sum_u64(h_el);
+    }
+    if (lane == 0) {
+        sh_e[w][0] = e_lj;
+        sh_e[w][1] = e_sh;
+        sh_e[w][2] = e_re;
+        sh_e[w][3] = vir;
+        sh_n[w][0] = n_lj;
+        sh_n[w][1] = n_el;
+        sh_n[w][2] = h_lj;
+        sh_n[w][3] = h_el;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < MSPMS_CL) {
+        const int sl = ci * MSPMS_CL + t;
+        if (__float_as_int(slot_par[sl].w) & SLOT_VALID) {
+            double g[3] = {0.0, 0.0, 0.0};
+            for (int ww = 0; ww < WPC; ww++)
+                for (int d = 0; d < 3; d++) g[d] += sh_f[ww][t][d];
+            const int o = slot_orig[sl];
+            fx[o] = g[0];
+            fy[o] = g[1];
+            fz[o] = g[2];
+        }
+    } else if (t >= 64 && t < 68) {
+        const int k = t - 64;
+        double sum = 0.0;
+        unsigned long long cnt = 0;
+        for (int ww = 0; ww < WPC; ww++) {
+            sum += sh_e[ww][k];
+            cnt += sh_n[ww][k];
+        }
+        epart[(size_t)ci * 4 + k] = sum;
+        ipart[(size_t)ci * 4 + k] = cnt;
+    }
+}
+
 template <int ELEC>
 void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* fx, double* fy,
               double* fz, double* epart, unsigned long long* ipart, int want_hash, double* gq,
@@ -355,10 +449,10 @@ void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* f
                            b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
                            fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
     else
-        hipLaunchKernelGGL((k_pair<ELEC, false, false>), dim3(nb8), dim3(256), 0, c.stream,
-                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
-                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
-                           fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((k_pair_mx<ELEC>), dim3((b.maxcl + 7) / 8 * 8), dim3(64 * WPC), 0,
+                           c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center, b.plist,
+                           b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz,
+                           epart, ipart, want_hash);
 }
 
 }  // namespace

@@ -28,6 +28,7 @@ struct SpeciesDev {
     int term0, nintra;     // frclong intramolecular pair terms
     int bterm0, nbterm;    // frcshort terms, in the reference's order
     int pad[2];
+    double mw;             // molecular mass, sum of aw in site order
 };
 
 #define ST_SINGULAR 8  // improper geometry the reference exit(1)s on

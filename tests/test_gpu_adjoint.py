@@ -15,7 +15,7 @@ import copy
 import numpy as np
 import pytest
 
-from mspms2_amd import configs
+from mspms2_amd import abi, configs
 from mspms2_amd.engine import Engine
 from oracle.oracle import Oracle
 from tests.util import assert_long_close, energy_scale, force_close, system_of
@@ -32,6 +32,8 @@ CASES = {
     "mixture_wolf_switch": lambda: configs.mixture_box(elec="wolf", kappa=0.2, switch=True,
                                                        seed=13),
     "bonded_zoo": lambda: configs.bonded_zoo(),
+    "mixture_water4_vacuum": lambda: configs.mixture_box(water4=True, bc=abi.EWALD_BC_VACUUM,
+                                                         seed=12),
 }
 
 

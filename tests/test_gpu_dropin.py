@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 HARNESS = os.path.join(os.path.dirname(O.__file__), "_ref", "mspms_dropin_harness")
 
 CASES = ["lj864_jitter", "water216_ewald", "water216_wolf", "butane64", "mixture_ewald",
-         "mixture_wolf_switch", "bonded_zoo", "examples_water"]
+         "mixture_wolf_switch", "bonded_zoo", "examples_water", "mixture_water4_vacuum",
+         "water216_coulomb"]
 
 
 def _golden(name):

@@ -41,6 +41,8 @@ CASES = {
                                                      tube_r=15.0, rc=10.0),
     "tube300_grid": "grid",
     "examples_water": "golden",
+    "mixture_water4_vacuum": "golden",
+    "water216_coulomb": "golden",
 }
 
 _cache = {}
