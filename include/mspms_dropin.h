@@ -30,6 +30,20 @@ extern "C" {
 int frclong(void);   /* replaces src/frclong.c:20 */
 int frcshort(void);  /* replaces src/frcshort.c:20 */
 
+/* Term-level functions with the reference's signatures (src/funcs.h:19-30),
+ * evaluated by the engine's term code on the host with the program's
+ * globals (archive member dropin_terms.o: linked only where the program
+ * leaves them undefined, so the reference's own term objects still win). */
+int ljfrc(double rijsq, double sigmaij, double epsilonij, double* uij, double* fij,
+          double* uijshift);                                        /* src/ljfrc.c:21 */
+int ewald_real_frc(double rijsq, double chargeij, double* uij, double* fij); /* src/elecfrc.c:17 */
+int ewald_excl_frc(double rijsq, double chargeij, double* uij, double* fij); /* src/elecfrc.c:35 */
+int wolf_real_frc(double rijsq, double chargeij, double* uij, double* fij);  /* src/elecfrc.c:48 */
+int bndfrc(int iSpecie, int iBond, int iabs, double* uij, double* virial_ij);  /* src/bndfrc.c:17 */
+int aglfrc(int iSpecie, int iAngle, int iabs, double* uij, double* virial_ij); /* src/aglfrc.c:17 */
+int dihfrc(int iSpecie, int iDih, int iabs, double* uij, double* virial_ij);   /* src/dihfrc.c:17 */
+int impfrc(int iSpecie, int iImp, int iabs, double* uij, double* virial_ij);   /* src/impfrc.c:17 */
+
 /* Releases the device context (optional; the OS reclaims it at exit). */
 void mspms_dropin_release(void);
 

@@ -290,6 +290,36 @@ int mspms_frcshort_device(mspms_ctx* ctx, const double* d_xx,
                           double* d_fxs, double* d_fys, double* d_fzs,
                           mspms_short_terms* d_terms, void* stream);
 
+/* Term-level entry points, evaluated on the host by the term code of the GPU
+ * kernels (FP64, the reference's expression order), with the run globals
+ * the reference's term functions read passed as `p`.
+ * mspms_term_pair replaces, with which =
+ *   MSPMS_TERM_LJ          ljfrc(rijsq, sigmaij=a, epsilonij=b, &u, &f, &ushift)
+ *                          (src/ljfrc.c:21-71)
+ *   MSPMS_TERM_EWALD_REAL  ewald_real_frc(rijsq, chargeij=a, &u, &f)
+ *                          (src/elecfrc.c:17-33)
+ *   MSPMS_TERM_EWALD_EXCL  ewald_excl_frc(rijsq, chargeij=a, &u, &f)
+ *                          (src/elecfrc.c:35-46)
+ *   MSPMS_TERM_WOLF_REAL   wolf_real_frc(rijsq, chargeij=a, &u, &f)
+ *                          (src/elecfrc.c:48-60)
+ * mspms_term_bonded replaces bndfrc / aglfrc / dihfrc / impfrc(iSpecie,
+ * index, iabs, &u, &virial) (src/bndfrc.c:17, src/aglfrc.c:17,
+ * src/dihfrc.c:17, src/impfrc.c:17) for the molecule whose sites are at
+ * xx[0..natom) (the reference's xx + iabs), adding its forces into
+ * fxs[0..natom) like the reference does into fxs + iabs.  A singular
+ * improper returns MSPMS_EINVAL where impfrc exits. */
+enum {
+    MSPMS_TERM_LJ = 0, MSPMS_TERM_EWALD_REAL = 1, MSPMS_TERM_EWALD_EXCL = 2,
+    MSPMS_TERM_WOLF_REAL = 3
+};
+enum { MSPMS_TERM_BOND = 0, MSPMS_TERM_ANGLE = 1, MSPMS_TERM_DIH = 2, MSPMS_TERM_IMP = 3 };
+int mspms_term_pair(const mspms_params* p, int which, double rijsq, double a,
+                    double b, double* uij, double* fij, double* ushift);
+int mspms_term_bonded(const mspms_species* sp, int kind, int index,
+                      const double* xx, const double* yy, const double* zz,
+                      double* fxs, double* fys, double* fzs, double* u,
+                      double* vir);
+
 /* Adjoint of the nonbonded energy with respect to the per-site force-field
  * parameters, in FP64 whatever the context precision.  Host buffers; grad
  * arrays hold nreplica * nsite entries.  Also returns the FP64 frclong

@@ -22,7 +22,7 @@ ARCH = os.environ.get("MSPMS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_cluster.hip", "k_pair.hip", "k_intra.hip", "k_recip.hip", "k_integrate.hip",
                "k_sf.hip",
-               "engine.cpp"]
+               "engine.cpp", "terms_host.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
     # packed FP32 instructions are kept out of the generated code
@@ -53,13 +53,16 @@ def build(verbose: bool = False) -> str:
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     # drop-in objects: plain C over the reference's globals
-    dsrc = os.path.join(CSRC, "dropin.c")
-    if os.path.exists(dsrc):
-        dobj = os.path.join(BUILD, "dropin.o")
-        _run(["gcc", "-std=c99", "-O2", "-fPIC", "-fcommon", "-Wall", "-c", dsrc, "-o", dobj])
-        if os.path.exists(DROPIN):
-            os.remove(DROPIN)
-        _run(["ar", "rcs", DROPIN, dobj])
+    # (dropin_terms.c: the term-level functions, an archive member of its own)
+    dobjs = []
+    for name in ("dropin.c", "dropin_terms.c"):
+        dobj = os.path.join(BUILD, os.path.splitext(name)[0] + ".o")
+        _run(["gcc", "-std=c99", "-O2", "-fPIC", "-fcommon", "-Wall", "-c",
+              os.path.join(CSRC, name), "-o", dobj])
+        dobjs.append(dobj)
+    if os.path.exists(DROPIN):
+        os.remove(DROPIN)
+    _run(["ar", "rcs", DROPIN] + dobjs)
     return LIB
 
 

@@ -259,3 +259,7 @@ void mspms_dropin_release(void)
     g_ctx = NULL;
     g_nspecie = g_natom = -1;
 }
+
+/* for the term-level shims of dropin_terms.c */
+void mspms_dropin_read_params(mspms_params* p) { read_params(p); }
+void mspms_dropin_species(mspms_species* sp) { build_species(sp); }

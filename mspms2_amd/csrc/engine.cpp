@@ -96,6 +96,11 @@ std::vector<T> copy_arr(const T* p, size_t n) {
 
 }  // namespace
 
+// the error path of the other translation units (mspms_last_error)
+__attribute__((visibility("hidden"))) int mspms_fail(int code, const char* what) {
+    return fail(code, "%s", what);
+}
+
 // -------------------------------------------------------------------------
 struct mspms_ctx {
     int device = 0, precision = MSPMS_PREC_MIXED;

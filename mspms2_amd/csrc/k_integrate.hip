@@ -54,7 +54,9 @@ __global__ void __launch_bounds__(256) k_respa_inner(
         vir = 0.0;
         for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
             double u = 0.0, v = 0.0;
-            int cls = bonded_term(terms[t], a0, x, y, z, fxs, fys, fzs, &u, &v, &st->flags);
+            int sing = 0;
+            int cls = bonded_term(terms[t], a0, x, y, z, fxs, fys, fzs, &u, &v, &sing);
+            if (sing) atomicOr(&st->flags, ST_SINGULAR);
             e[cls] += u;
             vir += v;
         }
@@ -126,7 +128,9 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
         vir = 0.0;
         for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
             double u = 0.0, v = 0.0;
-            int cls = bonded_term(terms[t], l, lx, ly, lz, lfx, lfy, lfz, &u, &v, &st->flags, 64);
+            int sing = 0;
+            int cls = bonded_term(terms[t], l, lx, ly, lz, lfx, lfy, lfz, &u, &v, &sing, 64);
+            if (sing) atomicOr(&st->flags, ST_SINGULAR);
             e[cls] += u;
             vir += v;
         }

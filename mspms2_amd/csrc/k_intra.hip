@@ -227,7 +227,9 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restri
     for (int t = S.bterm0; t < S.bterm0 + S.nbterm; t++) {
         const TermDev T = terms[t];
         double u = 0.0, v = 0.0;
-        int cls = bonded_term(T, base, x, y, z, fx, fy, fz, &u, &v, &st->flags);
+        int sing = 0;
+        int cls = bonded_term(T, base, x, y, z, fx, fy, fz, &u, &v, &sing);
+        if (sing) atomicOr(&st->flags, ST_SINGULAR);
         e[cls] += u;
         vir += v;
     }

@@ -11,7 +11,7 @@
 
 // ---- FP64 ---------------------------------------------------------------
 // ljfrc (src/ljfrc.c:21-71)
-__device__ inline void lj64(double rijsq, double sigmaij, double epsilonij, int switch_on,
+__host__ __device__ inline void lj64(double rijsq, double sigmaij, double epsilonij, int switch_on,
                             double rcutoffsq, double rcutonsq, double roff2_ron2_cube,
                             double shift1, double* uij, double* fij, double* ushift) {
 #pragma clang fp contract(off)
@@ -49,7 +49,7 @@ __device__ inline void lj64(double rijsq, double sigmaij, double epsilonij, int 
 // Parameter derivatives of the ljfrc energy: du/dsigma_ij and du/deps_ij of
 // u = 4 eps (s^12 - s^6) [* sw(r)], s = sigma / r; the switch factor does
 // not depend on the parameters.
-__device__ inline void lj64_dparam(double rijsq, double sigmaij, double epsilonij, int switch_on,
+__host__ __device__ inline void lj64_dparam(double rijsq, double sigmaij, double epsilonij, int switch_on,
                                    double rcutoffsq, double rcutonsq, double roff2_ron2_cube,
                                    double* du_dsigma, double* du_deps) {
     double sw = 1.0;
@@ -65,13 +65,13 @@ __device__ inline void lj64_dparam(double rijsq, double sigmaij, double epsiloni
 
 // d eps_ij / d eps_i of the mixing rule eps_ij = sqrt(eps_i eps_j): infinite
 // at eps_i = 0 < eps_j, where the square root has no derivative
-__device__ inline double deps_mix(double eij, double eps_j) {
+__host__ __device__ inline double deps_mix(double eij, double eps_j) {
     if (eij > 0.0) return 0.5 * eps_j / eij;
     return eps_j == 0.0 ? 0.0 : __builtin_huge_val();
 }
 
 // ewald_real_frc (src/elecfrc.c:17-33)
-__device__ inline void ewald_real64(double rijsq, double qq, double C, double kappa, double* uij,
+__host__ __device__ inline void ewald_real64(double rijsq, double qq, double C, double kappa, double* uij,
                                     double* fij) {
 #pragma clang fp contract(off)
     double rij = sqrt(rijsq);
@@ -83,7 +83,7 @@ __device__ inline void ewald_real64(double rijsq, double qq, double C, double ka
 }
 
 // ewald_excl_frc (src/elecfrc.c:35-46)
-__device__ inline void ewald_excl64(double rijsq, double qq, double C, double* uij, double* fij) {
+__host__ __device__ inline void ewald_excl64(double rijsq, double qq, double C, double* uij, double* fij) {
 #pragma clang fp contract(off)
     double rij = sqrt(rijsq);
     *uij = C * qq / rij;
@@ -91,7 +91,7 @@ __device__ inline void ewald_excl64(double rijsq, double qq, double C, double* u
 }
 
 // wolf_real_frc (src/elecfrc.c:48-60)
-__device__ inline void wolf_real64(double rijsq, double qq, double C, double kappa, double rc,
+__host__ __device__ inline void wolf_real64(double rijsq, double qq, double C, double kappa, double rc,
                                    double v1, double v2, double f1, double f2, double* uij,
                                    double* fij) {
 #pragma clang fp contract(off)

@@ -33,7 +33,7 @@ struct SpeciesDev {
 
 #define ST_SINGULAR 8  // improper geometry the reference exit(1)s on
 
-__device__ inline void charmm_geom(int i1, int i2, int i3, int i4, const double* xx,
+__host__ __device__ inline void charmm_geom(int i1, int i2, int i3, int i4, const double* xx,
                                    const double* yy, const double* zz, double g[18]) {
 #pragma clang fp contract(off)
     double rxjk = xx[i3] - xx[i2], ryjk = yy[i3] - yy[i2], rzjk = zz[i3] - zz[i2];
@@ -54,7 +54,7 @@ __device__ inline void charmm_geom(int i1, int i2, int i3, int i4, const double*
 
 // the four-atom gradient shared by src/dihfrc.c:423-468 and
 // src/impfrc.c:140-181
-__device__ inline void charmm_forces(int i1, int i2, int i3, int i4, const double* xx,
+__host__ __device__ inline void charmm_forces(int i1, int i2, int i3, int i4, const double* xx,
                                      const double* yy, const double* zz, double* fxs,
                                      double* fys, double* fzs, double temp, const double g[18]) {
 #pragma clang fp contract(off)
@@ -90,9 +90,12 @@ __device__ inline void charmm_forces(int i1, int i2, int i3, int i4, const doubl
 // Returns the energy class (0 bond, 1 angle, 2 dihedral, 3 improper).
 // Atom k of the term sits at index base + idx[k] * stride of the arrays
 // (stride 1: the global SoA arrays; a per-molecule LDS slice otherwise).
-__device__ inline int bonded_term(const TermDev& T, int base, const double* xx, const double* yy,
-                                  const double* zz, double* fxs, double* fys, double* fzs,
-                                  double* u, double* vir, int* flags, int stride = 1) {
+// *singular is set for an improper at the geometry impfrc exits on
+// (src/impfrc.c:96-134); the term then adds nothing.  Host and device.
+__host__ __device__ inline int bonded_term(const TermDev& T, int base, const double* xx,
+                                           const double* yy, const double* zz, double* fxs,
+                                           double* fys, double* fzs, double* u, double* vir,
+                                           int* singular, int stride = 1) {
 #pragma clang fp contract(off)
     const double* pr = T.para;
     if (T.kind == TERM_BOND) {
@@ -279,12 +282,12 @@ __device__ inline int bonded_term(const TermDev& T, int base, const double* xx, 
         *u = uu;
         *vir = 0.0;
         double ang = acos(alpha) * 180 / MSPMS_PI;
-        bool singular = ((fabs(om0) <= 1e-5) && (fabs(ang) <= 1e-20)) ||
+        bool sing = ((fabs(om0) <= 1e-5) && (fabs(ang) <= 1e-20)) ||
                         ((fabs(om0 - 180) <= 1e-5) && (fabs(ang - 180) <= 1e-20)) ||
                         ((fabs(om0) <= 1e-5) && (fabs(ang - 180) <= 1e-20)) ||
                         ((fabs(om0 - 180) <= 1e-5) && (fabs(ang) <= 1e-20));
-        if (singular) {
-            atomicOr(flags, ST_SINGULAR);
+        if (sing) {
+            *singular = 1;
             return 3;
         }
         double temp = -2.0 * kom * (phi - om0) * MSPMS_PI / 180 / sin(phi * MSPMS_PI / 180);

@@ -1,0 +1,87 @@
+// terms_host.cpp -- term-level entry points of the C ABI, evaluated on the
+// host by the same term code the gfx950 kernels run (physics.h, terms.h),
+// with the reference's run globals passed as an mspms_params block:
+//   mspms_term_pair    ljfrc (src/ljfrc.c:21-71), ewald_real_frc,
+//                      ewald_excl_frc, wolf_real_frc (src/elecfrc.c:17-60)
+//   mspms_term_bonded  bndfrc / aglfrc / dihfrc / impfrc (src/bndfrc.c:17-78,
+//                      src/aglfrc.c:17-190, src/dihfrc.c:17-284,
+//                      src/impfrc.c:17-191) on one molecule's coordinates
+// They exist for term-level parity tests and for callers that evaluate
+// single terms; the force evaluation itself runs on the GPU only.
+#include "engine.h"
+#include "physics.h"
+#include "terms.h"
+#include "../../include/mspms_gpu.h"
+
+int mspms_fail(int code, const char* what);
+
+extern "C" int mspms_term_pair(const mspms_params* p, int which, double rijsq, double a,
+                               double b, double* uij, double* fij, double* ushift) {
+    if (!p || !uij || !fij || !ushift) return mspms_fail(MSPMS_EINVAL, "null argument");
+    *ushift = 0.0;
+    switch (which) {
+    case MSPMS_TERM_LJ:
+        lj64(rijsq, a, b, p->isLJswitchOn, p->rcutoffsq, p->rcutonsq, p->roff2_minus_ron2_cube,
+             p->shift1, uij, fij, ushift);
+        return 0;
+    case MSPMS_TERM_EWALD_REAL:
+        ewald_real64(rijsq, a, p->coulomb_prefactor, p->kappa, uij, fij);
+        return 0;
+    case MSPMS_TERM_EWALD_EXCL:
+        ewald_excl64(rijsq, a, p->coulomb_prefactor, uij, fij);
+        return 0;
+    case MSPMS_TERM_WOLF_REAL:
+        wolf_real64(rijsq, a, p->coulomb_prefactor, p->kappa, p->rcutoffelec, p->wolfvcon1,
+                    p->wolfvcon2, p->wolffcon1, p->wolffcon2, uij, fij);
+        return 0;
+    default:
+        return mspms_fail(MSPMS_EINVAL, "unknown pair term");
+    }
+}
+
+extern "C" int mspms_term_bonded(const mspms_species* q, int kind, int k, const double* xx,
+                                 const double* yy, const double* zz, double* fxs, double* fys,
+                                 double* fzs, double* u, double* vir) {
+    if (!q || !xx || !yy || !zz || !fxs || !fys || !fzs || !u || !vir)
+        return mspms_fail(MSPMS_EINVAL, "null argument");
+    TermDev T{};
+    switch (kind) {
+    case MSPMS_TERM_BOND:
+        if (k < 0 || k >= q->nbond) return mspms_fail(MSPMS_EINVAL, "bond index");
+        T.kind = TERM_BOND;
+        T.type = q->bnd_type[k];
+        for (int i = 0; i < 2; i++) T.idx[i] = q->bnd_idx[2 * k + i];
+        for (int i = 0; i < 3; i++) T.para[i] = q->bnd_para[3 * k + i];
+        break;
+    case MSPMS_TERM_ANGLE:
+        if (k < 0 || k >= q->nangle) return mspms_fail(MSPMS_EINVAL, "angle index");
+        T.kind = TERM_ANGLE;
+        T.type = q->agl_type[k];
+        for (int i = 0; i < 3; i++) T.idx[i] = q->agl_idx[3 * k + i];
+        for (int i = 0; i < 5; i++) T.para[i] = q->agl_para[5 * k + i];
+        break;
+    case MSPMS_TERM_DIH:
+        if (k < 0 || k >= q->ndih) return mspms_fail(MSPMS_EINVAL, "dihedral index");
+        T.kind = TERM_DIH;
+        T.type = q->dih_type[k];
+        for (int i = 0; i < 4; i++) T.idx[i] = q->dih_idx[4 * k + i];
+        for (int i = 0; i < 4; i++) T.para[i] = q->dih_para[4 * k + i];
+        break;
+    case MSPMS_TERM_IMP:
+        if (k < 0 || k >= q->nimp) return mspms_fail(MSPMS_EINVAL, "improper index");
+        T.kind = TERM_IMP;
+        T.type = q->imp_type[k];
+        for (int i = 0; i < 4; i++) T.idx[i] = q->imp_idx[4 * k + i];
+        for (int i = 0; i < 2; i++) T.para[i] = q->imp_para[2 * k + i];
+        break;
+    default:
+        return mspms_fail(MSPMS_EINVAL, "unknown bonded term");
+    }
+    int singular = 0;
+    *u = 0.0;
+    *vir = 0.0;
+    bonded_term(T, 0, xx, yy, zz, fxs, fys, fzs, u, vir, &singular);
+    if (singular)
+        return mspms_fail(MSPMS_EINVAL, "improper at a singular geometry (src/impfrc.c:96-134)");
+    return 0;
+}

@@ -25,7 +25,7 @@ EXPORTS = [
     "mspms_natom", "mspms_nreplica", "mspms_frclong_host", "mspms_frcshort_host",
     "mspms_frclong_device", "mspms_frcshort_device", "mspms_energy_param_grad",
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
-    "mspms_vver_step_device", "mspms_vver_run_device",
+    "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
 ]
 
 

@@ -1441,3 +1441,31 @@ void oracle_ranmar_next(oracle_ranmar* r, double* rvec, int len)
         rvec[v] = uni;
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* term-level entry points: the reference's pair terms and bonded terms with  */
+/* the globals passed explicitly (checker for the engine's mspms_term_*)      */
+
+void oracle_term_pair(const mspms_params* p, int which, double rijsq, double a,
+                      double b, double* u, double* f, double* ushift)
+{
+    *ushift = 0.0;
+    switch (which) {
+    case 0: lj_term(p, rijsq, a, b, u, f, ushift); break; /* ljfrc */
+    case 1: ewald_real(p, rijsq, a, u, f); break;         /* ewald_real_frc */
+    case 2: ewald_excl(p, rijsq, a, u, f); break;         /* ewald_excl_frc */
+    default: wolf_real(p, rijsq, a, u, f); break;         /* wolf_real_frc */
+    }
+}
+
+int oracle_term_bonded(const mspms_species* q, int kind, int k, const double* xx,
+                       const double* yy, const double* zz, double* fxs,
+                       double* fys, double* fzs, double* u, double* vir)
+{
+    switch (kind) {
+    case 0: bond_term(q, k, 0, xx, yy, zz, fxs, fys, fzs, u, vir); return 0;
+    case 1: angle_term(q, k, 0, xx, yy, zz, fxs, fys, fzs, u, vir); return 0;
+    case 2: dih_term(q, k, 0, xx, yy, zz, fxs, fys, fzs, u, vir); return 0;
+    default: return imp_term(q, k, 0, xx, yy, zz, fxs, fys, fzs, u, vir);
+    }
+}

@@ -61,6 +61,16 @@ typedef struct oracle_ranmar { double u[98]; double c, cd, cm; int i97, j97; } o
 void oracle_rmarin(oracle_ranmar* r, int ij, int kl);
 void oracle_ranmar_next(oracle_ranmar* r, double* rvec, int len);
 
+/* Term level: which = 0 ljfrc (a sigma_ij, b epsilon_ij), 1 ewald_real_frc,
+ * 2 ewald_excl_frc, 3 wolf_real_frc (a = q_i q_j); kind = 0 bndfrc, 1 aglfrc,
+ * 2 dihfrc, 3 impfrc on the molecule coordinates xx.. (site indices of the
+ * template), forces added into fxs.. */
+void oracle_term_pair(const mspms_params* p, int which, double rijsq, double a,
+                      double b, double* u, double* f, double* ushift);
+int oracle_term_bonded(const mspms_species* q, int kind, int k, const double* xx,
+                       const double* yy, const double* zz, double* fxs,
+                       double* fys, double* fzs, double* u, double* vir);
+
 #ifdef __cplusplus
 }
 #endif
