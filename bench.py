@@ -85,6 +85,8 @@ def run_gpu(system, precision, steps, warmup, profile=True):
         fl.copy_(torch.tensor(f0))
         fs.copy_(torch.tensor(fs0))
     torch.cuda.synchronize()
+    # launch shape (cell size x waves per i-cluster) picked by timing
+    tune = eng.autotune(pos, repeats=5)
     delt = system.delt
     for _ in range(warmup):
         eng.vver_step_device(pos, vel, fl, fs, delt, NSTEP_INNER, lterms, sterms, ukin, stream)
@@ -107,7 +109,8 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     raw = lterms.cpu().numpy().view(np.uint8)
     lt = abi.LongTerms.from_buffer_copy(raw.tobytes()).as_dict()
     out = dict(ms_per_step=ms, wall_ms_per_step=1e3 * wall / steps, groups=groups, terms=lt,
-               finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0)
+               finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0,
+               tune=tune)
     eng.close()
     return out
 
@@ -273,7 +276,10 @@ def main():
         "data": "synthetic (seeded jittered-lattice water, random orientations)",
         "config": {"workload": "water-12288 Ewald RESPA 2 fs / 8 x 0.25 fs", "natom": natom,
                    "kvectors": nk, "rc_A": 10.0, "kappa_per_A": 0.28, "box_A": 49.66,
-                   "precision": "mixed"},
+                   "precision": "mixed",
+                   "launch_shape": {k: mixed["tune"][k] for k in ("cell_atoms",
+                                                                  "waves_per_cluster")}},
+        "autotune_candidates_ms": mixed["tune"]["candidates"],
         "pair_interactions_per_s": pairs_per_s,
         "pairs_in_cutoff": n_el,
         "fp64_ns_per_day": DT_FS * 1e-6 * 86400e3 / fp64["ms_per_step"],

@@ -371,6 +371,29 @@ int mspms_sync(mspms_ctx* ctx);
 /* The context's own HIP stream (hipStream_t). */
 void* mspms_stream(const mspms_ctx* ctx);
 
+/* Launch-shape autotuner.  Times `repeats` frclong evaluations of the
+ * device positions d_xx.. for each candidate shape -- sites per binning
+ * cell (6, 8, 12, 16; this sets the cluster geometry and so the pair-list
+ * length) x waves sharing an i-cluster's pair-list row in the mixed pair
+ * kernel (1, 2, 4, 8) -- and keeps the fastest for the context (until the
+ * next mspms_autotune / mspms_set_launch_shape).  Results are the same for
+ * every shape up to FP64 summation order.  No counterpart in the
+ * reference (a CPU code with nothing to tune). */
+#define MSPMS_TUNE_MAX 16
+typedef struct mspms_tune_result {
+    double cell_atoms;               /* chosen sites per binning cell   */
+    int    waves_per_cluster;        /* chosen waves per i-cluster      */
+    int    ncand;                    /* candidates timed                */
+    double ms;                       /* mean frclong time of the choice */
+    double cand_cell_atoms[MSPMS_TUNE_MAX];
+    int    cand_wpc[MSPMS_TUNE_MAX];
+    double cand_ms[MSPMS_TUNE_MAX];
+} mspms_tune_result;
+int mspms_autotune(mspms_ctx* ctx, const double* d_xx, const double* d_yy,
+                   const double* d_zz, int repeats, mspms_tune_result* out);
+/* Sets the launch shape by hand (cell_atoms in 2..64; waves 1, 2, 4, 8). */
+int mspms_set_launch_shape(mspms_ctx* ctx, double cell_atoms, int waves_per_cluster);
+
 /* Kernel-group timing with HIP events on the evaluation stream: switch on,
  * evaluate, then read the mean milliseconds per call of each group
  * (0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,

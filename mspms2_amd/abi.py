@@ -109,6 +109,23 @@ class ParamGrad(C.Structure):
     _fields_ = [("dE_depsilon", _dp), ("dE_dsigma", _dp), ("dE_dcharge", _dp)]
 
 
+TUNE_MAX = 16
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("cell_atoms", C.c_double), ("waves_per_cluster", C.c_int),
+                ("ncand", C.c_int), ("ms", C.c_double),
+                ("cand_cell_atoms", C.c_double * TUNE_MAX), ("cand_wpc", C.c_int * TUNE_MAX),
+                ("cand_ms", C.c_double * TUNE_MAX)]
+
+    def as_dict(self):
+        n = self.ncand
+        return {"cell_atoms": self.cell_atoms, "waves_per_cluster": self.waves_per_cluster,
+                "ms": self.ms,
+                "candidates": [(self.cand_cell_atoms[k], self.cand_wpc[k], self.cand_ms[k])
+                               for k in range(n)]}
+
+
 def dptr(a):
     """double* of a contiguous float64 numpy array (or NULL for None)."""
     if a is None:

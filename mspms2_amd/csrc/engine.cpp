@@ -117,6 +117,10 @@ struct mspms_ctx {
     bool any_intra = false, any_bonded = false, any_recip = false, any_sf = false;
     bool any_vacuum = false, any_coul = false;
     int want_hash = -1;                  // -1: FP64 only
+    // launch shape, set by mspms_autotune: sites per binning cell and waves
+    // per i-cluster of the mixed pair kernel
+    double cell_atoms = 8.0;
+    int pair_wpc = 4;
 
     // static device tables
     DevBuf<SiteDev> d_sites;
@@ -209,6 +213,7 @@ LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     L.nmol = c->nmol;
     L.nsite = c->nsite;
     L.nrep = c->nrep;
+    L.pair_wpc = c->pair_wpc;
     L.max_mol_atoms = 0;
     for (const SpeciesHost& q : c->sp) L.max_mol_atoms = std::max(L.max_mol_atoms, q.natom);
     L.sites = c->d_sites.p;
@@ -683,10 +688,10 @@ int build(mspms_ctx* c) {
         R.twopi_over_3v = P.twopi_over_3v;
         R.ewald_bc = P.iChargeType == MSPMS_ELEC_EWALD ? P.fEwald_BC : 0;
         if (R.ewald_bc == MSPMS_EWALD_BC_VACUUM) c->any_vacuum = true;
-        // cell grid: cubes holding ~8 real sites; columns along z
+        // cell grid: cubes holding ~cell_atoms real sites; columns along z
         double vol = P.boxlx * P.boxly * P.boxlz;
         double dens = std::max(nreal, 1) / vol;
-        double side = std::cbrt(8.0 / dens);
+        double side = std::cbrt(c->cell_atoms / dens);
         int nc[3];
         for (int d = 0; d < 3; d++) {
             nc[d] = std::max(1, (int)std::floor(R.L[d] / side));
@@ -1467,6 +1472,94 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
                                 hipMemcpyDeviceToDevice, s));
     }
     return 0;
+}
+
+// ---- autotuning -------------------------------------------------------------
+// Times frclong on the given device positions for each launch shape
+// (sites per binning cell x waves per i-cluster of the mixed pair kernel),
+// keeps the fastest.  Every shape gives the same result up to summation
+// order; only the time differs.
+extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, const double* z,
+                              int repeats, mspms_tune_result* out) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!x || !y || !z || repeats < 1) return fail(MSPMS_EINVAL, "bad arguments");
+    HIPC(hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    const double cells[] = {6.0, 8.0, 12.0, 16.0};
+    const int wpcs_mixed[] = {1, 2, 4, 8};
+    const int nw = c->precision == MSPMS_PREC_MIXED ? 4 : 1;
+    mspms_tune_result res{};
+    double best = 1e300;
+    int rc = 0;
+    for (double cell : cells) {
+        c->cell_atoms = cell;
+        if ((rc = build(c))) break;
+        for (int iw = 0; iw < nw && !rc; iw++) {
+            c->pair_wpc = nw > 1 ? wpcs_mixed[iw] : c->pair_wpc;
+            const size_t n = (size_t)c->natom;
+            double *fx = c->d_fl.p, *fy = fx + n, *fz = fy + n;
+            // one untimed call, regrowing the lists until they fit
+            for (int attempt = 0; attempt < 4; attempt++) {
+                hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
+                if ((rc = enqueue_long(c, x, y, z, fx, fy, fz, c->d_long.p, s))) break;
+                if ((rc = read_status(c, s))) break;
+                if (!(c->h_status->flags & ST_OVERFLOW)) break;
+                c->cap_hint = std::max(c->cap_hint * 2, c->h_status->max_list + 64);
+                if ((rc = build(c))) break;
+                fx = c->d_fl.p;
+                fy = fx + n;
+                fz = fy + n;
+            }
+            if (rc) break;
+            HIPC(hipEventRecord(e0, s));
+            for (int r = 0; r < repeats && !rc; r++)
+                rc = enqueue_long(c, x, y, z, fx, fy, fz, c->d_long.p, s);
+            HIPC(hipEventRecord(e1, s));
+            HIPC(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms / repeats;
+            if (res.ncand < MSPMS_TUNE_MAX) {
+                res.cand_cell_atoms[res.ncand] = cell;
+                res.cand_wpc[res.ncand] = c->pair_wpc;
+                res.cand_ms[res.ncand] = t;
+                res.ncand++;
+            }
+            if (t < best) {
+                best = t;
+                res.cell_atoms = cell;
+                res.waves_per_cluster = c->pair_wpc;
+                res.ms = t;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->profiling = prof;
+    if (rc) return rc;
+    c->cell_atoms = res.cell_atoms;
+    c->pair_wpc = res.waves_per_cluster;
+    if ((rc = build(c))) return rc;
+    if (out) *out = res;
+    return 0;
+}
+
+extern "C" int mspms_set_launch_shape(mspms_ctx* c, double cell_atoms, int waves_per_cluster) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    if (!(cell_atoms >= 2.0 && cell_atoms <= 64.0))
+        return fail(MSPMS_EINVAL, "cell_atoms must be within 2..64");
+    if (waves_per_cluster != 1 && waves_per_cluster != 2 && waves_per_cluster != 4 &&
+        waves_per_cluster != 8)
+        return fail(MSPMS_EINVAL, "waves_per_cluster must be 1, 2, 4 or 8");
+    c->cell_atoms = cell_atoms;
+    c->pair_wpc = waves_per_cluster;
+    return c->sp.empty() ? 0 : build(c);
 }
 
 // ---- profiling ------------------------------------------------------------

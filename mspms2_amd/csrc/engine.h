@@ -120,6 +120,7 @@ struct LaunchCtx {
     hipStream_t stream;
     int natom, nmol, nsite, nrep;
     int max_mol_atoms;        // largest molecule (atoms) of any species
+    int pair_wpc;             // waves per i-cluster of the mixed pair kernel
     const SiteDev* sites;
     const MolDev* mols;
     const int* atom_mol;

@@ -242,9 +242,7 @@ __global__ void __launch_bounds__(256) k_pair(
 // i are combined through LDS in wave order (deterministic).  Per entry a
 // lane loads two float4 (offset + charge, sigma / sqrt(eps) / molecule /
 // flags) and the wave one FP64 cluster centre.
-constexpr int WPC = 4;
-
-template <int ELEC>
+template <int ELEC, int WPC>
 __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const double4* __restrict__ cl_center, const uint32_t* __restrict__ plist,
@@ -418,8 +416,8 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             fy[o] = g[1];
             fz[o] = g[2];
         }
-    } else if (t >= 64 && t < 68) {
-        const int k = t - 64;
+    } else if (t >= MSPMS_CL && t < MSPMS_CL + 4) {
+        const int k = t - MSPMS_CL;
         double sum = 0.0;
         unsigned long long cnt = 0;
         for (int ww = 0; ww < WPC; ww++) {
@@ -429,6 +427,15 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
         epart[(size_t)ci * 4 + k] = sum;
         ipart[(size_t)ci * 4 + k] = cnt;
     }
+}
+
+template <int ELEC, int WPC>
+void launch_mx(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
+               double* epart, unsigned long long* ipart, int want_hash) {
+    hipLaunchKernelGGL((k_pair_mx<ELEC, WPC>), dim3((b.maxcl + 7) / 8 * 8), dim3(64 * WPC), 0,
+                       c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center, b.plist, b.plist_n,
+                       b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz, epart, ipart,
+                       want_hash);
 }
 
 template <int ELEC>
@@ -448,11 +455,14 @@ void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* f
                            b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
                            b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
                            fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
+    else if (c.pair_wpc <= 1)
+        launch_mx<ELEC, 1>(c, b, fx, fy, fz, epart, ipart, want_hash);
+    else if (c.pair_wpc == 2)
+        launch_mx<ELEC, 2>(c, b, fx, fy, fz, epart, ipart, want_hash);
+    else if (c.pair_wpc <= 4)
+        launch_mx<ELEC, 4>(c, b, fx, fy, fz, epart, ipart, want_hash);
     else
-        hipLaunchKernelGGL((k_pair_mx<ELEC>), dim3((b.maxcl + 7) / 8 * 8), dim3(64 * WPC), 0,
-                           c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center, b.plist,
-                           b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz,
-                           epart, ipart, want_hash);
+        launch_mx<ELEC, 8>(c, b, fx, fy, fz, epart, ipart, want_hash);
 }
 
 }  // namespace

@@ -26,6 +26,7 @@ EXPORTS = [
     "mspms_frclong_device", "mspms_frcshort_device", "mspms_energy_param_grad",
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
     "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
+    "mspms_autotune", "mspms_set_launch_shape",
 ]
 
 
@@ -71,6 +72,8 @@ def lib():
                                         vp, vp, vp, vp, vp, C.c_int, vp]
     L.mspms_energy_param_grad.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.POINTER(abi.LongTerms),
                                           C.POINTER(abi.ParamGrad)]
+    L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
+    L.mspms_set_launch_shape.argtypes = [vp, C.c_double, C.c_int]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -276,6 +279,20 @@ class Engine:
             p(lterms), p(sterms), p(ukin), None if lhist is None else p(lhist),
             None if shist is None else p(shist), None if ukin_hist is None else p(ukin_hist),
             1 if graph else 0, stream), "mspms_vver_run_device")
+
+    def autotune(self, pos, repeats=3) -> dict:
+        """Picks the fastest launch shape for frclong on the (3, N) device
+        positions `pos`; returns the chosen shape and every candidate's time."""
+        p = self._ptr(pos)
+        n = self.natom
+        r = abi.TuneResult()
+        _check(lib().mspms_autotune(self._h, p, p + 8 * n, p + 16 * n, int(repeats), C.byref(r)),
+               "mspms_autotune")
+        return r.as_dict()
+
+    def set_launch_shape(self, cell_atoms=8.0, waves_per_cluster=4):
+        _check(lib().mspms_set_launch_shape(self._h, float(cell_atoms), int(waves_per_cluster)),
+               "mspms_set_launch_shape")
 
     def sync(self):
         _check(lib().mspms_sync(self._h), "mspms_sync")

@@ -51,7 +51,8 @@ PROBE = r"""
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
   S(mspms_species) S(mspms_params) S(mspms_long_terms) S(mspms_short_terms)
-  S(mspms_sf_grid) S(mspms_param_grad)
+  S(mspms_sf_grid) S(mspms_param_grad) S(mspms_tune_result)
+  O(mspms_tune_result, cand_ms) O(mspms_tune_result, cand_wpc)
   O(mspms_species, nnbp) O(mspms_species, imp_para) O(mspms_species, dih_unique)
   O(mspms_params, kappa) O(mspms_params, coulomb_prefactor) O(mspms_params, wolffcon2)
   O(mspms_params, ntube) O(mspms_params, solid_natom) O(mspms_params, grid)
@@ -73,7 +74,8 @@ def test_ctypes_layout_matches_c():
     got = dict(ln.rsplit(" ", 1) for ln in out.strip().splitlines())
     py = {"mspms_species": abi.Species, "mspms_params": abi.Params,
           "mspms_long_terms": abi.LongTerms, "mspms_short_terms": abi.ShortTerms,
-          "mspms_sf_grid": abi.SFGrid, "mspms_param_grad": abi.ParamGrad}
+          "mspms_sf_grid": abi.SFGrid, "mspms_param_grad": abi.ParamGrad,
+          "mspms_tune_result": abi.TuneResult}
     for k, v in got.items():
         if "." in k:
             t, f = k.split(".")
