@@ -108,11 +108,95 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     groups = eng.timings() if profile else {}
     raw = lterms.cpu().numpy().view(np.uint8)
     lt = abi.LongTerms.from_buffer_copy(raw.tobytes()).as_dict()
+    # the drop-in path (host buffers: positions up, forces down, every call)
+    eng.set_profiling(False)
+    hx, hy, hz = (np.ascontiguousarray(v) for v in pos.cpu().numpy())
+    eng.frclong(hx, hy, hz)
+    t_host = time.perf_counter()
+    for _ in range(5):
+        eng.frclong(hx, hy, hz)
+    host_ms = 1e3 * (time.perf_counter() - t_host) / 5
     out = dict(ms_per_step=ms, wall_ms_per_step=1e3 * wall / steps, groups=groups, terms=lt,
                finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0,
-               tune=tune)
+               tune=tune, host_frclong_ms=host_ms)
     eng.close()
     return out
+
+
+def run_drift(system, precision, steps=1000):
+    """NVE energy conservation: `steps` RESPA outer steps replayed from the
+    captured graph with per-step terms recorded on the device; returns the
+    least-squares slope of H = ukin + uinter + uintra in K per ns per site
+    (reduced energies are K here: epsilon_base = 1) and the rms fluctuation
+    of H about the fit."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from mspms2_amd import abi
+    dev = torch.device("cuda:0")
+    eng = Engine(system, precision=precision)
+    n = system.natom
+    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
+    vel = torch.tensor(maxwell_velocities(system), device=dev)
+    f0, _ = eng.frclong()
+    fs0, _ = eng.frcshort()
+    fl, fs = torch.tensor(f0, device=dev), torch.tensor(fs0, device=dev)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(1, dtype=torch.float64, device=dev)
+    lh = torch.zeros(steps * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    sh = torch.zeros(steps * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    kh = torch.zeros(steps, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    eng.vver_run_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, steps, lt, st, uk, lhist=lh,
+                        shist=sh, ukin_hist=kh, graph=True)
+    torch.cuda.synchronize()
+    eng.sync()
+    L = (abi.LongTerms * steps).from_buffer_copy(lh.cpu().numpy().tobytes())
+    S = (abi.ShortTerms * steps).from_buffer_copy(sh.cpu().numpy().tobytes())
+    H = np.array([L[k].uinter + S[k].uintra for k in range(steps)]) + kh.cpu().numpy()
+    t_ns = DT_FS * 1e-6 * np.arange(1, steps + 1)
+    slope, icpt = np.polyfit(t_ns, H, 1)
+    eng.close()
+    return {"steps": steps, "H_start_K": float(H[0]),
+            "drift_K_per_ns_per_site": float(slope / n),
+            "rms_about_fit_K_per_site": float(np.sqrt(np.mean((H - (slope * t_ns + icpt)) ** 2))
+                                              / n),
+            "finite": bool(np.isfinite(H).all())}
+
+
+PROBE_COPY_MIB = 256
+
+
+def run_probe(system, calls=5):
+    """A short fixed workload for hardware-counter passes: the autotuned
+    frclong of the benchmark system `calls` times (mixed, device buffers)
+    and a plain device-to-device copy of PROBE_COPY_MIB MiB, the reference
+    transfer that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, Engine
+    dev = torch.device("cuda:0")
+    eng = Engine(system, precision="mixed")
+    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
+    f = torch.zeros_like(pos)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    eng.frclong()
+    torch.cuda.synchronize()
+    tune = eng.autotune(pos, repeats=2)
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for _ in range(calls):
+        eng.frclong_device(pos, f, lt, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    eng.sync()
+    n = PROBE_COPY_MIB * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    eng.close()
+    print(json.dumps({"probe": "frclong x %d + copy %d MiB x 3" % (calls, PROBE_COPY_MIB),
+                      "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
 
 
 def make_replicas(nrep=256, seed=2024):
@@ -216,12 +300,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replicas", type=int, default=256,
                     help="boxes in the batched-replica leg (0: skip it)")
+    ap.add_argument("--drift-steps", type=int, default=1000,
+                    help="NVE steps of the energy-drift leg, per precision (0: skip it)")
+    ap.add_argument("--probe", action="store_true",
+                    help="only the short frclong + reference-copy workload for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU per process; --gpus {args.gpus} is not "
               f"supported (replica batches shard across GPUs by replica)", file=sys.stderr)
         sys.exit(2)
     rd, system = make_workload()
+    if args.probe:
+        run_probe(system)
+        return
     mixed = run_gpu(system, "mixed", args.steps, args.warmup)
     fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False)
 
@@ -280,6 +371,7 @@ def main():
                    "launch_shape": {k: mixed["tune"][k] for k in ("cell_atoms",
                                                                   "waves_per_cluster")}},
         "autotune_candidates_ms": mixed["tune"]["candidates"],
+        "dropin_frclong_ms_pcie_inclusive": mixed["host_frclong_ms"],
         "pair_interactions_per_s": pairs_per_s,
         "pairs_in_cutoff": n_el,
         "fp64_ns_per_day": DT_FS * 1e-6 * 86400e3 / fp64["ms_per_step"],
@@ -292,6 +384,9 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
+    if args.drift_steps > 0:
+        line["energy_drift"] = {p: run_drift(system, p, args.drift_steps)
+                                for p in ("mixed", "fp64")}
     if args.replicas > 0:
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
     print(json.dumps(line))

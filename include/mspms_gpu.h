@@ -360,6 +360,32 @@ int mspms_vver_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
                           mspms_short_terms* shist, double* ukin_hist,
                           int use_graph, void* stream);
 
+/* Batched hybrid Monte Carlo, one independent chain per replica: hmc()
+ * (src/hmc.c:23-70) with canonical (MD) moves, md_move (src/dispmd.c:12-103)
+ * and velinit (src/initvars.c:151-232), all on the device.
+ * mspms_hmc_init seeds each replica's RANMAR stream (rmarin(ij[r], kl[r]),
+ * src/random.c:14-57; 0 <= ij <= 31328, 0 <= kl <= 30081) and draws its
+ * initial velocities into d_vel ((3, N) device array); treq[r] is the
+ * reduced temperature and nconstraint[r] the constrained degrees of freedom
+ * (nfree = 3 N_r - nconstraint[r], src/initvars.c:588).
+ * mspms_hmc_run_device runs nmove moves of every chain: the move-type draw,
+ * fresh velocities (all but the first move), forces of the current
+ * positions, nmd RESPA steps of delt (nstep_inner inner steps), then
+ * Metropolis on dH = [d uinter + d uintra + d ukin + d ushift] / treq;
+ * a rejected chain gets its positions back.  use_graph replays one
+ * captured move.  mspms_hmc_stats gives per replica the accepted and
+ * attempted moves and the last dH; mspms_hmc_draw takes n numbers from a
+ * replica's stream (the generator's known-answer test). */
+int mspms_hmc_init(mspms_ctx* ctx, const int* ij, const int* kl, const double* treq,
+                   const int* nconstraint, double* d_vel);
+int mspms_hmc_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
+                         double* fs, double delt, int nstep_inner, int nmd, int nmove,
+                         mspms_long_terms* lterms, mspms_short_terms* sterms,
+                         double* ukin, int use_graph, void* stream);
+int mspms_hmc_stats(mspms_ctx* ctx, long long* accepted, long long* moves,
+                    double* last_dH);
+int mspms_hmc_draw(mspms_ctx* ctx, int replica, int n, double* out);
+
 /* Synchronises the device and reports the status accumulated by the
  * device-buffer evaluations since the previous mspms_sync (list overflow,
  * non-finite input, grid out of range, singular improper), which the

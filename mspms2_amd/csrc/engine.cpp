@@ -186,6 +186,13 @@ struct mspms_ctx {
     // valid for the argument set in gkey until the next build
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool graph_is_hmc = false;           // the cached graph is one HMC move
+
+    // batched HMC chains (mspms_hmc_init)
+    DevBuf<RanmarDev> d_rng;
+    DevBuf<HmcRep> d_hmc;
+    DevBuf<double> d_aw_atom, d_pos_old;
+    bool hmc_ready = false;
     struct GraphKey {
         void *pos, *vel, *fl, *fs, *lt, *st, *uk;
         double delt;
@@ -488,6 +495,10 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_grad.release();
     c->d_grad_out.release();
     c->d_grad_work.release();
+    c->d_rng.release();
+    c->d_hmc.release();
+    c->d_aw_atom.release();
+    c->d_pos_old.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1024,6 +1035,7 @@ extern "C" int mspms_set_system(mspms_ctx* c, int nspecie, const mspms_species* 
     c->nmole.resize(nspecie);
     for (int k = 0; k < nspecie; k++) c->nmole[k] = sp[k].nmole;
     c->params.assign(1, *p);
+    c->hmc_ready = false;  // the chains belong to the previous layout
     return build(c);
 }
 
@@ -1050,6 +1062,7 @@ extern "C" int mspms_set_replicas(mspms_ctx* c, int nrep, const int* nmole,
     c->nrep = nrep;
     c->nmole.assign(nmole, nmole + (size_t)nrep * ns);
     c->params.assign(params, params + nrep);
+    c->hmc_ready = false;
     return build(c);
 }
 
@@ -1428,7 +1441,7 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
     const int nr = c->nrep;
     if (use_graph) {
         mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
-        if (!c->graph_exec || !(k == c->gkey)) {
+        if (!c->graph_exec || !(k == c->gkey) || c->graph_is_hmc) {
             drop_graph(c);
             // the capture stream must not have work in flight from the caller
             HIPC(hipStreamSynchronize(s));
@@ -1452,6 +1465,7 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
                 return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
             }
             c->gkey = k;
+            c->graph_is_hmc = false;
         }
     }
     for (int t = 0; t < nsteps; t++) {
@@ -1471,6 +1485,191 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
             HIPC(hipMemcpyAsync(ukin_hist + (size_t)t * nr, ukin, sizeof(double) * nr,
                                 hipMemcpyDeviceToDevice, s));
     }
+    return 0;
+}
+
+// ---- batched HMC ------------------------------------------------------------
+namespace {
+
+// rmarin (src/random.c:14-57): the 97 lags and the arithmetic sequence of
+// Marsaglia and Zaman's generator from the two seeds (James' formulation)
+void ranmar_seed(RanmarDev& r, int ij, int kl) {
+    int i = (ij / 177) % 177 + 2, j = ij % 177 + 2, k = (kl / 169) % 178 + 1, l = kl % 169;
+    r.u[0] = 0.0;
+    for (int lag = 1; lag <= 97; lag++) {
+        double s = 0.0, t = 0.5;
+        for (int bit = 0; bit < 24; bit++) {
+            const int m = (((i * j) % 179) * k) % 179;
+            i = j;
+            j = k;
+            k = m;
+            l = (53 * l + 1) % 169;
+            if ((l * m) % 64 >= 32) s += t;
+            t *= 0.5;
+        }
+        r.u[lag] = s;
+    }
+    r.c = 362436.0 / 16777216.0;
+    r.cd = 7654321.0 / 16777216.0;
+    r.cm = 16777213.0 / 16777216.0;
+    r.i97 = 97;
+    r.j97 = 33;
+    r.iset = 0;
+    r.pad = 0;
+    r.gset = 0.0;
+}
+
+// one HMC move of every replica (kernels only: capturable)
+int enqueue_hmc_move(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs, double delt,
+                     int nstep_inner, int nmd, mspms_long_terms* lterms,
+                     mspms_short_terms* sterms, double* ukin, hipStream_t s) {
+    const size_t n = (size_t)c->natom;
+    LaunchCtx L = launch_ctx(c, s);
+    launch_hmc_begin(L, c->d_hmc.p, c->d_rng.p, c->d_aw_atom.p, pos, c->d_pos_old.p, vel);
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s);
+    if (rc) return rc;
+    rc = enqueue_short(c, pos, pos + n, pos + 2 * n, fs, fs + n, fs + 2 * n, sterms, s);
+    if (rc) return rc;
+    launch_hmc_mark(L, c->d_hmc.p, lterms, sterms);
+    for (int t = 0; t < nmd; t++)
+        if ((rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s)))
+            return rc;
+    launch_hmc_end(L, c->d_hmc.p, c->d_rng.p, lterms, sterms, ukin, pos, c->d_pos_old.p);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mspms_hmc_init(mspms_ctx* c, const int* ij, const int* kl, const double* treq,
+                              const int* nconstraint, double* d_vel) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!ij || !kl || !treq || !nconstraint || !d_vel) return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    const int nr = c->nrep, ns = (int)c->sp.size();
+    std::vector<RanmarDev> rng(nr);
+    std::vector<HmcRep> hmc(nr);
+    std::vector<double> aw_atom;
+    aw_atom.reserve(c->natom);
+    for (int r = 0; r < nr; r++) {
+        if (ij[r] < 0 || ij[r] > 31328 || kl[r] < 0 || kl[r] > 30081)
+            return fail(MSPMS_EINVAL, "RANMAR seeds out of range (0..31328, 0..30081)");
+        if (!(treq[r] > 0.0)) return fail(MSPMS_EINVAL, "treq must be positive");
+        ranmar_seed(rng[r], ij[r], kl[r]);
+        HmcRep h{};
+        h.treq = treq[r];
+        double mass = 0.0;
+        int natom_r = 0;
+        for (int k = 0; k < ns; k++) {
+            const SpeciesHost& q = c->sp[k];
+            const int nm = c->nmole[(size_t)r * ns + k];
+            double mw = 0.0;
+            for (int a = 0; a < q.natom; a++) mw += q.aw[a];
+            mass += mw * nm;  // src/initvars.c:566-570
+            natom_r += nm * q.natom;
+            for (int m = 0; m < nm; m++)
+                for (int a = 0; a < q.natom; a++) aw_atom.push_back(q.aw[a]);
+        }
+        h.system_mass = mass;
+        h.nfree = 3 * natom_r - nconstraint[r];  // src/initvars.c:588
+        if (h.nfree <= 0) return fail(MSPMS_EINVAL, "no degrees of freedom");
+        hmc[r] = h;
+    }
+    int rc = 0;
+    const hipStream_t s = c->stream;
+    if ((rc = c->d_rng.upload(rng, s)) || (rc = c->d_hmc.upload(hmc, s)) ||
+        (rc = c->d_aw_atom.upload(aw_atom, s)) ||
+        (rc = c->d_pos_old.ensure((size_t)3 * c->natom)))
+        return rc;
+    drop_graph(c);
+    LaunchCtx L = launch_ctx(c, s);
+    launch_hmc_init(L, c->d_hmc.p, c->d_rng.p, c->d_aw_atom.p, d_vel);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s));
+    c->hmc_ready = true;
+    return 0;
+}
+
+extern "C" int mspms_hmc_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                    double* fs, double delt, int nstep_inner, int nmd, int nmove,
+                                    mspms_long_terms* lterms, mspms_short_terms* sterms,
+                                    double* ukin, int use_graph, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first");
+    if (nstep_inner < 1 || nmd < 1 || nmove < 0) return fail(MSPMS_EINVAL, "bad step counts");
+    if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (use_graph) {
+        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt,
+                              nstep_inner * 65536 + nmd};
+        if (!c->graph_exec || !(k == c->gkey) || !c->graph_is_hmc) {
+            drop_graph(c);
+            HIPC(hipStreamSynchronize(s));
+            HIPC(hipStreamSynchronize(c->stream));
+            c->capturing = true;
+            HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_hmc_move(c, pos, vel, fl, fs, delt, nstep_inner, nmd, lterms, sterms,
+                                      ukin, c->stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->capturing = false;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(MSPMS_EIO, "graph capture: %s", hipGetErrorString(e));
+            c->graph = g;
+            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                drop_graph(c);
+                return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
+            }
+            c->gkey = k;
+            c->graph_is_hmc = true;
+        }
+        for (int m = 0; m < nmove; m++) HIPC(hipGraphLaunch(c->graph_exec, s));
+        return 0;
+    }
+    for (int m = 0; m < nmove; m++) {
+        int rc = enqueue_hmc_move(c, pos, vel, fl, fs, delt, nstep_inner, nmd, lterms, sterms,
+                                  ukin, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int mspms_hmc_stats(mspms_ctx* c, long long* accepted, long long* moves,
+                               double* last_dH) {
+    if (!c || !c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first");
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipDeviceSynchronize());
+    std::vector<HmcRep> h(c->nrep);
+    HIPC(hipMemcpy(h.data(), c->d_hmc.p, sizeof(HmcRep) * c->nrep, hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; r++) {
+        if (accepted) accepted[r] = h[r].accepted;
+        if (moves) moves[r] = h[r].moves;
+        if (last_dH) last_dH[r] = h[r].last_dH;
+    }
+    return 0;
+}
+
+extern "C" int mspms_hmc_draw(mspms_ctx* c, int replica, int n, double* out) {
+    if (!c || !c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first");
+    if (replica < 0 || replica >= c->nrep || n < 0 || !out)
+        return fail(MSPMS_EINVAL, "bad arguments");
+    if (n == 0) return 0;
+    HIPC(hipSetDevice(c->device));
+    DevBuf<double> tmp;
+    int rc = tmp.ensure((size_t)n);
+    if (rc) return rc;
+    LaunchCtx L = launch_ctx(c, c->stream);
+    launch_hmc_draw(L, c->d_rng.p, replica, n, tmp.p);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(out, tmp.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    tmp.release();
     return 0;
 }
 

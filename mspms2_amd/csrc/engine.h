@@ -159,6 +159,35 @@ struct SFDev {
     const double* min_pe;    // [ntype] min node F (mylibtasos/grid.f90:102)
 };
 
+// RANMAR stream of one replica (src/random.c:14-109) and the cached second
+// deviate of gaussran (src/random.c:113-138)
+struct RanmarDev {
+    double u[98];
+    double c, cd, cm;
+    int i97, j97;
+    int iset, pad;
+    double gset;
+};
+
+// batched HMC chain state of one replica
+struct HmcRep {
+    double treq, system_mass;   // reduced temperature, sum of the site masses
+    int nfree, first;           // 3N - nconstraint; first move pending
+    double ukin;                // kinetic energy of the current velocities
+    double uinter_old, uintra_old, ukin_old, ushift_old;
+    double last_dH;
+    long long accepted, moves;
+};
+
+void launch_hmc_init(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,
+                     double* vel);
+void launch_hmc_begin(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,
+                      const double* pos, double* pos_old, double* vel);
+void launch_hmc_mark(const LaunchCtx& c, HmcRep* hmc, const void* lterms, const void* sterms);
+void launch_hmc_end(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const void* lterms,
+                    const void* sterms, const double* ukin, double* pos, const double* pos_old);
+void launch_hmc_draw(const LaunchCtx& c, RanmarDev* rng, int rep, int n, double* out);
+
 void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,
                double* fz, double* usf);

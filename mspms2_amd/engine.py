@@ -26,7 +26,8 @@ EXPORTS = [
     "mspms_frclong_device", "mspms_frcshort_device", "mspms_energy_param_grad",
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
     "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
-    "mspms_autotune", "mspms_set_launch_shape",
+    "mspms_autotune", "mspms_set_launch_shape", "mspms_hmc_init", "mspms_hmc_run_device",
+    "mspms_hmc_stats", "mspms_hmc_draw",
 ]
 
 
@@ -72,6 +73,12 @@ def lib():
                                         vp, vp, vp, vp, vp, C.c_int, vp]
     L.mspms_energy_param_grad.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.POINTER(abi.LongTerms),
                                           C.POINTER(abi.ParamGrad)]
+    L.mspms_hmc_init.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), dp,
+                                 C.POINTER(C.c_int), vp]
+    L.mspms_hmc_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       vp, vp, vp, C.c_int, vp]
+    L.mspms_hmc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
+    L.mspms_hmc_draw.argtypes = [vp, C.c_int, C.c_int, dp]
     L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
     L.mspms_set_launch_shape.argtypes = [vp, C.c_double, C.c_int]
     L.mspms_sync.argtypes = [vp]
@@ -279,6 +286,42 @@ class Engine:
             p(lterms), p(sterms), p(ukin), None if lhist is None else p(lhist),
             None if shist is None else p(shist), None if ukin_hist is None else p(ukin_hist),
             1 if graph else 0, stream), "mspms_vver_run_device")
+
+    # ---- batched HMC (one chain per replica) ---------------------------------
+    def hmc_init(self, ij, kl, treq, nconstraint, vel):
+        """Seeds the replicas' RANMAR streams and draws initial velocities
+        into the (3, N) device tensor `vel`."""
+        n = self.nrep
+        ij = np.ascontiguousarray(np.broadcast_to(ij, n), dtype=np.int32)
+        kl = np.ascontiguousarray(np.broadcast_to(kl, n), dtype=np.int32)
+        tq = np.ascontiguousarray(np.broadcast_to(treq, n), dtype=np.float64)
+        nc = np.ascontiguousarray(np.broadcast_to(nconstraint, n), dtype=np.int32)
+        _check(lib().mspms_hmc_init(self._h, abi.iptr(ij), abi.iptr(kl), abi.dptr(tq),
+                                    abi.iptr(nc), self._ptr(vel)), "mspms_hmc_init")
+
+    def hmc_run_device(self, pos, vel, fl, fs, delt, nstep_inner, nmd, nmove, lterms, sterms,
+                       ukin, graph=True, stream=None):
+        p = self._ptr
+        _check(lib().mspms_hmc_run_device(self._h, p(pos), p(vel), p(fl), p(fs), float(delt),
+                                          int(nstep_inner), int(nmd), int(nmove), p(lterms),
+                                          p(sterms), p(ukin), 1 if graph else 0, stream),
+               "mspms_hmc_run_device")
+
+    def hmc_stats(self):
+        n = self.nrep
+        acc = np.zeros(n, dtype=np.int64)
+        mov = np.zeros(n, dtype=np.int64)
+        dh = np.zeros(n)
+        _check(lib().mspms_hmc_stats(self._h, acc.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                     mov.ctypes.data_as(C.POINTER(C.c_longlong)), abi.dptr(dh)),
+               "mspms_hmc_stats")
+        return acc, mov, dh
+
+    def hmc_draw(self, replica, n):
+        out = np.zeros(n)
+        _check(lib().mspms_hmc_draw(self._h, int(replica), int(n), abi.dptr(out)),
+               "mspms_hmc_draw")
+        return out
 
     def autotune(self, pos, repeats=3) -> dict:
         """Picks the fastest launch shape for frclong on the (3, N) device
