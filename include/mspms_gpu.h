@@ -400,8 +400,8 @@ void* mspms_stream(const mspms_ctx* ctx);
 /* Launch-shape autotuner.  Times `repeats` frclong evaluations of the
  * device positions d_xx.. for each candidate shape -- sites per binning
  * cell (6, 8, 12, 16; this sets the cluster geometry and so the pair-list
- * length) x waves sharing an i-cluster's pair-list row in the mixed pair
- * kernel (1, 2, 4, 8) -- and keeps the fastest for the context (until the
+ * length) x waves sharing an i-cluster's pair-list row in the pair kernel
+ * (1, 2, 4, 8) -- and keeps the fastest for the context (until the
  * next mspms_autotune / mspms_set_launch_shape).  Results are the same for
  * every shape up to FP64 summation order.  No counterpart in the
  * reference (a CPU code with nothing to tune). */

@@ -62,7 +62,7 @@ def _min_image_dist2(a, b, L):
 def lj_fcc(ncell: int = 6, rho: float = 0.8442, temp: float = 0.722,
            rc: float = 2.5, jitter: float = 0.0, seed: int = 1,
            ensemble: int = 1, nstep: int = 1000) -> RunDir:
-    """Config 1: N=4*ncell^3 LJ atoms on an FCC lattice (reduced units given
+    """LJ fluid: N=4*ncell^3 atoms on an FCC lattice (reduced units given
     via argon bases sigma 3.405 A, epsilon 119.8 K, mass 1 reduced)."""
     sb, eb = 3.405, 119.8
     aw = 0.039948
@@ -202,8 +202,8 @@ def water_box(n_side: int = 16, L: float = 49.66, seed: int = 2,
               ksqmax: int = 64, rc: float = 10.0, jitter: float = 0.3,
               nstep_inner: int = 8, delt_fs: float = 2.0, min_dist: float = 1.6,
               ensemble: int = 0) -> RunDir:
-    """Config 2 (Ewald, n_side=16, 12,288 sites) and config 4 (Wolf): water on
-    a jittered cubic lattice with uniform random orientations."""
+    """Flexible three-site water on a jittered cubic lattice with uniform
+    random orientations (n_side=16: 12,288 sites; Ewald or Wolf)."""
     rng = np.random.default_rng(seed)
     nmol = n_side ** 3
     a = L / n_side
@@ -369,7 +369,7 @@ def butane_species(nmole: int) -> SpeciesSpec:
 def butane_box(nmol: int = 128, seed: int = 3, rc: float = 14.0,
                min_dist: float = 2.8, delt_fs: float = 1.0,
                nstep_inner: int = 5) -> RunDir:
-    """Config 3 (one replica): n-butane at 0.58 g/cm^3, rc = min(rc, L/2)."""
+    """One box of united-atom n-butane at 0.58 g/cm^3, rc = min(rc, L/2)."""
     rng = np.random.default_rng(seed)
     L = (nmol / BUTANE_DENSITY) ** (1.0 / 3.0)
     rcut = min(rc, 0.5 * L * (1 - 1e-12))
@@ -402,7 +402,7 @@ def methane_tube(natom: int = 2000, length: float = 400.0, box_xy: float = 100.0
                  tube_r: float = 50.0, seed: int = 5, rc: float = 12.0,
                  sf: int = abi.SF_HYPERGEO, r_fill: Optional[float] = None,
                  min_dist: float = 3.0) -> RunDir:
-    """Config 5: LJ methane (3.73 A, 148 K) in a smooth carbon tube of radius
+    """LJ methane (3.73 A, 148 K) in a smooth carbon tube of radius
     tube_r centred on the z axis, axial box `length` (Angstrom)."""
     rng = np.random.default_rng(seed)
     r_fill = tube_r - 4.0 if r_fill is None else r_fill

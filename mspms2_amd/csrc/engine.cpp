@@ -1690,8 +1690,8 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     const double cells[] = {6.0, 8.0, 12.0, 16.0};
-    const int wpcs_mixed[] = {1, 2, 4, 8};
-    const int nw = c->precision == MSPMS_PREC_MIXED ? 4 : 1;
+    const int wpcs[] = {1, 2, 4, 8};
+    const int nw = 4;
     mspms_tune_result res{};
     double best = 1e300;
     int rc = 0;
@@ -1699,7 +1699,7 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
         c->cell_atoms = cell;
         if ((rc = build(c))) break;
         for (int iw = 0; iw < nw && !rc; iw++) {
-            c->pair_wpc = nw > 1 ? wpcs_mixed[iw] : c->pair_wpc;
+            c->pair_wpc = wpcs[iw];
             const size_t n = (size_t)c->natom;
             double *fx = c->d_fl.p, *fy = fx + n, *fz = fy + n;
             // one untimed call, regrowing the lists until they fit

@@ -68,10 +68,11 @@ __device__ inline int xcd_block(int b, int nb) {
     return lb;
 }
 
-// Exact mode (FP64, the reference's operation order), one wave per
-// i-cluster; GRAD adds the per-atom parameter derivatives.
-template <int ELEC, bool FP64, bool GRAD>
-__global__ void __launch_bounds__(256) k_pair(
+// Exact mode (FP64, the reference's operation order), WPC waves per
+// i-cluster combined through LDS in wave order; GRAD adds the per-atom
+// parameter derivatives.
+template <int ELEC, bool FP64, bool GRAD, int WPC>
+__global__ void __launch_bounds__(64 * WPC) k_pair(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const SiteDev* __restrict__ sites, const double4* __restrict__ cl_center,
     const uint32_t* __restrict__ plist, const int* __restrict__ plist_n, int cap,
@@ -81,12 +82,13 @@ __global__ void __launch_bounds__(256) k_pair(
     double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash,
     double* __restrict__ gq, double* __restrict__ ge, double* __restrict__ gs) {
     static_assert(FP64, "the mixed-precision path is k_pair_mx");
-    const int nb = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, nb);
-    const int ci = lb * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    __shared__ double sh_f[WPC][MSPMS_CL][6];
+    __shared__ double sh_e[WPC][4];
+    __shared__ unsigned long long sh_n[WPC][4];
+    const int ci = xcd_block(blockIdx.x, gridDim.x);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ncl = min(*ncl_total, maxcl);
-    if (lb >= nb || ci >= ncl) return;
+    if (ci >= ncl) return;  // whole workgroup
     const int isub = lane >> 3, jsub = lane & 7;
 
     const double4 Ci = cl_center[ci];
@@ -109,7 +111,7 @@ __global__ void __launch_bounds__(256) k_pair(
 
     {
         const double4 pi4 = slot_x64[islot];
-        for (int e = 0; e < nrow; e++) {
+        for (int e = w; e < nrow; e += WPC) {
             const uint32_t ent = row[e];
             const int cj = (int)(ent >> 7);
             const int jslot = cj * MSPMS_CL + jsub;
@@ -194,24 +196,22 @@ __global__ void __launch_bounds__(256) k_pair(
         }
     }
 
-    // per-atom force: 8 j-lanes of each i
+    // per-atom partials: 8 j-lanes of each i, then the WPC waves in order
     afx = sum8(afx);
     afy = sum8(afy);
     afz = sum8(afz);
-    if (jsub == 0 && vi) {
-        fx[oi] = afx;
-        fy[oi] = afy;
-        fz[oi] = afz;
-    }
     if (GRAD) {
         g_q = sum8(g_q);
         g_e = sum8(g_e);
         g_s = sum8(g_s);
-        if (jsub == 0 && vi) {
-            gq[oi] = g_q;
-            ge[oi] = g_e;
-            gs[oi] = g_s;
-        }
+    }
+    if (jsub == 0) {
+        sh_f[w][isub][0] = afx;
+        sh_f[w][isub][1] = afy;
+        sh_f[w][isub][2] = afz;
+        sh_f[w][isub][3] = g_q;
+        sh_f[w][isub][4] = g_e;
+        sh_f[w][isub][5] = g_s;
     }
     e_lj = wave_sum(e_lj);
     e_sh = wave_sum(e_sh);
@@ -224,16 +224,43 @@ __global__ void __launch_bounds__(256) k_pair(
         h_el = wave_sum_u64(h_el);
     }
     if (lane == 0) {
-        double* ep = epart + (size_t)ci * 4;
-        ep[0] = e_lj;
-        ep[1] = e_sh;
-        ep[2] = e_re;
-        ep[3] = vir;
-        unsigned long long* ip = ipart + (size_t)ci * 4;
-        ip[0] = n_lj;
-        ip[1] = n_el;
-        ip[2] = h_lj;
-        ip[3] = h_el;
+        sh_e[w][0] = e_lj;
+        sh_e[w][1] = e_sh;
+        sh_e[w][2] = e_re;
+        sh_e[w][3] = vir;
+        sh_n[w][0] = n_lj;
+        sh_n[w][1] = n_el;
+        sh_n[w][2] = h_lj;
+        sh_n[w][3] = h_el;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < MSPMS_CL) {
+        const int sl = ci * MSPMS_CL + t;
+        if (slot_info[sl].y >= 0) {
+            double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int ww = 0; ww < WPC; ww++)
+                for (int d = 0; d < (GRAD ? 6 : 3); d++) g[d] += sh_f[ww][t][d];
+            const int o = slot_orig[sl];
+            fx[o] = g[0];
+            fy[o] = g[1];
+            fz[o] = g[2];
+            if (GRAD) {
+                gq[o] = g[3];
+                ge[o] = g[4];
+                gs[o] = g[5];
+            }
+        }
+    } else if (t >= MSPMS_CL && t < MSPMS_CL + 4) {
+        const int k = t - MSPMS_CL;
+        double sum = 0.0;
+        unsigned long long cnt = 0;
+        for (int ww = 0; ww < WPC; ww++) {
+            sum += sh_e[ww][k];
+            cnt += sh_n[ww][k];
+        }
+        epart[(size_t)ci * 4 + k] = sum;
+        ipart[(size_t)ci * 4 + k] = cnt;
     }
 }
 
@@ -429,6 +456,18 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     }
 }
 
+// one workgroup per i-cluster; the grid is padded to a multiple of 8 so the
+// XCD remap stays a bijection
+template <int ELEC, bool GRAD, int WPC>
+void launch_64(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
+               double* epart, unsigned long long* ipart, int want_hash, double* gq, double* ge,
+               double* gs) {
+    hipLaunchKernelGGL((k_pair<ELEC, true, GRAD, WPC>), dim3((b.maxcl + 7) / 8 * 8),
+                       dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, c.sites,
+                       b.cl_center, b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_info,
+                       b.slot_x64, b.slot_off, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+}
+
 template <int ELEC, int WPC>
 void launch_mx(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
                double* epart, unsigned long long* ipart, int want_hash) {
@@ -442,19 +481,16 @@ template <int ELEC>
 void launch_t(const LaunchCtx& c, const ClusterBufs& b, int precision, double* fx, double* fy,
               double* fz, double* epart, unsigned long long* ipart, int want_hash, double* gq,
               double* ge, double* gs) {
-    int nb = (b.maxcl + 3) / 4;
-    // pad the grid to a multiple of 8 so the XCD remap stays a bijection
-    int nb8 = (nb + 7) / 8 * 8;
     if (gq)
-        hipLaunchKernelGGL((k_pair<ELEC, true, true>), dim3(nb8), dim3(256), 0, c.stream,
-                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
-                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
-                           fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+        launch_64<ELEC, true, 4>(c, b, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+    else if (precision == MSPMS_PREC_FP64 && c.pair_wpc <= 1)
+        launch_64<ELEC, false, 1>(c, b, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+    else if (precision == MSPMS_PREC_FP64 && c.pair_wpc == 2)
+        launch_64<ELEC, false, 2>(c, b, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
+    else if (precision == MSPMS_PREC_FP64 && c.pair_wpc <= 4)
+        launch_64<ELEC, false, 4>(c, b, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
     else if (precision == MSPMS_PREC_FP64)
-        hipLaunchKernelGGL((k_pair<ELEC, true, false>), dim3(nb8), dim3(256), 0, c.stream,
-                           b.ncl_total, b.maxcl, c.reps, c.sites, b.cl_center, b.plist,
-                           b.plist_n, b.cap, b.slot_orig, b.slot_info, b.slot_x64, b.slot_off,
-                           fx, fy, fz, epart, ipart, want_hash, nullptr, nullptr, nullptr);
+        launch_64<ELEC, false, 8>(c, b, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
     else if (c.pair_wpc <= 1)
         launch_mx<ELEC, 1>(c, b, fx, fy, fz, epart, ipart, want_hash);
     else if (c.pair_wpc == 2)

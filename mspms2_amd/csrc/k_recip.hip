@@ -90,7 +90,36 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
         for (int t = 0; t < NKZ; t++) sr[t] = si[t] = 0.0;
         const int ax = K.kx < 0 ? -K.kx : K.kx, ay = K.ky < 0 ? -K.ky : K.ky;
         const int alo = K.kz_lo < 0 ? -K.kz_lo : K.kz_lo;
-        for (int a = 0; a < na; a++) {
+        int a = 0;
+        // four atoms at a time: four independent kz recurrences in flight
+        for (; a + 4 <= na; a += 4) {
+            cplx c4[4], s4[4];
+            double q4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double2* ta = tab + (a + u) * ntab;
+                const double2 tx = ta[ax], ty = ta[kmx + 1 + ay], tz = ta[kmx + kmy + 2 + alo],
+                              t1 = ta[kmx + kmy + 2 + 1];
+                const cplx cx = {tx.x, K.kx < 0 ? -tx.y : tx.y};
+                const cplx cy = {ty.x, K.ky < 0 ? -ty.y : ty.y};
+                const cplx cz = {tz.x, K.kz_lo < 0 ? -tz.y : tz.y};
+                s4[u] = {t1.x, t1.y};
+                c4[u] = cmul(cmul(cx, cy), cz);
+                q4[u] = qsh[a + u];
+            }
+#pragma unroll
+            for (int t = 0; t < NKZ; t++) {
+                if (t < nk) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        sr[t] += q4[u] * c4[u].re;
+                        si[t] += q4[u] * c4[u].im;
+                        c4[u] = cmul(c4[u], s4[u]);
+                    }
+                }
+            }
+        }
+        for (; a < na; a++) {
             const double2* ta = tab + a * ntab;
             double2 tx = ta[ax], ty = ta[kmx + 1 + ay], tz = ta[kmx + kmy + 2 + alo],
                     t1 = ta[kmx + kmy + 2 + 1];
@@ -150,10 +179,13 @@ __global__ void __launch_bounds__(256) k_sred(int nk_total, const int* __restric
     ak[k] = make_double2(s * sr, s * si);
 }
 
-// work: (rep, first atom, first row, end row); 64 atoms per wave.
+// work: (rep, first atom, first row, end row); 64 atoms per wave.  Each
+// lane keeps exp(i n 2pi z/L), n < NZ, in registers: the kz terms of a row
+// are then independent products cxy * z^|kz| (conjugated for kz < 0)
+// instead of a serial recurrence along kz.
 // GRAD: also dU_fourier/dq_i = Vf C sum_k kvec 2 (S_r cos + S_i sin)
 //     = sum_k (A_k cos + B_k sin), added into gq.
-template <bool GRAD>
+template <bool GRAD, int NZ>
 __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                                                const RepDev* __restrict__ reps,
                                                const KRow* __restrict__ rows,
@@ -179,7 +211,10 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
         sincos(R.twopi_l[2] * z[ai], &s, &c);
         e[2] = {c, s};
     }
-    const cplx ezc = conjc(e[2]);
+    cplx zp[NZ];
+    zp[0] = {1.0, 0.0};
+#pragma unroll
+    for (int n = 1; n < NZ; n++) zp[n] = cmul(zp[n - 1], e[2]);
     double gx = 0.0, gy = 0.0, gz = 0.0, dq = 0.0;
     int pkx = 1 << 30, pky = 1 << 30;
     cplx cx = {1.0, 0.0}, cy = {1.0, 0.0};
@@ -204,25 +239,22 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
             gsum += g;
             if (GRAD) dq += ab.x * cxy.re + ab.y * cxy.im;
         }
-        cplx c = cxy;
-        for (int kz = 1; kz <= K.kz_hi; kz++) {
-            c = cmul(c, e[2]);
-            if (kz >= K.kz_lo) {
-                const double2 ab = A[kz];
+#pragma unroll
+        for (int n = 1; n < NZ; n++) {
+            if (n >= K.kz_lo && n <= K.kz_hi) {  // kz = +n (uniform branch)
+                const cplx c = cmul(cxy, zp[n]);
+                const double2 ab = A[n];
                 const double g = ab.x * c.im - ab.y * c.re;
                 gsum += g;
-                gzr += g * kz;
+                gzr += g * n;
                 if (GRAD) dq += ab.x * c.re + ab.y * c.im;
             }
-        }
-        c = cxy;
-        for (int kz = -1; kz >= K.kz_lo; kz--) {
-            c = cmul(c, ezc);
-            if (kz <= K.kz_hi) {
-                const double2 ab = A[kz];
+            if (-n >= K.kz_lo && -n <= K.kz_hi) {  // kz = -n
+                const cplx c = cmul(cxy, conjc(zp[n]));
+                const double2 ab = A[-n];
                 const double g = ab.x * c.im - ab.y * c.re;
                 gsum += g;
-                gzr += g * kz;
+                gzr -= g * n;
                 if (GRAD) dq += ab.x * c.re + ab.y * c.im;
             }
         }
@@ -263,11 +295,24 @@ void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x, co
 
 void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
                     const double* z, double* fx, double* fy, double* fz, double* gq) {
-    if (gq)
-        hipLaunchKernelGGL(k_kforce<true>, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,
-                           c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz, gq);
-    else
-        hipLaunchKernelGGL(k_kforce<false>, dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,
-                           c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz,
-                           nullptr);
+    // z powers kept per lane: |kz| <= kmaxz, nkz_max = 2 kmaxz + 1
+#define KFORCE(G, NZ)                                                                          \
+    hipLaunchKernelGGL((k_kforce<G, NZ>), dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,   \
+                       c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz, gq)
+    if (gq) {
+        if (rb.nkz_max <= 9)
+            KFORCE(true, 5);
+        else if (rb.nkz_max <= 17)
+            KFORCE(true, 9);
+        else
+            KFORCE(true, 17);
+    } else {
+        if (rb.nkz_max <= 9)
+            KFORCE(false, 5);
+        else if (rb.nkz_max <= 17)
+            KFORCE(false, 9);
+        else
+            KFORCE(false, 17);
+    }
+#undef KFORCE
 }

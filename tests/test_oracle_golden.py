@@ -96,7 +96,7 @@ def test_positions_reduced_like_the_reference():
 
 @pytest.mark.slow
 def test_water12288_regenerates_and_matches():
-    """Config 2 (12,288-site Ewald water): the seeded generator reproduces
+    """12,288-site Ewald water: the seeded generator reproduces
     the run files the fixture was made from (SHA-256), and the oracle
     matches the reference's energies and sampled forces bit for bit."""
     name = "water12288_ewald"
