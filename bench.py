@@ -219,6 +219,62 @@ def run_probe(system, calls=5):
                       "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_water12288.json")
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """Per-kernel HBM bytes per launch from two rocprofv3 --pmc passes over
+    `bench.py --probe` (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each
+    scaled so that the reference copy of PROBE_COPY_MIB MiB reads and writes
+    exactly its size.  Kernels are keyed by name (the launches of the chosen
+    launch shape: the last `calls` dispatches of each name)."""
+    import csv
+    import re
+
+    def load(path, counter):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = re.sub(r"\(anonymous namespace\)::", "", r["Kernel_Name"])
+            name = re.sub(r"\(.*", "", name).replace("void ", "").strip()
+            per.setdefault(name, []).append(float(r["Counter_Value"]) * 1024.0)
+        return per
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy = "__amd_rocclr_copyBuffer"
+    size = PROBE_COPY_MIB * float(1 << 20)
+    big = lambda v: [x for x in v if x > 0.5 * size]
+    f_scale = size / np.mean(big(fetch[copy])[-3:])
+    w_scale = size / np.mean(big(write[copy])[-3:])
+    out = {"fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for name in sorted(set(fetch) & set(write)):
+        if name == copy:
+            continue
+        f, w = fetch[name][-5:], write[name][-5:]
+        out["kernels"][name] = {"fetch_bytes": float(np.mean(f)) * f_scale,
+                                "write_bytes": float(np.mean(w)) * w_scale}
+    return out
+
+
+def traffic_of(kernels):
+    """HBM bytes per launch summed over `kernels` (name prefixes) from the
+    committed counter summary, or None where it is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    total, hit = 0.0, 0
+    for pre in kernels:
+        for name, v in t.items():
+            if name.startswith(pre):
+                total += v["fetch_bytes"] + v["write_bytes"]
+                hit += 1
+                break
+    return total if hit == len(kernels) else None
+
+
 def make_replicas(nrep=256, seed=2024):
     """Batched-replica workload: n-butane boxes of U{64..512} molecules at
     0.58 g/cm^3, rc = min(14 A, L/2), one seed per box."""
@@ -324,11 +380,16 @@ def main():
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
     ap.add_argument("--probe", action="store_true",
                     help="only the short frclong + reference-copy workload for counter runs")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the per-kernel HBM traffic of two --pmc passes over --probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU per process; --gpus {args.gpus} is not "
               f"supported (replica batches shard across GPUs by replica)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
     rd, system = make_workload()
     if args.probe:
         run_probe(system)
@@ -352,15 +413,22 @@ def main():
     # FLOP-eq per i<j pair in range; reciprocal: 29 FP64 FLOP-eq per (k, atom))
     pair_flop = 20.0 * n_el + 24.0 * n_lj + 25.0 * n_el
     recip_flop = 29.0 * nk * natom
+    wpc = mixed["tune"]["waves_per_cluster"]
     cand = {
-        "k_pair": (g.get("pair", 0.0), pair_flop, PEAK_FP32_TFLOPS),
-        "k_sk+k_kforce": (g.get("recip_s", 0.0) + g.get("recip_f", 0.0), recip_flop,
-                          PEAK_FP64_TFLOPS),
+        "k_pair": (g.get("pair", 0.0), pair_flop, PEAK_FP32_TFLOPS,
+                   ["k_pair_mx<1, %d>" % wpc]),
+        "k_sk+k_sred+k_kforce": (g.get("recip_s", 0.0) + g.get("recip_f", 0.0), recip_flop,
+                                 PEAK_FP64_TFLOPS, ["k_sk<", "k_sred", "k_kforce<"]),
     }
-    name, (kms, flop, peak) = max(cand.items(), key=lambda kv: kv[1][0])
+    name, (kms, flop, peak, knames) = max(cand.items(), key=lambda kv: kv[1][0])
     achieved = flop / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    # HBM bytes per launch of the same kernels from the committed counter
+    # summary (two rocprofv3 --pmc passes over `bench.py --probe`)
+    traffic = traffic_of(knames)
     roofline = {"bound": "valu", "kernel": name, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                "traffic_unit": "bytes per launch", "flop_per_launch": flop,
+                "hbm_GBps_at_traffic": (traffic / (kms * 1e-3) / 1e9) if traffic else None,
                 "kernel_ms": kms, "groups_ms": g}
 
     cpu = None

@@ -106,6 +106,8 @@ struct mspms_ctx {
     int device = 0, precision = MSPMS_PREC_MIXED;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side = nullptr;          // concurrent branch of frclong
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     std::vector<SpeciesHost> sp;
     int nrep = 0;
@@ -168,6 +170,7 @@ struct mspms_ctx {
 
     // host-API staging and integrator scratch
     DevBuf<double> d_pos, d_fl, d_fs, d_kin;
+    DevBuf<double> d_frec;               // (3, N) reciprocal forces (side stream)
     // parameter adjoint: per-atom derivatives, per-(replica, site) sums
     DevBuf<double> d_grad, d_grad_out;
     DevBuf<int4> d_grad_work;
@@ -422,6 +425,14 @@ extern "C" int mspms_create(mspms_ctx** out, int device, int precision) {
         return fail(MSPMS_EIO, "hipStreamCreate failed");
     }
     c->own_stream = true;
+    // side stream (default priority) for the reciprocal S(k) pass, which
+    // overlaps the clustering and the pair kernel; joined by events
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        mspms_destroy(c);
+        return fail(MSPMS_EIO, "hipStreamCreate / hipEventCreate failed");
+    }
     if (c->d_status.ensure(1) ||
         hipHostMalloc((void**)&c->h_status, sizeof(Status), hipHostMallocDefault) != hipSuccess) {
         delete c;
@@ -492,6 +503,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_fl.release();
     c->d_fs.release();
     c->d_kin.release();
+    c->d_frec.release();
     c->d_grad.release();
     c->d_grad_out.release();
     c->d_grad_work.release();
@@ -500,6 +512,10 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_aw_atom.release();
     c->d_pos_old.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -974,6 +990,7 @@ int build(mspms_ctx* c) {
         (rc = c->d_ep_short.ensure((size_t)c->nmol * 5)) ||
         (rc = c->d_ep_sf.ensure((size_t)c->natom)) || (rc = c->d_long.ensure(c->nrep)) ||
         (rc = c->d_short.ensure(c->nrep)) || (rc = c->d_pos.ensure((size_t)3 * c->natom)) ||
+        (rc = c->d_frec.ensure((size_t)3 * c->natom)) ||
         (rc = c->d_fl.ensure((size_t)3 * c->natom)) || (rc = c->d_fs.ensure((size_t)3 * c->natom)) ||
         (rc = c->d_kin.ensure((size_t)c->natom)))
         return rc;
@@ -1125,7 +1142,31 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     // the status word is sticky: the host API clears it before each call,
     // device-API runs accumulate it until mspms_sync reads it
     LaunchCtx L = launch_ctx(c, s);
-    const int hash =c->want_hash < 0 ? (c->precision == MSPMS_PREC_FP64) : c->want_hash;
+    const int hash = c->want_hash < 0 ? (c->precision == MSPMS_PREC_FP64) : c->want_hash;
+    // fork: the Ewald S(k) pass only reads positions, so it runs on the side
+    // stream beside the clustering and the pair kernel; the force pass,
+    // which adds into the forces the pair kernel writes, waits for it
+    // Without the adjoint the force pass runs there too, into its own (3, N)
+    // buffer that joins the long forces in k_intra (or k_add_frec); the
+    // adjoint keeps it on the main stream (its charge derivative adds into
+    // the buffer the pair kernel writes).
+    const bool recip_side = c->any_recip && !grad;
+    double* frec = recip_side ? c->d_frec.p : nullptr;
+    if (c->any_recip) {
+        HIPC(hipEventRecord(c->ev_fork, s));
+        HIPC(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        LaunchCtx Ls = launch_ctx(c, c->side);
+        {
+            ProfScope p(c, c->side, PROF_RECIP_S);
+            launch_recip_s(Ls, c->rb, x, y, z);
+        }
+        if (recip_side) {
+            ProfScope p(c, c->side, PROF_RECIP_F);
+            launch_zero(Ls, frec, 3 * n);
+            launch_recip_f(Ls, c->rb, x, y, z, frec, frec + n, frec + 2 * n, nullptr);
+        }
+        HIPC(hipEventRecord(c->ev_join, c->side));
+    }
     {
         ProfScope p(c, s, PROF_CLUSTER);
         launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, c->precision);
@@ -1135,18 +1176,19 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         launch_pair(L, c->cb, grad ? MSPMS_PREC_FP64 : c->precision, c->charge_union, fx, fy,
                     fz, c->d_ep_pair.p, c->d_ip_pair.p, grad ? 1 : hash, gq, ge, gs);
     }
+    if (c->any_recip) HIPC(hipStreamWaitEvent(s, c->ev_join, 0));  // join
     if (c->any_intra) {
         ProfScope p(c, s, PROF_INTRA);
         launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p, gq,
-                     ge, gs);
+                     ge, gs, frec);
+    } else if (recip_side) {
+        launch_add_frec(L, fx, fy, fz, frec);
     }
     if (c->any_recip) {
-        {
-            ProfScope p(c, s, PROF_RECIP_S);
-            launch_recip_s(L, c->rb, x, y, z);
+        if (!recip_side) {
+            ProfScope p(c, s, PROF_RECIP_F);
+            launch_recip_f(L, c->rb, x, y, z, fx, fy, fz, gq);
         }
-        ProfScope p(c, s, PROF_RECIP_F);
-        launch_recip_f(L, c->rb, x, y, z, fx, fy, fz, gq);
         if (c->any_vacuum)
             launch_vacuum(L, c->d_species.p, x, y, z, fx, fy, fz, c->d_uvac.p, gq);
     }
@@ -1362,6 +1404,9 @@ extern "C" int mspms_energy_param_grad(mspms_ctx* c, const double* xx, const dou
     HIPC(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
     for (int attempt = 0; attempt < 4; attempt++) {
         hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, c->d_status.p);
+        // the pair kernel writes the entries of binned sites only; GHOST_FULL
+        // sites (never binned) start from zero for the kernels that add
+        HIPC(hipMemsetAsync(c->d_grad.p, 0, 3 * n * sizeof(double), s));
         rc = enqueue_long(c, dx, dx + n, dx + 2 * n, dfx, dfy, dfz, c->d_long.p, s, c->d_grad.p);
         if (rc) return rc;
         if ((rc = read_status(c, s))) return rc;

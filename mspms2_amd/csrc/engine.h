@@ -206,7 +206,11 @@ void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const TermDev* terms, const double* x, const double* y,
                   const double* z, double* fx, double* fy, double* fz,
                   double* epart, double* gq = nullptr, double* ge = nullptr,
-                  double* gs = nullptr);
+                  double* gs = nullptr, const double* frec = nullptr);
+// fx += frec ((3, N) reciprocal-space forces) and a capturable zero fill
+void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
+                     const double* frec);
+void launch_zero(const LaunchCtx& c, double* p, size_t n);
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
                    const TermDev* terms, const double* x, const double* y,
                    const double* z, double* fx, double* fy, double* fz,

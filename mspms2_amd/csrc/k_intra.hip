@@ -142,7 +142,8 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, double* fx,
                                                double* fy, double* fz, double* epart,
-                                               GradOut G) {
+                                               GradOut G, int natom_total,
+                                               const double* __restrict__ frec) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
@@ -151,6 +152,16 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
     IntraAcc acc = {0.0, 0.0, 0.0, 0.0};
     bool same = true;  // isSameMole carried into the 1-4 section
     const int base = M.atom0;
+    if (frec) {
+        // the reciprocal-space forces, computed into their own (3, N) buffer
+        // beside the pair kernel, join the long forces here
+        const size_t n = (size_t)natom_total;
+        for (int k = 0; k < M.natom; k++) {
+            fx[base + k] += frec[base + k];
+            fy[base + k] += frec[n + base + k];
+            fz[base + k] += frec[2 * n + base + k];
+        }
+    }
     for (int t = S.term0; t < S.term0 + S.nintra; t++) {
         const TermDev T = terms[t];
         const int mm = T.idx[0], nn = T.idx[1];
@@ -617,16 +628,45 @@ void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species, const int4
 
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
                   const double* x, const double* y, const double* z, double* fx, double* fy,
-                  double* fz, double* epart, double* gq, double* ge, double* gs) {
+                  double* fz, double* epart, double* gq, double* ge, double* gs,
+                  const double* frec) {
     int g = (c.nmol + 255) / 256;
     if (gq)
         hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps,
                            c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{gq, ge, gs});
+                           GradOut{gq, ge, gs}, c.natom, frec);
     else
         hipLaunchKernelGGL(k_intra<false>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
                            c.reps, c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{nullptr, nullptr, nullptr});
+                           GradOut{nullptr, nullptr, nullptr}, c.natom, frec);
+}
+
+// fx += frec (3, N): the reciprocal forces where no k_intra runs
+__global__ void __launch_bounds__(256) k_add_frec(int n, double* __restrict__ fx,
+                                                  double* __restrict__ fy,
+                                                  double* __restrict__ fz,
+                                                  const double* __restrict__ frec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fx[i] += frec[i];
+    fy[i] += frec[(size_t)n + i];
+    fz[i] += frec[2 * (size_t)n + i];
+}
+
+__global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
+                     const double* frec) {
+    hipLaunchKernelGGL(k_add_frec, dim3((c.natom + 255) / 256), dim3(256), 0, c.stream, c.natom,
+                       fx, fy, fz, frec);
+}
+
+// zero fill by a kernel (capturable; no memset nodes in the step graph)
+void launch_zero(const LaunchCtx& c, double* p, size_t n) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n, p);
 }
 
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
