@@ -301,15 +301,31 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const float kappa = (float)R.kappa, Cf = (float)R.C, shift1 = (float)R.shift1;
     const float rcel = (float)R.rc_el, v1 = (float)R.wvc1, v2 = (float)R.wvc2,
                 f1 = (float)R.wfc1, f2 = (float)R.wfc2;
-    const float rcon2 = (float)R.rcon2, sw_den = (float)R.roff2_ron2_cube;
+    const float rcon2 = (float)R.rcon2, inv_den = (float)(1.0 / R.roff2_ron2_cube);
 
+    // FP64 accumulators, fed from FP32 partials that cover at most FLUSH row
+    // entries of this wave (FLUSH pair terms per lane)
     double afx = 0.0, afy = 0.0, afz = 0.0;
     double e_lj = 0.0, e_sh = 0.0, e_re = 0.0, vir = 0.0;
-    unsigned long long n_lj = 0, n_el = 0, h_lj = 0, h_el = 0;
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe_lj = 0.f, pe_sh = 0.f, pe_re = 0.f, pvir = 0.f;
+    unsigned int c_lj = 0, c_el = 0;
+    unsigned long long h_lj = 0, h_el = 0;
+    constexpr int FLUSH = 8;
+    auto flush = [&]() {
+        afx += (double)pfx;
+        afy += (double)pfy;
+        afz += (double)pfz;
+        e_lj += (double)pe_lj;
+        e_sh += (double)pe_sh;
+        e_re += (double)pe_re;
+        vir += (double)pvir;
+        pfx = pfy = pfz = pe_lj = pe_sh = pe_re = pvir = 0.f;
+    };
 
     const int nrow = plist_n[ci];
     const uint32_t* row = plist + (size_t)ci * cap;
-    for (int e = w; e < nrow; e += WPC) {
+    for (int e = w, it = 0; e < nrow; e += WPC, ++it) {
+        if (it > 0 && it % FLUSH == 0) flush();  // uniform over the wave
         const uint32_t ent = row[e];
         const int cj = (int)(ent >> 7);
         const int jslot = cj * MSPMS_CL + jsub;
@@ -342,7 +358,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
         if (!(in_lj || in_el)) continue;
         const bool count = islot < jslot;
         const int oj = want_hash ? slot_orig[jslot] - R.atom0 : 0;
-        const float rinv2 = __frcp_rn(rsq);
+        const float rinv2 = __builtin_amdgcn_rcpf(rsq);  // v_rcp_f32 (~1 ulp)
         float fsum = 0.f;
         if (in_lj) {
             const float sij = 0.5f * (qi4.x + qj4.x);
@@ -357,20 +373,20 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             if (R.switch_on) {
                 if (rsq >= rcon2) {
                     const float a = rc2lj - rsq;
-                    const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * rcon2) / sw_den;
-                    f = f * sw - u * 12.0f * a * (rcon2 - rsq) / sw_den;
+                    const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * rcon2) * inv_den;
+                    f = f * sw - u * 12.0f * a * (rcon2 - rsq) * inv_den;
                     u *= sw;
                 }
             } else {
                 const float sg6 = s2 * s2 * s2;
                 ush = eij * sg6 * (sg6 * shift1 - 1.0f);
             }
-            e_lj += (double)u;
-            e_sh += (double)ush;
-            vir += (double)(f * rsq);
+            pe_lj += u;
+            pe_sh += ush;
+            pvir += f * rsq;
             fsum += f;
             if (count) {
-                n_lj++;
+                c_lj++;
                 if (want_hash) h_lj += pair_key(oi, oj);
             }
         }
@@ -388,19 +404,21 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             } else {
                 u = cq * (ec * rinv + v1 + v2 * (r - rcel));
                 f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
-                vir += (double)(f * rsq);
+                pvir += f * rsq;
             }
-            e_re += (double)u;
+            pe_re += u;
             fsum += f;
             if (count) {
-                n_el++;
+                c_el++;
                 if (want_hash) h_el += pair_key(oi, oj);
             }
         }
-        afx += (double)(fsum * rx);
-        afy += (double)(fsum * ry);
-        afz += (double)(fsum * rz);
+        pfx += fsum * rx;
+        pfy += fsum * ry;
+        pfz += fsum * rz;
     }
+    flush();
+    unsigned long long n_lj = c_lj, n_el = c_el;
 
     afx = sum8(afx);
     afy = sum8(afy);
