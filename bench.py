@@ -61,7 +61,7 @@ def maxwell_velocities(system, seed=7):
     return v - p
 
 
-def run_gpu(system, precision, steps, warmup, profile=True):
+def run_gpu(system, precision, steps, warmup, profile=True, skin=0.0):
     import torch
     from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
     from mspms2_amd import abi
@@ -92,6 +92,9 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     torch.cuda.synchronize()
     # launch shape (cell size x waves per i-cluster) picked by timing
     tune = eng.autotune(pos, repeats=5)
+    # pair-list buffer (reduced units): lists reused until a site has moved
+    # skin / 2 since the build
+    eng.set_list_skin(skin)
     delt = system.delt
     for _ in range(warmup):
         eng.vver_step_device(pos, vel, fl, fs, delt, NSTEP_INNER, lterms, sterms, ukin, stream)
@@ -117,6 +120,7 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     eng.vver_run_device(pos, vel, fl, fs, delt, NSTEP_INNER, 2, lterms, sterms, ukin, graph=True,
                         stream=stream)
     torch.cuda.synchronize()
+    builds0 = eng.list_builds()
     t_wall = time.perf_counter()
     start.record(st)
     eng.vver_run_device(pos, vel, fl, fs, delt, NSTEP_INNER, steps, lterms, sterms, ukin,
@@ -126,6 +130,7 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     wall = time.perf_counter() - t_wall
     eng.sync()
     ms = start.elapsed_time(stop) / steps
+    builds = eng.list_builds() - builds0
     raw = lterms.cpu().numpy().view(np.uint8)
     lt = abi.LongTerms.from_buffer_copy(raw.tobytes()).as_dict()
     # the drop-in path (host buffers: positions up, forces down, every call)
@@ -138,12 +143,13 @@ def run_gpu(system, precision, steps, warmup, profile=True):
     host_ms = 1e3 * (time.perf_counter() - t_host) / 5
     out = dict(ms_per_step=ms, wall_ms_per_step=1e3 * wall / steps, groups=groups, terms=lt,
                finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0,
-               tune=tune, host_frclong_ms=host_ms, eager_ms_per_step=ms_eager)
+               tune=tune, host_frclong_ms=host_ms, eager_ms_per_step=ms_eager,
+               list_builds_per_step=builds / steps)
     eng.close()
     return out
 
 
-def run_drift(system, precision, steps=1000):
+def run_drift(system, precision, steps=1000, skin=0.0):
     """NVE energy conservation: `steps` RESPA outer steps replayed from the
     captured graph with per-step terms recorded on the device; returns the
     least-squares slope of H = ukin + uinter + uintra in K per ns per site
@@ -154,6 +160,7 @@ def run_drift(system, precision, steps=1000):
     from mspms2_amd import abi
     dev = torch.device("cuda:0")
     eng = Engine(system, precision=precision)
+    eng.set_list_skin(skin)
     n = system.natom
     pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
     vel = torch.tensor(maxwell_velocities(system), device=dev)
@@ -374,6 +381,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--fp64-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skin", type=float, default=1.0,
+                    help="pair-list buffer in Angstrom (lists reused until a site moves half of it)")
     ap.add_argument("--replicas", type=int, default=256,
                     help="boxes in the batched-replica leg (0: skip it)")
     ap.add_argument("--drift-steps", type=int, default=1000,
@@ -394,8 +403,10 @@ def main():
     if args.probe:
         run_probe(system)
         return
-    mixed = run_gpu(system, "mixed", args.steps, args.warmup)
-    fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False)
+    skin = args.skin / system.params["sigma_base"]
+    mixed = run_gpu(system, "mixed", args.steps, args.warmup, skin=skin)
+    fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False,
+                   skin=skin)
 
     ms = mixed["ms_per_step"]
     ns_day = DT_FS * 1e-6 * 86400e3 / ms
@@ -458,7 +469,9 @@ def main():
                    "kvectors": nk, "rc_A": 10.0, "kappa_per_A": 0.28, "box_A": 49.66,
                    "precision": "mixed",
                    "launch_shape": {k: mixed["tune"][k] for k in ("cell_atoms",
-                                                                  "waves_per_cluster")}},
+                                                                  "waves_per_cluster")},
+                   "list_skin_A": args.skin},
+        "list_builds_per_step": mixed["list_builds_per_step"],
         "autotune_candidates_ms": mixed["tune"]["candidates"],
         "dropin_frclong_ms_pcie_inclusive": mixed["host_frclong_ms"],
         "pair_interactions_per_s": pairs_per_s,
@@ -474,7 +487,7 @@ def main():
     if cpu:
         line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
     if args.drift_steps > 0:
-        line["energy_drift"] = {p: run_drift(system, p, args.drift_steps)
+        line["energy_drift"] = {p: run_drift(system, p, args.drift_steps, skin)
                                 for p in ("mixed", "fp64")}
     if args.replicas > 0:
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))

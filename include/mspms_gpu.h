@@ -420,6 +420,21 @@ int mspms_autotune(mspms_ctx* ctx, const double* d_xx, const double* d_yy,
 /* Sets the launch shape by hand (cell_atoms in 2..64; waves 1, 2, 4, 8). */
 int mspms_set_launch_shape(mspms_ctx* ctx, double cell_atoms, int waves_per_cluster);
 
+/* Pair-list buffer.  The reference tests every molecule pair on every call
+ * (src/frclong.c:52-125); this engine finds candidate pairs through a
+ * cluster-pair list built with radius max(rcutoff, rcutoffelec) + skin and
+ * reuses it while every site stays within skin / 2 of its position at the
+ * list's build, checked on the device at each frclong (no host round
+ * trip; a reuse only refreshes the sites' positions).  Which pairs
+ * interact is decided per pair with the cutoff test either way, so the
+ * pair sets and results do not depend on the skin; a larger skin trades
+ * longer pair-list rows for fewer builds.  skin in 0..8 (length units of
+ * the positions, default 0: a list is reused only for unchanged
+ * positions).  mspms_list_builds reports the number of builds since the
+ * last change of system, parameters, launch shape or skin. */
+int mspms_set_list_skin(mspms_ctx* ctx, double skin);
+int mspms_list_builds(mspms_ctx* ctx, long long* builds);
+
 /* Kernel-group timing with HIP events on the evaluation stream: switch on,
  * evaluate, then read the mean milliseconds per call of each group
  * (0 clustering + pair list, 1 nonbonded pairs, 2 intramolecular,

@@ -192,8 +192,14 @@ static void ensure_context(void)
         const char* prec = getenv("MSPMS_GPU_PRECISION");
         const char* dev = getenv("MSPMS_GPU_DEVICE");
         int precision = (prec && !strcmp(prec, "fp64")) ? MSPMS_PREC_FP64 : MSPMS_PREC_MIXED;
+        const char* skin = getenv("MSPMS_GPU_SKIN");  /* Angstrom */
         rc = mspms_create(&g_ctx, dev ? atoi(dev) : 0, precision);
         if (rc) die("mspms_create", rc);
+        if (skin) {
+            /* positions are in units of sigma_base */
+            rc = mspms_set_list_skin(g_ctx, atof(skin) / p.sigma_base);
+            if (rc) die("mspms_set_list_skin", rc);
+        }
     }
     if (!same) {
         build_species(sp);

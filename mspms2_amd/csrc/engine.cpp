@@ -123,6 +123,11 @@ struct mspms_ctx {
     // per i-cluster of the mixed pair kernel
     double cell_atoms = 8.0;
     int pair_wpc = 4;
+    // pair-list buffer (mspms_set_list_skin): lists are built with cutoff +
+    // skin and reused while every atom stays within skin / 2 of its
+    // position at the build; force_rebuild: rebuild on every call
+    double skin = 0.0;
+    bool force_rebuild = false;
 
     // static device tables
     DevBuf<SiteDev> d_sites;
@@ -139,7 +144,8 @@ struct mspms_ctx {
         d_col_cell0, d_col_rep, d_col_cl0, d_cl_col, d_ncl_total, d_slot_orig, d_plist_n;
     DevBuf<double> d_atom_wz;
     DevBuf<int2> d_slot_info;
-    DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center;
+    DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center, d_atom_x0;
+    DevBuf<int> d_lstate;
     DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext;
     DevBuf<uint32_t> d_plist;
     int cap_hint = 0;
@@ -473,6 +479,8 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_slot_info.release();
     c->d_slot_x64.release();
     c->d_slot_w64.release();
+    c->d_atom_x0.release();
+    c->d_lstate.release();
     c->d_cl_center.release();
     c->d_slot_off.release();
     c->d_slot_par.release();
@@ -690,7 +698,7 @@ int build(mspms_ctx* c) {
         R.rc2_el = P.rcutoffelecsq;
         double rmax = P.iChargeType == MSPMS_ELEC_NONE ? P.rcutoff
                                                        : std::max(P.rcutoff, P.rcutoffelec);
-        R.rc_list = rmax * (1.0 + 1e-9) + 1e-9;
+        R.rc_list = rmax * (1.0 + 1e-9) + 1e-9 + c->skin;
         R.rcon2 = P.rcutonsq;
         R.roff2_ron2_cube = P.roff2_minus_ron2_cube;
         R.shift1 = P.shift1;
@@ -856,13 +864,15 @@ int build(mspms_ctx* c) {
         (rc = c->d_ncl_total.ensure(1)) || (rc = c->d_slot_orig.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_info.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_x64.ensure(maxcl * MSPMS_CL)) ||
-        (rc = c->d_slot_w64.ensure(maxcl * MSPMS_CL)) ||
+        (rc = c->d_slot_w64.ensure(maxcl * MSPMS_CL)) || (rc = c->d_atom_x0.ensure(c->natom)) ||
+        (rc = c->d_lstate.ensure(LS_N)) ||
         (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_par.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
         (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
         (rc = c->d_plist_n.ensure(maxcl)))
         return rc;
     HIPC(hipMemsetAsync(c->d_cell_count.p, 0, sizeof(int) * (size_t)ncell, s));
+    HIPC(hipMemsetAsync(c->d_lstate.p, 0, sizeof(int) * LS_N, s));  // no list yet
     b.atom_cell = c->d_atom_cell.p;
     b.atom_slot = c->d_atom_slot.p;
     b.atom_wz = c->d_atom_wz.p;
@@ -878,6 +888,8 @@ int build(mspms_ctx* c) {
     b.slot_info = c->d_slot_info.p;
     b.slot_x64 = c->d_slot_x64.p;
     b.slot_w64 = c->d_slot_w64.p;
+    b.atom_x0 = c->d_atom_x0.p;
+    b.lstate = c->d_lstate.p;
     b.slot_off = c->d_slot_off.p;
     b.slot_par = c->d_slot_par.p;
     b.cl_center = c->d_cl_center.p;
@@ -1169,7 +1181,8 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     }
     {
         ProfScope p(c, s, PROF_CLUSTER);
-        launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, c->precision);
+        launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, 0.25 * c->skin * c->skin,
+                             c->force_rebuild ? 1 : 0);
     }
     {
         ProfScope p(c, s, PROF_PAIR);
@@ -1206,7 +1219,7 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
                              c->d_ep_intra.p, c->any_intra ? 1 : 0, c->rb.uk,
                              c->any_recip ? 1 : 0, c->d_ep_sf.p, c->any_sf ? 1 : 0,
                              c->any_vacuum ? c->d_uvac.p : nullptr,
-                             c->any_coul ? c->d_ep_coul.p : nullptr, terms);
+                             c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate);
     }
     HIPC(hipGetLastError());
     return 0;
@@ -1731,6 +1744,8 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
     const hipStream_t s = c->stream;
     const bool prof = c->profiling;
     c->profiling = false;
+    // every timed call builds its list, as a call after a move would
+    c->force_rebuild = true;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
@@ -1786,6 +1801,7 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->profiling = prof;
+    c->force_rebuild = false;
     if (rc) return rc;
     c->cell_atoms = res.cell_atoms;
     c->pair_wpc = res.waves_per_cluster;
@@ -1804,6 +1820,25 @@ extern "C" int mspms_set_launch_shape(mspms_ctx* c, double cell_atoms, int waves
     c->cell_atoms = cell_atoms;
     c->pair_wpc = waves_per_cluster;
     return c->sp.empty() ? 0 : build(c);
+}
+
+extern "C" int mspms_set_list_skin(mspms_ctx* c, double skin) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    if (!(skin >= 0.0 && skin <= 8.0)) return fail(MSPMS_EINVAL, "skin must be within 0..8");
+    c->skin = skin;
+    return c->sp.empty() ? 0 : build(c);
+}
+
+extern "C" int mspms_list_builds(mspms_ctx* c, long long* builds) {
+    if (!c || !builds) return fail(MSPMS_EINVAL, "null argument");
+    *builds = 0;
+    if (c->sp.empty() || !c->cb.lstate) return 0;
+    HIPC(hipSetDevice(c->device));
+    int st[LS_N];
+    HIPC(hipMemcpyAsync(st, c->cb.lstate, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *builds = (long long)(unsigned)st[LS_BUILDS];
+    return 0;
 }
 
 // ---- profiling ------------------------------------------------------------

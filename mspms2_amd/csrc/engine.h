@@ -88,8 +88,11 @@ struct ClusterBufs {
     int* ncl_total;      // [1]
     int* slot_orig;      // [8*maxcl]
     int2* slot_info;     // [8*maxcl] (mol, site)
-    double4* slot_x64;   // [8*maxcl] FP64 unwrapped (x,y,z,-)
-    double4* slot_w64;   // [8*maxcl] FP64 wrapped  (x,y,z,-)
+    double4* slot_x64;   // [8*maxcl] FP64 unwrapped (x,y,z,-), current call
+    double4* slot_w64;   // [8*maxcl] FP64 wrapped at the last build (x,y,z,charge)
+    double4* atom_x0;    // [natom] unwrapped position at the last build
+    int* lstate;         // [4] list state: LS_VALID, LS_NEED (rebuild wanted
+                         //   by this call), LS_BUILDS (builds so far)
     float4* slot_off;    // [8*maxcl] FP32 offset from centre + charge
     float4* slot_par;    // [8*maxcl] FP32 sigma, sqrt(eps) (0: no LJ),
                          //   molecule id and flags as int bits (SLOT_*)
@@ -99,6 +102,7 @@ struct ClusterBufs {
     int* plist_n;        // [maxcl]
     int maxcl, cap, ncell, ncol;
 };
+enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_N = 4 };
 
 struct Status {
     int flags;           // bit 0: non-finite position, bit 1: list overflow,
@@ -192,9 +196,13 @@ void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,
                double* fz, double* usf);
 
+// Buffered (Verlet) cluster list: the list is built with rc_list = cutoff +
+// skin and reused while every atom stays within skin/2 of its position at
+// the build (force: rebuild regardless); reused calls only refresh the slot
+// positions.  A rebuild request made here is cleared by k_finalize_long.
 void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x,
                           const double* y, const double* z, double* fx,
-                          double* fy, double* fz, int precision);
+                          double* fy, double* fz, double half_skin2, int force);
 // gq/ge/gs: per-atom derivatives of the energy with respect to the atom's
 // own charge, epsilon and sigma (parameter adjoint, FP64); null: none
 void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision,
@@ -235,7 +243,7 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_intra, int have_intra,
                           const double* uk, int have_recip,
                           const double* ep_sf, int have_sf, const double* uvac,
-                          const double* ep_coul, void* terms);
+                          const double* ep_coul, void* terms, int* lstate);
 // simple Coulomb between molecules (iChargeType 3): COMs into com[nmol],
 // per-molecule energy partials into ep[nmol]; molecules of <= 16 sites
 int coulomb_mol_supported(int max_mol_atoms);

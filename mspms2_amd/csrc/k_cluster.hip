@@ -6,6 +6,10 @@
 // with the reference's cutoff test (k_pair.hip).
 //
 // Pipeline (one launch each, all replicas at once):
+//   k_list_check  per atom: displacement since the last build against half
+//              the skin -> rebuild request; per slot: refresh of the slot
+//              positions for a reused list (overwritten when it rebuilds)
+// and, only when a rebuild is requested (the others return at once):
 //   k_bin      per atom: wrap into the box, integer cell (cx,cy,cz), count
 //   k_scan     one workgroup: exclusive scans over cells and columns
 //              (cells of a column are contiguous, ordered by cz)
@@ -16,10 +20,60 @@
 //   k_plist    one wave per i-cluster: j-clusters whose boxes come within
 //              rc_list, compacted with a wave ballot (deterministic order)
 // The ranks make the final order independent of atomic arrival order.
+//
+// Reuse: a list built with rc_list = cutoff + skin holds every pair within
+// the cutoff while no atom has moved more than skin/2 since the build (the
+// pair's separation at the build was then below rc_list, so the boxes of the
+// build met within rc_list, and the image recorded for the cluster pair is
+// the pair's minimum image: the column window is narrower than the box, and
+// the z image is marked ambiguous when Ei.z + Ej.z + rc_list >= Lz/2).  A
+// reused slot keeps its build-time wrapped position and cluster centre; its
+// offset becomes (wrapped - centre) + (x - x_build), continuous when the
+// atom crosses a box face.
 #include "engine.h"
 #include "../../include/mspms_gpu.h"
 
 namespace {
+
+__global__ void __launch_bounds__(256) k_list_check(
+    int natom, int nslot, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ atom_site,
+    const SiteDev* __restrict__ sites, double* __restrict__ fx, double* __restrict__ fy,
+    double* __restrict__ fz, const double4* __restrict__ atom_x0,
+    const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ slot_orig,
+    const double4* __restrict__ slot_w64, const double4* __restrict__ cl_center,
+    double4* __restrict__ slot_x64, float4* __restrict__ slot_off, int* lstate,
+    double half_skin2, int force) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = lstate[LS_VALID] != 0 && !force;
+    bool need = !valid;
+    if (t < natom) {
+        if (sites[atom_site[t]].ghost == MSPMS_GHOST_FULL) {
+            // not binned: its force starts from zero (src/frclong.c:94,107)
+            fx[t] = 0.0;
+            fy[t] = 0.0;
+            fz[t] = 0.0;
+        } else if (valid) {
+            const double4 p0 = atom_x0[t];
+            const double dx = x[t] - p0.x, dy = y[t] - p0.y, dz = z[t] - p0.z;
+            need = !(dx * dx + dy * dy + dz * dz <= half_skin2);  // NaN: rebuild
+        }
+    }
+    if (valid && t < nslot && t < MSPMS_CL * min(*ncl_total, maxcl)) {
+        const int o = slot_orig[t];
+        if (o >= 0) {
+            const double4 p0 = atom_x0[o];
+            const double xi = x[o], yi = y[o], zi = z[o];
+            const double4 w = slot_w64[t];
+            const double4 C = cl_center[t / MSPMS_CL];
+            slot_x64[t] = make_double4(xi, yi, zi, 0.0);
+            slot_off[t] = make_float4((float)((w.x - C.x) + (xi - p0.x)),
+                                      (float)((w.y - C.y) + (yi - p0.y)),
+                                      (float)((w.z - C.z) + (zi - p0.z)), (float)w.w);
+        }
+    }
+    if (__ballot(need) && (threadIdx.x & 63) == 0) atomicOr(&lstate[LS_NEED], 1);
+}
 
 __device__ inline double wrap(double x, double L, double invL) {
     double w = x - L * floor(x * invL);
@@ -37,7 +91,8 @@ __global__ void k_bin(int natom, const double* __restrict__ x, const double* __r
                       int* __restrict__ atom_cell, int* __restrict__ atom_slot,
                       double* __restrict__ atom_wz, int* __restrict__ cell_count,
                       double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-                      Status* st) {
+                      Status* st, const int* lstate) {
+    if (!lstate[LS_NEED]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
     int s = atom_site[i];
@@ -105,7 +160,8 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
                                                const int* __restrict__ col_cell0,
                                                int* __restrict__ col_cl0, int* __restrict__ cl_col,
                                                int* __restrict__ ncl_total, int maxcl,
-                                               Status* st) {
+                                               Status* st, const int* lstate) {
+    if (!lstate[LS_NEED]) return;
     __shared__ int sh[32];
     int carry = 0;
     for (int base = 0; base < ncell; base += 1024) {
@@ -149,7 +205,8 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
 
 __global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
                           const int* __restrict__ atom_slot, const int* __restrict__ cell_start,
-                          int* __restrict__ cell_members) {
+                          int* __restrict__ cell_members, const int* lstate) {
+    if (!lstate[LS_NEED]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
     int c = atom_cell[i];
@@ -167,7 +224,9 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
                         const RepDev* __restrict__ reps, const int* __restrict__ col_cell0,
                         const int* __restrict__ col_cl0, int* __restrict__ slot_orig,
                         int2* __restrict__ slot_info, double4* __restrict__ slot_x64,
-                        double4* __restrict__ slot_w64, int maxcl) {
+                        double4* __restrict__ slot_w64, double4* __restrict__ atom_x0,
+                        const SiteDev* __restrict__ sites, int maxcl, const int* lstate) {
+    if (!lstate[LS_NEED]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
     int c = atom_cell[i];
@@ -192,7 +251,8 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
     slot_info[slot] = make_int2(atom_mol[i], atom_site[i]);
     slot_x64[slot] = make_double4(x[i], y[i], z[i], 0.0);
     slot_w64[slot] = make_double4(wrap(xi, R.L[0], R.invL[0]), wrap(yi, R.L[1], R.invL[1]), wz,
-                                  0.0);
+                                  (double)sites[atom_site[i]].charge_f);
+    atom_x0[i] = make_double4(x[i], y[i], z[i], 0.0);
 }
 
 // 8 lanes per cluster: bounding box, centre, FP32 offsets; pads marked.
@@ -203,10 +263,17 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
                        int2* __restrict__ slot_info, const double4* __restrict__ slot_w64,
                        float4* __restrict__ slot_off, float4* __restrict__ slot_par,
                        double4* __restrict__ cl_center,
-                       float4* __restrict__ cl_ext) {
+                       float4* __restrict__ cl_ext, int* lstate) {
+    if (!lstate[LS_NEED]) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int c = t >> 3, k = t & 7;
     int ncl = min(*ncl_total, maxcl);
+    if (t == 0) {
+        // usable for reuse unless the clusters overflowed (k_plist clears it
+        // again on a row overflow)
+        lstate[LS_VALID] = *ncl_total <= maxcl;
+        lstate[LS_BUILDS] += 1;
+    }
     bool live = c < ncl;
     int col = live ? cl_col[c] : 0;
     int cnt = live ? (cell_start[col_cell0[col + 1]] - cell_start[col_cell0[col]]) : 0;
@@ -269,7 +336,9 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                                                const double4* __restrict__ cl_center,
                                                const float4* __restrict__ cl_ext,
                                                uint32_t* __restrict__ plist,
-                                               int* __restrict__ plist_n, int cap, Status* st) {
+                                               int* __restrict__ plist_n, int cap, Status* st,
+                                               int* lstate) {
+    if (!lstate[LS_NEED]) return;
     int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int lane = threadIdx.x & 63;
     int ncl = min(*ncl_total, maxcl);
@@ -370,7 +439,10 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
     }
     if (lane == 0) {
         plist_n[ci] = min(n, cap);
-        if (n > cap) atomicOr(&st->flags, ST_OVERFLOW);
+        if (n > cap) {
+            atomicOr(&st->flags, ST_OVERFLOW);
+            lstate[LS_VALID] = 0;  // truncated row: never reuse
+        }
         atomicMax(&st->max_list, n);
     }
 }
@@ -378,28 +450,34 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
 }  // namespace
 
 void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, const double* y,
-                          const double* z, double* fx, double* fy, double* fz, int precision) {
-    (void)precision;
+                          const double* z, double* fx, double* fy, double* fz, double half_skin2,
+                          int force) {
     const int T = 256;
     int gA = (c.natom + T - 1) / T;
+    const int nslot = b.maxcl * MSPMS_CL;
+    hipLaunchKernelGGL(k_list_check, dim3((std::max(c.natom, nslot) + T - 1) / T), dim3(T), 0,
+                       c.stream, c.natom, nslot, x, y, z, c.atom_site, c.sites, fx, fy, fz,
+                       b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
+                       b.slot_x64, b.slot_off, b.lstate, half_skin2, force);
     hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
-                       b.cell_count, fx, fy, fz, c.status);
+                       b.cell_count, fx, fy, fz, c.status, b.lstate);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, b.ncell, b.ncol, b.cell_count,
                        b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total, b.maxcl,
-                       c.status);
+                       c.status, b.lstate);
     hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
-                       b.atom_slot, b.cell_start, b.cell_members);
+                       b.atom_slot, b.cell_start, b.cell_members, b.lstate);
     hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,
                        b.atom_wz, b.cell_start, b.cell_members, c.atom_mol, c.atom_site, c.mols,
                        c.reps, b.col_cell0, b.col_cl0, b.slot_orig, b.slot_info, b.slot_x64,
-                       b.slot_w64, b.maxcl);
+                       b.slot_w64, b.atom_x0, c.sites, b.maxcl, b.lstate);
     int gB = (b.maxcl * MSPMS_CL + T - 1) / T;
     hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,
-                       b.slot_info, b.slot_w64, b.slot_off, b.slot_par, b.cl_center, b.cl_ext);
+                       b.slot_info, b.slot_w64, b.slot_off, b.slot_par, b.cl_center, b.cl_ext,
+                       b.lstate);
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,
-                       b.cap, c.status);
+                       b.cap, c.status, b.lstate);
 }

@@ -302,11 +302,13 @@ __global__ void __launch_bounds__(1024) k_finalize_long(
     int have_pair, const double* __restrict__ ep_intra, int have_intra,
     const double* __restrict__ uk, int have_recip, const double* __restrict__ ep_sf, int have_sf,
     const double* __restrict__ uvac, const double* __restrict__ ep_coul,
-    mspms_long_terms* __restrict__ out) {
+    mspms_long_terms* __restrict__ out, int* lstate) {
 #pragma clang fp contract(off)
     __shared__ double sh[4 * 16];
     __shared__ unsigned long long shu[4 * 16];
     const RepDev& R = reps[blockIdx.x];
+    // the cluster kernels of this call have read the rebuild request
+    if (lstate && blockIdx.x == 0 && threadIdx.x == 0) lstate[LS_NEED] = 0;
     double p[4] = {0, 0, 0, 0};
     unsigned long long q[4] = {0, 0, 0, 0};
     if (have_pair) {
@@ -681,10 +683,11 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* 
                           const unsigned long long* ip_pair, int have_pair,
                           const double* ep_intra, int have_intra, const double* uk,
                           int have_recip, const double* ep_sf, int have_sf,
-                          const double* uvac, const double* ep_coul, void* terms) {
+                          const double* uvac, const double* ep_coul, void* terms,
+                          int* lstate) {
     hipLaunchKernelGGL(k_finalize_long, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, col_cl0,
                        ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip, ep_sf,
-                       have_sf, uvac, ep_coul, (mspms_long_terms*)terms);
+                       have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate);
 }
 
 int coulomb_mol_supported(int max_mol_atoms) { return max_mol_atoms <= 16; }

@@ -27,7 +27,7 @@ EXPORTS = [
     "mspms_sync", "mspms_stream", "mspms_set_profiling", "mspms_last_timings",
     "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
     "mspms_autotune", "mspms_set_launch_shape", "mspms_hmc_init", "mspms_hmc_run_device",
-    "mspms_hmc_stats", "mspms_hmc_draw",
+    "mspms_hmc_stats", "mspms_hmc_draw", "mspms_set_list_skin", "mspms_list_builds",
 ]
 
 
@@ -81,6 +81,8 @@ def lib():
     L.mspms_hmc_draw.argtypes = [vp, C.c_int, C.c_int, dp]
     L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
     L.mspms_set_launch_shape.argtypes = [vp, C.c_double, C.c_int]
+    L.mspms_set_list_skin.argtypes = [vp, C.c_double]
+    L.mspms_list_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -336,6 +338,16 @@ class Engine:
     def set_launch_shape(self, cell_atoms=8.0, waves_per_cluster=4):
         _check(lib().mspms_set_launch_shape(self._h, float(cell_atoms), int(waves_per_cluster)),
                "mspms_set_launch_shape")
+
+    def set_list_skin(self, skin: float):
+        """Pair-list buffer: lists are built with the cutoff + skin and reused
+        while every site stays within skin / 2 of its position at the build."""
+        _check(lib().mspms_set_list_skin(self._h, C.c_double(float(skin))), "mspms_set_list_skin")
+
+    def list_builds(self) -> int:
+        n = C.c_longlong(0)
+        _check(lib().mspms_list_builds(self._h, C.byref(n)), "mspms_list_builds")
+        return int(n.value)
 
     def sync(self):
         _check(lib().mspms_sync(self._h), "mspms_sync")
