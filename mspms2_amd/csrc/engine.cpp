@@ -161,6 +161,10 @@ struct mspms_ctx {
     // partial sums and outputs
     DevBuf<double> d_ep_pair, d_ep_intra, d_ep_short, d_ep_sf, d_uvac, d_ep_coul;
     DevBuf<double4> d_mcom;
+    FinBufs fin{};
+    DevBuf<double> d_fin_part;
+    DevBuf<unsigned long long> d_fin_partu;
+    DevBuf<int> d_fin_count;
 
     // solid-fluid tables
     SFDev sf{};
@@ -503,6 +507,9 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_uvac.release();
     c->d_ep_coul.release();
     c->d_mcom.release();
+    c->d_fin_part.release();
+    c->d_fin_partu.release();
+    c->d_fin_count.release();
     c->d_ip_pair.release();
     c->d_long.release();
     c->d_short.release();
@@ -1004,8 +1011,15 @@ int build(mspms_ctx* c) {
         (rc = c->d_short.ensure(c->nrep)) || (rc = c->d_pos.ensure((size_t)3 * c->natom)) ||
         (rc = c->d_frec.ensure((size_t)3 * c->natom)) ||
         (rc = c->d_fl.ensure((size_t)3 * c->natom)) || (rc = c->d_fs.ensure((size_t)3 * c->natom)) ||
-        (rc = c->d_kin.ensure((size_t)c->natom)))
+        (rc = c->d_kin.ensure((size_t)c->natom)) ||
+        (rc = c->d_fin_part.ensure((size_t)c->nrep * FIN_BLOCKS * FIN_ND)) ||
+        (rc = c->d_fin_partu.ensure((size_t)c->nrep * FIN_BLOCKS * 4)) ||
+        (rc = c->d_fin_count.ensure((size_t)c->nrep)))
         return rc;
+    HIPC(hipMemsetAsync(c->d_fin_count.p, 0, sizeof(int) * (size_t)c->nrep, s));
+    c->fin.part = c->d_fin_part.p;
+    c->fin.partu = c->d_fin_partu.p;
+    c->fin.count = c->d_fin_count.p;
     HIPC(hipStreamSynchronize(s));
     return 0;
 }
@@ -1142,11 +1156,25 @@ __global__ void k_reset_status(Status* st) {
     st->max_list = 0;
 }
 
+// The frclong terms from the partial sums; with sterms, also the frcshort
+// terms of the inner steps (ep_short) and the kinetic energy (kin).
+void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
+                     mspms_short_terms* sterms = nullptr, double* ukin = nullptr) {
+    launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1, c->d_ep_intra.p,
+                         c->any_intra ? 1 : 0, c->rb.uk, c->any_recip ? 1 : 0, c->d_ep_sf.p,
+                         c->any_sf ? 1 : 0, c->any_vacuum ? c->d_uvac.p : nullptr,
+                         c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate,
+                         sterms ? c->d_ep_short.p : nullptr, sterms, sterms ? c->d_kin.p : nullptr,
+                         sterms ? ukin : nullptr, c->fin);
+}
+
 // grad (optional): per-atom parameter derivatives gq | ge | gs, 3 * natom;
 // with it the pair path is the FP64 one whatever the context precision
+// finalize = false leaves the partial sums for the caller's finalize pass
+// (enqueue_vver reduces them together with the step's other partials)
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
                  double* fy, double* fz, mspms_long_terms* terms, hipStream_t s,
-                 double* grad = nullptr) {
+                 double* grad = nullptr, bool finalize = true) {
     const size_t n = (size_t)c->natom;
     double* gq = grad;
     double* ge = grad ? grad + n : nullptr;
@@ -1213,13 +1241,9 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         ProfScope p(c, s, PROF_SF);
         launch_sf(L, c->sf, x, y, z, fx, fy, fz, c->d_ep_sf.p);
     }
-    {
+    if (finalize) {
         ProfScope p(c, s, PROF_FINAL);
-        launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1,
-                             c->d_ep_intra.p, c->any_intra ? 1 : 0, c->rb.uk,
-                             c->any_recip ? 1 : 0, c->d_ep_sf.p, c->any_sf ? 1 : 0,
-                             c->any_vacuum ? c->d_uvac.p : nullptr,
-                             c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate);
+        launch_finalize(c, L, terms);
     }
     HIPC(hipGetLastError());
     return 0;
@@ -1249,12 +1273,17 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
         launch_respa_inner(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, vel,
                            vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
                            deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
-        launch_finalize_short(L, c->d_ep_short.p, sterms);
     }
-    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s);
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
+                          nullptr, false);
     if (rc) return rc;
     launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
-                       c->d_kin.p, ukin);
+                       c->d_kin.p);
+    {
+        // frclong, frcshort (last inner step) and kinetic terms in one pass
+        ProfScope p(c, s, PROF_FINAL);
+        launch_finalize(c, L, lterms, sterms, ukin);
+    }
     HIPC(hipGetLastError());
     return 0;
 }

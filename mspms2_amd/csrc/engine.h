@@ -104,6 +104,18 @@ struct ClusterBufs {
 };
 enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_N = 4 };
 
+// frclong / RESPA-step finalize: FIN_BLOCKS workgroups of FIN_T threads per
+// replica, their sums in part / partu, per-replica arrival counters (zero
+// between calls)
+#define FIN_BLOCKS 16
+#define FIN_T 256
+#define FIN_ND 17
+struct FinBufs {
+    double* part;              // [nrep][FIN_BLOCKS][FIN_ND]
+    unsigned long long* partu; // [nrep][FIN_BLOCKS][4]
+    int* count;                // [nrep]
+};
+
 struct Status {
     int flags;           // bit 0: non-finite position, bit 1: list overflow,
                          // bit 2: grid out of range
@@ -199,7 +211,7 @@ void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
 // Buffered (Verlet) cluster list: the list is built with rc_list = cutoff +
 // skin and reused while every atom stays within skin/2 of its position at
 // the build (force: rebuild regardless); reused calls only refresh the slot
-// positions.  A rebuild request made here is cleared by k_finalize_long.
+// positions.  A rebuild request made here is cleared by the finalize kernel.
 void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x,
                           const double* y, const double* z, double* fx,
                           double* fy, double* fz, double half_skin2, int force);
@@ -243,7 +255,9 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_intra, int have_intra,
                           const double* uk, int have_recip,
                           const double* ep_sf, int have_sf, const double* uvac,
-                          const double* ep_coul, void* terms, int* lstate);
+                          const double* ep_coul, void* terms, int* lstate,
+                          const double* ep_short, void* sterms, const double* kin,
+                          double* ukin, const FinBufs& fb);
 // simple Coulomb between molecules (iChargeType 3): COMs into com[nmol],
 // per-molecule energy partials into ep[nmol]; molecules of <= 16 sites
 int coulomb_mol_supported(int max_mol_atoms);
@@ -264,7 +278,8 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species,
                         const double* fyl, const double* fzl, double* fxs,
                         double* fys, double* fzs, double deltby2, double delts,
                         double deltsby2, int nstep_inner, double* epart);
+// closing half kick; kin[i] = v.v / r_mass per atom (the step's finalize
+// kernel reduces it into ukin)
 void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
                         const double* fxl, const double* fyl,
-                        const double* fzl, double deltby2, double* kin,
-                        double* ukin);
+                        const double* fzl, double deltby2, double* kin);

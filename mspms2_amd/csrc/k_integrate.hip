@@ -182,24 +182,6 @@ __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __res
     kin[i] = (a * a + b * b + c * c) / r_mass;
 }
 
-// per replica: ukin = 0.5 * sum (v.v)/r_mass in atom order blocks
-__global__ void __launch_bounds__(1024) k_kinetic(const RepDev* __restrict__ reps,
-                                                 const double* __restrict__ kin,
-                                                 double* __restrict__ ukin) {
-    __shared__ double sh[16];
-    const RepDev& R = reps[blockIdx.x];
-    double s = 0.0;
-    for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x) s += kin[a];
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
-        ukin[blockIdx.x] = t * 0.5;
-    }
-}
-
 }  // namespace
 
 void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
@@ -229,9 +211,8 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
 
 void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
                         const double* fxl, const double* fyl, const double* fzl,
-                        double deltby2, double* kin, double* ukin) {
+                        double deltby2, double* kin) {
     int g = (c.natom + 255) / 256;
     hipLaunchKernelGGL(k_respa_final, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site,
                        c.sites, vx, vy, vz, fxl, fyl, fzl, deltby2, kin);
-    hipLaunchKernelGGL(k_kinetic, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, kin, ukin);
 }

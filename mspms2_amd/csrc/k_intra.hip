@@ -6,8 +6,9 @@
 //                    real space, :597-787)
 //   k_bonded         frcshort: bonds, angles, dihedrals, impropers
 //                    (src/frcshort.c:20-97 and the term files)
-//   k_finalize_long  per-replica deterministic reduction of every partial
-//   k_finalize_short   sum and the reference's energy bookkeeping
+//   k_finalize       per-replica deterministic reduction of every partial
+//   k_finalize_short   sum and the reference's energy bookkeeping (one
+//                    RESPA step: long, short and kinetic in one pass)
 //                    (src/frclong.c:985, :1048-1061, :1128, :1142, :1223)
 // One thread owns one molecule and therefore its atoms: forces are
 // read-modify-written without atomics, in the reference's term order.
@@ -253,6 +254,10 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restri
 }
 
 // ---- deterministic block reductions ------------------------------------
+// NV sums over the workgroup (at most 16 waves): a shuffle tree in each
+// wave, then a 16-lane shuffle tree over the wave partials in wave 0.  The
+// order is fixed, so the result is deterministic; the sums end in lanes
+// 0..15 of wave 0 (read them in thread 0).  sh: NV * (number of waves).
 template <int NV>
 __device__ void block_sum(double (&v)[NV], double* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -261,13 +266,15 @@ __device__ void block_sum(double (&v)[NV], double* sh) {
         double s = v[k];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) sh[wid * NV + k] = s;
+        if (lane == 0) sh[k * nw + wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
+#pragma unroll
         for (int k = 0; k < NV; k++) {
-            double s = 0.0;
-            for (int w = 0; w < nw; w++) s += sh[w * NV + k];
+            double s = lane < nw ? sh[k * nw + lane] : 0.0;
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
             v[k] = s;
         }
     }
@@ -282,83 +289,146 @@ __device__ void block_sum_u64(unsigned long long (&v)[NV], unsigned long long* s
         unsigned long long s = v[k];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) sh[wid * NV + k] = s;
+        if (lane == 0) sh[k * nw + wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
+#pragma unroll
         for (int k = 0; k < NV; k++) {
-            unsigned long long s = 0;
-            for (int w = 0; w < nw; w++) s += sh[w * NV + k];
+            unsigned long long s = lane < nw ? sh[k * nw + lane] : 0ull;
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
             v[k] = s;
         }
     }
     __syncthreads();
 }
 
-// One workgroup per replica.
-__global__ void __launch_bounds__(1024) k_finalize_long(
+// Every partial of frclong into its terms, FIN_BLOCKS workgroups per
+// replica: each reduces a strided share of the replica's partials and
+// stores its sums; the replica's last workgroup to finish (counter, reset
+// by that workgroup) adds the FIN_BLOCKS sums in block order, so the result
+// does not depend on arrival order.  STEP (one RESPA outer step) also
+// reduces the bonded partials of the inner steps into the frcshort terms and
+// the per-atom 2 x kinetic energies into ukin, in the same pass.  Slots of
+// d[]: 0-3 pair (uvdw, shift, ureal, virial), 4-7 intramolecular (uvdw,
+// ureal, uexcl, virial), 8 reciprocal, 9 solid-fluid, 10 simple Coulomb,
+// 11-15 bonded (bond, angle, dihedral, improper, virial), 16 kinetic.
+template <bool STEP>
+__global__ void __launch_bounds__(FIN_T) k_finalize(
     const RepDev* __restrict__ reps, const int* __restrict__ col_cl0,
     const double* __restrict__ ep_pair, const unsigned long long* __restrict__ ip_pair,
     int have_pair, const double* __restrict__ ep_intra, int have_intra,
     const double* __restrict__ uk, int have_recip, const double* __restrict__ ep_sf, int have_sf,
     const double* __restrict__ uvac, const double* __restrict__ ep_coul,
-    mspms_long_terms* __restrict__ out, int* lstate) {
+    mspms_long_terms* __restrict__ out, int* lstate, const double* __restrict__ ep_short,
+    mspms_short_terms* __restrict__ sout, const double* __restrict__ kin,
+    double* __restrict__ ukin, FinBufs fb) {
 #pragma clang fp contract(off)
-    __shared__ double sh[4 * 16];
-    __shared__ unsigned long long shu[4 * 16];
-    const RepDev& R = reps[blockIdx.x];
-    // the cluster kernels of this call have read the rebuild request
-    if (lstate && blockIdx.x == 0 && threadIdx.x == 0) lstate[LS_NEED] = 0;
-    double p[4] = {0, 0, 0, 0};
+    constexpr int ND = STEP ? 17 : 11;
+    static_assert(ND <= FIN_ND, "finalize partial slots");
+    __shared__ double sh[ND * (FIN_T / 64)];
+    __shared__ unsigned long long shu[4 * (FIN_T / 64)];
+    __shared__ double tot[ND];
+    __shared__ unsigned long long totu[4];
+    __shared__ int last;
+    const int r = blockIdx.x / FIN_BLOCKS, b = blockIdx.x % FIN_BLOCKS;
+    const RepDev& R = reps[r];
+    const int t0 = b * FIN_T + threadIdx.x, nt = FIN_BLOCKS * FIN_T;
+    double d[ND];
+#pragma unroll
+    for (int k = 0; k < ND; k++) d[k] = 0.0;
     unsigned long long q[4] = {0, 0, 0, 0};
     if (have_pair) {
-        int c0 = col_cl0[R.col0], c1 = col_cl0[R.col0 + R.ncol];
-        for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-            for (int k = 0; k < 4; k++) {
-                p[k] += ep_pair[(size_t)c * 4 + k];
-                q[k] += ip_pair[(size_t)c * 4 + k];
-            }
+        const int c0 = col_cl0[R.col0], c1 = col_cl0[R.col0 + R.ncol];
+        for (int c = c0 + t0; c < c1; c += nt) {
+            const double4 e = reinterpret_cast<const double4*>(ep_pair)[c];
+            const ulonglong4 n = reinterpret_cast<const ulonglong4*>(ip_pair)[c];
+            d[0] += e.x;
+            d[1] += e.y;
+            d[2] += e.z;
+            d[3] += e.w;
+            q[0] += n.x;
+            q[1] += n.y;
+            q[2] += n.z;
+            q[3] += n.w;
         }
     }
-    block_sum<4>(p, sh);
-    block_sum_u64<4>(q, shu);
-    double w[4] = {0, 0, 0, 0};
-    if (have_intra) {
-        for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x)
-            for (int k = 0; k < 4; k++) w[k] += ep_intra[(size_t)m * 4 + k];
-    }
-    block_sum<4>(w, sh);
-    double f[1] = {0.0};
+    if (have_intra)
+        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt) {
+            const double4 e = reinterpret_cast<const double4*>(ep_intra)[m];
+            d[4] += e.x;
+            d[5] += e.y;
+            d[6] += e.z;
+            d[7] += e.w;
+        }
     if (have_recip)
-        for (int k = R.k0 + threadIdx.x; k < R.k0 + R.nk; k += blockDim.x) f[0] += uk[k];
-    block_sum<1>(f, sh);
-    double sf[1] = {0.0};
+        for (int k = R.k0 + t0; k < R.k0 + R.nk; k += nt) d[8] += uk[k];
     if (have_sf)
-        for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x)
-            sf[0] += ep_sf[a];
-    block_sum<1>(sf, sh);
-    double uc[1] = {0.0};
+        for (int a = R.atom0 + t0; a < R.atom0 + R.natom; a += nt) d[9] += ep_sf[a];
     if (ep_coul && R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB)
-        for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x) uc[0] += ep_coul[m];
-    block_sum<1>(uc, sh);
+        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt) d[10] += ep_coul[m];
+    if (STEP) {
+        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt)
+            for (int k = 0; k < 5; k++) d[11 + k] += ep_short[(size_t)m * 5 + k];
+        for (int a = R.atom0 + t0; a < R.atom0 + R.natom; a += nt) d[ND - 1] += kin[a];
+    }
+    block_sum<ND>(d, sh);
+    block_sum_u64<4>(q, shu);
+    if (threadIdx.x == 0) {
+        double* P = fb.part + ((size_t)r * FIN_BLOCKS + b) * FIN_ND;
+        unsigned long long* Q = fb.partu + ((size_t)r * FIN_BLOCKS + b) * 4;
+        for (int k = 0; k < ND; k++) P[k] = d[k];
+        for (int k = 0; k < 4; k++) Q[k] = q[k];
+        __threadfence();
+        last = atomicAdd(&fb.count[r], 1) == FIN_BLOCKS - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // the replica's last workgroup: the FIN_BLOCKS sums in block order, read
+    // past the (non-coherent) vector L1
+    __threadfence();
+    if (threadIdx.x < ND) {
+        double s = 0.0;
+        for (int bb = 0; bb < FIN_BLOCKS; bb++)
+            s += __hip_atomic_load(fb.part + ((size_t)r * FIN_BLOCKS + bb) * FIN_ND + threadIdx.x,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tot[threadIdx.x] = s;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 68) {
+        const int k = threadIdx.x - 64;
+        unsigned long long s = 0;
+        for (int bb = 0; bb < FIN_BLOCKS; bb++)
+            s += __hip_atomic_load(fb.partu + ((size_t)r * FIN_BLOCKS + bb) * 4 + k,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        totu[k] = s;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    fb.count[r] = 0;  // ready for the next call
+    // the cluster kernels of this call have read the rebuild request
+    if (lstate && r == 0) lstate[LS_NEED] = 0;
+#pragma unroll
+    for (int k = 0; k < ND; k++) d[k] = tot[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = totu[k];
+
     mspms_long_terms t;
     // full lists visit every inter-molecular pair twice
-    t.uvdw = 0.5 * p[0] + w[0];
-    t.ushift = (0.5 * p[1]) * R.shift4;   // src/frclong.c:985
-    t.ureal = 0.5 * p[2] + w[1];
-    t.uexcl = w[2];
-    double vir = 0.5 * p[3] + w[3];
+    t.uvdw = 0.5 * d[0] + d[4];
+    t.ushift = (0.5 * d[1]) * R.shift4;   // src/frclong.c:985
+    t.ureal = 0.5 * d[2] + d[5];
+    t.uexcl = d[6];
+    double vir = 0.5 * d[3] + d[7];
     t.ufourier = 0.0;
     t.uself = 0.0;
     t.uvacuum = 0.0;
     t.uelec = 0.0;
     if (R.charge_type == MSPMS_ELEC_EWALD) {
-        t.ufourier = f[0] * (R.vfactor * R.C);                        // :1048
+        t.ufourier = d[8] * (R.vfactor * R.C);                        // :1048
         t.uself = R.self_sum_q2 * (R.C * sqrt(R.kappa * R.kappa / MSPMS_PI));  // :1058
         t.uelec = t.ureal + t.ufourier - t.uself - t.uexcl;           // :1061
         if (uvac && R.ewald_bc == MSPMS_EWALD_BC_VACUUM) {
-            t.uvacuum = uvac[blockIdx.x];
+            t.uvacuum = uvac[r];
             t.uelec += t.uvacuum;                                     // :1125
         }
         vir += t.uelec;                                               // :1128
@@ -368,10 +438,10 @@ __global__ void __launch_bounds__(1024) k_finalize_long(
     } else if (R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB) {
         // every molecule pair twice; the Coulomb virial is its energy
         // (src/elecfrc.c:134-135)
-        t.uelec = 0.5 * uc[0];
+        t.uelec = 0.5 * d[10];
         vir += t.uelec;
     }
-    t.usflj = sf[0];
+    t.usflj = d[9];
     t.uotherff = 0.0;
     t.uinter = t.uvdw + t.uelec + t.usflj + t.uotherff;               // :1223
     t.virial_inter = vir;
@@ -379,7 +449,18 @@ __global__ void __launch_bounds__(1024) k_finalize_long(
     t.n_elec_pairs = (int64_t)q[1];
     t.lj_pair_hash = q[2];
     t.elec_pair_hash = q[3];
-    out[blockIdx.x] = t;
+    out[r] = t;
+    if (STEP) {
+        mspms_short_terms u;
+        u.ubond = d[11];
+        u.uangle = d[12];
+        u.udih = d[13];
+        u.uimp = d[14];
+        u.uintra = u.ubond + u.uangle + u.udih + u.uimp;  // src/frcshort.c:87
+        u.virial_intra = d[15];
+        sout[r] = u;
+        ukin[r] = d[ND - 1] * 0.5;
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_finalize_short(const RepDev* __restrict__ reps,
@@ -684,10 +765,19 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* 
                           const double* ep_intra, int have_intra, const double* uk,
                           int have_recip, const double* ep_sf, int have_sf,
                           const double* uvac, const double* ep_coul, void* terms,
-                          int* lstate) {
-    hipLaunchKernelGGL(k_finalize_long, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, col_cl0,
-                       ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip, ep_sf,
-                       have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate);
+                          int* lstate, const double* ep_short, void* sterms,
+                          const double* kin, double* ukin, const FinBufs& fb) {
+    const dim3 grid(c.nrep * FIN_BLOCKS);
+    if (ep_short)
+        hipLaunchKernelGGL(k_finalize<true>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
+                           ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
+                           ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
+                           ep_short, (mspms_short_terms*)sterms, kin, ukin, fb);
+    else
+        hipLaunchKernelGGL(k_finalize<false>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
+                           ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
+                           ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
+                           nullptr, nullptr, nullptr, nullptr, fb);
 }
 
 int coulomb_mol_supported(int max_mol_atoms) { return max_mol_atoms <= 16; }
