@@ -226,6 +226,38 @@ def run_probe(system, calls=5):
                       "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
 
 
+def run_trace(system, steps, skin):
+    """Only the timed region of the headline: `steps` mixed RESPA steps
+    replayed from the captured graph (after the autotuner and 10 eager
+    steps), for kernel-trace runs; the last `steps` dispatches of each
+    kernel are the graph's."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    dev = torch.device("cuda:0")
+    eng = Engine(system, precision="mixed")
+    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
+    vel = torch.tensor(maxwell_velocities(system), device=dev)
+    fl = torch.tensor(eng.frclong()[0], device=dev)
+    fs = torch.tensor(eng.frcshort()[0], device=dev)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(1, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    tune = eng.autotune(pos, repeats=2)
+    eng.set_list_skin(skin)
+    for _ in range(10):
+        eng.vver_step_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, lt, st, uk)
+    torch.cuda.synchronize()
+    b0 = eng.list_builds()
+    eng.vver_run_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, steps, lt, st, uk, graph=True)
+    torch.cuda.synchronize()
+    eng.sync()
+    print(json.dumps({"trace": "%d graph-replayed mixed RESPA steps" % steps,
+                      "list_builds": eng.list_builds() - b0,
+                      "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
+    eng.close()
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_water12288.json")
 
 
@@ -389,6 +421,8 @@ def main():
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
     ap.add_argument("--probe", action="store_true",
                     help="only the short frclong + reference-copy workload for counter runs")
+    ap.add_argument("--trace-steps", type=int, default=0,
+                    help="only this many graph-replayed mixed steps, for kernel-trace runs")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-kernel HBM traffic of two --pmc passes over --probe")
     args = ap.parse_args()
@@ -402,6 +436,9 @@ def main():
     rd, system = make_workload()
     if args.probe:
         run_probe(system)
+        return
+    if args.trace_steps > 0:
+        run_trace(system, args.trace_steps, args.skin / system.params["sigma_base"])
         return
     skin = args.skin / system.params["sigma_base"]
     mixed = run_gpu(system, "mixed", args.steps, args.warmup, skin=skin)

@@ -133,6 +133,13 @@ REF_SCALARS = [
     "delts", "const_3pisq_theta"]
 
 
+# the md[] block of a version-2 dump (oracle/ref_harness.c, --md-steps)
+MD_SCALARS = [
+    "nmd", "what_ensemble", "nfree", "nstep_inner", "treq", "preq", "delt", "delts", "qq",
+    "qqs", "gg", "ggs", "ss", "ps", "sss", "pss", "unhts", "unhtss", "ukin", "tinst", "Qts",
+    "Qbs", "vts", "vbs", "rts", "utsbs", "boxv", "pinst"]
+
+
 def read_ref_dump(path: str) -> dict:
     with open(path, "rb") as f:
         raw = f.read()
@@ -146,6 +153,15 @@ def read_ref_dump(path: str) -> dict:
     out = {n: float(v) for n, v in zip(REF_SCALARS, sc)}
     out.update(xx=arr[0].copy(), yy=arr[1].copy(), zz=arr[2].copy(),
                fl=arr[3:6].copy(), fs=arr[6:9].copy(), natom=int(natom))
+    if ver >= 2:
+        off += 8 * 9 * natom
+        (nmd,) = np.frombuffer(raw[off:off + 4], dtype=np.int32)
+        off += 4
+        md = np.frombuffer(raw[off:off + 8 * nmd], dtype=np.float64)
+        off += 8 * nmd
+        out["md"] = {n: float(v) for n, v in zip(MD_SCALARS, md)}
+        v = np.frombuffer(raw[off:off + 8 * 6 * natom], dtype=np.float64).reshape(6, natom)
+        out.update(v0=v[0:3].copy(), v=v[3:6].copy())
     return out
 
 
@@ -153,9 +169,12 @@ def reference_available() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
-def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None) -> dict:
+def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
+                  md_steps: int = 0) -> dict:
     """Runs the reference's start-up path + frclong + frcshort on a run
-    directory (in a scratch copy) and returns its dump."""
+    directory (in a scratch copy) and returns its dump; md_steps > 0 then
+    runs that many steps of the input's integrator (vver / vver_nh_3 /
+    npt_respa) and adds the velocities and the thermostat state."""
     if not reference_available():
         raise FileNotFoundError(REF_HARNESS)
     with tempfile.TemporaryDirectory(prefix="mspms_ref_") as tmp:
@@ -164,7 +183,8 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None)
                     open(os.path.join(tmp, fn), "wb") as dst:
                 dst.write(src.read())
         out = os.path.join(tmp, "dump.bin")
-        cmd = [REF_HARNESS, tmp, out] + (["--repeat", str(repeat)] if repeat else [])
+        cmd = [REF_HARNESS, tmp, out] + (["--repeat", str(repeat)] if repeat else []) + \
+            (["--md-steps", str(md_steps)] if md_steps else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stderr[-2000:]}")

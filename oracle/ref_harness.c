@@ -12,7 +12,15 @@
  *   "MSPMSREF" | int32 version=1 | int32 natom | int32 nscalar |
  *   double scalar[nscalar] | double xx,yy,zz,fxl,fyl,fzl,fxs,fys,fzs [natom]
  *
- * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K]
+ * With --md-steps N (version 2) it then runs N steps of the integrator the
+ * input selects, as the reference's main loop does (src/mspms2.c:64-77,
+ * :96-98: vver, vver_nh_3 or npt_respa), and the dump holds the state after
+ * them, followed by
+ *   int32 nmd | double md[nmd] (MD_SCALARS below) |
+ *   double vx0,vy0,vz0 [natom] (velinit's velocities, before step 1) |
+ *   double vx,vy,vz [natom] (after step N)
+ *
+ * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N]
  *   --repeat K times K further frclong() calls and prints seconds/call.
  *
  * Configurations that would reach code absent from this build (the
@@ -30,9 +38,11 @@
 
 static void put(FILE* f, const double* v, int n) { fwrite(v, sizeof(double), (size_t)n, f); }
 
+static double v0[3][NATOM_MAX];
+
 int main(int argc, char** argv)
 {
-    int repeat = 0, k;
+    int repeat = 0, nmd = 0, k;
     char outpath[4096];
     FILE* fo;
     if (argc < 3) {
@@ -40,7 +50,10 @@ int main(int argc, char** argv)
         return 2;
     }
     for (k = 3; k < argc; k++)
-        if (!strcmp(argv[k], "--repeat") && k + 1 < argc) repeat = atoi(argv[++k]);
+        if (!strcmp(argv[k], "--repeat") && k + 1 < argc)
+            repeat = atoi(argv[++k]);
+        else if (!strcmp(argv[k], "--md-steps") && k + 1 < argc)
+            nmd = atoi(argv[++k]);
     if (argv[2][0] == '/') {
         snprintf(outpath, sizeof outpath, "%s", argv[2]);
     } else {
@@ -74,6 +87,19 @@ int main(int argc, char** argv)
         printf("frclong_seconds_per_call %.9f\n",
                ((t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec)) / repeat);
     }
+    for (k = 0; k < natom; k++) {
+        v0[0][k] = vx[k];
+        v0[1][k] = vy[k];
+        v0[2][k] = vz[k];
+    }
+    for (k = 0; k < nmd; k++) {
+        if (what_ensemble == NVT)
+            vver_nh_3();
+        else if (what_ensemble == NPT)
+            npt_respa();
+        else
+            vver();
+    }
     fo = fopen(outpath, "wb");
     if (!fo) {
         perror("fopen");
@@ -91,7 +117,7 @@ int main(int argc, char** argv)
             wolfvcon1, wolfvcon2, wolfvcon3, wolffcon1, wolffcon2,
             f0, sigma_base, epsilon_base, uljlrc, pljlrc, time_base, delt,
             delts, const_3pisq_theta};
-        int nsc = (int)(sizeof sc / sizeof sc[0]), ver = 1;
+        int nsc = (int)(sizeof sc / sizeof sc[0]), ver = nmd > 0 ? 2 : 1;
         fwrite("MSPMSREF", 1, 8, fo);
         fwrite(&ver, sizeof ver, 1, fo);
         fwrite(&natom, sizeof natom, 1, fo);
@@ -107,6 +133,22 @@ int main(int argc, char** argv)
     put(fo, fxs, natom);
     put(fo, fys, natom);
     put(fo, fzs, natom);
+    if (nmd > 0) {
+        /* MD_SCALARS */
+        const double md[] = {
+            (double)nmd, (double)what_ensemble, (double)nfree, (double)nstep_inner, treq,
+            preq, delt, delts, qq, qqs, gg, ggs, ss, ps, sss, pss, unhts, unhtss, ukin, tinst,
+            Qts, Qbs, vts, vbs, rts, utsbs, boxv, pinst};
+        int n = (int)(sizeof md / sizeof md[0]);
+        fwrite(&n, sizeof n, 1, fo);
+        put(fo, md, n);
+        put(fo, v0[0], natom);
+        put(fo, v0[1], natom);
+        put(fo, v0[2], natom);
+        put(fo, vx, natom);
+        put(fo, vy, natom);
+        put(fo, vz, natom);
+    }
     fclose(fo);
     return 0;
 }

@@ -12,6 +12,12 @@ all scalars and the forces of every 12th atom.
 
 examples_water holds the input files of the reference's Examples/Water
 (flexible Toukan-Rahman water with Morse bonds), unchanged.
+
+MD cases (md_*): the run directory and md<N>.npz with the state after N
+steps of the reference's own integrator for the input's ensemble (vver,
+vver_nh_3), run from velinit's velocities (stored as v0): positions,
+velocities, both forces, the energies of the last evaluation and the
+thermostat scalars.
 """
 import hashlib
 import os
@@ -56,6 +62,29 @@ BIG = {
 }
 
 
+MD_CASES = {
+    # the LJ-864 argon workload: FCC, T* 0.722, NVT qq = qqs = 100, dt* 0.005
+    "md_lj864_nvt": (lambda: configs.lj_fcc(jitter=0.05, seed=7), (1, 10, 100)),
+    # flexible water, Ewald, RESPA with 4 inner steps, NVT qq = qqs = 200
+    "md_water216_nvt": (lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
+                                                  kmax=5, ksqmax=27, seed=3, nstep_inner=4,
+                                                  ensemble=1), (1, 10)),
+    # the same water, NVE (vver)
+    "md_water216_nve": (lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
+                                                  kmax=5, ksqmax=27, seed=3, nstep_inner=4,
+                                                  ensemble=0), (1, 10)),
+}
+
+
+def save_md(path, d, nstep):
+    sc = {k: np.float64(v) for k, v in d.items() if isinstance(v, float)}
+    md = {f"md_{k}": np.float64(v) for k, v in d["md"].items()}
+    np.savez_compressed(os.path.join(path, f"md{nstep}.npz"),
+                        x=np.stack([d["xx"], d["yy"], d["zz"]]), v=d["v"], v0=d["v0"],
+                        fl=d["fl"], fs=d["fs"], natom=d["natom"],
+                        **{f"s_{k}": v for k, v in sc.items()}, **md)
+
+
 def _sha(path):
     h = hashlib.sha256()
     for fn in ("in.mspms", "cfg.mspms", "xyz.mspms"):
@@ -88,6 +117,14 @@ def main():
     d = O.run_reference(ew)
     save(ew, d)
     print("examples_water", d["natom"], d["uinter"], d["uintra"])
+    for name, (fn, steps) in MD_CASES.items():
+        path = os.path.join(HERE, name)
+        os.makedirs(path, exist_ok=True)
+        fn().write(path)
+        for n in steps:
+            d = O.run_reference(path, md_steps=n)
+            save_md(path, d, n)
+            print(name, n, d["md"]["ukin"], d["md"]["ps"])
     for name, fn in BIG.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)
