@@ -386,6 +386,34 @@ int mspms_hmc_stats(mspms_ctx* ctx, long long* accepted, long long* moves,
                     double* last_dH);
 int mspms_hmc_draw(mspms_ctx* ctx, int replica, int n, double* out);
 
+/* Nose-Hoover NVT molecular dynamics on device buffers: the reference's
+ * vver_nh_3 (src/nvtnh.c:11-221, selected by src/mspms2.c:66-69), one
+ * thermostat on the outer step (ss, ps, mass qq) and one on the RESPA
+ * inner steps (sss, pss, mass qqs), degrees of freedom gg = ggs = nfree
+ * (init_nvt, src/initvars.c:298-347: ss = ps = sss = pss = 0 at the
+ * start).  mspms_nvt_set uploads one state per replica (all fields but
+ * the outputs unhts, unhtss, ukin and tinst); mspms_nvt_run_device runs
+ * nsteps steps of every replica from pos / vel with fl / fs holding the
+ * long and short forces of pos (as after the reference's start-up
+ * frclong / frcshort), leaving lterms / sterms of the last step and ukin
+ * [nrep]; replayed from one captured graph when use_graph is set.
+ * mspms_nvt_get downloads the states (thermostat variables, energies
+ * unhts / unhtss, ukin, tinst = 2 ukin / nfree). */
+typedef struct mspms_nvt_state {
+    double qq, qqs;            /* thermostat masses (outer, inner)          */
+    double gg, ggs;            /* thermostat degrees of freedom (nfree)     */
+    double treq;               /* reduced temperature                       */
+    double ss, ps, sss, pss;   /* thermostat positions and momenta          */
+    double unhts, unhtss;      /* thermostat energies (outputs)             */
+    double ukin, tinst;        /* kinetic energy, temperature (outputs)     */
+    int nfree, pad;
+} mspms_nvt_state;
+int mspms_nvt_set(mspms_ctx* ctx, const mspms_nvt_state* states);
+int mspms_nvt_get(mspms_ctx* ctx, mspms_nvt_state* states);
+int mspms_nvt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, double* fs,
+                         double delt, int nstep_inner, int nsteps, mspms_long_terms* lterms,
+                         mspms_short_terms* sterms, double* ukin, int use_graph, void* stream);
+
 /* Synchronises the device and reports the status accumulated by the
  * device-buffer evaluations since the previous mspms_sync (list overflow,
  * non-finite input, grid out of range, singular improper), which the

@@ -126,6 +126,16 @@ class TuneResult(C.Structure):
                                for k in range(n)]}
 
 
+class NvtState(C.Structure):
+    """mspms_nvt_state: one replica's Nose-Hoover thermostats (vver_nh_3)."""
+    _fields_ = [(k, C.c_double) for k in ("qq", "qqs", "gg", "ggs", "treq", "ss", "ps", "sss",
+                                          "pss", "unhts", "unhtss", "ukin", "tinst")] + \
+        [("nfree", C.c_int), ("pad", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 def dptr(a):
     """double* of a contiguous float64 numpy array (or NULL for None)."""
     if a is None:

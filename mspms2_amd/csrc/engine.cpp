@@ -199,13 +199,19 @@ struct mspms_ctx {
     // valid for the argument set in gkey until the next build
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    bool graph_is_hmc = false;           // the cached graph is one HMC move
+    int graph_kind = -1;                 // GRAPH_*: what the cached graph runs
 
     // batched HMC chains (mspms_hmc_init)
     DevBuf<RanmarDev> d_rng;
     DevBuf<HmcRep> d_hmc;
     DevBuf<double> d_aw_atom, d_pos_old;
     bool hmc_ready = false;
+
+    // Nose-Hoover NVT (mspms_nvt_set): states, the last solve's sum m v^2,
+    // the Newton iterate (3, N)
+    DevBuf<mspms_nvt_state> d_nvt;
+    DevBuf<double> d_nvt_carry, d_nvt_vn;
+    bool nvt_ready = false;
     struct GraphKey {
         void *pos, *vel, *fl, *fs, *lt, *st, *uk;
         double delt;
@@ -224,6 +230,40 @@ void drop_graph(mspms_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
+}
+
+enum { GRAPH_VVER = 0, GRAPH_HMC = 1, GRAPH_NVT = 2 };
+
+// Makes the cached graph the capture of enqueue(stream) for (kind, key),
+// capturing it again when either differs.
+template <typename F>
+int ensure_graph(mspms_ctx* c, hipStream_t s, int kind, const mspms_ctx::GraphKey& key,
+                 F&& enqueue) {
+    if (c->graph_exec && key == c->gkey && c->graph_kind == kind) return 0;
+    drop_graph(c);
+    // the capture stream must not have work in flight from the caller
+    HIPC(hipStreamSynchronize(s));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->capturing = true;
+    HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->capturing = false;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(MSPMS_EIO, "graph capture: %s", hipGetErrorString(e));
+    c->graph = g;
+    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(c);
+        return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    c->gkey = key;
+    c->graph_kind = kind;
+    return 0;
 }
 
 LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
@@ -526,6 +566,9 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_hmc.release();
     c->d_aw_atom.release();
     c->d_pos_old.release();
+    c->d_nvt.release();
+    c->d_nvt_carry.release();
+    c->d_nvt_vn.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -1020,6 +1063,14 @@ int build(mspms_ctx* c) {
     c->fin.part = c->d_fin_part.p;
     c->fin.partu = c->d_fin_partu.p;
     c->fin.count = c->d_fin_count.p;
+    {
+        // one partial of each kind per thread in the largest replica
+        size_t most = 1;
+        for (const RepDev& R : c->reps)
+            most = std::max({most, (size_t)R.natom, (size_t)R.nmol, (size_t)R.nk,
+                             (size_t)(R.natom / MSPMS_CL + R.ncol + 1)});
+        c->fin.nb = (int)std::min<size_t>(FIN_BLOCKS, (most + FIN_T - 1) / FIN_T);
+    }
     HIPC(hipStreamSynchronize(s));
     return 0;
 }
@@ -1079,6 +1130,7 @@ extern "C" int mspms_set_system(mspms_ctx* c, int nspecie, const mspms_species* 
     for (int k = 0; k < nspecie; k++) c->nmole[k] = sp[k].nmole;
     c->params.assign(1, *p);
     c->hmc_ready = false;  // the chains belong to the previous layout
+    c->nvt_ready = false;
     return build(c);
 }
 
@@ -1106,6 +1158,7 @@ extern "C" int mspms_set_replicas(mspms_ctx* c, int nrep, const int* nmole,
     c->nmole.assign(nmole, nmole + (size_t)nrep * ns);
     c->params.assign(params, params + nrep);
     c->hmc_ready = false;
+    c->nvt_ready = false;
     return build(c);
 }
 
@@ -1171,10 +1224,14 @@ void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
 // grad (optional): per-atom parameter derivatives gq | ge | gs, 3 * natom;
 // with it the pair path is the FP64 one whatever the context precision
 // finalize = false leaves the partial sums for the caller's finalize pass
-// (enqueue_vver reduces them together with the step's other partials)
+// (enqueue_vver reduces them together with the step's other partials).
+// kick (optional): the RESPA step's closing half kick, applied by the last
+// kernel that adds long forces when no kernel adds after it; *kicked tells
+// whether it was (otherwise the caller runs it).
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
                  double* fy, double* fz, mspms_long_terms* terms, hipStream_t s,
-                 double* grad = nullptr, bool finalize = true) {
+                 double* grad = nullptr, bool finalize = true, const Kick* kick = nullptr,
+                 bool* kicked = nullptr) {
     const size_t n = (size_t)c->natom;
     double* gq = grad;
     double* ge = grad ? grad + n : nullptr;
@@ -1218,12 +1275,16 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
                     fz, c->d_ep_pair.p, c->d_ip_pair.p, grad ? 1 : hash, gq, ge, gs);
     }
     if (c->any_recip) HIPC(hipStreamWaitEvent(s, c->ev_join, 0));  // join
+    // nothing adds long forces after k_intra / k_add_frec
+    const bool fuse = kick && !grad && !c->any_vacuum && !c->any_coul && !c->any_sf &&
+                      (c->any_intra || recip_side);
+    if (kicked) *kicked = fuse;
     if (c->any_intra) {
         ProfScope p(c, s, PROF_INTRA);
         launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p, gq,
-                     ge, gs, frec);
+                     ge, gs, frec, fuse ? kick : nullptr);
     } else if (recip_side) {
-        launch_add_frec(L, fx, fy, fz, frec);
+        launch_add_frec(L, fx, fy, fz, frec, fuse ? kick : nullptr);
     }
     if (c->any_recip) {
         if (!recip_side) {
@@ -1274,11 +1335,14 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
                            vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
                            deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
     }
+    const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2};
+    bool kicked = false;
     int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
-                          nullptr, false);
+                          nullptr, false, &kick, &kicked);
     if (rc) return rc;
-    launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
-                       c->d_kin.p);
+    if (!kicked)
+        launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
+                           c->d_kin.p);
     {
         // frclong, frcshort (last inner step) and kinetic terms in one pass
         ProfScope p(c, s, PROF_FINAL);
@@ -1528,32 +1592,10 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
     const int nr = c->nrep;
     if (use_graph) {
         mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
-        if (!c->graph_exec || !(k == c->gkey) || c->graph_is_hmc) {
-            drop_graph(c);
-            // the capture stream must not have work in flight from the caller
-            HIPC(hipStreamSynchronize(s));
-            HIPC(hipStreamSynchronize(c->stream));
-            c->capturing = true;
-            HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin,
-                                  c->stream);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            c->capturing = false;
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (e != hipSuccess) return fail(MSPMS_EIO, "graph capture: %s", hipGetErrorString(e));
-            c->graph = g;
-            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                drop_graph(c);
-                return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
-            }
-            c->gkey = k;
-            c->graph_is_hmc = false;
-        }
+        int rc = ensure_graph(c, s, GRAPH_VVER, k, [&](hipStream_t cs) {
+            return enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, cs);
+        });
+        if (rc) return rc;
     }
     for (int t = 0; t < nsteps; t++) {
         if (use_graph) {
@@ -1691,37 +1733,110 @@ extern "C" int mspms_hmc_run_device(mspms_ctx* c, double* pos, double* vel, doub
     if (use_graph) {
         mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt,
                               nstep_inner * 65536 + nmd};
-        if (!c->graph_exec || !(k == c->gkey) || !c->graph_is_hmc) {
-            drop_graph(c);
-            HIPC(hipStreamSynchronize(s));
-            HIPC(hipStreamSynchronize(c->stream));
-            c->capturing = true;
-            HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_hmc_move(c, pos, vel, fl, fs, delt, nstep_inner, nmd, lterms, sterms,
-                                      ukin, c->stream);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            c->capturing = false;
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (e != hipSuccess) return fail(MSPMS_EIO, "graph capture: %s", hipGetErrorString(e));
-            c->graph = g;
-            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                drop_graph(c);
-                return fail(MSPMS_EIO, "graph instantiate: %s", hipGetErrorString(e));
-            }
-            c->gkey = k;
-            c->graph_is_hmc = true;
-        }
+        int rc = ensure_graph(c, s, GRAPH_HMC, k, [&](hipStream_t cs) {
+            return enqueue_hmc_move(c, pos, vel, fl, fs, delt, nstep_inner, nmd, lterms, sterms,
+                                    ukin, cs);
+        });
+        if (rc) return rc;
         for (int m = 0; m < nmove; m++) HIPC(hipGraphLaunch(c->graph_exec, s));
         return 0;
     }
     for (int m = 0; m < nmove; m++) {
         int rc = enqueue_hmc_move(c, pos, vel, fl, fs, delt, nstep_inner, nmd, lterms, sterms,
                                   ukin, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// ---- Nose-Hoover NVT ----------------------------------------------------------
+namespace {
+
+// One vver_nh_3 step (src/nvtnh.c:11-221) of every replica on device (3, N)
+// buffers; kernels only, so that it can be captured.
+int enqueue_nvt(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs, double delt,
+                int nstep_inner, mspms_long_terms* lterms, mspms_short_terms* sterms,
+                double* ukin, hipStream_t s) {
+    const size_t n = (size_t)c->natom;
+    // src/initvars.c:604-607, :329-330
+    const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
+    const double delt_sqby2 = delt * delt / 2.0, delts_sqby2 = delts * delts / 2.0;
+    LaunchCtx L = launch_ctx(c, s);
+    {
+        ProfScope p(c, s, PROF_BONDED);
+        for (int ll = 0; ll < nstep_inner; ll++) {
+            launch_nvt_drift(L, c->d_nvt.p, ll == 0, pos, pos + n, pos + 2 * n, vel, vel + n,
+                             vel + 2 * n, fl, fs, deltby2, delts, deltsby2, delts_sqby2);
+            launch_bonded(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, fs, fs + n,
+                          fs + 2 * n, c->d_ep_short.p);
+            launch_nvt_solve(L, c->d_nvt.p, false, c->d_nvt_carry.p, vel, vel + n, vel + 2 * n,
+                             fs, c->d_nvt_vn.p, delts, deltsby2, delts_sqby2, nullptr);
+        }
+    }
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
+                          nullptr, false);
+    if (rc) return rc;
+    launch_nvt_solve(L, c->d_nvt.p, true, c->d_nvt_carry.p, vel, vel + n, vel + 2 * n, fl,
+                     c->d_nvt_vn.p, delt, deltby2, delt_sqby2, ukin);
+    {
+        ProfScope p(c, s, PROF_FINAL);
+        launch_finalize(c, L, lterms, sterms, nullptr);
+    }
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mspms_nvt_set(mspms_ctx* c, const mspms_nvt_state* st) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!st) return fail(MSPMS_EINVAL, "null argument");
+    for (int r = 0; r < c->nrep; r++)
+        if (!(st[r].qq > 0.0) || !(st[r].qqs > 0.0) || !(st[r].treq > 0.0) || st[r].nfree < 1)
+            return fail(MSPMS_EINVAL, "replica %d: qq, qqs, treq and nfree must be positive", r);
+    HIPC(hipSetDevice(c->device));
+    int rc;
+    if ((rc = c->d_nvt.ensure(c->nrep)) || (rc = c->d_nvt_carry.ensure(c->nrep)) ||
+        (rc = c->d_nvt_vn.ensure((size_t)3 * c->natom)))
+        return rc;
+    HIPC(hipMemcpyAsync(c->d_nvt.p, st, sizeof(mspms_nvt_state) * c->nrep,
+                        hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->nvt_ready = true;
+    return 0;
+}
+
+extern "C" int mspms_nvt_get(mspms_ctx* c, mspms_nvt_state* st) {
+    if (!c || !c->nvt_ready) return fail(MSPMS_EINVAL, "mspms_nvt_set first");
+    if (!st) return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipDeviceSynchronize());
+    HIPC(hipMemcpy(st, c->d_nvt.p, sizeof(mspms_nvt_state) * c->nrep, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mspms_nvt_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                    double* fs, double delt, int nstep_inner, int nsteps,
+                                    mspms_long_terms* lterms, mspms_short_terms* sterms,
+                                    double* ukin, int use_graph, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!c->nvt_ready) return fail(MSPMS_EINVAL, "mspms_nvt_set first");
+    if (nstep_inner < 1 || nsteps < 0) return fail(MSPMS_EINVAL, "bad step counts");
+    if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (use_graph) {
+        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
+        int rc = ensure_graph(c, s, GRAPH_NVT, k, [&](hipStream_t cs) {
+            return enqueue_nvt(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, cs);
+        });
+        if (rc) return rc;
+        for (int t = 0; t < nsteps; t++) HIPC(hipGraphLaunch(c->graph_exec, s));
+        return 0;
+    }
+    for (int t = 0; t < nsteps; t++) {
+        int rc = enqueue_nvt(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
         if (rc) return rc;
     }
     return 0;

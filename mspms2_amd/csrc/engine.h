@@ -104,16 +104,18 @@ struct ClusterBufs {
 };
 enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_N = 4 };
 
-// frclong / RESPA-step finalize: FIN_BLOCKS workgroups of FIN_T threads per
-// replica, their sums in part / partu, per-replica arrival counters (zero
-// between calls)
-#define FIN_BLOCKS 16
+// frclong / RESPA-step finalize: nb (<= FIN_BLOCKS) workgroups of FIN_T
+// threads per replica, enough for one partial of each kind per thread in the
+// largest replica; their sums in part / partu, per-replica arrival counters
+// (zero between calls)
+#define FIN_BLOCKS 64
 #define FIN_T 256
 #define FIN_ND 17
 struct FinBufs {
-    double* part;              // [nrep][FIN_BLOCKS][FIN_ND]
-    unsigned long long* partu; // [nrep][FIN_BLOCKS][4]
+    double* part;              // [nrep][nb][FIN_ND]
+    unsigned long long* partu; // [nrep][nb][4]
     int* count;                // [nrep]
+    int nb;
 };
 
 struct Status {
@@ -204,6 +206,16 @@ void launch_hmc_end(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const void*
                     const void* sterms, const double* ukin, double* pos, const double* pos_old);
 void launch_hmc_draw(const LaunchCtx& c, RanmarDev* rng, int rep, int n, double* out);
 
+// Nose-Hoover NVT (k_nvt.hip), one workgroup per replica
+struct mspms_nvt_state;
+void launch_nvt_drift(const LaunchCtx& c, mspms_nvt_state* st, bool first, double* x, double* y,
+                      double* z, double* vx, double* vy, double* vz, const double* fl,
+                      const double* fs, double deltby2, double delts, double deltsby2,
+                      double delts_sqby2);
+void launch_nvt_solve(const LaunchCtx& c, mspms_nvt_state* st, bool outer, double* carry,
+                      double* vx, double* vy, double* vz, const double* f, double* vn,
+                      double dt, double dtby2, double dt_sqby2, double* ukin);
+
 void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,
                double* fz, double* usf);
@@ -222,14 +234,22 @@ void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision,
                  double* epart, unsigned long long* ipart, int want_hash,
                  double* gq = nullptr, double* ge = nullptr,
                  double* gs = nullptr);
+// The closing half kick of a RESPA step (src/md.c:75-80), fused into the
+// last kernel that adds long forces when nothing adds after it: per atom
+// v += deltby2 f / m and kin = m v.v (vx null: no kick).
+struct Kick {
+    double *vx, *vy, *vz, *kin;
+    double deltby2;
+};
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const TermDev* terms, const double* x, const double* y,
                   const double* z, double* fx, double* fy, double* fz,
                   double* epart, double* gq = nullptr, double* ge = nullptr,
-                  double* gs = nullptr, const double* frec = nullptr);
+                  double* gs = nullptr, const double* frec = nullptr,
+                  const Kick* kick = nullptr);
 // fx += frec ((3, N) reciprocal-space forces) and a capturable zero fill
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
-                     const double* frec);
+                     const double* frec, const struct Kick* kick = nullptr);
 void launch_zero(const LaunchCtx& c, double* p, size_t n);
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
                    const TermDev* terms, const double* x, const double* y,

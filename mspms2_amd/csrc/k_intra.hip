@@ -17,6 +17,7 @@
 #include "engine.h"
 #include "physics.h"
 #include "terms.h"
+#include "wave.h"
 #include "../../include/mspms_gpu.h"
 
 namespace {
@@ -133,6 +134,20 @@ __device__ inline bool is_long(const RepDev& R, double rx, double ry, double rz)
     return fabs(rx) > R.L[0] * 0.5 || fabs(ry) > R.L[1] * 0.5 || fabs(rz) > R.L[2] * 0.5;
 }
 
+// closing half kick of atom i (k_respa_final's arithmetic)
+__device__ inline void kick_atom(const Kick& K, int i, double aw, const double* fx,
+                                 const double* fy, const double* fz) {
+#pragma clang fp contract(off)
+    const double r_mass = 1.0 / aw;
+    const double a = K.vx[i] + (K.deltby2 * fx[i] * r_mass);
+    const double b = K.vy[i] + (K.deltby2 * fy[i] * r_mass);
+    const double c = K.vz[i] + (K.deltby2 * fz[i] * r_mass);
+    K.vx[i] = a;
+    K.vy[i] = b;
+    K.vz[i] = c;
+    K.kin[i] = (a * a + b * b + c * c) / r_mass;
+}
+
 template <bool GRAD>
 __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restrict__ mols,
                                                const RepDev* __restrict__ reps,
@@ -144,7 +159,7 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
                                                const double* __restrict__ z, double* fx,
                                                double* fy, double* fz, double* epart,
                                                GradOut G, int natom_total,
-                                               const double* __restrict__ frec) {
+                                               const double* __restrict__ frec, Kick K) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
@@ -215,6 +230,10 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
     ep[1] = acc.ureal;
     ep[2] = acc.uexcl;
     ep[3] = acc.vir;
+    // this thread owns the molecule's atoms: their long forces are final
+    if (K.vx)
+        for (int k = 0; k < M.natom; k++)
+            kick_atom(K, base + k, sites[S.site0 + k].aw, fx, fy, fz);
 }
 
 __global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restrict__ mols,
@@ -254,61 +273,44 @@ __global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restri
 }
 
 // ---- deterministic block reductions ------------------------------------
-// NV sums over the workgroup (at most 16 waves): a shuffle tree in each
-// wave, then a 16-lane shuffle tree over the wave partials in wave 0.  The
-// order is fixed, so the result is deterministic; the sums end in lanes
-// 0..15 of wave 0 (read them in thread 0).  sh: NV * (number of waves).
-template <int NV>
-__device__ void block_sum(double (&v)[NV], double* sh) {
+// NV sums over the workgroup (at most 16 waves): a butterfly in each wave,
+// then a 16-lane butterfly over the wave partials in wave 0.  The order is
+// fixed, so the result is deterministic; the sums end in lanes 0..15 of wave
+// 0 (read them in thread 0).  v is overwritten.  sh: NV * (number of waves).
+
+template <typename T, int NV>
+__device__ void block_sum_t(T (&v)[NV], T* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    butterfly<T, NV>(v, 32);
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = v[k];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) sh[k * nw + wid] = s;
+        for (int k = 0; k < NV; k++) sh[k * nw + wid] = v[k];
     }
     __syncthreads();
     if (wid == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = lane < nw ? sh[k * nw + lane] : 0.0;
-#pragma unroll
-            for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-            v[k] = s;
-        }
+        for (int k = 0; k < NV; k++) v[k] = lane < nw ? sh[k * nw + lane] : T(0);
+        butterfly<T, NV>(v, 8);
     }
     __syncthreads();
+}
+
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* sh) {
+    block_sum_t<double, NV>(v, sh);
 }
 
 template <int NV>
 __device__ void block_sum_u64(unsigned long long (&v)[NV], unsigned long long* sh) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        unsigned long long s = v[k];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) sh[k * nw + wid] = s;
-    }
-    __syncthreads();
-    if (wid == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            unsigned long long s = lane < nw ? sh[k * nw + lane] : 0ull;
-#pragma unroll
-            for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-            v[k] = s;
-        }
-    }
-    __syncthreads();
+    block_sum_t<unsigned long long, NV>(v, sh);
 }
 
-// Every partial of frclong into its terms, FIN_BLOCKS workgroups per
-// replica: each reduces a strided share of the replica's partials and
-// stores its sums; the replica's last workgroup to finish (counter, reset
-// by that workgroup) adds the FIN_BLOCKS sums in block order, so the result
-// does not depend on arrival order.  STEP (one RESPA outer step) also
+// Every partial of frclong into its terms, nb workgroups per replica (one
+// partial of each kind per thread where the replica fits): each reduces a
+// strided share of the replica's partials, every load of a pass issued
+// before the first add, and stores its sums; the replica's last workgroup
+// to finish (counter, reset by that workgroup) adds the nb sums in block
+// order, so the result does not depend on arrival order.  STEP (one RESPA outer step) also
 // reduces the bonded partials of the inner steps into the frcshort terms and
 // the per-atom 2 x kinetic energies into ukin, in the same pass.  Slots of
 // d[]: 0-3 pair (uvdw, shift, ureal, virial), 4-7 intramolecular (uvdw,
@@ -323,7 +325,7 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     const double* __restrict__ uvac, const double* __restrict__ ep_coul,
     mspms_long_terms* __restrict__ out, int* lstate, const double* __restrict__ ep_short,
     mspms_short_terms* __restrict__ sout, const double* __restrict__ kin,
-    double* __restrict__ ukin, FinBufs fb) {
+    double* __restrict__ ukin, FinBufs fb, int nb) {
 #pragma clang fp contract(off)
     constexpr int ND = STEP ? 17 : 11;
     static_assert(ND <= FIN_ND, "finalize partial slots");
@@ -331,75 +333,88 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     __shared__ unsigned long long shu[4 * (FIN_T / 64)];
     __shared__ double tot[ND];
     __shared__ unsigned long long totu[4];
+    __shared__ double lp[FIN_BLOCKS * FIN_ND];
+    __shared__ unsigned long long lq[FIN_BLOCKS * 4];
     __shared__ int last;
-    const int r = blockIdx.x / FIN_BLOCKS, b = blockIdx.x % FIN_BLOCKS;
+    const int r = blockIdx.x / nb, b = blockIdx.x % nb;
     const RepDev& R = reps[r];
-    const int t0 = b * FIN_T + threadIdx.x, nt = FIN_BLOCKS * FIN_T;
+    const int t0 = b * FIN_T + threadIdx.x, nt = nb * FIN_T;
     double d[ND];
 #pragma unroll
     for (int k = 0; k < ND; k++) d[k] = 0.0;
     unsigned long long q[4] = {0, 0, 0, 0};
-    if (have_pair) {
-        const int c0 = col_cl0[R.col0], c1 = col_cl0[R.col0 + R.ncol];
-        for (int c = c0 + t0; c < c1; c += nt) {
-            const double4 e = reinterpret_cast<const double4*>(ep_pair)[c];
-            const ulonglong4 n = reinterpret_cast<const ulonglong4*>(ip_pair)[c];
-            d[0] += e.x;
-            d[1] += e.y;
-            d[2] += e.z;
-            d[3] += e.w;
-            q[0] += n.x;
-            q[1] += n.y;
-            q[2] += n.z;
-            q[3] += n.w;
+    const int c0 = have_pair ? col_cl0[R.col0] : 0;
+    const int ncl = have_pair ? col_cl0[R.col0 + R.ncol] - c0 : 0;
+    const int nmi = have_intra ? R.nmol : 0, nkr = have_recip ? R.nk : 0;
+    const int nsf = have_sf ? R.natom : 0;
+    const int nco = ep_coul && R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB ? R.nmol : 0;
+    const int nsh = STEP ? R.nmol : 0, nki = STEP && kin ? R.natom : 0;
+    const int nmax = max(max(max(ncl, nmi), max(nkr, nsf)), max(nco, max(nsh, nki)));
+    const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int i = t0; i < nmax; i += nt) {
+        // the loads of every kind first (one memory round trip per pass)
+        const double4 e = i < ncl ? reinterpret_cast<const double4*>(ep_pair)[c0 + i] : z4;
+        const ulonglong4 n = i < ncl ? reinterpret_cast<const ulonglong4*>(ip_pair)[c0 + i]
+                                     : make_ulonglong4(0, 0, 0, 0);
+        const double4 w = i < nmi ? reinterpret_cast<const double4*>(ep_intra)[R.mol0 + i] : z4;
+        const double vk = i < nkr ? uk[R.k0 + i] : 0.0;
+        const double vs = i < nsf ? ep_sf[R.atom0 + i] : 0.0;
+        const double vc = i < nco ? ep_coul[R.mol0 + i] : 0.0;
+        double vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (i < nsh)
+            for (int k = 0; k < 5; k++) vb[k] = ep_short[(size_t)(R.mol0 + i) * 5 + k];
+        const double vt = i < nki ? kin[R.atom0 + i] : 0.0;
+        d[0] += e.x;
+        d[1] += e.y;
+        d[2] += e.z;
+        d[3] += e.w;
+        q[0] += n.x;
+        q[1] += n.y;
+        q[2] += n.z;
+        q[3] += n.w;
+        d[4] += w.x;
+        d[5] += w.y;
+        d[6] += w.z;
+        d[7] += w.w;
+        d[8] += vk;
+        d[9] += vs;
+        d[10] += vc;
+        if (STEP) {
+            for (int k = 0; k < 5; k++) d[11 + k] += vb[k];
+            d[ND - 1] += vt;
         }
-    }
-    if (have_intra)
-        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt) {
-            const double4 e = reinterpret_cast<const double4*>(ep_intra)[m];
-            d[4] += e.x;
-            d[5] += e.y;
-            d[6] += e.z;
-            d[7] += e.w;
-        }
-    if (have_recip)
-        for (int k = R.k0 + t0; k < R.k0 + R.nk; k += nt) d[8] += uk[k];
-    if (have_sf)
-        for (int a = R.atom0 + t0; a < R.atom0 + R.natom; a += nt) d[9] += ep_sf[a];
-    if (ep_coul && R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB)
-        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt) d[10] += ep_coul[m];
-    if (STEP) {
-        for (int m = R.mol0 + t0; m < R.mol0 + R.nmol; m += nt)
-            for (int k = 0; k < 5; k++) d[11 + k] += ep_short[(size_t)m * 5 + k];
-        for (int a = R.atom0 + t0; a < R.atom0 + R.natom; a += nt) d[ND - 1] += kin[a];
     }
     block_sum<ND>(d, sh);
     block_sum_u64<4>(q, shu);
     if (threadIdx.x == 0) {
-        double* P = fb.part + ((size_t)r * FIN_BLOCKS + b) * FIN_ND;
-        unsigned long long* Q = fb.partu + ((size_t)r * FIN_BLOCKS + b) * 4;
+        double* P = fb.part + ((size_t)r * nb + b) * FIN_ND;
+        unsigned long long* Q = fb.partu + ((size_t)r * nb + b) * 4;
         for (int k = 0; k < ND; k++) P[k] = d[k];
         for (int k = 0; k < 4; k++) Q[k] = q[k];
         __threadfence();
-        last = atomicAdd(&fb.count[r], 1) == FIN_BLOCKS - 1;
+        last = atomicAdd(&fb.count[r], 1) == nb - 1;
     }
     __syncthreads();
     if (!last) return;
-    // the replica's last workgroup: the FIN_BLOCKS sums in block order, read
-    // past the (non-coherent) vector L1
+    // the replica's last workgroup: the nb sums, read past the
+    // (non-coherent) vector L1 by all threads at once into LDS, then added
+    // in block order
     __threadfence();
+    for (int t = threadIdx.x; t < nb * FIN_ND; t += FIN_T)
+        lp[t] = __hip_atomic_load(fb.part + (size_t)r * nb * FIN_ND + t, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    for (int t = threadIdx.x; t < nb * 4; t += FIN_T)
+        lq[t] = __hip_atomic_load(fb.partu + (size_t)r * nb * 4 + t, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     if (threadIdx.x < ND) {
         double s = 0.0;
-        for (int bb = 0; bb < FIN_BLOCKS; bb++)
-            s += __hip_atomic_load(fb.part + ((size_t)r * FIN_BLOCKS + bb) * FIN_ND + threadIdx.x,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int bb = 0; bb < nb; bb++) s += lp[bb * FIN_ND + threadIdx.x];
         tot[threadIdx.x] = s;
     } else if (threadIdx.x >= 64 && threadIdx.x < 68) {
         const int k = threadIdx.x - 64;
         unsigned long long s = 0;
-        for (int bb = 0; bb < FIN_BLOCKS; bb++)
-            s += __hip_atomic_load(fb.partu + ((size_t)r * FIN_BLOCKS + bb) * 4 + k,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int bb = 0; bb < nb; bb++) s += lq[bb * 4 + k];
         totu[k] = s;
     }
     __syncthreads();
@@ -459,7 +474,7 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
         u.uintra = u.ubond + u.uangle + u.udih + u.uimp;  // src/frcshort.c:87
         u.virial_intra = d[15];
         sout[r] = u;
-        ukin[r] = d[ND - 1] * 0.5;
+        if (ukin) ukin[r] = d[ND - 1] * 0.5;
     }
 }
 
@@ -712,28 +727,32 @@ void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species, const int4
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
                   const double* x, const double* y, const double* z, double* fx, double* fy,
                   double* fz, double* epart, double* gq, double* ge, double* gs,
-                  const double* frec) {
+                  const double* frec, const Kick* kick) {
     int g = (c.nmol + 255) / 256;
+    const Kick K = kick ? *kick : Kick{nullptr, nullptr, nullptr, nullptr, 0.0};
     if (gq)
         hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps,
                            c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{gq, ge, gs}, c.natom, frec);
+                           GradOut{gq, ge, gs}, c.natom, frec, K);
     else
         hipLaunchKernelGGL(k_intra<false>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
                            c.reps, c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{nullptr, nullptr, nullptr}, c.natom, frec);
+                           GradOut{nullptr, nullptr, nullptr}, c.natom, frec, K);
 }
 
 // fx += frec (3, N): the reciprocal forces where no k_intra runs
 __global__ void __launch_bounds__(256) k_add_frec(int n, double* __restrict__ fx,
                                                   double* __restrict__ fy,
                                                   double* __restrict__ fz,
-                                                  const double* __restrict__ frec) {
+                                                  const double* __restrict__ frec,
+                                                  const int* __restrict__ atom_site,
+                                                  const SiteDev* __restrict__ sites, Kick K) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     fx[i] += frec[i];
     fy[i] += frec[(size_t)n + i];
     fz[i] += frec[2 * (size_t)n + i];
+    if (K.vx) kick_atom(K, i, sites[atom_site[i]].aw, fx, fy, fz);
 }
 
 __global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) {
@@ -742,9 +761,10 @@ __global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) 
 }
 
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
-                     const double* frec) {
+                     const double* frec, const Kick* kick) {
+    const Kick K = kick ? *kick : Kick{nullptr, nullptr, nullptr, nullptr, 0.0};
     hipLaunchKernelGGL(k_add_frec, dim3((c.natom + 255) / 256), dim3(256), 0, c.stream, c.natom,
-                       fx, fy, fz, frec);
+                       fx, fy, fz, frec, c.atom_site, c.sites, K);
 }
 
 // zero fill by a kernel (capturable; no memset nodes in the step graph)
@@ -767,17 +787,17 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* 
                           const double* uvac, const double* ep_coul, void* terms,
                           int* lstate, const double* ep_short, void* sterms,
                           const double* kin, double* ukin, const FinBufs& fb) {
-    const dim3 grid(c.nrep * FIN_BLOCKS);
+    const dim3 grid(c.nrep * fb.nb);
     if (ep_short)
         hipLaunchKernelGGL(k_finalize<true>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
                            ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
                            ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
-                           ep_short, (mspms_short_terms*)sterms, kin, ukin, fb);
+                           ep_short, (mspms_short_terms*)sterms, kin, ukin, fb, fb.nb);
     else
         hipLaunchKernelGGL(k_finalize<false>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
                            ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
                            ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
-                           nullptr, nullptr, nullptr, nullptr, fb);
+                           nullptr, nullptr, nullptr, nullptr, fb, fb.nb);
 }
 
 int coulomb_mol_supported(int max_mol_atoms) { return max_mol_atoms <= 16; }

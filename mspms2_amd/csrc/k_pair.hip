@@ -24,6 +24,7 @@
 //          polynomial erfc; FP64 accumulation of forces, energies, virial.
 #include "engine.h"
 #include "physics.h"
+#include "wave.h"
 #include "../../include/mspms_gpu.h"
 
 namespace {
@@ -33,23 +34,6 @@ __device__ inline uint64_t splitmix64(uint64_t x) {
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
     return x ^ (x >> 31);
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline double sum8(double v) {
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    return v;
 }
 
 // pair-set hash key of (i, j), replica-local atom ids (the numbering a
@@ -197,41 +181,25 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
     }
 
     // per-atom partials: 8 j-lanes of each i, then the WPC waves in order
-    afx = sum8(afx);
-    afy = sum8(afy);
-    afz = sum8(afz);
+    double fr[6] = {afx, afy, afz, g_q, g_e, g_s};
     if (GRAD) {
-        g_q = sum8(g_q);
-        g_e = sum8(g_e);
-        g_s = sum8(g_s);
+        butterfly<double, 6>(fr, 4);
+    } else {
+        double f3[3] = {afx, afy, afz};
+        butterfly<double, 3>(f3, 4);
+        for (int d = 0; d < 3; d++) fr[d] = f3[d];
     }
-    if (jsub == 0) {
-        sh_f[w][isub][0] = afx;
-        sh_f[w][isub][1] = afy;
-        sh_f[w][isub][2] = afz;
-        sh_f[w][isub][3] = g_q;
-        sh_f[w][isub][4] = g_e;
-        sh_f[w][isub][5] = g_s;
-    }
-    e_lj = wave_sum(e_lj);
-    e_sh = wave_sum(e_sh);
-    e_re = wave_sum(e_re);
-    vir = wave_sum(vir);
-    n_lj = wave_sum_u64(n_lj);
-    n_el = wave_sum_u64(n_el);
-    if (want_hash) {
-        h_lj = wave_sum_u64(h_lj);
-        h_el = wave_sum_u64(h_el);
-    }
+    if (jsub == 0)
+        for (int d = 0; d < 6; d++) sh_f[w][isub][d] = fr[d];
+    double er[4] = {e_lj, e_sh, e_re, vir};
+    unsigned long long nr[4] = {n_lj, n_el, h_lj, h_el};
+    butterfly<double, 4>(er, 32);
+    butterfly<unsigned long long, 4>(nr, 32);
     if (lane == 0) {
-        sh_e[w][0] = e_lj;
-        sh_e[w][1] = e_sh;
-        sh_e[w][2] = e_re;
-        sh_e[w][3] = vir;
-        sh_n[w][0] = n_lj;
-        sh_n[w][1] = n_el;
-        sh_n[w][2] = h_lj;
-        sh_n[w][3] = h_el;
+        for (int k = 0; k < 4; k++) {
+            sh_e[w][k] = er[k];
+            sh_n[w][k] = nr[k];
+        }
     }
     __syncthreads();
     const int t = threadIdx.x;
@@ -418,35 +386,22 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
         pfz += fsum * rz;
     }
     flush();
-    unsigned long long n_lj = c_lj, n_el = c_el;
-
-    afx = sum8(afx);
-    afy = sum8(afy);
-    afz = sum8(afz);
+    double fr[3] = {afx, afy, afz};
+    butterfly<double, 3>(fr, 4);
     if (jsub == 0) {
-        sh_f[w][isub][0] = afx;
-        sh_f[w][isub][1] = afy;
-        sh_f[w][isub][2] = afz;
+        sh_f[w][isub][0] = fr[0];
+        sh_f[w][isub][1] = fr[1];
+        sh_f[w][isub][2] = fr[2];
     }
-    e_lj = wave_sum(e_lj);
-    e_sh = wave_sum(e_sh);
-    e_re = wave_sum(e_re);
-    vir = wave_sum(vir);
-    n_lj = wave_sum_u64(n_lj);
-    n_el = wave_sum_u64(n_el);
-    if (want_hash) {
-        h_lj = wave_sum_u64(h_lj);
-        h_el = wave_sum_u64(h_el);
-    }
+    double er[4] = {e_lj, e_sh, e_re, vir};
+    unsigned long long nr[4] = {c_lj, c_el, h_lj, h_el};
+    butterfly<double, 4>(er, 32);
+    butterfly<unsigned long long, 4>(nr, 32);
     if (lane == 0) {
-        sh_e[w][0] = e_lj;
-        sh_e[w][1] = e_sh;
-        sh_e[w][2] = e_re;
-        sh_e[w][3] = vir;
-        sh_n[w][0] = n_lj;
-        sh_n[w][1] = n_el;
-        sh_n[w][2] = h_lj;
-        sh_n[w][3] = h_el;
+        for (int k = 0; k < 4; k++) {
+            sh_e[w][k] = er[k];
+            sh_n[w][k] = nr[k];
+        }
     }
     __syncthreads();
     const int t = threadIdx.x;

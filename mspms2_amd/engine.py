@@ -28,6 +28,7 @@ EXPORTS = [
     "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
     "mspms_autotune", "mspms_set_launch_shape", "mspms_hmc_init", "mspms_hmc_run_device",
     "mspms_hmc_stats", "mspms_hmc_draw", "mspms_set_list_skin", "mspms_list_builds",
+    "mspms_nvt_set", "mspms_nvt_get", "mspms_nvt_run_device",
 ]
 
 
@@ -82,6 +83,10 @@ def lib():
     L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
     L.mspms_set_launch_shape.argtypes = [vp, C.c_double, C.c_int]
     L.mspms_set_list_skin.argtypes = [vp, C.c_double]
+    L.mspms_nvt_set.argtypes = [vp, C.POINTER(abi.NvtState)]
+    L.mspms_nvt_get.argtypes = [vp, C.POINTER(abi.NvtState)]
+    L.mspms_nvt_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp,
+                                       vp, C.c_int, vp]
     L.mspms_list_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
@@ -308,6 +313,39 @@ class Engine:
                                           int(nstep_inner), int(nmd), int(nmove), p(lterms),
                                           p(sterms), p(ukin), 1 if graph else 0, stream),
                "mspms_hmc_run_device")
+
+    def nvt_set(self, qq, qqs, treq, nfree, ss=0.0, ps=0.0, sss=0.0, pss=0.0):
+        """Nose-Hoover thermostats of every replica (init_nvt's start when
+        the positions and momenta are left at zero); scalars or per-replica
+        sequences; gg = ggs = nfree."""
+        n = self.nrep
+        arr = (abi.NvtState * n)()
+        b = lambda v: np.broadcast_to(np.asarray(v, dtype=np.float64), n)  # noqa: E731
+        cols = dict(qq=b(qq), qqs=b(qqs), treq=b(treq), ss=b(ss), ps=b(ps), sss=b(sss),
+                    pss=b(pss), nf=np.broadcast_to(np.asarray(nfree), n))
+        for r in range(n):
+            st = arr[r]
+            st.qq, st.qqs, st.treq = cols["qq"][r], cols["qqs"][r], cols["treq"][r]
+            st.gg = st.ggs = float(cols["nf"][r])
+            st.nfree = int(cols["nf"][r])
+            st.ss, st.ps, st.sss, st.pss = cols["ss"][r], cols["ps"][r], cols["sss"][r], \
+                cols["pss"][r]
+        _check(lib().mspms_nvt_set(self._h, arr), "mspms_nvt_set")
+
+    def nvt_get(self):
+        arr = (abi.NvtState * self.nrep)()
+        _check(lib().mspms_nvt_get(self._h, arr), "mspms_nvt_get")
+        return [a.as_dict() for a in arr]
+
+    def nvt_run_device(self, pos, vel, fl, fs, delt, nstep_inner, nsteps, lterms, sterms, ukin,
+                       graph=True, stream=None):
+        """nsteps vver_nh_3 steps of every replica on (3, N) device tensors;
+        fl / fs must hold the forces of pos."""
+        p = self._ptr
+        _check(lib().mspms_nvt_run_device(self._h, p(pos), p(vel), p(fl), p(fs), float(delt),
+                                          int(nstep_inner), int(nsteps), p(lterms), p(sterms),
+                                          p(ukin), 1 if graph else 0, stream),
+               "mspms_nvt_run_device")
 
     def hmc_stats(self):
         n = self.nrep

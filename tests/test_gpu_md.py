@@ -1,0 +1,156 @@
+"""Device integrators against the reference's own trajectories
+(tests/golden/md_*: its compiled integrators run from velinit's
+velocities): Nose-Hoover NVT (mspms_nvt_run_device, vver_nh_3) and NVE
+RESPA (mspms_vver_run_device, vver); NVT replica batches against the CPU
+restatement (tests/md_cpu.py).
+
+Tolerances: FP64 mode, ||dx|| / ||x|| and ||dv|| / ||v|| <= 1e-9 after 1 and
+10 steps and <= 1e-7 after 100; thermostat momenta within 1e-8 of their
+scale.  The device adds the sums over atoms in a tree order instead of left
+to right (1e-16 relative per sum), and a sum that lands within that of the
+Newton iteration's 1e-10 stopping test can take one pass more or less,
+which moves the thermostat momentum by up to ~1e-10 of itself; the growth of
+such differences over the trajectory sets the bounds.  Mixed mode: 1e-5
+after 10 steps."""
+import os
+
+import numpy as np
+import pytest
+
+from mspms2_amd import configs, runio
+from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+from mspms2_amd.system import System
+from tests.md_cpu import NvtCpu
+from tests.util import GOLDEN, system_of
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    d = float(np.linalg.norm(np.asarray(a) - b))
+    n = float(np.linalg.norm(b))
+    return d / n if n > 0 else d
+
+
+def _load(name, n):
+    path = os.path.join(GOLDEN, name)
+    rd = runio.RunDir.read(path)
+    return rd, System.from_rundir(rd), np.load(os.path.join(path, f"md{n}.npz"))
+
+
+def _run(eng, s, rd, v0, nsteps, graph, pos0=None):
+    import torch
+    dev = torch.device("cuda:0")
+    x0 = np.stack([s.xx, s.yy, s.zz]) if pos0 is None else pos0
+    f0, _ = eng.frclong(*x0)
+    fs0, _ = eng.frcshort(*x0)
+    pos = torch.tensor(x0, dtype=torch.float64, device=dev).contiguous()
+    vel = torch.tensor(v0, dtype=torch.float64, device=dev).contiguous()
+    fl = torch.tensor(f0, dtype=torch.float64, device=dev).contiguous()
+    fs = torch.tensor(fs0, dtype=torch.float64, device=dev).contiguous()
+    lt = torch.zeros(eng.nrep * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(eng.nrep * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(eng.nrep, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    eng.nvt_run_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, nsteps, lt, st, uk,
+                       graph=graph)
+    torch.cuda.synchronize()
+    eng.sync()
+    return pos.cpu().numpy(), vel.cpu().numpy(), fl.cpu().numpy(), uk.cpu().numpy()
+
+
+@pytest.mark.parametrize("name,n", [("md_lj864_nvt", 1), ("md_lj864_nvt", 10),
+                                    ("md_lj864_nvt", 100), ("md_water216_nvt", 1),
+                                    ("md_water216_nvt", 10)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_fp64_matches_reference_trajectory(name, n, graph):
+    rd, s, g = _load(name, n)
+    print(f"case {name} steps={n} graph={graph} natom={s.natom}", flush=True)
+    eng = Engine(s, precision="fp64")
+    nfree = 3 * s.natom - rd.run.nconstraint
+    eng.nvt_set(rd.run.qq, rd.run.qqs, s.treq, nfree)
+    x, v, fl, uk = _run(eng, s, rd, g["v0"], n, graph)
+    tol = 1e-7 if n >= 100 else 1e-9
+    ex, ev = rel(x, g["x"]), rel(v, g["v"])
+    print(f"  x {ex:.2e} v {ev:.2e} fl {rel(fl, g['fl']):.2e}")
+    assert ex <= tol and ev <= tol
+    assert rel(fl, g["fl"]) <= 100 * tol
+    st = eng.nvt_get()[0]
+    for k in ("ps", "pss"):
+        ref = float(g[f"md_{k}"])
+        scale = max(abs(ref), 1e-3)
+        assert abs(st[k] - ref) <= 10 * tol * scale, (k, st[k], ref)
+    assert abs(st["ukin"] - float(g["md_ukin"])) <= tol * abs(float(g["md_ukin"]))
+    assert abs(uk[0] - st["ukin"]) <= 1e-15 * abs(st["ukin"])
+    assert abs(st["tinst"] - float(g["md_tinst"])) <= tol * abs(float(g["md_tinst"]))
+    eng.close()
+
+
+def test_mixed_follows_reference_trajectory():
+    rd, s, g = _load("md_water216_nvt", 10)
+    eng = Engine(s, precision="mixed")
+    eng.nvt_set(rd.run.qq, rd.run.qqs, s.treq, 3 * s.natom - rd.run.nconstraint)
+    x, v, _, _ = _run(eng, s, rd, g["v0"], 10, True)
+    print(f"mixed x {rel(x, g['x']):.2e} v {rel(v, g['v']):.2e}")
+    assert rel(x, g["x"]) <= 1e-5 and rel(v, g["v"]) <= 1e-5
+    eng.close()
+
+
+def test_replica_batch_matches_per_replica_restatement():
+    """Three LJ boxes of different sizes, seeds and temperatures in one
+    table: every replica's chain equals the CPU restatement of its own."""
+    boxes = [configs.lj_fcc(ncell=3, jitter=0.05, seed=11, temp=0.72),
+             configs.lj_fcc(ncell=4, jitter=0.05, seed=12, temp=1.1),
+             configs.lj_fcc(ncell=3, jitter=0.04, seed=13, temp=0.9)]
+    systems = [system_of(b) for b in boxes]
+    rng = np.random.default_rng(3)
+    v0s = [rng.standard_normal((3, s.natom)) * np.sqrt(s.treq / s.masses()) for s in systems]
+    nsteps = 8
+    ref = []
+    for b, s, v0 in zip(boxes, systems, v0s):
+        c = NvtCpu(s, v0, b.run.qq, b.run.qqs, b.run.nconstraint)
+        for _ in range(nsteps):
+            c.step()
+        ref.append(c)
+    eng = Engine.replicas(systems, precision="fp64")
+    eng.nvt_set([b.run.qq for b in boxes], [b.run.qqs for b in boxes],
+                [s.treq for s in systems],
+                [3 * s.natom - b.run.nconstraint for b, s in zip(boxes, systems)])
+    s0 = systems[0]
+    x, v, _, uk = _run(eng, s0, boxes[0], eng.pack(v0s), nsteps, True,
+                       pos0=eng.packed_positions())
+    states = eng.nvt_get()
+    for r, c in enumerate(ref):
+        xr, vr = eng.unpack(x)[r], eng.unpack(v)[r]
+        print(f"replica {r}: x {rel(xr, c.x):.2e} v {rel(vr, c.v):.2e}")
+        assert rel(xr, c.x) <= 1e-9 and rel(vr, c.v) <= 1e-9
+        assert abs(states[r]["ps"] - c.ps) <= 1e-8 * max(abs(c.ps), 1e-3)
+        assert abs(uk[r] - c.ukin) <= 1e-9 * c.ukin
+    eng.close()
+
+
+@pytest.mark.parametrize("n", [1, 10])
+def test_nve_fp64_matches_reference_trajectory(n):
+    """The device NVE RESPA step (mspms_vver_run_device) against the
+    reference's own vver trajectory (tests/golden/md_water216_nve)."""
+    import torch
+    rd, s, g = _load("md_water216_nve", n)
+    eng = Engine(s, precision="fp64")
+    dev = torch.device("cuda:0")
+    x0 = np.stack([s.xx, s.yy, s.zz])
+    f0, _ = eng.frclong(*x0)
+    fs0, _ = eng.frcshort(*x0)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    pos, vel, fl, fs = t(x0), t(g["v0"]), t(f0), t(fs0)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(1, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    eng.vver_run_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, n, lt, st, uk, graph=True)
+    torch.cuda.synchronize()
+    eng.sync()
+    x, v = pos.cpu().numpy(), vel.cpu().numpy()
+    print(f"nve {n}: x {rel(x, g['x']):.2e} v {rel(v, g['v']):.2e}")
+    assert rel(x, g["x"]) <= 1e-9 and rel(v, g["v"]) <= 1e-9
+    assert abs(uk.item() - float(g["md_ukin"])) <= 1e-9 * float(g["md_ukin"])
+    eng.close()

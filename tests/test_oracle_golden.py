@@ -20,7 +20,8 @@ from oracle import oracle as O
 from tests.util import GOLDEN
 
 SMALL = sorted(d for d in os.listdir(GOLDEN)
-               if os.path.isfile(os.path.join(GOLDEN, d, "in.mspms")))
+               if os.path.isfile(os.path.join(GOLDEN, d, "in.mspms"))
+               and os.path.isfile(os.path.join(GOLDEN, d, "ref.npz")))
 LONG_KEYS = ("uvdw", "ushift", "ureal", "uexcl", "ufourier", "uself", "uelec", "usflj",
              "uotherff", "uinter", "virial_inter")
 SHORT_KEYS = ("ubond", "uangle", "udih", "uimp", "uintra", "virial_intra")
