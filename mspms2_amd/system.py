@@ -138,6 +138,7 @@ class System:
     time_base: float = 1.0
     delt: float = 0.0
     treq: float = 0.0
+    preq: float = 0.0                # reduced pressure (src/readins.c:38-47)
     grid: Optional[dict] = None      # SF_TASOS grid (mspms_sf_grid fields)
     _keep: list = field(default_factory=list, repr=False)
 
@@ -180,8 +181,10 @@ class System:
                      tube_r=np.array([t[2] / sb for t in r.tubes]),
                      const_3pisq_theta=C3_PI_SQ_THETA * sb * sb,
                      solid_sigma=r.tube_sigma / sb, solid_epsilon=r.tube_epsilon / eb)
+        pressure_base = BOLTZMAN_CONSTANT * eb * 1.0e25 / (sb * sb * sb)
         s = System(species=species, params=p, xx=xx, yy=yy, zz=zz, run=r,
-                   time_base=time_base, delt=r.delt / time_base, treq=r.treq / eb)
+                   time_base=time_base, delt=r.delt / time_base, treq=r.treq / eb,
+                   preq=r.preq / pressure_base)
         return s
 
     # ------------------------------------------------------------------ #

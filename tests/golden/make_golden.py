@@ -69,6 +69,11 @@ MD_CASES = {
     "md_water216_nvt": (lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
                                                   kmax=5, ksqmax=27, seed=3, nstep_inner=4,
                                                   ensemble=1), (1, 10)),
+    # NPT (npt_respa): LJ with the tail correction, and the water
+    "md_lj864_npt": (lambda: configs.lj_fcc(jitter=0.05, seed=7, ensemble=2), (1, 10)),
+    "md_water216_npt": (lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
+                                                  kmax=5, ksqmax=27, seed=3, nstep_inner=4,
+                                                  ensemble=2), (1, 5)),
     # the same water, NVE (vver)
     "md_water216_nve": (lambda: configs.water_box(n_side=6, L=6 * 3.104, rc=9.0, kappa=0.3,
                                                   kmax=5, ksqmax=27, seed=3, nstep_inner=4,
@@ -81,6 +86,7 @@ def save_md(path, d, nstep):
     md = {f"md_{k}": np.float64(v) for k, v in d["md"].items()}
     np.savez_compressed(os.path.join(path, f"md{nstep}.npz"),
                         x=np.stack([d["xx"], d["yy"], d["zz"]]), v=d["v"], v0=d["v0"],
+                        box=np.array([d["boxlx"], d["boxly"], d["boxlz"]]),
                         fl=d["fl"], fs=d["fs"], natom=d["natom"],
                         **{f"s_{k}": v for k, v in sc.items()}, **md)
 

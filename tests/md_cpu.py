@@ -11,6 +11,8 @@ tests the atom velocities, but with the loop index left at natom
 zero, so 0/0 compares false and only the thermostat test decides."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from oracle.oracle import Oracle
@@ -141,3 +143,136 @@ class NveCpu:
         self.v = v
         self.ukin = seqsum((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / rm) * 0.5
         self.tinst = 2.0 * self.ukin / self.nfree
+
+
+PI = 3.141592653589793  # src/defs.h:98
+
+
+def ljlrc_pressure_terms(system) -> np.ndarray:
+    """pljlrc_term[m][n] of InitLJlrcCommonTerms (src/initvars.c:474-521)."""
+    s = system
+    rc = s.params["rcutoff"]
+    t1 = (32.0 / 9.0) * PI * rc ** -9.0
+    t2 = -(16.0 / 3.0) * PI * rc ** -3.0
+    ns = len(s.species)
+    out = np.zeros((ns, ns))
+    for m, a in enumerate(s.species):
+        for n, b in enumerate(s.species):
+            acc = 0.0
+            for i in range(a.natom):
+                for j in range(b.natom):
+                    sij = 0.5 * (a.sigma[i] + b.sigma[j])
+                    eij = math.sqrt(a.epsilon[i] * b.epsilon[j])
+                    temp2 = eij * sij ** 3.0
+                    temp3 = sij ** 9.0 * t1 + sij ** 3.0 * t2
+                    acc += temp3 * temp2
+            out[m, n] = acc
+    return out
+
+
+class NptCpu:
+    """The reference's NPT RESPA step, npt_respa with npt_nh_operator
+    (src/nptnh.c:10-234; set-up init_npt_respa, src/initvars.c:351-396),
+    on the oracle, whose box-dependent parameters follow the box as the
+    reference's globals do (src/nptnh.c:180-199)."""
+
+    def __init__(self, system, v0, Qts: float, Qbs: float, nconstraint: int = 3):
+        s = system
+        self.s = s
+        self.o = Oracle(s)
+        self.rm = 1.0 / s.masses()
+        self.treq, self.preq = s.treq, s.preq
+        self.natom = s.natom
+        self.nfree = 3 * s.natom - nconstraint
+        self.Qts, self.Qbs = Qts, Qbs
+        self.vts = math.sqrt((self.nfree + 1.0) / Qts)
+        self.vbs = math.sqrt((self.nfree + 1.0) / Qbs)
+        self.rts = 0.0
+        self.box = [s.params["boxlx"], s.params["boxly"], s.params["boxlz"]]
+        self.boxv = self.box[0] * self.box[1] * self.box[2]
+        self.nmole = [sp.nmole for sp in s.species]
+        self.pterm = ljlrc_pressure_terms(s) if s.run.isLJlrcOn else np.zeros(
+            (len(s.species), len(s.species)))
+        self.pljlrc = self._ljlrc()
+        delt, nin = s.delt, s.run.nstep_inner
+        self.nstep_inner = nin
+        self.deltby2 = delt / 2.0
+        self.delts = delt / nin
+        self.deltsby2 = self.delts / 2.0
+        self.dt_outer2, self.dt_outer4, self.dt_outer8 = self.deltby2, delt / 4.0, delt / 8.0
+        self.ewald = s.params["iChargeType"] == 1
+        self.x = np.stack([s.xx, s.yy, s.zz]).astype(np.float64)
+        self.v = np.array(v0, dtype=np.float64)
+        self.fl, self.tl = self.o.frclong(*self.x)
+        self.fs, self.ts = self.o.frcshort(*self.x)
+        self.utsbs = self.ukin = self.tinst = self.pinst = 0.0
+
+    def _ljlrc(self):
+        """calculate_ljlrc's pressure part (src/misc.c:131-148)"""
+        p = 0.0
+        for m in range(len(self.nmole)):
+            for n in range(len(self.nmole)):
+                p += self.pterm[m, n] * self.nmole[m] * self.nmole[n] / self.boxv / self.boxv
+        return p
+
+    def op(self):
+        """npt_nh_operator (src/nptnh.c:10-130)"""
+        rm, treq, nfree = self.rm, self.treq, self.nfree
+        v = self.v
+        N_plus1RT = (nfree + 1) * treq
+        one_3N = 1.0 + 3.0 / nfree
+        mvsq = seqsum((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / rm)
+        pdiff = self.tl["virial_inter"] + self.ts["virial_intra"] + \
+            self.pljlrc * self.boxv * 3.0 - self.preq * self.boxv * 3.0
+        Gts = (mvsq + self.Qbs * self.vbs * self.vbs - N_plus1RT) / self.Qts
+        self.vts = self.vts + self.dt_outer4 * Gts
+        BB = math.exp(-self.vts * self.dt_outer8)
+        self.vbs = self.vbs * BB
+        Gbs = (one_3N * mvsq + pdiff) / self.Qbs
+        self.vbs = self.vbs + self.dt_outer4 * Gbs
+        self.vbs = self.vbs * BB
+        self.rts = self.rts + self.deltby2 * self.vts
+        AA = math.exp(-self.dt_outer2 * (self.vts + one_3N * self.vbs))
+        self.v = v * AA
+        self.vbs = self.vbs * BB
+        mvsq = mvsq * AA * AA
+        Gbs = (one_3N * mvsq + pdiff) / self.Qbs
+        self.vbs = self.vbs + self.dt_outer4 * Gbs
+        self.vbs = self.vbs * BB
+        Gts = (mvsq + self.Qbs * self.vbs * self.vbs - N_plus1RT) / self.Qts
+        self.vts = self.vts + self.dt_outer4 * Gts
+        self.utsbs = 0.5 * self.Qbs * self.vbs * self.vbs + 0.5 * self.Qts * self.vts * \
+            self.vts + (nfree + 1) * treq * self.rts + self.preq * self.boxv
+        self.ukin = mvsq / 2.0
+        self.tinst = 2.0 * self.ukin / nfree
+        pideal = self.natom / self.boxv * self.tinst
+        self.pinst = pideal + (self.tl["virial_inter"] + self.ts["virial_intra"]) / 3.0 / \
+            self.boxv + self.pljlrc
+
+    def step(self):
+        rm = self.rm
+        self.op()
+        v = self.v + self.deltby2 * self.fl * rm
+        for _ in range(self.nstep_inner):
+            AA = math.exp(self.deltsby2 * self.vbs)
+            v = v + self.deltsby2 * self.fs * rm
+            x = self.x * AA
+            x = x + self.delts * v
+            self.x = x * AA
+            expfactor = math.exp(self.delts * self.vbs)
+            self.box = [b * expfactor for b in self.box]
+            self.boxv = self.box[0] * self.box[1] * self.box[2]
+            self.pljlrc = self._ljlrc()
+            p = self.o.p
+            p.boxlx, p.boxly, p.boxlz = self.box
+            if self.ewald:
+                p.vfactor_ewald = 2.0 * PI / self.boxv
+                p.twopi_lx = 2.0 * PI / self.box[0]
+                p.twopi_ly = 2.0 * PI / self.box[1]
+                p.twopi_lz = 2.0 * PI / self.box[2]
+                p.twopi_over_3v = 2.0 * PI / 3.0 / self.boxv
+            self.fs, self.ts = self.o.frcshort(*self.x)
+            v = v + self.deltsby2 * self.fs * rm
+        self.fl, self.tl = self.o.frclong(*self.x)
+        self.v = v + self.deltby2 * self.fl * rm
+        self.op()

@@ -12,11 +12,13 @@ import pytest
 
 from mspms2_amd import runio
 from mspms2_amd.system import System
-from tests.md_cpu import NveCpu, NvtCpu
+from tests.md_cpu import NptCpu, NveCpu, NvtCpu
 from tests.util import GOLDEN
 
 CASES = [("md_lj864_nvt", 1), ("md_lj864_nvt", 10), ("md_lj864_nvt", 100),
          ("md_water216_nvt", 1), ("md_water216_nvt", 10),
+         ("md_lj864_npt", 1), ("md_lj864_npt", 10),
+         ("md_water216_npt", 1), ("md_water216_npt", 5),
          ("md_water216_nve", 1), ("md_water216_nve", 10)]
 
 
@@ -39,6 +41,8 @@ def test_restatement_matches_reference_trajectory(name, n):
     rd, s, g = load(name, n)
     if rd.run.what_ensemble == 1:
         c = NvtCpu(s, g["v0"], rd.run.qq, rd.run.qqs, rd.run.nconstraint)
+    elif rd.run.what_ensemble == 2:
+        c = NptCpu(s, g["v0"], rd.run.Qts, rd.run.Qbs, rd.run.nconstraint)
     else:
         c = NveCpu(s, g["v0"], rd.run.nconstraint)
     for _ in range(n):
@@ -51,6 +55,11 @@ def test_restatement_matches_reference_trajectory(name, n):
     assert abs(c.ukin - g["md_ukin"]) <= tol * abs(g["md_ukin"])
     if rd.run.what_ensemble == 1:
         for k in ("ps", "pss", "ss", "sss", "unhts", "unhtss"):
+            ref = float(g[f"md_{k}"])
+            assert abs(getattr(c, k) - ref) <= tol * max(abs(ref), 1e-300), k
+    if rd.run.what_ensemble == 2:
+        assert np.allclose(c.box, g["box"], rtol=tol, atol=0)
+        for k in ("vts", "vbs", "rts", "utsbs", "pinst", "tinst"):
             ref = float(g[f"md_{k}"])
             assert abs(getattr(c, k) - ref) <= tol * max(abs(ref), 1e-300), k
 
