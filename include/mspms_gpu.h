@@ -414,6 +414,41 @@ int mspms_nvt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, d
                          double delt, int nstep_inner, int nsteps, mspms_long_terms* lterms,
                          mspms_short_terms* sterms, double* ukin, int use_graph, void* stream);
 
+/* NPT molecular dynamics on device buffers: the reference's npt_respa with
+ * the Nose-Hoover thermostat / barostat operator npt_nh_operator
+ * (src/nptnh.c:10-234, selected by src/mspms2.c:70-73).  The box scales by
+ * exp(delts vbs) after every RESPA inner step, and with it the replica's
+ * box-dependent tables (cell grid, Ewald weights, 2 pi / L, 2 pi / V) and,
+ * with lj_tail (the reference's isLJlrcOn), the LJ tail pressure
+ * pljlrc = sum pljlrc_term[m][n] n_m n_n / V^2 (src/misc.c:131-148).
+ * mspms_npt_set resets every replica to the box of its parameters and
+ * uploads one state per replica (Qts, Qbs, treq, preq, vts, vbs, rts,
+ * nfree, lj_tail; init_npt_respa, src/initvars.c:351-396, starts from
+ * vts = sqrt((nfree+1)/Qts), vbs = sqrt((nfree+1)/Qbs), rts = 0).
+ * mspms_npt_run_device runs nsteps steps of every replica; on entry fl /
+ * fs and lterms / sterms hold the forces and terms of pos (the barostat
+ * reads their virials), on exit those of the new positions, with ukin
+ * [nrep].  The pair lists are rebuilt every step (the box moves).
+ * mspms_npt_get downloads the states, current boxes included (outputs
+ * utsbs, the conserved quantity's thermostat / barostat part, ukin, tinst
+ * and pinst).  After a run, mspms_set_params with the new box keeps later
+ * rebuilds of the tables (launch shape, skin, parameters) consistent. */
+typedef struct mspms_npt_state {
+    double Qts, Qbs;             /* thermostat and barostat masses            */
+    double treq, preq;           /* reduced temperature and pressure          */
+    double vts, vbs, rts;        /* thermostat and barostat velocities,
+                                    thermostat position                        */
+    double pljlrc;               /* LJ tail pressure of the box (output)      */
+    double utsbs, ukin, tinst, pinst;  /* outputs                             */
+    double boxl[3], boxv;        /* current box (output)                      */
+    int nfree, lj_tail;
+} mspms_npt_state;
+int mspms_npt_set(mspms_ctx* ctx, const mspms_npt_state* states);
+int mspms_npt_get(mspms_ctx* ctx, mspms_npt_state* states);
+int mspms_npt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, double* fs,
+                         double delt, int nstep_inner, int nsteps, mspms_long_terms* lterms,
+                         mspms_short_terms* sterms, double* ukin, int use_graph, void* stream);
+
 /* Synchronises the device and reports the status accumulated by the
  * device-buffer evaluations since the previous mspms_sync (list overflow,
  * non-finite input, grid out of range, singular improper), which the

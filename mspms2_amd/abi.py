@@ -136,6 +136,19 @@ class NvtState(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class NptState(C.Structure):
+    """mspms_npt_state: one replica's NPT thermostat / barostat (npt_respa)."""
+    _fields_ = [(k, C.c_double) for k in ("Qts", "Qbs", "treq", "preq", "vts", "vbs", "rts",
+                                          "pljlrc", "utsbs", "ukin", "tinst", "pinst")] + \
+        [("boxl", C.c_double * 3), ("boxv", C.c_double), ("nfree", C.c_int),
+         ("lj_tail", C.c_int)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "boxl"}
+        d["boxl"] = list(self.boxl)
+        return d
+
+
 def dptr(a):
     """double* of a contiguous float64 numpy array (or NULL for None)."""
     if a is None:

@@ -212,6 +212,9 @@ struct mspms_ctx {
     DevBuf<mspms_nvt_state> d_nvt;
     DevBuf<double> d_nvt_carry, d_nvt_vn;
     bool nvt_ready = false;
+    // NPT (mspms_npt_set)
+    DevBuf<NptDev> d_npt;
+    bool npt_ready = false;
     struct GraphKey {
         void *pos, *vel, *fl, *fs, *lt, *st, *uk;
         double delt;
@@ -232,7 +235,7 @@ void drop_graph(mspms_ctx* c) {
     c->graph = nullptr;
 }
 
-enum { GRAPH_VVER = 0, GRAPH_HMC = 1, GRAPH_NVT = 2 };
+enum { GRAPH_VVER = 0, GRAPH_HMC = 1, GRAPH_NVT = 2, GRAPH_NPT = 3 };
 
 // Makes the cached graph the capture of enqueue(stream) for (kind, key),
 // capturing it again when either differs.
@@ -567,6 +570,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_aw_atom.release();
     c->d_pos_old.release();
     c->d_nvt.release();
+    c->d_npt.release();
     c->d_nvt_carry.release();
     c->d_nvt_vn.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
@@ -964,6 +968,7 @@ int build(mspms_ctx* c) {
             (rc = c->d_ak.ensure(kvec.size())) || (rc = c->d_uk.ensure(kvec.size())))
             return rc;
         R.rows = c->d_rows.p;
+        R.nrow = (int)rows.size();
         R.rows_s = c->d_rows_s.p;
         R.kvec = c->d_kvec.p;
         R.k_rep = c->d_k_rep.p;
@@ -1131,6 +1136,7 @@ extern "C" int mspms_set_system(mspms_ctx* c, int nspecie, const mspms_species* 
     c->params.assign(1, *p);
     c->hmc_ready = false;  // the chains belong to the previous layout
     c->nvt_ready = false;
+    c->npt_ready = false;
     return build(c);
 }
 
@@ -1159,6 +1165,7 @@ extern "C" int mspms_set_replicas(mspms_ctx* c, int nrep, const int* nmole,
     c->params.assign(params, params + nrep);
     c->hmc_ready = false;
     c->nvt_ready = false;
+    c->npt_ready = false;
     return build(c);
 }
 
@@ -1837,6 +1844,169 @@ extern "C" int mspms_nvt_run_device(mspms_ctx* c, double* pos, double* vel, doub
     }
     for (int t = 0; t < nsteps; t++) {
         int rc = enqueue_nvt(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// ---- NPT -----------------------------------------------------------------------
+namespace {
+
+// One npt_respa step (src/nptnh.c:133-234) of every replica on device (3, N)
+// buffers; lterms / sterms hold the terms of the current positions on
+// entry (their virials drive the barostat) and of the new ones on exit.
+int enqueue_npt(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs, double delt,
+                int nstep_inner, mspms_long_terms* lterms, mspms_short_terms* sterms,
+                double* ukin, hipStream_t s) {
+    const size_t n = (size_t)c->natom;
+    // src/initvars.c:604-610
+    const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
+    const double dt_outer4 = delt / 4.0, dt_outer8 = delt / 8.0;
+    LaunchCtx L = launch_ctx(c, s);
+    NptDev* st = c->d_npt.p;
+    {
+        ProfScope p(c, s, PROF_BONDED);
+        launch_npt_op(L, st, false, vel, fl, fs, lterms, sterms, deltby2, deltsby2, dt_outer4,
+                      dt_outer8, nullptr);
+        for (int ll = 0; ll < nstep_inner; ll++) {
+            launch_npt_inner(L, c->d_reps.p, st, ll == 0, pos, vel, fs, delts, deltsby2);
+            launch_bonded(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, fs, fs + n,
+                          fs + 2 * n, c->d_ep_short.p);
+        }
+    }
+    // the box changed: new Ewald weights, and the pair lists are rebuilt
+    if (c->any_recip) launch_kvec(L, c->rb, c->rb.nrow);
+    const bool force = c->force_rebuild;
+    c->force_rebuild = true;
+    int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
+                          nullptr, false);
+    c->force_rebuild = force;
+    if (rc) return rc;
+    {
+        ProfScope p(c, s, PROF_FINAL);
+        launch_finalize(c, L, lterms, sterms, nullptr);
+    }
+    launch_npt_op(L, st, true, vel, fl, fs, lterms, sterms, deltby2, deltsby2, dt_outer4,
+                  dt_outer8, ukin);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mspms_npt_set(mspms_ctx* c, const mspms_npt_state* st) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!st) return fail(MSPMS_EINVAL, "null argument");
+    const int ns = (int)c->sp.size();
+    if (ns * ns > NPT_MAX_LRC) return fail(MSPMS_EINVAL, "NPT supports at most 4 species");
+    for (int r = 0; r < c->nrep; r++)
+        if (!(st[r].Qts > 0.0) || !(st[r].Qbs > 0.0) || !(st[r].treq > 0.0) || st[r].nfree < 1)
+            return fail(MSPMS_EINVAL, "replica %d: Qts, Qbs, treq and nfree must be positive", r);
+    // every replica back to the box of its parameters (an earlier NPT run
+    // moved the device copy)
+    int rc = build(c);
+    if (rc) return rc;
+    std::vector<NptDev> h(c->nrep);
+    for (int r = 0; r < c->nrep; r++) {
+        NptDev& D = h[r];
+        D = NptDev{};
+        D.Qts = st[r].Qts;
+        D.Qbs = st[r].Qbs;
+        D.treq = st[r].treq;
+        D.preq = st[r].preq;
+        D.vts = st[r].vts;
+        D.vbs = st[r].vbs;
+        D.rts = st[r].rts;
+        D.nfree = st[r].nfree;
+        const mspms_params& P = c->params[r];
+        D.boxv = P.boxlx * P.boxly * P.boxlz;
+        if (st[r].lj_tail) {
+            // InitLJlrcCommonTerms (src/initvars.c:474-521) and the
+            // molecule-count products of calculate_ljlrc (src/misc.c:131-148)
+            const double pi = 3.141592653589793;
+            const double t1 = (32.0 / 9.0) * pi * std::pow(P.rcutoff, -9.0);
+            const double t2 = -(16.0 / 3.0) * pi * std::pow(P.rcutoff, -3.0);
+            for (int m = 0; m < ns; m++)
+                for (int nn = 0; nn < ns; nn++) {
+                    double term = 0.0;
+                    const SpeciesHost &A = c->sp[m], &B = c->sp[nn];
+                    for (int i = 0; i < A.natom; i++)
+                        for (int j = 0; j < B.natom; j++) {
+                            const double sij = 0.5 * (A.sigma[i] + B.sigma[j]);
+                            const double eij = std::sqrt(A.epsilon[i] * B.epsilon[j]);
+                            const double temp2 = eij * std::pow(sij, 3.0);
+                            const double temp3 = std::pow(sij, 9.0) * t1 + std::pow(sij, 3.0) * t2;
+                            term += temp3 * temp2;
+                        }
+                    D.lrc[D.nlrc++] = term * c->nmole[(size_t)r * ns + m] *
+                                      c->nmole[(size_t)r * ns + nn];
+                }
+            double p = 0.0;
+            for (int k = 0; k < D.nlrc; k++) p += D.lrc[k] / D.boxv / D.boxv;
+            D.pljlrc = p;
+        }
+    }
+    HIPC(hipSetDevice(c->device));
+    if ((rc = c->d_npt.upload(h, c->stream))) return rc;
+    HIPC(hipStreamSynchronize(c->stream));
+    c->npt_ready = true;
+    return 0;
+}
+
+extern "C" int mspms_npt_get(mspms_ctx* c, mspms_npt_state* st) {
+    if (!c || !c->npt_ready) return fail(MSPMS_EINVAL, "mspms_npt_set first");
+    if (!st) return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipDeviceSynchronize());
+    std::vector<NptDev> h(c->nrep);
+    std::vector<RepDev> reps(c->nrep);
+    HIPC(hipMemcpy(h.data(), c->d_npt.p, sizeof(NptDev) * c->nrep, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(reps.data(), c->d_reps.p, sizeof(RepDev) * c->nrep, hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; r++) {
+        const NptDev& D = h[r];
+        mspms_npt_state& o = st[r];
+        o.Qts = D.Qts;
+        o.Qbs = D.Qbs;
+        o.treq = D.treq;
+        o.preq = D.preq;
+        o.vts = D.vts;
+        o.vbs = D.vbs;
+        o.rts = D.rts;
+        o.pljlrc = D.pljlrc;
+        o.utsbs = D.utsbs;
+        o.ukin = D.ukin;
+        o.tinst = D.tinst;
+        o.pinst = D.pinst;
+        for (int d = 0; d < 3; d++) o.boxl[d] = reps[r].L[d];
+        o.boxv = D.boxv;
+        o.nfree = D.nfree;
+        o.lj_tail = D.nlrc > 0;
+    }
+    return 0;
+}
+
+extern "C" int mspms_npt_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                    double* fs, double delt, int nstep_inner, int nsteps,
+                                    mspms_long_terms* lterms, mspms_short_terms* sterms,
+                                    double* ukin, int use_graph, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!c->npt_ready) return fail(MSPMS_EINVAL, "mspms_npt_set first");
+    if (nstep_inner < 1 || nsteps < 0) return fail(MSPMS_EINVAL, "bad step counts");
+    if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
+        return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (use_graph) {
+        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
+        int rc = ensure_graph(c, s, GRAPH_NPT, k, [&](hipStream_t cs) {
+            return enqueue_npt(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, cs);
+        });
+        if (rc) return rc;
+        for (int t = 0; t < nsteps; t++) HIPC(hipGraphLaunch(c->graph_exec, s));
+        return 0;
+    }
+    for (int t = 0; t < nsteps; t++) {
+        int rc = enqueue_npt(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
         if (rc) return rc;
     }
     return 0;

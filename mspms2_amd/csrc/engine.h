@@ -162,6 +162,7 @@ struct RecipBufs {
     double2* ak;         // [nk_total] force coefficients
     double* uk;          // [nk_total] kvec |S|^2
     int nk_total, ntab_max, nkz_max;
+    int nrow;            // rows of all replicas
 };
 
 // solid-fluid model (shared by all replicas), passed by value to k_sf
@@ -205,6 +206,24 @@ void launch_hmc_mark(const LaunchCtx& c, HmcRep* hmc, const void* lterms, const 
 void launch_hmc_end(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const void* lterms,
                     const void* sterms, const double* ukin, double* pos, const double* pos_old);
 void launch_hmc_draw(const LaunchCtx& c, RanmarDev* rng, int rep, int n, double* out);
+
+// NPT state of one replica (k_npt.hip): the thermostat and barostat of
+// npt_nh_operator, the LJ tail-pressure coefficients and the outputs
+#define NPT_MAX_LRC 16
+struct NptDev {
+    double Qts, Qbs, treq, preq;
+    double vts, vbs, rts;
+    double pljlrc, boxv;
+    double utsbs, ukin, tinst, pinst;
+    double lrc[NPT_MAX_LRC];    // pljlrc_term[m][n] n_m n_n, (m, n) order
+    int nlrc, nfree;
+};
+void launch_npt_op(const LaunchCtx& c, NptDev* st, bool end, double* vel, const double* fl,
+                   const double* fs, const void* lterms, const void* sterms, double deltby2,
+                   double deltsby2, double dt_outer4, double dt_outer8, double* ukin);
+void launch_npt_inner(const LaunchCtx& c, RepDev* reps, NptDev* st, bool first, double* pos,
+                      double* vel, const double* fs, double delts, double deltsby2);
+void launch_kvec(const LaunchCtx& c, const RecipBufs& rb, int nrow);
 
 // Nose-Hoover NVT (k_nvt.hip), one workgroup per replica
 struct mspms_nvt_state;

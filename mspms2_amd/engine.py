@@ -28,7 +28,8 @@ EXPORTS = [
     "mspms_vver_step_device", "mspms_vver_run_device", "mspms_term_pair", "mspms_term_bonded",
     "mspms_autotune", "mspms_set_launch_shape", "mspms_hmc_init", "mspms_hmc_run_device",
     "mspms_hmc_stats", "mspms_hmc_draw", "mspms_set_list_skin", "mspms_list_builds",
-    "mspms_nvt_set", "mspms_nvt_get", "mspms_nvt_run_device",
+    "mspms_nvt_set", "mspms_nvt_get", "mspms_nvt_run_device", "mspms_npt_set", "mspms_npt_get",
+    "mspms_npt_run_device",
 ]
 
 
@@ -88,6 +89,10 @@ def lib():
     L.mspms_nvt_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp,
                                        vp, C.c_int, vp]
     L.mspms_list_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.mspms_npt_set.argtypes = [vp, C.POINTER(abi.NptState)]
+    L.mspms_npt_get.argtypes = [vp, C.POINTER(abi.NptState)]
+    L.mspms_npt_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp,
+                                       vp, C.c_int, vp]
     L.mspms_sync.argtypes = [vp]
     L.mspms_stream.argtypes = [vp]
     L.mspms_stream.restype = vp
@@ -346,6 +351,41 @@ class Engine:
                                           int(nstep_inner), int(nsteps), p(lterms), p(sterms),
                                           p(ukin), 1 if graph else 0, stream),
                "mspms_nvt_run_device")
+
+    def npt_set(self, Qts, Qbs, treq, preq, nfree, lj_tail=False, vts=None, vbs=None,
+                rts=0.0):
+        """NPT thermostat / barostat of every replica; vts / vbs default to
+        init_npt_respa's sqrt((nfree + 1) / Q) (src/initvars.c:379-381).
+        Resets every replica to the box of its parameters."""
+        n = self.nrep
+        b = lambda v: np.broadcast_to(np.asarray(v, dtype=np.float64), n)  # noqa: E731
+        qt, qb, tq, pq, rt = b(Qts), b(Qbs), b(treq), b(preq), b(rts)
+        nf = np.broadcast_to(np.asarray(nfree), n)
+        lt = np.broadcast_to(np.asarray(lj_tail), n)
+        arr = (abi.NptState * n)()
+        for r in range(n):
+            st = arr[r]
+            st.Qts, st.Qbs, st.treq, st.preq, st.rts = qt[r], qb[r], tq[r], pq[r], rt[r]
+            st.nfree = int(nf[r])
+            st.lj_tail = 1 if lt[r] else 0
+            st.vts = float(b(vts)[r]) if vts is not None else np.sqrt((nf[r] + 1.0) / qt[r])
+            st.vbs = float(b(vbs)[r]) if vbs is not None else np.sqrt((nf[r] + 1.0) / qb[r])
+        _check(lib().mspms_npt_set(self._h, arr), "mspms_npt_set")
+
+    def npt_get(self):
+        arr = (abi.NptState * self.nrep)()
+        _check(lib().mspms_npt_get(self._h, arr), "mspms_npt_get")
+        return [a.as_dict() for a in arr]
+
+    def npt_run_device(self, pos, vel, fl, fs, delt, nstep_inner, nsteps, lterms, sterms, ukin,
+                       graph=True, stream=None):
+        """nsteps npt_respa steps of every replica on (3, N) device tensors;
+        fl / fs and lterms / sterms must hold the forces and terms of pos."""
+        p = self._ptr
+        _check(lib().mspms_npt_run_device(self._h, p(pos), p(vel), p(fl), p(fs), float(delt),
+                                          int(nstep_inner), int(nsteps), p(lterms), p(sterms),
+                                          p(ukin), 1 if graph else 0, stream),
+               "mspms_npt_run_device")
 
     def hmc_stats(self):
         n = self.nrep

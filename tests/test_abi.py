@@ -54,6 +54,8 @@ int main(void) {
   S(mspms_sf_grid) S(mspms_param_grad) S(mspms_tune_result) S(mspms_nvt_state)
   O(mspms_tune_result, cand_ms) O(mspms_tune_result, cand_wpc)
   O(mspms_nvt_state, pss) O(mspms_nvt_state, tinst) O(mspms_nvt_state, nfree)
+  S(mspms_npt_state) O(mspms_npt_state, pinst) O(mspms_npt_state, boxl)
+  O(mspms_npt_state, boxv) O(mspms_npt_state, lj_tail)
   O(mspms_species, nnbp) O(mspms_species, imp_para) O(mspms_species, dih_unique)
   O(mspms_params, kappa) O(mspms_params, coulomb_prefactor) O(mspms_params, wolffcon2)
   O(mspms_params, ntube) O(mspms_params, solid_natom) O(mspms_params, grid)
@@ -76,7 +78,8 @@ def test_ctypes_layout_matches_c():
     py = {"mspms_species": abi.Species, "mspms_params": abi.Params,
           "mspms_long_terms": abi.LongTerms, "mspms_short_terms": abi.ShortTerms,
           "mspms_sf_grid": abi.SFGrid, "mspms_param_grad": abi.ParamGrad,
-          "mspms_tune_result": abi.TuneResult, "mspms_nvt_state": abi.NvtState}
+          "mspms_tune_result": abi.TuneResult, "mspms_nvt_state": abi.NvtState,
+          "mspms_npt_state": abi.NptState}
     for k, v in got.items():
         if "." in k:
             t, f = k.split(".")

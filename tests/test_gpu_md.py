@@ -154,3 +154,47 @@ def test_nve_fp64_matches_reference_trajectory(n):
     assert rel(x, g["x"]) <= 1e-9 and rel(v, g["v"]) <= 1e-9
     assert abs(uk.item() - float(g["md_ukin"])) <= 1e-9 * float(g["md_ukin"])
     eng.close()
+
+
+@pytest.mark.parametrize("name,n", [("md_lj864_npt", 1), ("md_lj864_npt", 10),
+                                    ("md_water216_npt", 1), ("md_water216_npt", 5)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_npt_fp64_matches_reference_trajectory(name, n, graph):
+    """The device NPT step (mspms_npt_run_device) against the reference's
+    own npt_respa trajectory: positions, velocities, the box, thermostat and
+    barostat velocities, the conserved quantity's extra part and the
+    instantaneous pressure (same bounds as the NVT trajectories)."""
+    import torch
+    rd, s, g = _load(name, n)
+    print(f"case {name} steps={n} graph={graph}", flush=True)
+    eng = Engine(s, precision="fp64")
+    nfree = 3 * s.natom - rd.run.nconstraint
+    eng.npt_set(rd.run.Qts, rd.run.Qbs, s.treq, s.preq, nfree, lj_tail=bool(rd.run.isLJlrcOn))
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    pos, vel = t(np.stack([s.xx, s.yy, s.zz])), t(g["v0"])
+    fl, fs = torch.zeros_like(pos), torch.zeros_like(pos)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(1, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    # forces and terms (virials) of the start, as after the reference's
+    # start-up frclong / frcshort
+    eng.frclong_device(pos, fl, lt)
+    eng.frcshort_device(pos, fs, st)
+    eng.sync()
+    eng.npt_run_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, n, lt, st, uk, graph=graph)
+    torch.cuda.synchronize()
+    eng.sync()
+    x, v = pos.cpu().numpy(), vel.cpu().numpy()
+    state = eng.npt_get()[0]
+    tol = 1e-9
+    print(f"  x {rel(x, g['x']):.2e} v {rel(v, g['v']):.2e} box {state['boxl'][0]!r} "
+          f"vs {float(g['box'][0])!r}")
+    assert rel(x, g["x"]) <= tol and rel(v, g["v"]) <= tol
+    assert np.allclose(state["boxl"], g["box"], rtol=1e-12, atol=0)
+    for k in ("vts", "vbs", "rts", "utsbs", "pinst", "ukin", "tinst"):
+        ref = float(g[f"md_{k}"])
+        assert abs(state[k] - ref) <= 10 * tol * max(abs(ref), 1e-3), (k, state[k], ref)
+    assert abs(uk.item() - state["ukin"]) <= 1e-15 * abs(state["ukin"])
+    eng.close()
