@@ -384,6 +384,54 @@ def run_replica_batch(systems, steps=20, warmup=3, nstep_inner=5):
             "batched_over_sequential": ms_seq / ms_b, "finite": fin and fin_s}
 
 
+def run_lj_nvt(steps=1000, cpu=True):
+    """The LJ-864 argon workload: FCC at rho* 0.8442, T* 0.722, rc* 2.5,
+    Nose-Hoover NVT (vver_nh_3, qq = qqs = 100), dt* 0.005, `steps` steps
+    graph-replayed on the device in each precision, against the
+    reference's own integrator timed on one host core over the same steps
+    (oracle/_ref, where it was built)."""
+    import torch
+    from mspms2_amd import configs
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from tests.util import system_of
+    rd = configs.lj_fcc(jitter=0.05, seed=7)
+    s = system_of(rd)
+    dev = torch.device("cuda:0")
+    out = {"workload": "LJ-864 FCC argon, NVT vver_nh_3, rc* 2.5, dt* 0.005", "steps": steps}
+    for prec in ("mixed", "fp64"):
+        eng = Engine(s, precision=prec)
+        eng.nvt_set(rd.run.qq, rd.run.qqs, s.treq, 3 * s.natom - rd.run.nconstraint)
+        eng.set_list_skin(0.3)  # sigma units
+        x0 = np.stack([s.xx, s.yy, s.zz])
+        t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa
+        pos, vel = t(x0), t(maxwell_velocities(s))
+        fl, fs = t(eng.frclong()[0]), t(eng.frcshort()[0])
+        lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        uk = torch.zeros(1, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        eng.nvt_run_device(pos, vel, fl, fs, s.delt, 1, 20, lt, st, uk, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.nvt_run_device(pos, vel, fl, fs, s.delt, 1, steps, lt, st, uk, graph=True)
+        torch.cuda.synchronize()
+        out[f"{prec}_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+        out[f"{prec}_tinst"] = eng.nvt_get()[0]["tinst"]
+        eng.sync()
+        eng.close()
+    if cpu:
+        from oracle import oracle as O
+        if O.reference_available():
+            with tempfile.TemporaryDirectory(prefix="mspms_bench_") as tmp:
+                rd.write(tmp)
+                d = O.run_reference(tmp, md_steps=steps, timeout=600)
+            out["cpu_reference_ms_per_step"] = 1e3 * d["md_seconds_per_step"]
+            out["cpu_cores"] = 1
+            out["speedup_mixed_vs_cpu_reference"] = \
+                out["cpu_reference_ms_per_step"] / out["mixed_ms_per_step"]
+    return out
+
+
 def run_reference_cpu(rd, repeat=2):
     """The reference's own frclong timed on one host core (oracle/_ref)."""
     from oracle import oracle as O
@@ -419,6 +467,8 @@ def main():
                     help="boxes in the batched-replica leg (0: skip it)")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
+    ap.add_argument("--lj-steps", type=int, default=1000,
+                    help="steps of the LJ-864 NVT leg (0: skip it)")
     ap.add_argument("--probe", action="store_true",
                     help="only the short frclong + reference-copy workload for counter runs")
     ap.add_argument("--trace-steps", type=int, default=0,
@@ -528,6 +578,8 @@ def main():
                                 for p in ("mixed", "fp64")}
     if args.replicas > 0:
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
+    if args.lj_steps > 0:
+        line["lj864_nvt"] = run_lj_nvt(args.lj_steps, cpu=not args.no_cpu)
     print(json.dumps(line))
 
 

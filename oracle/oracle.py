@@ -192,4 +192,6 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
         for ln in r.stdout.splitlines():
             if ln.startswith("frclong_seconds_per_call"):
                 d["frclong_seconds_per_call"] = float(ln.split()[1])
+            if ln.startswith("md_seconds_per_step"):
+                d["md_seconds_per_step"] = float(ln.split()[1])
         return d

@@ -21,7 +21,8 @@
  *   double vx,vy,vz [natom] (after step N)
  *
  * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N]
- *   --repeat K times K further frclong() calls and prints seconds/call.
+ *   --repeat K times K further frclong() calls and prints seconds/call;
+ *   --md-steps N also prints the integrator's seconds per step.
  *
  * Configurations that would reach code absent from this build (the
  * Fortran grid library, iSF_type 3; the GSL metal-cluster force field,
@@ -92,13 +93,20 @@ int main(int argc, char** argv)
         v0[1][k] = vy[k];
         v0[2][k] = vz[k];
     }
-    for (k = 0; k < nmd; k++) {
-        if (what_ensemble == NVT)
-            vver_nh_3();
-        else if (what_ensemble == NPT)
-            npt_respa();
-        else
-            vver();
+    if (nmd > 0) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        for (k = 0; k < nmd; k++) {
+            if (what_ensemble == NVT)
+                vver_nh_3();
+            else if (what_ensemble == NPT)
+                npt_respa();
+            else
+                vver();
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        printf("md_seconds_per_step %.9f\n",
+               ((t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec)) / nmd);
     }
     fo = fopen(outpath, "wb");
     if (!fo) {
