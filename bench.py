@@ -196,7 +196,8 @@ PROBE_COPY_MIB = 256
 
 def run_probe(system, calls=5):
     """A short fixed workload for hardware-counter passes: the autotuned
-    frclong of the benchmark system `calls` times (mixed, device buffers)
+    frclong of the benchmark system `calls` times (mixed, device buffers,
+    every call with a list build)
     and a plain device-to-device copy of PROBE_COPY_MIB MiB, the reference
     transfer that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
     import torch
@@ -210,8 +211,13 @@ def run_probe(system, calls=5):
     torch.cuda.synchronize()
     tune = eng.autotune(pos, repeats=2)
     st = torch.cuda.Stream()
+    # a 1e-7 nudge of every site before each call: each call rebuilds its
+    # pair list, as a call after a move does
+    nudge = torch.full_like(pos, 1e-7)
     torch.cuda.synchronize()
-    for _ in range(calls):
+    for k in range(calls):
+        pos.add_(nudge if k % 2 == 0 else -nudge)
+        torch.cuda.synchronize()
         eng.frclong_device(pos, f, lt, stream=st.cuda_stream)
     torch.cuda.synchronize()
     eng.sync()
