@@ -338,6 +338,10 @@ class RunDir:
     sysname: str
     species: List[SpeciesSpec]
     xyz: np.ndarray            # (natom, 3) Angstrom, reference atom order
+    # old.mspms, read where the starting option is not NEW (init_vars ->
+    # loadit, src/initvars.c:678-681): it replaces the positions, the
+    # velocities, the thermostat / barostat variables and the box
+    checkpoint: Optional[object] = None
 
     @property
     def natom(self) -> int:
@@ -355,6 +359,9 @@ class RunDir:
         write_in(os.path.join(path, "in.mspms"), self.run)
         write_cfg(os.path.join(path, "cfg.mspms"), self.sysname, self.species)
         write_xyz(os.path.join(path, "xyz.mspms"), self.atom_names(), self.xyz)
+        if self.checkpoint is not None:
+            from .rundata import LOAD, write_checkpoint
+            write_checkpoint(os.path.join(path, LOAD), self.checkpoint)
         return path
 
     @staticmethod
@@ -363,4 +370,9 @@ class RunDir:
         sysname, species = read_cfg(os.path.join(path, "cfg.mspms"))
         natom = sum(sp.natom * sp.nmole for sp in species)
         xyz = read_xyz(os.path.join(path, "xyz.mspms"), natom)
-        return RunDir(run=run, sysname=sysname, species=species, xyz=xyz)
+        ck = None
+        old = os.path.join(path, "old.mspms")
+        if run.iStart_option != 1 and os.path.exists(old):
+            from .rundata import read_checkpoint
+            ck = read_checkpoint(old, natom)
+        return RunDir(run=run, sysname=sysname, species=species, xyz=xyz, checkpoint=ck)

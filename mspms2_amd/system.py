@@ -139,6 +139,8 @@ class System:
     delt: float = 0.0
     treq: float = 0.0
     preq: float = 0.0                # reduced pressure (src/readins.c:38-47)
+    v0: Optional[np.ndarray] = None  # velocities of a restart (old.mspms), reduced
+    checkpoint: Optional[object] = None
     grid: Optional[dict] = None      # SF_TASOS grid (mspms_sf_grid fields)
     _keep: list = field(default_factory=list, repr=False)
 
@@ -181,10 +183,23 @@ class System:
                      tube_r=np.array([t[2] / sb for t in r.tubes]),
                      const_3pisq_theta=C3_PI_SQ_THETA * sb * sb,
                      solid_sigma=r.tube_sigma / sb, solid_epsilon=r.tube_epsilon / eb)
+        ck = rd.checkpoint
+        if ck is not None:
+            # loadit (src/loadsave.c:60-128): positions, velocities and box of
+            # the checkpoint, in reduced units.  The reference keeps the
+            # Ewald constants init_charge derived from the input file's box
+            # (src/initvars.c:660-681 run before loadit); they are derived
+            # from the checkpoint's box here, which is the same box for
+            # every run that does not change it (NVE, NVT, HMC).
+            xx = np.ascontiguousarray(ck.x[0])
+            yy = np.ascontiguousarray(ck.x[1])
+            zz = np.ascontiguousarray(ck.x[2])
+            p.update(boxlx=ck.box[0], boxly=ck.box[1], boxlz=ck.box[2])
         pressure_base = BOLTZMAN_CONSTANT * eb * 1.0e25 / (sb * sb * sb)
         s = System(species=species, params=p, xx=xx, yy=yy, zz=zz, run=r,
                    time_base=time_base, delt=r.delt / time_base, treq=r.treq / eb,
-                   preq=r.preq / pressure_base)
+                   preq=r.preq / pressure_base,
+                   v0=None if ck is None else np.array(ck.v), checkpoint=ck)
         return s
 
     # ------------------------------------------------------------------ #

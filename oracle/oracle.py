@@ -170,21 +170,27 @@ def reference_available() -> bool:
 
 
 def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
-                  md_steps: int = 0) -> dict:
+                  md_steps: int = 0, files_out: Optional[str] = None) -> dict:
     """Runs the reference's start-up path + frclong + frcshort on a run
     directory (in a scratch copy) and returns its dump; md_steps > 0 then
     runs that many steps of the input's integrator (vver / vver_nh_3 /
-    npt_respa) and adds the velocities and the thermostat state."""
+    npt_respa) and adds the velocities and the thermostat state.  With
+    files_out, the reference's own trajectory frame, snapshot and checkpoint
+    of the final state (trj.mspms, ss.mspms, sav.mspms) are copied there;
+    an old.mspms in the run directory is passed along (restart inputs)."""
     if not reference_available():
         raise FileNotFoundError(REF_HARNESS)
     with tempfile.TemporaryDirectory(prefix="mspms_ref_") as tmp:
-        for fn in ("in.mspms", "cfg.mspms", "xyz.mspms"):
+        for fn in ("in.mspms", "cfg.mspms", "xyz.mspms", "old.mspms"):
+            if fn == "old.mspms" and not os.path.exists(os.path.join(rundir, fn)):
+                continue
             with open(os.path.join(rundir, fn), "rb") as src, \
                     open(os.path.join(tmp, fn), "wb") as dst:
                 dst.write(src.read())
         out = os.path.join(tmp, "dump.bin")
         cmd = [REF_HARNESS, tmp, out] + (["--repeat", str(repeat)] if repeat else []) + \
-            (["--md-steps", str(md_steps)] if md_steps else [])
+            (["--md-steps", str(md_steps)] if md_steps else []) + \
+            (["--write-files"] if files_out else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stderr[-2000:]}")
@@ -194,4 +200,10 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
                 d["frclong_seconds_per_call"] = float(ln.split()[1])
             if ln.startswith("md_seconds_per_step"):
                 d["md_seconds_per_step"] = float(ln.split()[1])
+        if files_out:
+            os.makedirs(files_out, exist_ok=True)
+            for fn in ("trj.mspms", "ss.mspms", "sav.mspms"):
+                with open(os.path.join(tmp, fn), "rb") as src, \
+                        open(os.path.join(files_out, fn), "wb") as dst:
+                    dst.write(src.read())
         return d

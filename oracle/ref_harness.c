@@ -20,7 +20,11 @@
  *   double vx0,vy0,vz0 [natom] (velinit's velocities, before step 1) |
  *   double vx,vy,vz [natom] (after step N)
  *
- * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N]
+ * With --write-files it finally writes the reference's own trajectory
+ * frame, snapshot and checkpoint of that state into RUNDIR (trajectory(),
+ * snapshot(), saveit(): trj.mspms, ss.mspms, sav.mspms).
+ *
+ * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N] [--write-files]
  *   --repeat K times K further frclong() calls and prints seconds/call;
  *   --md-steps N also prints the integrator's seconds per step.
  *
@@ -43,7 +47,7 @@ static double v0[3][NATOM_MAX];
 
 int main(int argc, char** argv)
 {
-    int repeat = 0, nmd = 0, k;
+    int repeat = 0, nmd = 0, files = 0, k;
     char outpath[4096];
     FILE* fo;
     if (argc < 3) {
@@ -55,6 +59,8 @@ int main(int argc, char** argv)
             repeat = atoi(argv[++k]);
         else if (!strcmp(argv[k], "--md-steps") && k + 1 < argc)
             nmd = atoi(argv[++k]);
+        else if (!strcmp(argv[k], "--write-files"))
+            files = 1;
     if (argv[2][0] == '/') {
         snprintf(outpath, sizeof outpath, "%s", argv[2]);
     } else {
@@ -107,6 +113,12 @@ int main(int argc, char** argv)
         clock_gettime(CLOCK_MONOTONIC, &t1);
         printf("md_seconds_per_step %.9f\n",
                ((t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec)) / nmd);
+    }
+    if (files) {
+        trajectory();
+        fclose(fptrj);
+        snapshot();
+        saveit();
     }
     fo = fopen(outpath, "wb");
     if (!fo) {

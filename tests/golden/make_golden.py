@@ -81,6 +81,12 @@ MD_CASES = {
 }
 
 
+# the reference's own run-state files of a run (saveit / trajectory /
+# snapshot), and a restart from its checkpoint (starting option 2, loadit)
+FILE_CASE = ("md_water216_nvt", 10)
+RESTART = ("md_water216_nvt_restart", 5)
+
+
 def save_md(path, d, nstep):
     sc = {k: np.float64(v) for k, v in d.items() if isinstance(v, float)}
     md = {f"md_{k}": np.float64(v) for k, v in d["md"].items()}
@@ -104,6 +110,31 @@ def save(path, d, keep_every=1):
     idx = np.arange(0, d["natom"], keep_every)
     np.savez_compressed(os.path.join(path, "ref.npz"), fl=d["fl"][:, idx], fs=d["fs"][:, idx],
                         idx=idx, natom=d["natom"], **{f"s_{k}": v for k, v in sc.items()})
+
+
+def make_file_cases():
+    name, n = FILE_CASE
+    src = os.path.join(HERE, name)
+    with tempfile.TemporaryDirectory() as out:
+        O.run_reference(src, md_steps=n, files_out=out)
+        for fn in ("sav.mspms", "trj.mspms", "ss.mspms"):
+            shutil.copy(os.path.join(out, fn), os.path.join(src, f"ref{n}_{fn}"))
+    rname, rn = RESTART
+    dst = os.path.join(HERE, rname)
+    os.makedirs(dst, exist_ok=True)
+    for fn in ("cfg.mspms", "xyz.mspms"):
+        shutil.copy(os.path.join(src, fn), os.path.join(dst, fn))
+    with open(os.path.join(src, "in.mspms")) as f:
+        lines = f.read().split("\n")
+    # line 17 of in.mspms: the starting option (src/readins.c:62)
+    assert lines[16].split()[1] == "iStart_option"
+    lines[16] = f"{2:<24d} iStart_option"
+    with open(os.path.join(dst, "in.mspms"), "w") as f:
+        f.write("\n".join(lines))
+    shutil.copy(os.path.join(src, f"ref{n}_sav.mspms"), os.path.join(dst, "old.mspms"))
+    d = O.run_reference(dst, md_steps=rn)
+    save_md(dst, d, rn)
+    print(rname, rn, d["md"]["ukin"])
 
 
 def main():
@@ -131,6 +162,7 @@ def main():
             d = O.run_reference(path, md_steps=n)
             save_md(path, d, n)
             print(name, n, d["md"]["ukin"], d["md"]["ps"])
+    make_file_cases()
     for name, fn in BIG.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)

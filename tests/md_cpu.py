@@ -29,7 +29,8 @@ def seqsum(a: np.ndarray) -> float:
 class NvtCpu:
     """One Nose-Hoover NVT chain of vver_nh_3 steps on the oracle."""
 
-    def __init__(self, system, v0, qq: float, qqs: float, nconstraint: int = 3):
+    def __init__(self, system, v0, qq: float, qqs: float, nconstraint: int = 3,
+                 state: dict = None):
         s = system
         self.s = s
         self.o = Oracle(s)
@@ -39,6 +40,9 @@ class NvtCpu:
         self.qq, self.qqs = qq, qqs
         self.gg = self.ggs = float(self.nfree)  # src/initvars.c:332-335
         self.ss = self.ps = self.sss = self.pss = 0.0
+        # a restart's thermostats (loadit, src/loadsave.c:86-93)
+        for k, v in (state or {}).items():
+            setattr(self, k, v)
         self.unhts = self.unhtss = 0.0
         self.ukin = self.tinst = 0.0
         delt = s.delt

@@ -198,3 +198,22 @@ def test_npt_fp64_matches_reference_trajectory(name, n, graph):
         assert abs(state[k] - ref) <= 10 * tol * max(abs(ref), 1e-3), (k, state[k], ref)
     assert abs(uk.item() - state["ukin"]) <= 1e-15 * abs(state["ukin"])
     eng.close()
+
+
+def test_nvt_restart_from_checkpoint_matches_reference():
+    """A run directory with starting option 2: positions, velocities and both
+    thermostats come from old.mspms (the reference's loadit); the device
+    continues the run as the reference's own integrator does."""
+    rd = runio.RunDir.read(os.path.join(GOLDEN, "md_water216_nvt_restart"))
+    s = System.from_rundir(rd)
+    ck = rd.checkpoint
+    g = np.load(os.path.join(GOLDEN, "md_water216_nvt_restart", "md5.npz"))
+    eng = Engine(s, precision="fp64")
+    eng.nvt_set(ck.qq, ck.qqs, s.treq, 3 * s.natom - rd.run.nconstraint, ss=ck.ss, ps=ck.ps,
+                sss=ck.sss, pss=ck.pss)
+    x, v, _, _ = _run(eng, s, rd, s.v0, 5, True)
+    print(f"restart x {rel(x, g['x']):.2e} v {rel(v, g['v']):.2e}")
+    assert rel(x, g["x"]) <= 1e-9 and rel(v, g["v"]) <= 1e-9
+    st = eng.nvt_get()[0]
+    assert abs(st["ps"] - float(g["md_ps"])) <= 1e-8 * max(abs(float(g["md_ps"])), 1e-3)
+    eng.close()
