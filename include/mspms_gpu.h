@@ -414,6 +414,23 @@ int mspms_nvt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, d
                          double delt, int nstep_inner, int nsteps, mspms_long_terms* lterms,
                          mspms_short_terms* sterms, double* ukin, int use_graph, void* stream);
 
+/* Simulated annealing of every replica on device buffers: the reference's
+ * siman (src/siman.c:61-182; init_siman, :8-59).  Cycle k of replica r runs
+ * at T = t_initial[r] / mu_t[r]^k (reduced): velinit at T from the
+ * replica's RANMAR stream, frclong / frcshort of the current positions,
+ * then n_tries vver_nh_3 steps from zeroed thermostats (qq, qqs, nfree of
+ * mspms_nvt_set).  A replica stops after its first cycle run below t_min[r]
+ * (later cycles of the others leave it as it is), at most max_cycles
+ * cycles.  Needs mspms_hmc_init (streams seeded as rmarin(ij, kl) and the
+ * start-up velinit done, as in init_vars) and mspms_nvt_set.  Outputs:
+ * cycles[r] run and t_final[r], the temperature of the last cycle; pos,
+ * vel, fl, fs, lterms, sterms, ukin as after mspms_nvt_run_device. */
+int mspms_siman_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, double* fs,
+                           double delt, int nstep_inner, int n_tries, const double* t_initial,
+                           const double* t_min, const double* mu_t, int max_cycles,
+                           int* cycles, double* t_final, mspms_long_terms* lterms,
+                           mspms_short_terms* sterms, double* ukin, void* stream);
+
 /* NPT molecular dynamics on device buffers: the reference's npt_respa with
  * the Nose-Hoover thermostat / barostat operator npt_nh_operator
  * (src/nptnh.c:10-234, selected by src/mspms2.c:70-73).  The box scales by

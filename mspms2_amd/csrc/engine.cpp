@@ -211,6 +211,7 @@ struct mspms_ctx {
     // the Newton iterate (3, N)
     DevBuf<mspms_nvt_state> d_nvt;
     DevBuf<double> d_nvt_carry, d_nvt_vn;
+    DevBuf<int> d_active;                // replicas the NVT kernels advance
     bool nvt_ready = false;
     // NPT (mspms_npt_set)
     DevBuf<NptDev> d_npt;
@@ -573,6 +574,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_npt.release();
     c->d_nvt_carry.release();
     c->d_nvt_vn.release();
+    c->d_active.release();
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -1773,18 +1775,20 @@ int enqueue_nvt(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs, 
         ProfScope p(c, s, PROF_BONDED);
         for (int ll = 0; ll < nstep_inner; ll++) {
             launch_nvt_drift(L, c->d_nvt.p, ll == 0, pos, pos + n, pos + 2 * n, vel, vel + n,
-                             vel + 2 * n, fl, fs, deltby2, delts, deltsby2, delts_sqby2);
+                             vel + 2 * n, fl, fs, deltby2, delts, deltsby2, delts_sqby2,
+                             c->d_active.p);
             launch_bonded(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, fs, fs + n,
                           fs + 2 * n, c->d_ep_short.p);
             launch_nvt_solve(L, c->d_nvt.p, false, c->d_nvt_carry.p, vel, vel + n, vel + 2 * n,
-                             fs, c->d_nvt_vn.p, delts, deltsby2, delts_sqby2, nullptr);
+                             fs, c->d_nvt_vn.p, delts, deltsby2, delts_sqby2, nullptr,
+                             c->d_active.p);
         }
     }
     int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
                           nullptr, false);
     if (rc) return rc;
     launch_nvt_solve(L, c->d_nvt.p, true, c->d_nvt_carry.p, vel, vel + n, vel + 2 * n, fl,
-                     c->d_nvt_vn.p, delt, deltby2, delt_sqby2, ukin);
+                     c->d_nvt_vn.p, delt, deltby2, delt_sqby2, ukin, c->d_active.p);
     {
         ProfScope p(c, s, PROF_FINAL);
         launch_finalize(c, L, lterms, sterms, nullptr);
@@ -1804,7 +1808,8 @@ extern "C" int mspms_nvt_set(mspms_ctx* c, const mspms_nvt_state* st) {
     HIPC(hipSetDevice(c->device));
     int rc;
     if ((rc = c->d_nvt.ensure(c->nrep)) || (rc = c->d_nvt_carry.ensure(c->nrep)) ||
-        (rc = c->d_nvt_vn.ensure((size_t)3 * c->natom)))
+        (rc = c->d_nvt_vn.ensure((size_t)3 * c->natom)) ||
+        (rc = c->d_active.upload(std::vector<int>(c->nrep, 1), c->stream)))
         return rc;
     HIPC(hipMemcpyAsync(c->d_nvt.p, st, sizeof(mspms_nvt_state) * c->nrep,
                         hipMemcpyHostToDevice, c->stream));
@@ -1819,6 +1824,86 @@ extern "C" int mspms_nvt_get(mspms_ctx* c, mspms_nvt_state* st) {
     HIPC(hipSetDevice(c->device));
     HIPC(hipDeviceSynchronize());
     HIPC(hipMemcpy(st, c->d_nvt.p, sizeof(mspms_nvt_state) * c->nrep, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Simulated annealing (siman, src/siman.c:61-182) of every replica: cycles
+// of velinit at T (each replica's RANMAR stream), the forces of the current
+// positions and n_tries NVT steps from zeroed thermostats; T /= mu_t after
+// each cycle, a replica stopping after its first cycle run below t_min (its
+// later cycles leave it as it is).  Host-driven: one graph-replayed NVT run
+// per cycle; needs mspms_hmc_init (streams, masses) and mspms_nvt_set.
+extern "C" int mspms_siman_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
+                                      double* fs, double delt, int nstep_inner, int n_tries,
+                                      const double* t_initial, const double* t_min,
+                                      const double* mu_t, int max_cycles, int* cycles,
+                                      double* t_final, mspms_long_terms* lterms,
+                                      mspms_short_terms* sterms, double* ukin, void* stream) {
+    if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (!c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first (velocity streams)");
+    if (!c->nvt_ready) return fail(MSPMS_EINVAL, "mspms_nvt_set first");
+    if (!pos || !vel || !fl || !fs || !t_initial || !t_min || !mu_t || !lterms || !sterms ||
+        !ukin)
+        return fail(MSPMS_EINVAL, "null argument");
+    if (nstep_inner < 1 || n_tries < 0 || max_cycles < 0) return fail(MSPMS_EINVAL, "bad counts");
+    const int nr = c->nrep;
+    for (int r = 0; r < nr; r++)
+        if (!(t_initial[r] > 0.0) || !(mu_t[r] > 1.0))
+            return fail(MSPMS_EINVAL, "replica %d: t_initial must be positive, mu_t above 1", r);
+    HIPC(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const size_t n = (size_t)c->natom;
+    std::vector<double> T(t_initial, t_initial + nr);
+    std::vector<int> act(nr, 1), cyc(nr, 0);
+    std::vector<HmcRep> hmc(nr);
+    std::vector<mspms_nvt_state> nvt(nr);
+    int rc = 0;
+    for (int k = 0; k < max_cycles && !rc; k++) {
+        bool any = false;
+        for (int r = 0; r < nr; r++) any = any || act[r];
+        if (!any) break;
+        HIPC(hipStreamSynchronize(s));
+        HIPC(hipMemcpy(hmc.data(), c->d_hmc.p, sizeof(HmcRep) * nr, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(nvt.data(), c->d_nvt.p, sizeof(mspms_nvt_state) * nr,
+                       hipMemcpyDeviceToHost));
+        for (int r = 0; r < nr; r++) {
+            if (!act[r]) continue;
+            // src/siman.c:98 and the reset of :169-177 (init_nvt's at the start)
+            hmc[r].treq = T[r];
+            mspms_nvt_state& v = nvt[r];
+            v.treq = T[r];
+            v.ss = v.ps = v.sss = v.pss = v.unhts = v.unhtss = 0.0;
+            v.gg = v.ggs = (double)v.nfree;
+        }
+        HIPC(hipMemcpy(c->d_hmc.p, hmc.data(), sizeof(HmcRep) * nr, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(c->d_nvt.p, nvt.data(), sizeof(mspms_nvt_state) * nr,
+                       hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(c->d_active.p, act.data(), sizeof(int) * nr, hipMemcpyHostToDevice));
+        LaunchCtx L = launch_ctx(c, s);
+        launch_hmc_init(L, c->d_hmc.p, c->d_rng.p, c->d_aw_atom.p, vel, c->d_active.p);
+        if ((rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s)))
+            break;
+        if ((rc = enqueue_short(c, pos, pos + n, pos + 2 * n, fs, fs + n, fs + 2 * n, sterms, s)))
+            break;
+        rc = mspms_nvt_run_device(c, pos, vel, fl, fs, delt, nstep_inner, n_tries, lterms,
+                                  sterms, ukin, 1, s);
+        for (int r = 0; r < nr; r++) {
+            if (!act[r]) continue;
+            cyc[r]++;
+            if (T[r] < t_min[r])
+                act[r] = 0;  // src/siman.c:141-144
+            else
+                T[r] /= mu_t[r];
+        }
+    }
+    HIPC(hipStreamSynchronize(s));
+    std::vector<int> ones(nr, 1);
+    HIPC(hipMemcpy(c->d_active.p, ones.data(), sizeof(int) * nr, hipMemcpyHostToDevice));
+    if (rc) return rc;
+    for (int r = 0; r < nr; r++) {
+        if (cycles) cycles[r] = cyc[r];
+        if (t_final) t_final[r] = T[r];  // the last cycle's (the next one's if unfinished)
+    }
     return 0;
 }
 

@@ -198,8 +198,9 @@ struct HmcRep {
     long long accepted, moves;
 };
 
+// velinit of every replica (active: only where active[r] != 0; null: all)
 void launch_hmc_init(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,
-                     double* vel);
+                     double* vel, const int* active = nullptr);
 void launch_hmc_begin(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,
                       const double* pos, double* pos_old, double* vel);
 void launch_hmc_mark(const LaunchCtx& c, HmcRep* hmc, const void* lterms, const void* sterms);
@@ -230,10 +231,11 @@ struct mspms_nvt_state;
 void launch_nvt_drift(const LaunchCtx& c, mspms_nvt_state* st, bool first, double* x, double* y,
                       double* z, double* vx, double* vy, double* vz, const double* fl,
                       const double* fs, double deltby2, double delts, double deltsby2,
-                      double delts_sqby2);
+                      double delts_sqby2, const int* active);
 void launch_nvt_solve(const LaunchCtx& c, mspms_nvt_state* st, bool outer, double* carry,
                       double* vx, double* vy, double* vz, const double* f, double* vn,
-                      double dt, double dtby2, double dt_sqby2, double* ukin);
+                      double dt, double dtby2, double dt_sqby2, double* ukin,
+                      const int* active);
 
 void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,

@@ -131,11 +131,13 @@ __global__ void __launch_bounds__(256) k_hmc_init(const RepDev* __restrict__ rep
                                                   HmcRep* __restrict__ hmc,
                                                   RanmarDev* __restrict__ rng,
                                                   const double* __restrict__ aw_atom,
-                                                  double* vx, double* vy, double* vz) {
+                                                  double* vx, double* vy, double* vz,
+                                                  const int* __restrict__ active) {
     __shared__ double sh[4 * 4];
     __shared__ double red[4];
     __shared__ RanmarDev r;
     const int rep = blockIdx.x;
+    if (active && !active[rep]) return;  // whole workgroup
     if (threadIdx.x == 0) r = rng[rep];
     __syncthreads();
     velinit_rep(reps[rep], hmc[rep], r, aw_atom, vx, vy, vz, sh, red);
@@ -239,10 +241,10 @@ __global__ void k_hmc_draw(RanmarDev* rng, int rep, int n, double* out) {
 }  // namespace
 
 void launch_hmc_init(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,
-                     double* vel) {
+                     double* vel, const int* active) {
     const size_t n = (size_t)c.natom;
     hipLaunchKernelGGL(k_hmc_init, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, hmc, rng,
-                       aw_atom, vel, vel + n, vel + 2 * n);
+                       aw_atom, vel, vel + n, vel + 2 * n, active);
 }
 
 void launch_hmc_begin(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const double* aw_atom,

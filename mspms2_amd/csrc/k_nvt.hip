@@ -60,9 +60,10 @@ __global__ void __launch_bounds__(NVT_T) k_nvt_drift(
     const double* __restrict__ fyl, const double* __restrict__ fzl,
     const double* __restrict__ fxs, const double* __restrict__ fys,
     const double* __restrict__ fzs, double deltby2, double delts, double deltsby2,
-    double delts_sqby2) {
+    double delts_sqby2, const int* __restrict__ active) {
 #pragma clang fp contract(off)
     __shared__ double sh[16];
+    if (active && !active[blockIdx.x]) return;  // a frozen replica (whole workgroup)
     const RepDev& R = reps[blockIdx.x];
     mspms_nvt_state* S = st + blockIdx.x;
     const double ps = S->ps, pss = S->pss;
@@ -103,9 +104,10 @@ __global__ void __launch_bounds__(NVT_T) k_nvt_solve(
     double* __restrict__ carry, double* __restrict__ vx, double* __restrict__ vy,
     double* __restrict__ vz, const double* __restrict__ fx, const double* __restrict__ fy,
     const double* __restrict__ fz, double* __restrict__ vn, size_t n, double dt, double dtby2,
-    double dt_sqby2, double* __restrict__ ukin) {
+    double dt_sqby2, double* __restrict__ ukin, const int* __restrict__ active) {
 #pragma clang fp contract(off)
     __shared__ double sh[16];
+    if (active && !active[blockIdx.x]) return;  // a frozen replica (whole workgroup)
     const int r = blockIdx.x;
     const RepDev& R = reps[r];
     mspms_nvt_state* S = st + r;
@@ -199,28 +201,29 @@ __global__ void __launch_bounds__(NVT_T) k_nvt_solve(
 void launch_nvt_drift(const LaunchCtx& c, mspms_nvt_state* st, bool first, double* x, double* y,
                       double* z, double* vx, double* vy, double* vz, const double* fl,
                       const double* fs, double deltby2, double delts, double deltsby2,
-                      double delts_sqby2) {
+                      double delts_sqby2, const int* active) {
     const size_t n = (size_t)c.natom;
     if (first)
         hipLaunchKernelGGL(k_nvt_drift<true>, dim3(c.nrep), dim3(NVT_T), 0, c.stream, c.reps,
                            c.sites, c.atom_site, st, x, y, z, vx, vy, vz, fl, fl + n, fl + 2 * n,
-                           fs, fs + n, fs + 2 * n, deltby2, delts, deltsby2, delts_sqby2);
+                           fs, fs + n, fs + 2 * n, deltby2, delts, deltsby2, delts_sqby2, active);
     else
         hipLaunchKernelGGL(k_nvt_drift<false>, dim3(c.nrep), dim3(NVT_T), 0, c.stream, c.reps,
                            c.sites, c.atom_site, st, x, y, z, vx, vy, vz, fl, fl + n, fl + 2 * n,
-                           fs, fs + n, fs + 2 * n, deltby2, delts, deltsby2, delts_sqby2);
+                           fs, fs + n, fs + 2 * n, deltby2, delts, deltsby2, delts_sqby2, active);
 }
 
 void launch_nvt_solve(const LaunchCtx& c, mspms_nvt_state* st, bool outer, double* carry,
                       double* vx, double* vy, double* vz, const double* f, double* vn,
-                      double dt, double dtby2, double dt_sqby2, double* ukin) {
+                      double dt, double dtby2, double dt_sqby2, double* ukin,
+                      const int* active) {
     const size_t n = (size_t)c.natom;
     if (outer)
         hipLaunchKernelGGL(k_nvt_solve<true>, dim3(c.nrep), dim3(NVT_T), 0, c.stream, c.reps,
                            c.sites, c.atom_site, st, carry, vx, vy, vz, f, f + n, f + 2 * n, vn, n,
-                           dt, dtby2, dt_sqby2, ukin);
+                           dt, dtby2, dt_sqby2, ukin, active);
     else
         hipLaunchKernelGGL(k_nvt_solve<false>, dim3(c.nrep), dim3(NVT_T), 0, c.stream, c.reps,
                            c.sites, c.atom_site, st, carry, vx, vy, vz, f, f + n, f + 2 * n, vn, n,
-                           dt, dtby2, dt_sqby2, ukin);
+                           dt, dtby2, dt_sqby2, ukin, active);
 }

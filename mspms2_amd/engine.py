@@ -29,7 +29,7 @@ EXPORTS = [
     "mspms_autotune", "mspms_set_launch_shape", "mspms_hmc_init", "mspms_hmc_run_device",
     "mspms_hmc_stats", "mspms_hmc_draw", "mspms_set_list_skin", "mspms_list_builds",
     "mspms_nvt_set", "mspms_nvt_get", "mspms_nvt_run_device", "mspms_npt_set", "mspms_npt_get",
-    "mspms_npt_run_device",
+    "mspms_npt_run_device", "mspms_siman_run_device",
 ]
 
 
@@ -90,6 +90,8 @@ def lib():
                                        vp, C.c_int, vp]
     L.mspms_list_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.mspms_npt_set.argtypes = [vp, C.POINTER(abi.NptState)]
+    L.mspms_siman_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, dp, dp,
+                                         dp, C.c_int, C.POINTER(C.c_int), dp, vp, vp, vp, vp]
     L.mspms_npt_get.argtypes = [vp, C.POINTER(abi.NptState)]
     L.mspms_npt_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp,
                                        vp, C.c_int, vp]
@@ -351,6 +353,23 @@ class Engine:
                                           int(nstep_inner), int(nsteps), p(lterms), p(sterms),
                                           p(ukin), 1 if graph else 0, stream),
                "mspms_nvt_run_device")
+
+    def siman_run_device(self, pos, vel, fl, fs, delt, nstep_inner, n_tries, t_initial, t_min,
+                         mu_t, lterms, sterms, ukin, max_cycles=1000, stream=None):
+        """Simulated annealing of every replica (reduced temperatures; scalars
+        or per-replica sequences); returns (cycles, t_final) per replica."""
+        n = self.nrep
+        b = lambda v: np.ascontiguousarray(np.broadcast_to(v, n), dtype=np.float64)  # noqa
+        ti, tm, mu = b(t_initial), b(t_min), b(mu_t)
+        cyc = np.zeros(n, dtype=np.int32)
+        tf = np.zeros(n)
+        p = self._ptr
+        _check(lib().mspms_siman_run_device(
+            self._h, p(pos), p(vel), p(fl), p(fs), float(delt), int(nstep_inner), int(n_tries),
+            abi.dptr(ti), abi.dptr(tm), abi.dptr(mu), int(max_cycles),
+            cyc.ctypes.data_as(C.POINTER(C.c_int)), abi.dptr(tf), p(lterms), p(sterms),
+            p(ukin), stream), "mspms_siman_run_device")
+        return cyc, tf
 
     def npt_set(self, Qts, Qbs, treq, preq, nfree, lj_tail=False, vts=None, vbs=None,
                 rts=0.0):

@@ -123,6 +123,13 @@ class RunInput:
     delv: float = 100.0
     nstep_delt_adj_cycle: int = 100
     nstep_delv_adj_cycle: int = 100
+    # SIMAN (init_siman, src/siman.c:8-59): NVT steps per temperature, print
+    # interval; initial and final temperature (K), cooling divisor
+    n_tries: int = 20
+    iters_fixed_t: int = 10
+    t_initial: float = 300.0
+    t_min: float = 100.0
+    mu_t: float = 1.5
 
 
 _FIXED = [  # (attribute, format) in src/readins.c:31-82 order
@@ -175,6 +182,10 @@ def write_in(path: str, r: RunInput) -> None:
                        ("rreq_volm", "g"), ("delv", "g"), ("nstep_delt_adj_cycle", "d"),
                        ("nstep_delv_adj_cycle", "d")]:
         lines.append(f"{_fmt(getattr(r, name), kind):<24s} {name}")
+    lines.append("")
+    lines.append("   SIMAN")
+    lines.append(f"{r.n_tries} {r.iters_fixed_t}      n_tries iters_fixed_t")
+    lines.append(f"{_g(r.t_initial)} {_g(r.t_min)} {_g(r.mu_t)}      t_initial t_min mu_t (K)")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -221,6 +232,11 @@ def read_in(path: str) -> RunInput:
         r.nstep_md_per_hmc = int(vals[0])
         r.pdisp, r.pvolm, r.pmake, r.pkill, r.rreq_disp, r.rreq_volm, r.delv = map(float, vals[1:8])
         r.nstep_delt_adj_cycle, r.nstep_delv_adj_cycle = int(vals[8]), int(vals[9])
+    i = _section(lines, "SIMAN")
+    if i is not None:
+        a, b = lines[i].split(), lines[i + 1].split()
+        r.n_tries, r.iters_fixed_t = int(a[0]), int(a[1])
+        r.t_initial, r.t_min, r.mu_t = float(b[0]), float(b[1]), float(b[2])
     return r
 
 

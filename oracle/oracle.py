@@ -170,14 +170,16 @@ def reference_available() -> bool:
 
 
 def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
-                  md_steps: int = 0, files_out: Optional[str] = None) -> dict:
+                  md_steps: int = 0, files_out: Optional[str] = None,
+                  siman: bool = False) -> dict:
     """Runs the reference's start-up path + frclong + frcshort on a run
     directory (in a scratch copy) and returns its dump; md_steps > 0 then
     runs that many steps of the input's integrator (vver / vver_nh_3 /
     npt_respa) and adds the velocities and the thermostat state.  With
     files_out, the reference's own trajectory frame, snapshot and checkpoint
     of the final state (trj.mspms, ss.mspms, sav.mspms) are copied there;
-    an old.mspms in the run directory is passed along (restart inputs)."""
+    an old.mspms in the run directory is passed along (restart inputs).
+    siman runs the reference's simulated annealing driver instead."""
     if not reference_available():
         raise FileNotFoundError(REF_HARNESS)
     with tempfile.TemporaryDirectory(prefix="mspms_ref_") as tmp:
@@ -190,7 +192,7 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
         out = os.path.join(tmp, "dump.bin")
         cmd = [REF_HARNESS, tmp, out] + (["--repeat", str(repeat)] if repeat else []) + \
             (["--md-steps", str(md_steps)] if md_steps else []) + \
-            (["--write-files"] if files_out else [])
+            (["--write-files"] if files_out else []) + (["--siman"] if siman else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stderr[-2000:]}")

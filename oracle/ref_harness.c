@@ -24,7 +24,13 @@
  * frame, snapshot and checkpoint of that state into RUNDIR (trajectory(),
  * snapshot(), saveit(): trj.mspms, ss.mspms, sav.mspms).
  *
+ * With --siman it runs the reference's simulated annealing driver siman()
+ * (src/siman.c:61-182: cycles of velinit + vver_nh_3 steps, cooling
+ * between them) instead, and dumps the state it ends in (version 2, no
+ * integrator timing).
+ *
  * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N] [--write-files]
+ *                          [--siman]
  *   --repeat K times K further frclong() calls and prints seconds/call;
  *   --md-steps N also prints the integrator's seconds per step.
  *
@@ -47,7 +53,7 @@ static double v0[3][NATOM_MAX];
 
 int main(int argc, char** argv)
 {
-    int repeat = 0, nmd = 0, files = 0, k;
+    int repeat = 0, nmd = 0, files = 0, sa = 0, k;
     char outpath[4096];
     FILE* fo;
     if (argc < 3) {
@@ -61,6 +67,8 @@ int main(int argc, char** argv)
             nmd = atoi(argv[++k]);
         else if (!strcmp(argv[k], "--write-files"))
             files = 1;
+        else if (!strcmp(argv[k], "--siman"))
+            sa = 1;
     if (argv[2][0] == '/') {
         snprintf(outpath, sizeof outpath, "%s", argv[2]);
     } else {
@@ -99,7 +107,10 @@ int main(int argc, char** argv)
         v0[1][k] = vy[k];
         v0[2][k] = vz[k];
     }
-    if (nmd > 0) {
+    if (sa) {
+        siman();
+        nmd = 1;  /* version-2 dump of the final state */
+    } else if (nmd > 0) {
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
         for (k = 0; k < nmd; k++) {

@@ -112,6 +112,26 @@ def save(path, d, keep_every=1):
                         idx=idx, natom=d["natom"], **{f"s_{k}": v for k, v in sc.items()})
 
 
+def siman_lj256():
+    """LJ-256 annealed by the reference's siman: 150 K -> 100 K -> 66.7 K
+    (t_min 80 K, mu_t 1.5), 20 NVT steps per temperature."""
+    rd = configs.lj_fcc(ncell=4, jitter=0.05, seed=9)
+    rd.run.what_simulation, rd.run.what_ensemble = 2, 1
+    rd.run.n_tries, rd.run.iters_fixed_t = 20, 10
+    rd.run.t_initial, rd.run.t_min, rd.run.mu_t = 150.0, 80.0, 1.5
+    return rd
+
+
+def make_siman_case():
+    path = os.path.join(HERE, "siman_lj256")
+    os.makedirs(path, exist_ok=True)
+    siman_lj256().write(path)
+    d = O.run_reference(path, siman=True)
+    np.savez_compressed(os.path.join(path, "siman.npz"), x=np.stack([d["xx"], d["yy"], d["zz"]]),
+                        v=d["v"], treq=d["md"]["treq"], natom=d["natom"])
+    print("siman_lj256", d["md"]["treq"])
+
+
 def make_file_cases():
     name, n = FILE_CASE
     src = os.path.join(HERE, name)
@@ -163,6 +183,7 @@ def main():
             save_md(path, d, n)
             print(name, n, d["md"]["ukin"], d["md"]["ps"])
     make_file_cases()
+    make_siman_case()
     for name, fn in BIG.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)

@@ -30,7 +30,7 @@ class NvtCpu:
     """One Nose-Hoover NVT chain of vver_nh_3 steps on the oracle."""
 
     def __init__(self, system, v0, qq: float, qqs: float, nconstraint: int = 3,
-                 state: dict = None):
+                 state: dict = None, x0=None):
         s = system
         self.s = s
         self.o = Oracle(s)
@@ -54,7 +54,7 @@ class NvtCpu:
         self.deltsby2 = self.delts / 2.0
         self.delt_sqby2 = delt * delt / 2.0
         self.delts_sqby2 = self.delts * self.delts / 2.0
-        self.x = np.stack([s.xx, s.yy, s.zz]).astype(np.float64)
+        self.x = (np.stack([s.xx, s.yy, s.zz]) if x0 is None else np.array(x0)).astype(np.float64)
         self.v = np.array(v0, dtype=np.float64)
         self.fl, _ = self.o.frclong(*self.x)
         self.fs, _ = self.o.frcshort(*self.x)
@@ -280,3 +280,37 @@ class NptCpu:
         self.fl, self.tl = self.o.frclong(*self.x)
         self.v = v + self.deltby2 * self.fl * rm
         self.op()
+
+
+class SimanCpu:
+    """The reference's simulated annealing, siman (src/siman.c:61-182; set-up
+    init_siman, :8-59): cycles at T = t_initial, t_initial / mu_t, ... of
+    velinit at T (RANMAR seeded by rmarin(ij, jk) in init_vars, whose own
+    velinit the cycles follow) and n_tries vver_nh_3 steps from zeroed
+    thermostats, ending after the first cycle run below t_min."""
+
+    def __init__(self, system, run):
+        from tests.hmc_cpu import HmcChain
+        self.s, self.run = system, run
+        # rmarin + init_vars' velinit (its draws are consumed, its velocities
+        # replaced by the first cycle's)
+        self.chain = HmcChain(system, run.ij, run.jk, run.nconstraint)
+        self.x = self.chain.x
+        eb = system.params["epsilon_base"]
+        self.T, self.t_min = run.t_initial / eb, run.t_min / eb
+        self.cycles = 0
+
+    def anneal(self):
+        r = self.run
+        while True:
+            self.chain.treq = self.T
+            v = self.chain.velinit()
+            nvt = NvtCpu(self.s, v, r.qq, r.qqs, r.nconstraint, x0=self.x)
+            nvt.treq = self.T
+            for _ in range(r.n_tries):
+                nvt.step()
+            self.x, self.v, self.nvt = nvt.x, nvt.v, nvt
+            self.cycles += 1
+            if self.T < self.t_min:
+                return self
+            self.T /= r.mu_t

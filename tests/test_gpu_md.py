@@ -217,3 +217,81 @@ def test_nvt_restart_from_checkpoint_matches_reference():
     st = eng.nvt_get()[0]
     assert abs(st["ps"] - float(g["md_ps"])) <= 1e-8 * max(abs(float(g["md_ps"])), 1e-3)
     eng.close()
+
+
+def _siman_buffers(eng, x0):
+    import torch
+    dev = torch.device("cuda:0")
+    pos = torch.tensor(x0, dtype=torch.float64, device=dev).contiguous()
+    vel = torch.zeros_like(pos)
+    fl, fs = torch.zeros_like(pos), torch.zeros_like(pos)
+    lt = torch.zeros(eng.nrep * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(eng.nrep * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(eng.nrep, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    return pos, vel, fl, fs, lt, st, uk
+
+
+def test_siman_matches_reference():
+    """Device simulated annealing (mspms_siman_run_device) against the
+    reference's own siman (tests/golden/siman_lj256), FP64."""
+    import torch
+    path = os.path.join(GOLDEN, "siman_lj256")
+    rd = runio.RunDir.read(path)
+    s = System.from_rundir(rd)
+    g = np.load(os.path.join(path, "siman.npz"))
+    eb = s.params["epsilon_base"]
+    eng = Engine(s, precision="fp64")
+    pos, vel, fl, fs, lt, st, uk = _siman_buffers(eng, np.stack([s.xx, s.yy, s.zz]))
+    nfree = 3 * s.natom - rd.run.nconstraint
+    # rmarin(ij, jk) and init_vars' velinit at t_initial (init_siman sets treq)
+    eng.hmc_init(rd.run.ij, rd.run.jk, rd.run.t_initial / eb, rd.run.nconstraint, vel)
+    eng.nvt_set(rd.run.qq, rd.run.qqs, rd.run.t_initial / eb, nfree)
+    cyc, tf = eng.siman_run_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, rd.run.n_tries,
+                                   rd.run.t_initial / eb, rd.run.t_min / eb, rd.run.mu_t, lt, st,
+                                   uk)
+    torch.cuda.synchronize()
+    eng.sync()
+    x, v = pos.cpu().numpy(), vel.cpu().numpy()
+    print(f"siman cycles {cyc} T {tf} x {rel(x, g['x']):.2e} v {rel(v, g['v']):.2e}")
+    assert cyc[0] == 3 and tf[0] == float(g["treq"])
+    assert rel(x, g["x"]) <= 1e-9 and rel(v, g["v"]) <= 1e-9
+    eng.close()
+
+
+def test_siman_replica_batch_matches_restatement():
+    """Three LJ boxes with their own seeds and schedules (2, 3 and 4
+    cycles) annealed in one table; each equals the restatement of its own."""
+    import torch
+    from tests.md_cpu import SimanCpu
+    boxes, sched = [], [(150.0, 100.0, 1.6), (150.0, 80.0, 1.5), (200.0, 70.0, 1.5)]
+    for k, (ti, tm, mu) in enumerate(sched):
+        b = configs.lj_fcc(ncell=3, jitter=0.05, seed=30 + k)
+        b.run.what_simulation, b.run.what_ensemble = 2, 1
+        b.run.ij, b.run.jk = 1000 + k, 2000 + k
+        b.run.n_tries = 6
+        b.run.t_initial, b.run.t_min, b.run.mu_t = ti, tm, mu
+        boxes.append(b)
+    systems = [system_of(b) for b in boxes]
+    ref = [SimanCpu(s, b.run).anneal() for s, b in zip(systems, boxes)]
+    eng = Engine.replicas(systems, precision="fp64")
+    eb = systems[0].params["epsilon_base"]
+    pos, vel, fl, fs, lt, st, uk = _siman_buffers(eng, eng.packed_positions())
+    eng.hmc_init([b.run.ij for b in boxes], [b.run.jk for b in boxes],
+                 [b.run.t_initial / eb for b in boxes], [b.run.nconstraint for b in boxes], vel)
+    eng.nvt_set([b.run.qq for b in boxes], [b.run.qqs for b in boxes],
+                [b.run.t_initial / eb for b in boxes],
+                [3 * s.natom - b.run.nconstraint for s, b in zip(systems, boxes)])
+    cyc, tf = eng.siman_run_device(pos, vel, fl, fs, systems[0].delt, 1, 6,
+                                   [b.run.t_initial / eb for b in boxes],
+                                   [b.run.t_min / eb for b in boxes],
+                                   [b.run.mu_t for b in boxes], lt, st, uk)
+    torch.cuda.synchronize()
+    eng.sync()
+    x, v = eng.unpack(pos.cpu().numpy()), eng.unpack(vel.cpu().numpy())
+    for r, c in enumerate(ref):
+        print(f"replica {r}: cycles {cyc[r]} / {c.cycles} x {rel(x[r], c.x):.2e} "
+              f"v {rel(v[r], c.v):.2e}")
+        assert cyc[r] == c.cycles and tf[r] == c.T
+        assert rel(x[r], c.x) <= 1e-9 and rel(v[r], c.v) <= 1e-9
+    eng.close()

@@ -82,3 +82,19 @@ def test_nvt_extended_energy_is_conserved(name, nstep):
         h.append(c.ukin + up + c.unhts + c.unhtss)
         k.append(c.ukin)
     assert np.std(h) <= 0.1 * np.std(k)
+
+
+def test_siman_restatement_matches_reference():
+    """The annealing restatement (tests/md_cpu.py SimanCpu) against the
+    reference's own siman on LJ-256 (tests/golden/siman_lj256: three cycles,
+    150 K -> 100 K -> 66.7 K, 20 NVT steps each).  velinit's numpy sums are
+    taken in another order than the reference's loop (1e-16 per velocity),
+    so the bound is 1e-12."""
+    path = os.path.join(GOLDEN, "siman_lj256")
+    rd = runio.RunDir.read(path)
+    s = System.from_rundir(rd)
+    g = np.load(os.path.join(path, "siman.npz"))
+    from tests.md_cpu import SimanCpu
+    c = SimanCpu(s, rd.run).anneal()
+    assert c.cycles == 3 and c.T == float(g["treq"])
+    assert rel(c.x, g["x"]) <= 1e-12 and rel(c.v, g["v"]) <= 1e-12
