@@ -414,6 +414,32 @@ int mspms_nvt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, d
                          double delt, int nstep_inner, int nsteps, mspms_long_terms* lterms,
                          mspms_short_terms* sterms, double* ukin, int use_graph, void* stream);
 
+/* Solid-fluid grid generation on the device: the reference's offline
+ * generator tools/sfgridgen.c:71-352.  At every node (x0 + h i, y0 + h j,
+ * h k) of a grid over one unit cell of a z-periodic solid (nx = floor(uclx /
+ * h) + 2 nodes, likewise y and z; x0 = y0 = 0 as the generator centres the
+ * cell) it sums the fluid-solid LJ potential (Lorentz-Berthelot mixing,
+ * epsilon converted to J/mol, optional switch between rcuton and rcutoff)
+ * over the solid atoms and their z images within the cutoff, in the
+ * generator's order and FP64 expressions, so that the tables equal its
+ * out.sfgridgen bit for bit.  out: 13 fields of nx*ny*nz nodes in the
+ * generator's order (energy, dE/dx, dE/dy, dE/dz, then the Jacobian of the
+ * force -grad E, dFx/dx, dFx/dy, dFx/dz, dFy/dx .. dFz/dz), node = (i ny + j)
+ * nz + k.  With the switch on, a pair in the switching shell adds the Jacobian
+ * terms of the
+ * node's previous pair (the generator's empty else branch,
+ * tools/sfgridgen.c:254-256); they start at zero here. */
+typedef struct mspms_sfgridgen_spec {
+    double f_sigma, f_epsilon;        /* fluid site (Angstrom, K)          */
+    double grid_itvl, rcuton, rcutoff;  /* Angstrom                        */
+    int isLJswitch, s_natom;          /* solid atoms: 1..1000               */
+    double uclx, ucly, uclz;          /* unit cell (Angstrom)               */
+    double s_sigma, s_epsilon;        /* solid site (Angstrom, K)           */
+    const double *sx, *sy, *sz;       /* [s_natom] solid atoms (Angstrom)   */
+} mspms_sfgridgen_spec;
+int mspms_sfgridgen_dims(const mspms_sfgridgen_spec* spec, int* nx, int* ny, int* nz);
+int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* spec, double* out);
+
 /* Simulated annealing of every replica on device buffers: the reference's
  * siman (src/siman.c:61-182; init_siman, :8-59).  Cycle k of replica r runs
  * at T = t_initial[r] / mu_t[r]^k (reduced): velinit at T from the

@@ -172,3 +172,13 @@ def fptr(a):
     if a is None:
         return _fp()
     return a.ctypes.data_as(_fp)
+
+
+class SfGridGenSpec(C.Structure):
+    """mspms_sfgridgen_spec: the grid generator's input (tools/sfgridgen.c:104-117)."""
+    _fields_ = [("f_sigma", C.c_double), ("f_epsilon", C.c_double), ("grid_itvl", C.c_double),
+                ("rcuton", C.c_double), ("rcutoff", C.c_double), ("isLJswitch", C.c_int),
+                ("s_natom", C.c_int), ("uclx", C.c_double), ("ucly", C.c_double),
+                ("uclz", C.c_double), ("s_sigma", C.c_double), ("s_epsilon", C.c_double),
+                ("sx", C.POINTER(C.c_double)), ("sy", C.POINTER(C.c_double)),
+                ("sz", C.POINTER(C.c_double))]

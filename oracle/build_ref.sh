@@ -101,3 +101,11 @@ if [ -f "$PKG/libmspms_dropin.a" ] && [ -f "$PKG/libmspms_gpu.so" ]; then
         -Wl,--unresolved-symbols=ignore-in-object-files
     echo "build_ref.sh: built $OUT/mspms_dropin_harness"
 fi
+
+# The reference's offline solid-fluid grid generator (tools/sfgridgen.c,
+# plain C; it reads its input file and writes out.sfgridgen in the current
+# directory), compiled as it stands.
+if [ -f "$REF/tools/sfgridgen.c" ]; then
+    gcc -O2 -w -o "$OUT/sfgridgen" "$REF/tools/sfgridgen.c" -lm
+    echo "build_ref.sh: built $OUT/sfgridgen"
+fi

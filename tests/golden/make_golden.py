@@ -22,6 +22,7 @@ thermostat scalars.
 import hashlib
 import os
 import shutil
+import subprocess
 import sys
 import tempfile
 
@@ -132,6 +133,26 @@ def make_siman_case():
     print("siman_lj256", d["md"]["treq"])
 
 
+def make_sfgridgen_case():
+    """The reference's grid generator (oracle/_ref/sfgridgen, built from
+    tools/sfgridgen.c) on its own CH3 / (11,11) nanotube input
+    (tools/CH3-SWNT11x11.in) at a 1 A interval (16 x 16 x 4 nodes)."""
+    path = os.path.join(HERE, "sfgridgen_ch3_1A")
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(REF, "tools", "CH3-SWNT11x11.in")) as f:
+        lines = f.read().split("\n")
+    t = lines[1].split()
+    lines[1] = " ".join(["1.0"] + t[1:])
+    with open(os.path.join(path, "grid.in"), "w") as f:
+        f.write("\n".join(lines))
+    exe = os.path.join(ROOT, "oracle", "_ref", "sfgridgen")
+    with tempfile.TemporaryDirectory() as tmp:
+        subprocess.run([exe, os.path.join(path, "grid.in")], cwd=tmp, check=True,
+                       capture_output=True)
+        shutil.copy(os.path.join(tmp, "out.sfgridgen"), os.path.join(path, "out.sfgridgen"))
+    print("sfgridgen_ch3_1A", os.path.getsize(os.path.join(path, "out.sfgridgen")))
+
+
 def make_file_cases():
     name, n = FILE_CASE
     src = os.path.join(HERE, name)
@@ -184,6 +205,7 @@ def main():
             print(name, n, d["md"]["ukin"], d["md"]["ps"])
     make_file_cases()
     make_siman_case()
+    make_sfgridgen_case()
     for name, fn in BIG.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)

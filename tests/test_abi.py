@@ -61,6 +61,8 @@ int main(void) {
   O(mspms_params, ntube) O(mspms_params, solid_natom) O(mspms_params, grid)
   O(mspms_long_terms, virial_inter) O(mspms_long_terms, elec_pair_hash)
   O(mspms_sf_grid, nodes)
+  S(mspms_sfgridgen_spec) O(mspms_sfgridgen_spec, isLJswitch) O(mspms_sfgridgen_spec, uclx)
+  O(mspms_sfgridgen_spec, s_epsilon) O(mspms_sfgridgen_spec, sz)
   return 0;
 }
 """
@@ -79,7 +81,7 @@ def test_ctypes_layout_matches_c():
           "mspms_long_terms": abi.LongTerms, "mspms_short_terms": abi.ShortTerms,
           "mspms_sf_grid": abi.SFGrid, "mspms_param_grad": abi.ParamGrad,
           "mspms_tune_result": abi.TuneResult, "mspms_nvt_state": abi.NvtState,
-          "mspms_npt_state": abi.NptState}
+          "mspms_npt_state": abi.NptState, "mspms_sfgridgen_spec": abi.SfGridGenSpec}
     for k, v in got.items():
         if "." in k:
             t, f = k.split(".")
