@@ -69,7 +69,7 @@ extern "C" {
 #define MSPMS_SF_HYPERGEO          1
 #define MSPMS_SF_ATOM_EXPLICIT     2
 #define MSPMS_SF_TASOS             3 /* tricubic Hermite grid (mylibtasos) */
-#define MSPMS_SF_MY_INTERP         4 /* not supported: no generator exists */
+#define MSPMS_SF_MY_INTERP         4 /* 32-coefficient cube polynomials   */
 
 #define MSPMS_ELEC_NONE           0
 #define MSPMS_ELEC_EWALD          1
@@ -143,6 +143,26 @@ typedef struct mspms_sf_grid {
     const float* nodes;        /* [ntype][8][nz][ny][nx] FP32             */
 } mspms_sf_grid;
 
+/* Cube polynomials of the "myinterp" model, the data init_my_interp reads
+ * (src/sffrc.c:228-748, globals src/vars.h:319-472).  Per cube and table
+ * type, four polynomials (energy in J/mol, then the force components in
+ * J/mol/Angstrom) of 32 coefficients each over the monomials of
+ * Amatrix_ele_assign_value (src/sffrc.c:750-797) in ABSOLUTE Angstrom
+ * coordinates.  tables[t*128 + f*32 + m] points at the ncube values of
+ * coefficient m of polynomial f (the reference's ene0..31, fxa0..31,
+ * fya0..31, fza0..31 of type t), cube = (ix*ncy + iy)*ncz + iz.  A site at
+ * x (Angstrom) uses cube ix = (int)((x - xmin)/hx), likewise y; z is
+ * wrapped into [0, zmax) first (src/sffrc.c:969-983).  Table type =
+ * interp_type - 1 (init_my_interp decrements it, src/sffrc.c:249-257). */
+typedef struct mspms_sf_cubes {
+    int ntype;                 /* grid files (unique atoms), 1..5         */
+    int ncx, ncy, ncz;         /* cubes per axis = nodes - 1              */
+    double xmin, ymin, zmax;   /* Angstrom: xcenter - uclx/2, ycenter -
+                                  ucly/2, uclz                            */
+    double hx, hy, hz;         /* grid interval (Angstrom)                */
+    const double* const* tables;  /* [ntype*128] host arrays of ncube     */
+} mspms_sf_cubes;
+
 /* ---- run parameters ---------------------------------------------------------
  * Field names follow src/vars.h:30-212.  Fill the "input" fields, then call
  * mspms_derive_params() to compute the derived constants with the exact
@@ -186,6 +206,9 @@ typedef struct mspms_params {
 
     /* SF_TASOS */
     const mspms_sf_grid* grid;
+
+    /* SF_MY_INTERP */
+    const mspms_sf_cubes* cubes;
 } mspms_params;
 
 /* Outputs of frclong: the globals src/frclong.c writes (src/vars.h:173-210),
@@ -439,6 +462,28 @@ typedef struct mspms_sfgridgen_spec {
 } mspms_sfgridgen_spec;
 int mspms_sfgridgen_dims(const mspms_sfgridgen_spec* spec, int* nx, int* ny, int* nz);
 int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* spec, double* out);
+
+/* Generator of the myinterp cube polynomials (the reference reads them,
+ * src/sffrc.c:228-748, but ships no program that writes them).
+ *
+ * mspms_myinterp_nodes: at the nodes of mspms_sfgridgen (same spec, same
+ * order node = (i ny + j) nz + k, switch off only) the fluid-solid LJ sum
+ * and its derivatives up to third order, as node_data[q*nnode + node] with
+ * q = f*7 + d for f in (E, Fx, Fy, Fz) (J/mol and J/mol/Angstrom, F = -grad
+ * E) and d in (value, d/dx, d/dy, d/dz, d2/dxdy, d2/dxdz, d2/dydz).
+ *
+ * mspms_myinterp_fit: per cube of the node lattice (nx-1 by ny-1 by nz-1
+ * cubes, lower corner (x0 + hx i, y0 + hy j, z0 + hz k)) and per
+ * polynomial f, the 32 coefficients that fit the value and gradient at the
+ * eight corners (least squares: the 32 conditions have rank 29); the three
+ * functions that vanish with their gradient at every corner, u v (1-u)(1-v)
+ * and its xz and yz analogues in cube-local coordinates, are added with the
+ * weights that fit the mixed second derivatives at the corners.  A
+ * polynomial of the 32-monomial space is reproduced exactly.  Output as the
+ * file stores it: coef[(f*32 + m)*ncube + cube], absolute coordinates. */
+int mspms_myinterp_nodes(int device, const mspms_sfgridgen_spec* spec, double* node_data);
+int mspms_myinterp_fit(int device, int nx, int ny, int nz, double x0, double y0, double z0,
+                       double hx, double hy, double hz, const double* node_data, double* coef);
 
 /* Simulated annealing of every replica on device buffers: the reference's
  * siman (src/siman.c:61-182; init_siman, :8-59).  Cycle k of replica r runs

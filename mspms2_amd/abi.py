@@ -58,6 +58,16 @@ class SFGrid(C.Structure):
     ]
 
 
+class SFCubes(C.Structure):
+    """mspms_sf_cubes: myinterp cube polynomials (init_my_interp's data)."""
+    _fields_ = [
+        ("ntype", C.c_int), ("ncx", C.c_int), ("ncy", C.c_int), ("ncz", C.c_int),
+        ("xmin", C.c_double), ("ymin", C.c_double), ("zmax", C.c_double),
+        ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+        ("tables", C.POINTER(C.POINTER(C.c_double))),
+    ]
+
+
 class Params(C.Structure):
     _fields_ = [
         ("sigma_base", C.c_double), ("epsilon_base", C.c_double),
@@ -80,6 +90,7 @@ class Params(C.Structure):
         ("solid_epsilon", C.c_double),
         ("solid_natom", C.c_int), ("solid_xx", _dp), ("solid_yy", _dp), ("solid_zz", _dp),
         ("grid", C.POINTER(SFGrid)),
+        ("cubes", C.POINTER(SFCubes)),
     ]
 
 

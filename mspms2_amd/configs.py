@@ -450,3 +450,71 @@ def methane_tube(natom: int = 2000, length: float = 400.0, box_xy: float = 100.0
         Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.016043, epsilon=148.0,
              sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
     return RunDir(run=run, sysname="methane-tube", species=[sp], xyz=xyz)
+
+
+# --------------------------------------------------------------------------- #
+def tube_cell_fluid(uclx: float, ucly: float, uclz: float, ncell_z: int = 12,
+                    n_ch3: int = 16, n_dimer: int = 4, r_fill: float = 3.5,
+                    rc: float = 7.0, seed: int = 21, min_dist: float = 3.3,
+                    files=("ch3.cubes", "ch2.cubes")) -> RunDir:
+    """United-atom CH3 sites (3.75 A, 98 K, table type 1) and CH2-CH2 dimers
+    (3.95 A, 46 K, bond 1.54 A, table type 2) inside a nanotube whose unit
+    cell (uclx, ucly, uclz, Angstrom) the myinterp cube files describe; the
+    box is one cell across and ncell_z cells long, the fluid within r_fill of
+    the axis (uclx/2, ucly/2).  The MYINTERP section names `files`; the
+    caller supplies their contents (RunDir.cube_files)."""
+    rng = np.random.default_rng(seed)
+    Lz = ncell_z * uclz
+    cx, cy = uclx / 2.0, ucly / 2.0
+    pts = []
+
+    def fits(p):
+        for q in pts:
+            d = p - q
+            d[2] -= Lz * np.rint(d[2] / Lz)
+            if d @ d < min_dist * min_dist:
+                return False
+        return True
+
+    def point():
+        r = r_fill * math.sqrt(rng.random())
+        th = 2 * math.pi * rng.random()
+        return np.array([cx + r * math.cos(th), cy + r * math.sin(th), Lz * rng.random()])
+
+    ch3 = []
+    while len(ch3) < n_ch3:
+        p = point()
+        if fits(p):
+            pts.append(p)
+            ch3.append(p)
+    dim = []
+    bond = 1.54
+    while len(dim) < n_dimer:
+        p = point()
+        u = rng.normal(size=3)
+        u[:2] *= 0.3                      # roughly along the axis
+        u /= np.linalg.norm(u)
+        q = p + bond * u
+        if math.hypot(q[0] - cx, q[1] - cy) > r_fill or not fits(p) or not fits(q):
+            continue
+        pts += [p, q]
+        dim.append((p, q))
+    xyz = np.array(ch3 + [v for pq in dim for v in pq])
+    xyz[:, 2] %= Lz
+    sp1 = SpeciesSpec(name="CH3", nmole=n_ch3, sites=[
+        Site(type=0, name="CH3", xx=0.0, yy=0.0, zz=0.0, aw=0.015035, epsilon=98.0,
+             sigma=3.75, charge=0.0, ghost_type=0, interp_type=1)])
+    sp2 = SpeciesSpec(name="ethylene-ua", nmole=n_dimer, sites=[
+        Site(type=1, name="CH2", xx=0.0, yy=0.0, zz=0.0, aw=0.014027, epsilon=46.0,
+             sigma=3.95, charge=0.0, ghost_type=0, interp_type=2),
+        Site(type=1, name="CH2", xx=bond, yy=0.0, zz=0.0, aw=0.014027, epsilon=46.0,
+             sigma=3.95, charge=0.0, ghost_type=0, interp_type=2)],
+        bonds=[(0, 1, abi.BOND_HARMONIC, 905800.0, bond, 0.0)])
+    rcut = min(rc, 0.5 * min(uclx, ucly) * (1 - 1e-12))
+    run = RunInput(title="fluid in a nanotube cell (myinterp)", sigma_base=1.0,
+                   epsilon_base=1.0, mass_base=1.0, treq=300.0, boxlx=uclx, boxly=ucly,
+                   boxlz=Lz, rcuton=rcut, rcutoff=rcut, rcutoffelec=rcut, isLJlrcOn=0,
+                   delt=1.0, nstep_inner=4, iChargeType=abi.ELEC_NONE,
+                   iSF_type=abi.SF_MY_INTERP, mi_ucl=(uclx, ucly, uclz),
+                   mi_center=(cx, cy, 0.0), mi_files=list(files))
+    return RunDir(run=run, sysname="tube-fluid", species=[sp1, sp2], xyz=xyz)

@@ -44,6 +44,15 @@ extern double const_3pisq_theta;
 extern double *solid_sigma, *solid_epsilon;
 extern int solid_natom;
 extern double *solid_xx, *solid_yy, *solid_zz;
+/* myinterp cube tables (src/vars.h:319-472), NUNIQUE_ATOM_MAX = 5 */
+#define MI_32(X)                                                                        \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) \
+    X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29)  \
+    X(30) X(31)
+#define MI_DECL(k) extern double *ene##k[], *fxa##k[], *fya##k[], *fza##k[];
+MI_32(MI_DECL)
+extern double xmin, ymin, zmax, grid_itvl_x, grid_itvl_y, grid_itvl_z;
+extern int ncube_x, ncube_y, ncube_z;
 
 /* ---- cached device context ------------------------------------------------ */
 #define MAXSP 3 /* NSPECIE_MAX, src/molevars.h:15 */
@@ -54,6 +63,11 @@ static mspms_params g_params;
 static double g_bnd[MAXSP][3 * REF_SAMPLE_NTERM_MAX], g_agl[MAXSP][5 * REF_SAMPLE_NTERM_MAX],
     g_dih[MAXSP][4 * REF_SAMPLE_NTERM_MAX], g_imp[MAXSP][2 * REF_SAMPLE_NTERM_MAX];
 static unsigned char g_au[MAXSP][REF_SAMPLE_NTERM_MAX], g_du[MAXSP][REF_SAMPLE_NTERM_MAX];
+static const double* g_mi_tables[5 * 128];
+static mspms_sf_cubes g_cubes;
+/* interp_type as the cfg file gives it: init_my_interp decrements the
+ * reference's copy in place (src/sffrc.c:249-257) */
+static int g_interp[MAXSP][REF_SAMPLE_NATOM_MAX];
 
 static void die(const char* what, int rc)
 {
@@ -116,6 +130,31 @@ static void read_params(mspms_params* p)
         p->solid_zz = solid_zz;
         p->solid_sigma = solid_sigma ? *solid_sigma : 0.0;
         p->solid_epsilon = solid_epsilon ? *solid_epsilon : 0.0;
+    } else if (iSF_type == MSPMS_SF_MY_INTERP) {
+        /* init_my_interp filled ene0..fza31[t] for its nunique_atom files */
+        int t, n = 0;
+        while (n < 5 && ene0[n]) n++;
+        for (t = 0; t < n; t++) {
+#define MI_FILL(k)                                \
+    g_mi_tables[t * 128 + (k)] = ene##k[t];       \
+    g_mi_tables[t * 128 + 32 + (k)] = fxa##k[t];  \
+    g_mi_tables[t * 128 + 64 + (k)] = fya##k[t];  \
+    g_mi_tables[t * 128 + 96 + (k)] = fza##k[t];
+            MI_32(MI_FILL)
+#undef MI_FILL
+        }
+        g_cubes.ntype = n;
+        g_cubes.ncx = ncube_x;
+        g_cubes.ncy = ncube_y;
+        g_cubes.ncz = ncube_z;
+        g_cubes.xmin = xmin;
+        g_cubes.ymin = ymin;
+        g_cubes.zmax = zmax;
+        g_cubes.hx = grid_itvl_x;
+        g_cubes.hy = grid_itvl_y;
+        g_cubes.hz = grid_itvl_z;
+        g_cubes.tables = g_mi_tables;
+        p->cubes = &g_cubes;
     }
 }
 
@@ -129,7 +168,12 @@ static void build_species(mspms_species* out)
         q->natom = m->natom;
         q->nmole = nmole_per_specie[s];
         q->ghost_type = m->ghost_type;
-        q->interp_type = m->interp_type;
+        if (iSF_type == MSPMS_SF_MY_INTERP) {
+            for (k = 0; k < m->natom; k++) g_interp[s][k] = m->interp_type[k] + 1;
+            q->interp_type = g_interp[s];
+        } else {
+            q->interp_type = m->interp_type;
+        }
         q->aw = m->aw;
         q->sigma = m->sigma;
         q->epsilon = m->epsilon;

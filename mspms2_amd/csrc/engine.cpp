@@ -172,6 +172,8 @@ struct mspms_ctx {
     DevBuf<float> d_sf_nodes;
     size_t sf_nodes_uploaded = 0;
     const float* sf_nodes_src = nullptr;
+    DevBuf<double> d_sf_cubes;                  // [ntype][cube][128]
+    std::vector<const double*> sf_cubes_src;    // the tables last uploaded
     DevBuf<unsigned long long> d_ip_pair;
     DevBuf<mspms_long_terms> d_long;
     DevBuf<mspms_short_terms> d_short;
@@ -401,10 +403,8 @@ int validate_params(const mspms_params& p) {
             p.kmaxz > 16)
             return fail(MSPMS_EINVAL, "KMAX must be within 0..16");
     }
-    if (p.iSF_type < MSPMS_SF_NONE || p.iSF_type > MSPMS_SF_TASOS)
-        return fail(MSPMS_EINVAL,
-                    "solid-fluid type %d unsupported (4, my-interp, has no grid generator)",
-                    p.iSF_type);
+    if (p.iSF_type < MSPMS_SF_NONE || p.iSF_type > MSPMS_SF_MY_INTERP)
+        return fail(MSPMS_EINVAL, "solid-fluid type %d unsupported", p.iSF_type);
     if (p.iSF_type == MSPMS_SF_HYPERGEO && (p.ntube < 1 || !p.tube_x || !p.tube_y || !p.tube_r))
         return fail(MSPMS_EINVAL, "hypergeometric tube needs ntube >= 1 and tube arrays");
     if (p.iSF_type == MSPMS_SF_ATOM_EXPLICIT && p.solid_natom > 0 &&
@@ -414,6 +414,16 @@ int validate_params(const mspms_params& p) {
         (!p.grid || !p.grid->nodes || p.grid->ntype < 1 || p.grid->nx < 2 || p.grid->ny < 2 ||
          p.grid->nz < 2 || !(p.grid->hx > 0) || !(p.grid->hy > 0) || !(p.grid->hz > 0)))
         return fail(MSPMS_EINVAL, "SF grid missing or malformed");
+    if (p.iSF_type == MSPMS_SF_MY_INTERP) {
+        const mspms_sf_cubes* q = p.cubes;
+        // NUNIQUE_ATOM_MAX = 5 grid files (src/defs.h:78)
+        if (!q || !q->tables || q->ntype < 1 || q->ntype > 5 || q->ncx < 1 || q->ncy < 1 ||
+            q->ncz < 1 || !(q->hx > 0) || !(q->hy > 0) || !(q->hz > 0) || !(q->zmax > 0) ||
+            (double)q->ncx * q->ncy * q->ncz > 2147483647.0)
+            return fail(MSPMS_EINVAL, "myinterp cube tables missing or malformed");
+        for (int k = 0; k < 128 * q->ntype; k++)
+            if (!q->tables[k]) return fail(MSPMS_EINVAL, "myinterp table %d is null", k);
+    }
     return 0;
 }
 
@@ -1021,6 +1031,42 @@ int build(mspms_ctx* c) {
             if ((rc = c->d_sf_solid.upload(t, s))) return rc;
             F.nsolid = P.solid_natom;
             F.solid = c->d_sf_solid.p;
+        } else if (F.type == MSPMS_SF_MY_INTERP) {
+            const mspms_sf_cubes& Q = *P.cubes;
+            const size_t ncube = (size_t)Q.ncx * Q.ncy * Q.ncz, nt = (size_t)Q.ntype;
+            std::vector<const double*> src(Q.tables, Q.tables + 128 * nt);
+            // the tables are large (up to GiB): upload once per source
+            if (c->sf_cubes_src != src || c->d_sf_cubes.n < nt * ncube * 128) {
+                DevBuf<double> stage;
+                if ((rc = c->d_sf_cubes.ensure(nt * ncube * 128)) ||
+                    (rc = stage.ensure(nt * ncube * 128)))
+                    return rc;
+                hipError_t e = hipSuccess;
+                for (size_t k = 0; k < 128 * nt && e == hipSuccess; k++)
+                    e = hipMemcpyAsync(stage.p + k * ncube, Q.tables[k], ncube * sizeof(double),
+                                       hipMemcpyHostToDevice, s);
+                if (e == hipSuccess) {
+                    launch_cube_major(s, stage.p, c->d_sf_cubes.p, (int)nt, (int)ncube);
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                stage.release();
+                if (e != hipSuccess)
+                    return fail(MSPMS_EIO, "myinterp table upload: %s", hipGetErrorString(e));
+                c->sf_cubes_src = src;
+            }
+            F.ntype = Q.ntype;
+            F.nx = Q.ncx;
+            F.ny = Q.ncy;
+            F.nz = Q.ncz;
+            F.x0 = Q.xmin;
+            F.y0 = Q.ymin;
+            F.z0 = 0.0;
+            F.zmax = Q.zmax;
+            F.hx = Q.hx;
+            F.hy = Q.hy;
+            F.hz = Q.hz;
+            F.cubes = c->d_sf_cubes.p;
         } else {
             const mspms_sf_grid& G = *P.grid;
             size_t nn = (size_t)G.nx * G.ny * G.nz, tot = nn * 8 * (size_t)G.ntype;

@@ -176,6 +176,9 @@ struct SFDev {
     double sigma_base, epsilon_base;
     const float* nodes;      // [ntype][8][nz][ny][nx]
     const double* min_pe;    // [ntype] min node F (mylibtasos/grid.f90:102)
+    // SF_MY_INTERP: nx, ny, nz = cubes per axis, x0, y0 = xmin, ymin
+    double zmax;
+    const double* cubes;     // [ntype][cube][4][32]
 };
 
 // RANMAR stream of one replica (src/random.c:14-109) and the cached second
@@ -237,6 +240,8 @@ void launch_nvt_solve(const LaunchCtx& c, mspms_nvt_state* st, bool outer, doubl
                       double dt, double dtby2, double dt_sqby2, double* ukin,
                       const int* active);
 
+// [type][128][ncube] (the file's order) -> [type][cube][128]
+void launch_cube_major(hipStream_t s, const double* src, double* dst, int ntype, int ncube);
 void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,
                double* fz, double* usf);

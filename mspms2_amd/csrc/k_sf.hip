@@ -8,6 +8,9 @@
 //                     (call_tasos_forces, src/sffrc.c:1322-1337 ->
 //                     CForce_atom, mylibtasos/forces.f90:6-173): a gather of
 //                     8 corners x 8 fields = 256 B per site, FP64 arithmetic
+//   SF_MY_INTERP      32-monomial polynomials per cube (get_values_from_grid,
+//                     src/sffrc.c:951-1089): one contiguous KiB per site
+//                     (cube-major tables), FP64, the reference's order
 // Forces are added to the long-range forces (the site's own thread is the
 // only writer); the energy goes to a per-site partial for the finalize.
 #include "engine.h"
@@ -205,6 +208,53 @@ __global__ void __launch_bounds__(256) k_sf(int natom, const int* __restrict__ a
         fx[i] += -gx * fconv;
         fy[i] += -gy * fconv;
         fz[i] += -gz * fconv;
+    } else if (sf.type == MSPMS_SF_MY_INTERP) {
+        // get_values_from_grid (src/sffrc.c:951-1089) and its caller
+        // (src/frclong.c:1193-1207): the same expressions in the same order
+        const int ty = S.interp - 1;  // init_my_interp decrements interp_type
+        if (ty < 0 || ty >= sf.ntype) {
+            atomicOr(&st->flags, ST_GRID_EDOM);
+            usf[i] = 0.0;
+            return;
+        }
+        const double fxx = x[i] * sf.sigma_base, fyy = y[i] * sf.sigma_base;
+        double fzz = z[i] * sf.sigma_base;
+        const double tx = (fxx - sf.x0) / sf.hx, tyy = (fyy - sf.y0) / sf.hy;
+        fzz = fzz - sf.zmax * rint(fzz / sf.zmax);  // periodic in z
+        if (fzz < 0) fzz = sf.zmax + fzz;
+        const double tz = (fzz - 0.0) / sf.hz;      // zmin = 0
+        // a cube index outside the tables (the reference reads past them)
+        if (!(tx > -1.0 && tx < sf.nx && tyy > -1.0 && tyy < sf.ny && tz > -1.0 && tz < sf.nz)) {
+            atomicOr(&st->flags, ST_GRID_EDOM);
+            usf[i] = 0.0;
+            return;
+        }
+        const int iAx = (int)tx, iAy = (int)tyy, iAz = (int)tz;
+        const int cubeidx = iAx * sf.ny * sf.nz + iAy * sf.nz + iAz;
+        const double* q =
+            sf.cubes + ((size_t)ty * ((size_t)sf.nx * sf.ny * sf.nz) + (size_t)cubeidx) * 128;
+        // Amatrix_ele_assign_value (src/sffrc.c:750-803)
+        const double X = fxx, Y = fyy, Z = fzz;
+        const double x2 = X * X, x3 = x2 * X, y2 = Y * Y, y3 = y2 * Y, z2 = Z * Z, z3 = z2 * Z;
+        const double v[32] = {1.0,        X,          Y,          Z,      x2,     X * Y,
+                              X * Z,      y2,         Y * Z,      z2,     x3,     x2 * Y,
+                              X * y2,     x2 * Z,     X * z2,     y3,     y2 * Z, Y * z2,
+                              z3,         X * Y * Z,  x3 * Y,     x2 * y2, X * y3, x3 * Z,
+                              x2 * z2,    X * z3,     y3 * Z,     y2 * z2, Y * z3, x2 * Y * Z,
+                              X * y2 * Z, X * Y * z2};
+        double r[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            double a = q[f * 32] * v[0];
+#pragma unroll
+            for (int m = 1; m < 32; m++) a = a + q[f * 32 + m] * v[m];
+            r[f] = a;
+        }
+        const double R_RGAS = 0.120272219;  // src/defs.h:96
+        u = r[0] * R_RGAS / sf.epsilon_base;
+        fx[i] += r[1] * R_RGAS * sf.sigma_base / sf.epsilon_base;
+        fy[i] += r[2] * R_RGAS * sf.sigma_base / sf.epsilon_base;
+        fz[i] += r[3] * R_RGAS * sf.sigma_base / sf.epsilon_base;
     }
     usf[i] = u;
 }

@@ -130,6 +130,12 @@ class RunInput:
     t_initial: float = 300.0
     t_min: float = 100.0
     mu_t: float = 1.5
+    # MYINTERP (init_my_interp, src/sffrc.c:262-297): unit cell and centre
+    # (Angstrom), one cube-polynomial file per table type; written only
+    # when files are named
+    mi_ucl: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    mi_center: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    mi_files: List[str] = field(default_factory=list)
 
 
 _FIXED = [  # (attribute, format) in src/readins.c:31-82 order
@@ -186,6 +192,14 @@ def write_in(path: str, r: RunInput) -> None:
     lines.append("   SIMAN")
     lines.append(f"{r.n_tries} {r.iters_fixed_t}      n_tries iters_fixed_t")
     lines.append(f"{_g(r.t_initial)} {_g(r.t_min)} {_g(r.mu_t)}      t_initial t_min mu_t (K)")
+    if r.mi_files:
+        lines.append("")
+        lines.append("   MYINTERP")
+        lines.append(" ".join(repr(float(v)) for v in r.mi_ucl) + "      uclx ucly uclz (A)")
+        lines.append(" ".join(repr(float(v)) for v in r.mi_center) +
+                     "      xcenter ycenter zcenter (A)")
+        lines.append(f"{len(r.mi_files):<24d} nunique_atom")
+        lines += list(r.mi_files)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -237,6 +251,12 @@ def read_in(path: str) -> RunInput:
         a, b = lines[i].split(), lines[i + 1].split()
         r.n_tries, r.iters_fixed_t = int(a[0]), int(a[1])
         r.t_initial, r.t_min, r.mu_t = float(b[0]), float(b[1]), float(b[2])
+    i = _section(lines, "MYINTERP")
+    if i is not None:
+        r.mi_ucl = tuple(float(v) for v in lines[i].split()[:3])
+        r.mi_center = tuple(float(v) for v in lines[i + 1].split()[:3])
+        n = int(lines[i + 2].split()[0])
+        r.mi_files = [lines[i + 3 + k].split()[0] for k in range(n)]
     return r
 
 
@@ -358,6 +378,8 @@ class RunDir:
     # loadit, src/initvars.c:678-681): it replaces the positions, the
     # velocities, the thermostat / barostat variables and the box
     checkpoint: Optional[object] = None
+    # myinterp cube files (file name of the MYINTERP section -> CubeFile)
+    cube_files: Optional[dict] = None
 
     @property
     def natom(self) -> int:
@@ -378,6 +400,10 @@ class RunDir:
         if self.checkpoint is not None:
             from .rundata import LOAD, write_checkpoint
             write_checkpoint(os.path.join(path, LOAD), self.checkpoint)
+        if self.cube_files:
+            from .myinterp import write_cubes
+            for name, cf in self.cube_files.items():
+                write_cubes(os.path.join(path, name), cf)
         return path
 
     @staticmethod
@@ -391,4 +417,9 @@ class RunDir:
         if run.iStart_option != 1 and os.path.exists(old):
             from .rundata import read_checkpoint
             ck = read_checkpoint(old, natom)
-        return RunDir(run=run, sysname=sysname, species=species, xyz=xyz, checkpoint=ck)
+        cubes = None
+        if run.iSF_type == 4 and run.mi_files:
+            from .myinterp import read_cubes
+            cubes = {n: read_cubes(os.path.join(path, n)) for n in run.mi_files}
+        return RunDir(run=run, sysname=sysname, species=species, xyz=xyz, checkpoint=ck,
+                      cube_files=cubes)

@@ -142,6 +142,7 @@ class System:
     v0: Optional[np.ndarray] = None  # velocities of a restart (old.mspms), reduced
     checkpoint: Optional[object] = None
     grid: Optional[dict] = None      # SF_TASOS grid (mspms_sf_grid fields)
+    cubes: Optional[dict] = None     # SF_MY_INTERP tables (myinterp.engine_tables)
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -195,11 +196,18 @@ class System:
             yy = np.ascontiguousarray(ck.x[1])
             zz = np.ascontiguousarray(ck.x[2])
             p.update(boxlx=ck.box[0], boxly=ck.box[1], boxlz=ck.box[2])
+        cubes = None
+        if r.iSF_type == abi.SF_MY_INTERP:
+            from .myinterp import engine_tables
+            if not rd.cube_files:
+                raise ValueError("iSF_type 4 needs the MYINTERP section and its cube files")
+            cubes = engine_tables(r.mi_ucl, r.mi_center,
+                                  [rd.cube_files[n] for n in r.mi_files])
         pressure_base = BOLTZMAN_CONSTANT * eb * 1.0e25 / (sb * sb * sb)
         s = System(species=species, params=p, xx=xx, yy=yy, zz=zz, run=r,
                    time_base=time_base, delt=r.delt / time_base, treq=r.treq / eb,
                    preq=r.preq / pressure_base,
-                   v0=None if ck is None else np.array(ck.v), checkpoint=ck)
+                   v0=None if ck is None else np.array(ck.v), checkpoint=ck, cubes=cubes)
         return s
 
     # ------------------------------------------------------------------ #
@@ -270,5 +278,10 @@ class System:
             g.nodes = abi.fptr(nodes)
             keep.append(g)
             p.grid = C.pointer(g)
+        if self.cubes is not None:
+            from .myinterp import c_cubes
+            q, k2 = c_cubes(self.cubes)
+            keep += k2 + [q]
+            p.cubes = C.pointer(q)
         p._keep = keep
         return p

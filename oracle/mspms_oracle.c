@@ -682,6 +682,49 @@ int oracle_hermite_site(const mspms_sf_grid* g, int itype, double min_pe,
     return MSPMS_OK;
 }
 
+/* get_values_from_grid (src/sffrc.c:951-1089): the position in Angstrom
+ * (reduced times sigma_base), the cube of its lower corner, the 32 monomials
+ * of Amatrix_ele_assign_value (:750-803) and four left-to-right dot
+ * products; J/mol and J/mol/Angstrom out.  Returns EDOM for a cube index
+ * outside the tables (the reference reads past them). */
+static int oracle_myinterp_site(const mspms_sf_cubes* q, int type, double fxx, double fyy,
+                                double fzz, double* usf, double fsf[3])
+{
+    double tx, ty, tz, x2, x3, y2, y3, z2, z3, v[32];
+    const double* const* tb = q->tables + 128 * type;
+    int iAx, iAy, iAz, cube, f, m;
+    tx = (fxx - q->xmin) / q->hx;
+    ty = (fyy - q->ymin) / q->hy;
+    fzz = fzz - q->zmax * rint(fzz / q->zmax);
+    if (fzz < 0) fzz = q->zmax + fzz;
+    tz = (fzz - 0.0) / q->hz;
+    if (!(tx > -1.0 && tx < q->ncx && ty > -1.0 && ty < q->ncy && tz > -1.0 && tz < q->ncz))
+        return MSPMS_EDOM;
+    iAx = (int)tx;
+    iAy = (int)ty;
+    iAz = (int)tz;
+    cube = iAx * q->ncy * q->ncz + iAy * q->ncz + iAz;
+    x2 = fxx * fxx; x3 = x2 * fxx;
+    y2 = fyy * fyy; y3 = y2 * fyy;
+    z2 = fzz * fzz; z3 = z2 * fzz;
+    v[0] = 1.0; v[1] = fxx; v[2] = fyy; v[3] = fzz; v[4] = x2; v[5] = fxx * fyy;
+    v[6] = fxx * fzz; v[7] = y2; v[8] = fyy * fzz; v[9] = z2; v[10] = x3; v[11] = x2 * fyy;
+    v[12] = fxx * y2; v[13] = x2 * fzz; v[14] = fxx * z2; v[15] = y3; v[16] = y2 * fzz;
+    v[17] = fyy * z2; v[18] = z3; v[19] = fxx * fyy * fzz; v[20] = x3 * fyy; v[21] = x2 * y2;
+    v[22] = fxx * y3; v[23] = x3 * fzz; v[24] = x2 * z2; v[25] = fxx * z3; v[26] = y3 * fzz;
+    v[27] = y2 * z2; v[28] = fyy * z3; v[29] = x2 * fyy * fzz; v[30] = fxx * y2 * fzz;
+    v[31] = fxx * fyy * z2;
+    for (f = 0; f < 4; f++) {
+        double a = tb[32 * f][cube] * v[0];
+        for (m = 1; m < 32; m++) a = a + tb[32 * f + m][cube] * v[m];
+        if (f == 0)
+            *usf = a;
+        else
+            fsf[f - 1] = a;
+    }
+    return MSPMS_OK;
+}
+
 /* ------------------------------------------------------------------------- */
 int oracle_validate(int nspecie, const mspms_species* sp, const mspms_params* p)
 {
@@ -689,8 +732,11 @@ int oracle_validate(int nspecie, const mspms_species* sp, const mspms_params* p)
     if (nspecie < 1) return MSPMS_EINVAL;
     if (p->iExternal_FF_type != 0) return MSPMS_EINVAL;
     if (p->iChargeType < 0 || p->iChargeType > 3) return MSPMS_EINVAL;
-    if (p->iSF_type < 0 || p->iSF_type > 3) return MSPMS_EINVAL;
+    if (p->iSF_type < 0 || p->iSF_type > 4) return MSPMS_EINVAL;
     if (p->iSF_type == MSPMS_SF_TASOS && !p->grid) return MSPMS_EINVAL;
+    if (p->iSF_type == MSPMS_SF_MY_INTERP &&
+        (!p->cubes || !p->cubes->tables || p->cubes->ntype < 1))
+        return MSPMS_EINVAL;
     /* cal_sf_hypergeo leaves *uij unset when ntube == 0 */
     if (p->iSF_type == MSPMS_SF_HYPERGEO && p->ntube < 1) return MSPMS_EINVAL;
     for (s = 0; s < nspecie; s++) {
@@ -931,6 +977,21 @@ int oracle_frclong(int nspecie, const mspms_species* sp, const mspms_params* p,
                 fxl[ii] += f3[0];
                 fyl[ii] += f3[1];
                 fzl[ii] += f3[2];
+            } else if (p->iSF_type == MSPMS_SF_MY_INTERP) {
+                /* src/frclong.c:1193-1207; R_RGAS src/defs.h:96 */
+                const double R_RGAS = 0.120272219;
+                double f3[3] = {0.0, 0.0, 0.0}, ue = 0.0;
+                int ty = sp[spc].interp_type[site] - 1;
+                int r = (ty < 0 || ty >= p->cubes->ntype)
+                            ? MSPMS_EDOM
+                            : oracle_myinterp_site(p->cubes, ty, xx[ii] * p->sigma_base,
+                                                   yy[ii] * p->sigma_base,
+                                                   zz[ii] * p->sigma_base, &ue, f3);
+                if (r) return r;
+                t->usflj += ue * R_RGAS / p->epsilon_base;
+                fxl[ii] += f3[0] * R_RGAS * p->sigma_base / p->epsilon_base;
+                fyl[ii] += f3[1] * R_RGAS * p->sigma_base / p->epsilon_base;
+                fzl[ii] += f3[2] * R_RGAS * p->sigma_base / p->epsilon_base;
             }
         }
     }

@@ -178,13 +178,21 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
     npt_respa) and adds the velocities and the thermostat state.  With
     files_out, the reference's own trajectory frame, snapshot and checkpoint
     of the final state (trj.mspms, ss.mspms, sav.mspms) are copied there;
-    an old.mspms in the run directory is passed along (restart inputs).
+    an old.mspms in the run directory is passed along (restart inputs), and
+    so are the cube files a MYINTERP section names.
     siman runs the reference's simulated annealing driver instead."""
     if not reference_available():
         raise FileNotFoundError(REF_HARNESS)
     with tempfile.TemporaryDirectory(prefix="mspms_ref_") as tmp:
-        for fn in ("in.mspms", "cfg.mspms", "xyz.mspms", "old.mspms"):
-            if fn == "old.mspms" and not os.path.exists(os.path.join(rundir, fn)):
+        extra = []
+        try:
+            from mspms2_amd.runio import read_in
+            extra = list(read_in(os.path.join(rundir, "in.mspms")).mi_files)
+        except (OSError, ValueError, IndexError):
+            pass
+        for fn in ["in.mspms", "cfg.mspms", "xyz.mspms", "old.mspms"] + extra:
+            if fn not in ("in.mspms", "cfg.mspms", "xyz.mspms") and \
+                    not os.path.exists(os.path.join(rundir, fn)):
                 continue
             with open(os.path.join(rundir, fn), "rb") as src, \
                     open(os.path.join(tmp, fn), "wb") as dst:

@@ -36,7 +36,8 @@
  *
  * Configurations that would reach code absent from this build (the
  * Fortran grid library, iSF_type 3; the GSL metal-cluster force field,
- * iExternal_FF_type != 0) are refused before init_vars.
+ * iExternal_FF_type != 0) are refused before init_vars.  iSF_type 4
+ * (myinterp, src/sffrc.c:228-1089) is plain C and runs.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdio.h>
@@ -82,8 +83,7 @@ int main(int argc, char** argv)
     }
     opening();
     readins();
-    if (iSF_type == SF_NANOTUBE_TASOS || iSF_type == SF_NANOTUBE_MY_INTERP ||
-        iExternal_FF_type != FF_NONE) {
+    if (iSF_type == SF_NANOTUBE_TASOS || iExternal_FF_type != FF_NONE) {
         fprintf(stderr, "ref_harness: iSF_type=%d / iExternal_FF_type=%d need "
                         "libraries absent from this build\n",
                 iSF_type, iExternal_FF_type);

@@ -153,6 +153,35 @@ def make_sfgridgen_case():
     print("sfgridgen_ch3_1A", os.path.getsize(os.path.join(path, "out.sfgridgen")))
 
 
+def make_myinterp_case():
+    """CH3 sites and CH2 dimers in the (11,11) nanotube cell of
+    sfgridgen_ch3_1A (the reference's tools/CH3-SWNT11x11.in), iSF_type 4:
+    two cube files at a 2 A interval written by the CPU restatement of the
+    generator (oracle/myinterp_gen.py), evaluated by the reference's own
+    init_my_interp / get_values_from_grid."""
+    from mspms2_amd import myinterp as MI
+    from mspms2_amd import sfgridgen as S
+    from oracle import myinterp_gen as MG
+    g = S.read_input(os.path.join(HERE, "sfgridgen_ch3_1A", "grid.in"))
+    g.grid_itvl = 2.0
+    files = {}
+    for fname, (fs, fe) in (("ch3.cubes", (3.75, 98.0)), ("ch2.cubes", (3.95, 46.0))):
+        g.f_sigma, g.f_epsilon = fs, fe
+        nodes = MG.lj_nodes(g)
+        files[fname] = MI.CubeFile(ucl=(g.uclx, g.ucly, g.uclz),
+                                   center=(g.uclx / 2.0, g.ucly / 2.0, 0.0),
+                                   ngrid=nodes.shape[2:], h=(2.0, 2.0, 2.0),
+                                   coef=MG.fit(nodes, (2.0, 2.0, 2.0)))
+    rd = configs.tube_cell_fluid(g.uclx, g.ucly, g.uclz, files=tuple(files))
+    rd.cube_files = files
+    path = os.path.join(HERE, "tube_cell_myinterp")
+    os.makedirs(path, exist_ok=True)
+    rd.write(path)
+    d = O.run_reference(path)
+    save(path, d)
+    print("tube_cell_myinterp", d["natom"], d["usflj"], d["uinter"])
+
+
 def make_file_cases():
     name, n = FILE_CASE
     src = os.path.join(HERE, name)
@@ -206,6 +235,7 @@ def main():
     make_file_cases()
     make_siman_case()
     make_sfgridgen_case()
+    make_myinterp_case()
     for name, fn in BIG.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)

@@ -24,7 +24,7 @@ HARNESS = os.path.join(os.path.dirname(O.__file__), "_ref", "mspms_dropin_harnes
 
 CASES = ["lj864_jitter", "water216_ewald", "water216_wolf", "butane64", "mixture_ewald",
          "mixture_wolf_switch", "bonded_zoo", "examples_water", "mixture_water4_vacuum",
-         "water216_coulomb"]
+         "water216_coulomb", "tube_cell_myinterp"]
 
 
 def _golden(name):
@@ -41,8 +41,9 @@ def test_dropin(name, precision):
     print(f"dropin {name} {precision}", flush=True)
     sc, fl, fs = _golden(name)
     with tempfile.TemporaryDirectory(prefix="mspms_dropin_") as tmp:
-        for fn in ("in.mspms", "cfg.mspms", "xyz.mspms"):
-            shutil.copy(os.path.join(GOLDEN, name, fn), tmp)
+        for fn in os.listdir(os.path.join(GOLDEN, name)):
+            if fn.endswith(".mspms") or fn.endswith(".cubes"):
+                shutil.copy(os.path.join(GOLDEN, name, fn), tmp)
         env = dict(os.environ, MSPMS_GPU_PRECISION=precision)
         out = os.path.join(tmp, "dump.bin")
         r = subprocess.run([HARNESS, tmp, out], env=env, capture_output=True, text=True,

@@ -43,6 +43,7 @@ CASES = {
     "examples_water": "golden",
     "mixture_water4_vacuum": "golden",
     "water216_coulomb": "golden",
+    "tube_cell_myinterp": "golden",
 }
 
 _cache = {}
