@@ -438,6 +438,59 @@ def run_lj_nvt(steps=1000, cpu=True):
     return out
 
 
+def run_sfgrid(h=0.1, cpu=True, cpu_h=None):
+    """The reference's grid-generator workload: its own CH3 / (11,11)
+    nanotube input (tools/CH3-SWNT11x11.in: 44 solid atoms, rcutoff 28 A,
+    kept as tests/golden/sfgridgen_ch3_1A/grid.in) at its shipped 0.1 A
+    interval.  GPU: the solid-fluid tables (bit-exact to the reference's
+    generator) and the myinterp cube polynomials, each call timed whole
+    (allocation, upload, kernels and the copy back to the host).  CPU: the
+    reference's compiled generator (oracle/_ref/sfgridgen, built from
+    tools/sfgridgen.c) on the same input at cpu_h (default: the same
+    interval, about 13 s on one core at 0.1 A), per node."""
+    cpu_h = h if cpu_h is None else cpu_h
+    from mspms2_amd import myinterp as MI
+    from mspms2_amd import sfgridgen as S
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                       "sfgridgen_ch3_1A", "grid.in")
+    g = S.read_input(src)
+    g.grid_itvl = h
+    nx, ny, nz = S.dims(g)
+    nn = nx * ny * nz
+    S.generate(g)                      # first call: module load, pools
+    t0 = time.perf_counter()
+    S.generate(g)
+    t_sfg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nodes = MI.node_data(g)
+    t_nodes = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    MI.fit(nodes, (h, h, h))
+    t_fit = time.perf_counter() - t0
+    out = {"workload": f"sfgridgen CH3 / (11,11) SWNT input, {h} A interval, "
+                       f"{nx}x{ny}x{nz} nodes",
+           "nodes": nn, "sfgridgen_ms": 1e3 * t_sfg, "sfgridgen_nodes_per_s": nn / t_sfg,
+           "myinterp_nodes_ms": 1e3 * t_nodes, "myinterp_fit_ms": 1e3 * t_fit,
+           "cubes": (nx - 1) * (ny - 1) * (nz - 1)}
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "sfgridgen")
+    if cpu and os.path.exists(exe):
+        with tempfile.TemporaryDirectory(prefix="mspms_bench_") as tmp:
+            lines = open(src).read().split("\n")
+            lines[1] = " ".join([repr(cpu_h)] + lines[1].split()[1:])
+            inp = os.path.join(tmp, "grid.in")
+            with open(inp, "w") as f:
+                f.write("\n".join(lines))
+            g2 = S.read_input(inp)
+            n2 = int(np.prod(S.dims(g2)))
+            t0 = time.perf_counter()
+            subprocess.run([exe, inp], cwd=tmp, check=True, capture_output=True, timeout=600)
+            t_cpu = time.perf_counter() - t0
+        out["cpu_reference"] = {"nodes_per_s": n2 / t_cpu, "cores": 1,
+                                "sample": f"the same input at {cpu_h} A ({n2} nodes)"}
+        out["speedup_vs_cpu_reference"] = out["sfgridgen_nodes_per_s"] / (n2 / t_cpu)
+    return out
+
+
 def run_reference_cpu(rd, repeat=2):
     """The reference's own frclong timed on one host core (oracle/_ref)."""
     from oracle import oracle as O
@@ -475,6 +528,10 @@ def main():
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
     ap.add_argument("--lj-steps", type=int, default=1000,
                     help="steps of the LJ-864 NVT leg (0: skip it)")
+    ap.add_argument("--sfgrid", type=float, default=0.1,
+                    help="grid-generator leg at this interval (A); 0 skips it")
+    ap.add_argument("--sfgrid-only", action="store_true",
+                    help="run only the grid-generator leg and print its record")
     ap.add_argument("--probe", action="store_true",
                     help="only the short frclong + reference-copy workload for counter runs")
     ap.add_argument("--trace-steps", type=int, default=0,
@@ -488,6 +545,9 @@ def main():
         sys.exit(2)
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
+    if args.sfgrid_only:
+        print(json.dumps(run_sfgrid(args.sfgrid, cpu=not args.no_cpu)))
         return
     rd, system = make_workload()
     if args.probe:
@@ -586,6 +646,8 @@ def main():
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
     if args.lj_steps > 0:
         line["lj864_nvt"] = run_lj_nvt(args.lj_steps, cpu=not args.no_cpu)
+    if args.sfgrid > 0:
+        line["sfgrid"] = run_sfgrid(args.sfgrid, cpu=not args.no_cpu)
     print(json.dumps(line))
 
 
