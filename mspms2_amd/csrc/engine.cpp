@@ -147,7 +147,7 @@ struct mspms_ctx {
     DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center, d_atom_x0;
     DevBuf<int> d_lstate;
     DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext;
-    DevBuf<uint32_t> d_plist;
+    DevBuf<float4> d_plist;
     int cap_hint = 0;
 
     // reciprocal
@@ -1321,6 +1321,10 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     }
     {
         ProfScope p(c, s, PROF_CLUSTER);
+        // the mixed pair kernel takes each cluster pair once; the exact and
+        // adjoint kernels want both rows (a list of the other layout is
+        // rebuilt on the device)
+        c->cb.half = (!grad && c->precision == MSPMS_PREC_MIXED) ? 1 : 0;
         launch_cluster_build(L, c->cb, x, y, z, fx, fy, fz, 0.25 * c->skin * c->skin,
                              c->force_rebuild ? 1 : 0);
     }

@@ -98,11 +98,18 @@ struct ClusterBufs {
                          //   molecule id and flags as int bits (SLOT_*)
     double4* cl_center;  // [maxcl]
     float4* cl_ext;      // [maxcl]
-    uint32_t* plist;     // [maxcl*cap]
+    float4* plist;       // [maxcl*cap] entries: FP32 centre separation
+                         //   Ci - (Cj + image shift) (Ci - Cj for an
+                         //   ambiguous image) and the entry bits in .w
     int* plist_n;        // [maxcl]
     int maxcl, cap, ncell, ncol;
+    int half;            // layout this call wants: 1 = each cluster pair in one
+                         //   row (mixed pair kernel, j forces by atomics),
+                         //   0 = both rows (exact and adjoint kernels)
 };
-enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_N = 4 };
+// LS_LAYOUT: the `half` of the list last built (a list of the other layout
+// is rebuilt, decided on the device so that captured graphs stay valid)
+enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_LAYOUT = 3, LS_N = 4 };
 
 // frclong / RESPA-step finalize: nb (<= FIN_BLOCKS) workgroups of FIN_T
 // threads per replica, enough for one partial of each kind per thread in the

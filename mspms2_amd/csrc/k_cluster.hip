@@ -43,17 +43,21 @@ __global__ void __launch_bounds__(256) k_list_check(
     const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ slot_orig,
     const double4* __restrict__ slot_w64, const double4* __restrict__ cl_center,
     double4* __restrict__ slot_x64, float4* __restrict__ slot_off, int* lstate,
-    double half_skin2, int force) {
+    double half_skin2, int force, int half) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = lstate[LS_VALID] != 0 && !force;
+    const bool valid = lstate[LS_VALID] != 0 && !force && lstate[LS_LAYOUT] == half;
     bool need = !valid;
     if (t < natom) {
-        if (sites[atom_site[t]].ghost == MSPMS_GHOST_FULL) {
-            // not binned: its force starts from zero (src/frclong.c:94,107)
+        const bool full_ghost = sites[atom_site[t]].ghost == MSPMS_GHOST_FULL;
+        // a full ghost is not binned: its force starts from zero
+        // (src/frclong.c:94,107); with one row per cluster pair every pair
+        // force arrives by atomic addition, onto zero
+        if (full_ghost || half) {
             fx[t] = 0.0;
             fy[t] = 0.0;
             fz[t] = 0.0;
-        } else if (valid) {
+        }
+        if (!full_ghost && valid) {
             const double4 p0 = atom_x0[t];
             const double dx = x[t] - p0.x, dy = y[t] - p0.y, dz = z[t] - p0.z;
             need = !(dx * dx + dy * dy + dz * dz <= half_skin2);  // NaN: rebuild
@@ -327,7 +331,8 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
 // every cluster of a candidate column is tested box-to-box.  Entry layout:
 //   bits 0-1 x image + 1, 2-3 y image + 1, 4-5 z image + 1,
 //   bit 6 "ambiguous image: apply the minimum image per pair",
-//   bits 7-31 j-cluster index.
+//   bits 7-31 j-cluster index,
+// stored in .w of a float4 whose x, y, z hold the FP32 centre separation.
 __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total, int maxcl,
                                                const int* __restrict__ cl_col,
                                                const int* __restrict__ col_cl0,
@@ -335,9 +340,9 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                                                const RepDev* __restrict__ reps,
                                                const double4* __restrict__ cl_center,
                                                const float4* __restrict__ cl_ext,
-                                               uint32_t* __restrict__ plist,
+                                               float4* __restrict__ plist,
                                                int* __restrict__ plist_n, int cap, Status* st,
-                                               int* lstate) {
+                                               int* lstate, int half) {
     if (!lstate[LS_NEED]) return;
     int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int lane = threadIdx.x & 63;
@@ -367,7 +372,7 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
     }
     const double rl2 = rl * rl;
     int n = 0;
-    uint32_t* row = plist + (size_t)ci * cap;
+    float4* row = plist + (size_t)ci * cap;
     // candidate columns in (y, x) order, 64 per batch; their clusters are
     // flattened so that each pass of the wave tests 64 clusters whatever the
     // column heights (a column holds ~ncz clusters, far fewer than 64)
@@ -409,6 +414,7 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
             const int qincl = __shfl(incl, qc & 63, 64);
             bool hit = false;
             uint32_t entry = 0;
+            float4 sep = make_float4(0.f, 0.f, 0.f, 0.f);
             if (f < total) {
                 const int cj = qa + (f - (qincl - qcnt));
                 double4 Cj = cl_center[cj];
@@ -423,6 +429,9 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                 double gy = amby ? 0.0 : fmax(0.0, fabs(dyc) - ((double)Ei.y + (double)Ej.y));
                 double gz = ambz ? 0.0 : fmax(0.0, fabs(dzc) - ((double)Ei.z + (double)Ej.z));
                 hit = gx * gx + gy * gy + gz * gz < rl2;
+                // one row per cluster pair: same index parity in the lower
+                // row, different parity in the higher (rows stay balanced)
+                if (half && cj != ci && ((((ci ^ cj) & 1) == 0) == (ci > cj))) hit = false;
                 int szi = (int)nz;  // j image: Cj + (sx, sy, sz) * L
                 bool a2 = ambx || amby || ambz || qx < -1 || qx > 1 || qy < -1 || qy > 1 ||
                           szi < -1 || szi > 1;
@@ -430,14 +439,26 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                 entry = ((uint32_t)cj << 7) | ((a2 ? 1u : 0u) << 6) |
                         ((uint32_t)(szi + 1) << 4) |
                         ((uint32_t)((a2 ? 0 : qy) + 1) << 2) | (uint32_t)((a2 ? 0 : qx) + 1);
+                // the centre separation the mixed kernel uses, rounded once
+                // to FP32; the centres stay fixed while the list is reused
+                if (a2) {
+                    sep = make_float4((float)(Ci.x - Cj.x), (float)(Ci.y - Cj.y),
+                                      (float)(Ci.z - Cj.z), 0.f);
+                } else {
+                    sep = make_float4((float)(Ci.x - (Cj.x + qx * R.L[0])),
+                                      (float)(Ci.y - (Cj.y + qy * R.L[1])),
+                                      (float)(Ci.z - (Cj.z + szi * R.L[2])), 0.f);
+                }
+                sep.w = __uint_as_float(entry);
             }
             uint64_t mask = __ballot(hit);
             int pre = __popcll(mask & ((1ull << lane) - 1ull));
-            if (hit && n + pre < cap) row[n + pre] = entry;
+            if (hit && n + pre < cap) row[n + pre] = sep;
             n += __popcll(mask);
         }
     }
     if (lane == 0) {
+        if (ci == 0) lstate[LS_LAYOUT] = half;
         plist_n[ci] = min(n, cap);
         if (n > cap) {
             atomicOr(&st->flags, ST_OVERFLOW);
@@ -458,7 +479,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     hipLaunchKernelGGL(k_list_check, dim3((std::max(c.natom, nslot) + T - 1) / T), dim3(T), 0,
                        c.stream, c.natom, nslot, x, y, z, c.atom_site, c.sites, fx, fy, fz,
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
-                       b.slot_x64, b.slot_off, b.lstate, half_skin2, force);
+                       b.slot_x64, b.slot_off, b.lstate, half_skin2, force, b.half);
     hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
                        b.cell_count, fx, fy, fz, c.status, b.lstate);
@@ -479,5 +500,5 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,
-                       b.cap, c.status, b.lstate);
+                       b.cap, c.status, b.lstate, b.half);
 }

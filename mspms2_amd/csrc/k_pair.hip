@@ -5,11 +5,13 @@
 // A wave64 walks (part of) an i-cluster's row of the cluster-pair list.
 // Lane (i = lane>>3, j = lane&7) evaluates one site pair per j-cluster;
 // each i's force is reduced over its 8 lanes with wave shuffles at the end,
-// and the energy / virial partials over the whole wave.  The lists are full
-// (both i->j and j->i), so every atom's force is written once, without
-// atomics, in a fixed order: the result is deterministic.  Energies and
-// virial are summed twice and halved in the finalize kernel; pair counts and
-// hashes are taken on i<j only.
+// and the energy / virial partials over the whole wave.  For the exact
+// kernel the lists are full (both i->j and j->i), so every atom's force is
+// written once, without atomics, in a fixed order: the result is
+// deterministic.  The mixed kernel takes each cluster pair once and adds
+// the j forces with FP64 atomics (half the pair terms; forces repeat to the
+// last bits).  Energies and virial reach the finalize doubled and are halved
+// there; pair counts and hashes are taken on i<j only.
 //   k_pair     exact mode: one wave per i-cluster
 //   k_pair_mx  mixed mode: WPC waves per i-cluster sharing its row, the
 //              partials combined through LDS in wave order
@@ -42,6 +44,13 @@ __device__ inline uint64_t pair_key(int i, int j) {
     return splitmix64(((uint64_t)(uint32_t)i << 32) | (uint32_t)j);
 }
 
+// a lane's float from another lane of its row by a DPP pattern (all lanes
+// active; row and bank masks full)
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
 // XCD-aware block order: consecutive i-clusters (spatial neighbours that
 // share j-clusters) land on the same XCD and its L2.
 __device__ inline int xcd_block(int b, int nb) {
@@ -59,7 +68,7 @@ template <int ELEC, bool FP64, bool GRAD, int WPC>
 __global__ void __launch_bounds__(64 * WPC) k_pair(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const SiteDev* __restrict__ sites, const double4* __restrict__ cl_center,
-    const uint32_t* __restrict__ plist, const int* __restrict__ plist_n, int cap,
+    const float4* __restrict__ plist, const int* __restrict__ plist_n, int cap,
     const int* __restrict__ slot_orig, const int2* __restrict__ slot_info,
     const double4* __restrict__ slot_x64, const float4* __restrict__ slot_off,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
@@ -91,12 +100,12 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
     unsigned long long n_lj = 0, n_el = 0, h_lj = 0, h_el = 0;
 
     const int nrow = plist_n[ci];
-    const uint32_t* row = plist + (size_t)ci * cap;
+    const float4* row = plist + (size_t)ci * cap;
 
     {
         const double4 pi4 = slot_x64[islot];
         for (int e = w; e < nrow; e += WPC) {
-            const uint32_t ent = row[e];
+            const uint32_t ent = __float_as_uint(row[e].w);
             const int cj = (int)(ent >> 7);
             const int jslot = cj * MSPMS_CL + jsub;
             const int2 infoj = slot_info[jslot];
@@ -234,13 +243,19 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
 
 // Mixed precision, WPC waves per i-cluster: wave w of the workgroup takes
 // row entries w, w + WPC, ...; the WPC partial forces and energies of each
-// i are combined through LDS in wave order (deterministic).  Per entry a
-// lane loads two float4 (offset + charge, sigma / sqrt(eps) / molecule /
-// flags) and the wave one FP64 cluster centre.
-template <int ELEC, int WPC>
+// i are combined through LDS in wave order.  Per entry a lane loads two
+// float4 (offset + charge, sigma / sqrt(eps) / molecule / flags) and the
+// wave one FP64 cluster centre.
+// HALF: each cluster pair sits in one row only (the self pair in its own,
+// where lanes with j > i take each site pair once).  A lane's pair force
+// goes to i as before and, negated, to j: per entry the eight i-lanes of
+// each j are summed with shuffles and added to j's force with an FP64
+// atomic; i's total is added atomically too (k_list_check zeroed both).
+// Energies and virial are then doubled, since the finalize halves them.
+template <int ELEC, int WPC, bool HALF>
 __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
-    const double4* __restrict__ cl_center, const uint32_t* __restrict__ plist,
+    const double4* __restrict__ cl_center, const float4* __restrict__ plist,
     const int* __restrict__ plist_n, int cap, const int* __restrict__ slot_orig,
     const float4* __restrict__ slot_off, const float4* __restrict__ slot_par,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
@@ -252,7 +267,9 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int ncl = min(*ncl_total, maxcl);
     if (ci >= ncl) return;  // whole workgroup
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int isub = lane >> 3, jsub = lane & 7;
+    // HALF: i = lane & 7, so that the eight i-lanes of a j are adjacent and
+    // their per-entry sum takes three DPP steps; otherwise i = lane >> 3
+    const int isub = HALF ? (lane & 7) : (lane >> 3), jsub = HALF ? (lane >> 3) : (lane & 7);
 
     const double4 Ci = cl_center[ci];
     const RepDev& R = reps[(int)Ci.w];
@@ -270,6 +287,10 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const float rcel = (float)R.rc_el, v1 = (float)R.wvc1, v2 = (float)R.wvc2,
                 f1 = (float)R.wfc1, f2 = (float)R.wfc2;
     const float rcon2 = (float)R.rcon2, inv_den = (float)(1.0 / R.roff2_ron2_cube);
+    // replica constants in registers: a reload inside the entry loop would
+    // wait for the prefetched loads
+    const bool sw_on = R.switch_on != 0;
+    const int atom0 = R.atom0;
 
     // FP64 accumulators, fed from FP32 partials that cover at most FLUSH row
     // entries of this wave (FLUSH pair terms per lane)
@@ -290,104 +311,182 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
         pfx = pfy = pfz = pe_lj = pe_sh = pe_re = pvir = 0.f;
     };
 
+    // This wave's row entries (w, w + WPC, ...) are loaded 64 at a time, one
+    // per lane, and read back with readlane.  The loop is unrolled by two
+    // with two register sets: each set's j data are loaded (unconditionally,
+    // the index clamped to the chunk) one entry before they are used, so at
+    // every use at least the other set's three loads are in flight and the
+    // wait does not cover them or the j-force atomics.
     const int nrow = plist_n[ci];
-    const uint32_t* row = plist + (size_t)ci * cap;
-    for (int e = w, it = 0; e < nrow; e += WPC, ++it) {
-        if (it > 0 && it % FLUSH == 0) flush();  // uniform over the wave
-        const uint32_t ent = row[e];
-        const int cj = (int)(ent >> 7);
-        const int jslot = cj * MSPMS_CL + jsub;
-        const float4 pj4 = slot_off[jslot];
-        const float4 qj4 = slot_par[jslot];
-        const int flj = __float_as_int(qj4.w);
-        if (!(vi && (flj & SLOT_VALID) && __float_as_int(qj4.z) != moli)) continue;
-        const double4 Cj = cl_center[cj];
-        float rx, ry, rz;
-        if (ent & 64u) {
-            rx = (float)(Ci.x - Cj.x) + (pi4.x - pj4.x);
-            ry = (float)(Ci.y - Cj.y) + (pi4.y - pj4.y);
-            rz = (float)(Ci.z - Cj.z) + (pi4.z - pj4.z);
-            rx -= Lx * rintf(rx * iLx);
-            ry -= Ly * rintf(ry * iLy);
-            rz -= Lz * rintf(rz * iLz);
-        } else {
-            const int sx = (int)(ent & 3u) - 1, sy = (int)((ent >> 2) & 3u) - 1,
-                      sz = (int)((ent >> 4) & 3u) - 1;
-            const float Dx = (float)(Ci.x - (Cj.x + sx * R.L[0]));
-            const float Dy = (float)(Ci.y - (Cj.y + sy * R.L[1]));
-            const float Dz = (float)(Ci.z - (Cj.z + sz * R.L[2]));
-            rx = (pi4.x - pj4.x) + Dx;
-            ry = (pi4.y - pj4.y) + Dy;
-            rz = (pi4.z - pj4.z) + Dz;
-        }
-        const float rsq = rx * rx + ry * ry + rz * rz;
-        const bool in_lj = rsq < rc2lj && (fli & flj & SLOT_LJ);
-        const bool in_el = ELEC != 0 && rsq < rc2el;
-        if (!(in_lj || in_el)) continue;
-        const bool count = islot < jslot;
-        const int oj = want_hash ? slot_orig[jslot] - R.atom0 : 0;
-        const float rinv2 = __builtin_amdgcn_rcpf(rsq);  // v_rcp_f32 (~1 ulp)
-        float fsum = 0.f;
-        if (in_lj) {
-            const float sij = 0.5f * (qi4.x + qj4.x);
-            const float eij = qi4.y * qj4.y;
-            const float s2 = sij * sij;
-            float s6 = s2 * rinv2;
-            s6 = s6 * s6 * s6;
-            const float s12 = s6 * s6;
-            float u = 4.0f * eij * (s12 - s6);
-            float f = 24.0f * eij * (2.0f * s12 - s6) * rinv2;
-            float ush = 0.f;
-            if (R.switch_on) {
-                if (rsq >= rcon2) {
-                    const float a = rc2lj - rsq;
-                    const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * rcon2) * inv_den;
-                    f = f * sw - u * 12.0f * a * (rcon2 - rsq) * inv_den;
-                    u *= sw;
+    const float4* row = plist + (size_t)ci * cap;
+    const int nmine = nrow > w ? (nrow - w + WPC - 1) / WPC : 0;
+    int it = 0;
+    for (int base = 0; base < nmine; base += 64) {
+        const int cnt = min(64, nmine - base);
+        const float4 my_ent = base + lane < nmine ? row[w + (base + lane) * WPC]
+                                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+        auto entry_at = [&](int k) {
+            return make_float4(
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ent.x), k)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ent.y), k)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ent.z), k)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ent.w), k)));
+        };
+        struct JSet {
+            float4 sep, pj, qj;
+            int oj;
+        };
+        auto fetch = [&](int k, JSet& J) {
+            J.sep = entry_at(min(k, cnt - 1));
+            const int cn = (int)(__float_as_uint(J.sep.w) >> 7);
+            J.pj = slot_off[cn * MSPMS_CL + jsub];
+            J.qj = slot_par[cn * MSPMS_CL + jsub];
+            J.oj = HALF ? slot_orig[cn * MSPMS_CL + jsub] : 0;
+        };
+        auto process = [&](const JSet& J) {
+            if (it > 0 && it % FLUSH == 0) flush();  // uniform over the wave
+            ++it;
+            const float4 sep = J.sep;
+            const uint32_t ent = __float_as_uint(sep.w);
+            const float4 pj4 = J.pj, qj4 = J.qj;
+            const int ojs = J.oj;  // j's atom (HALF)
+            const int cj = (int)(ent >> 7);
+            const int jslot = cj * MSPMS_CL + jsub;
+            const int flj = __float_as_int(qj4.w);
+            bool act = vi && (flj & SLOT_VALID) && __float_as_int(qj4.z) != moli;
+            if (HALF && cj == ci) act = act && jsub > isub;
+            float gx = 0.f, gy = 0.f, gz = 0.f;  // this lane's pair force on i
+            bool hit = false;
+            if (act) {
+                float rx = sep.x + (pi4.x - pj4.x);
+                float ry = sep.y + (pi4.y - pj4.y);
+                float rz = sep.z + (pi4.z - pj4.z);
+                if (ent & 64u) {
+                    rx -= Lx * rintf(rx * iLx);
+                    ry -= Ly * rintf(ry * iLy);
+                    rz -= Lz * rintf(rz * iLz);
                 }
-            } else {
-                const float sg6 = s2 * s2 * s2;
-                ush = eij * sg6 * (sg6 * shift1 - 1.0f);
+                const float rsq = rx * rx + ry * ry + rz * rz;
+                const bool in_lj = rsq < rc2lj && (fli & flj & SLOT_LJ);
+                const bool in_el = ELEC != 0 && rsq < rc2el;
+                if (in_lj || in_el) {
+                    hit = true;
+                    const bool count = HALF || islot < jslot;
+                    int oj = 0, ka = oi, kb = 0;
+                    if (want_hash) {
+                        oj = (HALF ? ojs : slot_orig[jslot]) - atom0;
+                        kb = oj;
+                        if (HALF && oj < oi) {
+                            ka = oj;
+                            kb = oi;
+                        }
+                    }
+                    const float rinv2 = __builtin_amdgcn_rcpf(rsq);  // v_rcp_f32 (~1 ulp)
+                    float fsum = 0.f;
+                    if (in_lj) {
+                        const float sij = 0.5f * (qi4.x + qj4.x);
+                        const float eij = qi4.y * qj4.y;
+                        const float s2 = sij * sij;
+                        float s6 = s2 * rinv2;
+                        s6 = s6 * s6 * s6;
+                        const float s12 = s6 * s6;
+                        float u = 4.0f * eij * (s12 - s6);
+                        float f = 24.0f * eij * (2.0f * s12 - s6) * rinv2;
+                        float ush = 0.f;
+                        if (sw_on) {
+                            if (rsq >= rcon2) {
+                                const float a = rc2lj - rsq;
+                                const float sw = a * a * (rc2lj + 2.0f * rsq - 3.0f * rcon2) * inv_den;
+                                f = f * sw - u * 12.0f * a * (rcon2 - rsq) * inv_den;
+                                u *= sw;
+                            }
+                        } else {
+                            const float sg6 = s2 * s2 * s2;
+                            ush = eij * sg6 * (sg6 * shift1 - 1.0f);
+                        }
+                        pe_lj += u;
+                        pe_sh += ush;
+                        pvir += f * rsq;
+                        fsum += f;
+                        if (count) {
+                            c_lj++;
+                            if (want_hash) h_lj += pair_key(ka, kb);
+                        }
+                    }
+                    if (in_el) {
+                        const float rinv = rsqrtf(rsq);
+                        const float r = rsq * rinv;
+                        const float x = kappa * r;
+                        const float ex = __expf(-x * x);
+                        const float ec = ex * erfc_scaled_poly(x);
+                        const float cq = Cf * q_i * pj4.w;
+                        float u, f;
+                        if (ELEC == MSPMS_ELEC_EWALD) {
+                            u = cq * rinv * ec;
+                            f = (u + cq * rinv * MSPMS_TWO_OVER_SQRTPI_F * x * ex) * rinv2;
+                        } else {
+                            u = cq * (ec * rinv + v1 + v2 * (r - rcel));
+                            f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
+                            pvir += f * rsq;
+                        }
+                        pe_re += u;
+                        fsum += f;
+                        if (count) {
+                            c_el++;
+                            if (want_hash) h_el += pair_key(ka, kb);
+                        }
+                    }
+                    gx = fsum * rx;
+                    gy = fsum * ry;
+                    gz = fsum * rz;
+                    pfx += gx;
+                    pfy += gy;
+                    pfz += gz;
+                }
             }
-            pe_lj += u;
-            pe_sh += ush;
-            pvir += f * rsq;
-            fsum += f;
-            if (count) {
-                c_lj++;
-                if (want_hash) h_lj += pair_key(oi, oj);
+            if (HALF && __ballot(hit)) {
+                // j's share of this entry: the sum over its eight adjacent i-lanes
+                // (lane ^ 1, lane ^ 2, then the mirror within the 8-lane half row)
+                float v[3] = {gx, gy, gz};
+    #pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    v[k] += dpp_f32<0xB1>(v[k]);   // quad_perm [1,0,3,2]
+                    v[k] += dpp_f32<0x4E>(v[k]);   // quad_perm [2,3,0,1]
+                    v[k] += dpp_f32<0x141>(v[k]);  // row_half_mirror
+                }
+                if (isub == 0 && (flj & SLOT_VALID) && (v[0] != 0.f || v[1] != 0.f || v[2] != 0.f)) {
+                    atomicAdd(&fx[ojs], -(double)v[0]);
+                    atomicAdd(&fy[ojs], -(double)v[1]);
+                    atomicAdd(&fz[ojs], -(double)v[2]);
+                }
+            }
+        };
+        JSet A, B;
+        fetch(0, A);
+        for (int tt = 0; tt < cnt; tt += 2) {
+            fetch(tt + 1, B);
+            process(A);
+            if (tt + 1 < cnt) {
+                fetch(tt + 2, A);
+                process(B);
             }
         }
-        if (in_el) {
-            const float rinv = rsqrtf(rsq);
-            const float r = rsq * rinv;
-            const float x = kappa * r;
-            const float ex = __expf(-x * x);
-            const float ec = ex * erfc_scaled_poly(x);
-            const float cq = Cf * q_i * pj4.w;
-            float u, f;
-            if (ELEC == MSPMS_ELEC_EWALD) {
-                u = cq * rinv * ec;
-                f = (u + cq * rinv * MSPMS_TWO_OVER_SQRTPI_F * x * ex) * rinv2;
-            } else {
-                u = cq * (ec * rinv + v1 + v2 * (r - rcel));
-                f = cq * rinv * (ec * rinv2 + f1 * ex * rinv + f2);
-                pvir += f * rsq;
-            }
-            pe_re += u;
-            fsum += f;
-            if (count) {
-                c_el++;
-                if (want_hash) h_el += pair_key(oi, oj);
-            }
-        }
-        pfx += fsum * rx;
-        pfy += fsum * ry;
-        pfz += fsum * rz;
     }
     flush();
     double fr[3] = {afx, afy, afz};
-    butterfly<double, 3>(fr, 4);
+    if (HALF) {
+        // i's lanes are lane & 7 + 8 j: sum over lane bits 3..5
+#pragma unroll
+        for (int o = 32; o >= 8; o >>= 1) {
+            double t[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) t[k] = __shfl_xor(fr[k], o, 64);
+#pragma unroll
+            for (int k = 0; k < 3; k++) fr[k] += t[k];
+        }
+    } else {
+        butterfly<double, 3>(fr, 4);
+    }
     if (jsub == 0) {
         sh_f[w][isub][0] = fr[0];
         sh_f[w][isub][1] = fr[1];
@@ -399,7 +498,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     butterfly<unsigned long long, 4>(nr, 32);
     if (lane == 0) {
         for (int k = 0; k < 4; k++) {
-            sh_e[w][k] = er[k];
+            sh_e[w][k] = HALF ? 2.0 * er[k] : er[k];
             sh_n[w][k] = nr[k];
         }
     }
@@ -412,9 +511,15 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             for (int ww = 0; ww < WPC; ww++)
                 for (int d = 0; d < 3; d++) g[d] += sh_f[ww][t][d];
             const int o = slot_orig[sl];
-            fx[o] = g[0];
-            fy[o] = g[1];
-            fz[o] = g[2];
+            if (HALF) {
+                atomicAdd(&fx[o], g[0]);
+                atomicAdd(&fy[o], g[1]);
+                atomicAdd(&fz[o], g[2]);
+            } else {
+                fx[o] = g[0];
+                fy[o] = g[1];
+                fz[o] = g[2];
+            }
         }
     } else if (t >= MSPMS_CL && t < MSPMS_CL + 4) {
         const int k = t - MSPMS_CL;
@@ -444,10 +549,16 @@ void launch_64(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy,
 template <int ELEC, int WPC>
 void launch_mx(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
                double* epart, unsigned long long* ipart, int want_hash) {
-    hipLaunchKernelGGL((k_pair_mx<ELEC, WPC>), dim3((b.maxcl + 7) / 8 * 8), dim3(64 * WPC), 0,
-                       c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center, b.plist, b.plist_n,
-                       b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz, epart, ipart,
-                       want_hash);
+    if (b.half)
+        hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, true>), dim3((b.maxcl + 7) / 8 * 8),
+                           dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
+                           b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy,
+                           fz, epart, ipart, want_hash);
+    else
+        hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, false>), dim3((b.maxcl + 7) / 8 * 8),
+                           dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
+                           b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy,
+                           fz, epart, ipart, want_hash);
 }
 
 template <int ELEC>
