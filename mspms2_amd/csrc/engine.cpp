@@ -865,16 +865,19 @@ int build(mspms_ctx* c) {
             nk_stride = std::max(nk_stride, R.nk);
             ntab_max = std::max(ntab_max, P.kmaxx + P.kmaxy + std::max(P.kmaxz, 1) + 3);
             nkz_max = std::max(nkz_max, 2 * P.kmaxz + 1);
-            // S pass: rows sorted by length; one workgroup per 64-atom chunk
+            // S pass: rows sorted by length; one workgroup per SK_CHUNK-atom
+            // chunk (the kernel's tables hold up to 64): smaller chunks give
+            // more waves (rows x chunks threads) for more partial sums
             std::vector<KRow> rs(rows.begin() + R.krow0, rows.end());
             std::stable_sort(rs.begin(), rs.end(),
                              [](const KRow& a, const KRow& b) { return a.nk > b.nk; });
             rows_s.insert(rows_s.end(), rs.begin(), rs.end());
-            int nchunk = (R.natom + 63) / 64;
+            const int SK_CHUNK = 32;
+            int nchunk = (R.natom + SK_CHUNK - 1) / SK_CHUNK;
             for (int ch = 0; ch < nchunk; ch++) {
-                int a0 = R.atom0 + 64 * ch;
+                int a0 = R.atom0 + SK_CHUNK * ch;
                 work_s.push_back(make_int4(r, rep_chunk0.back() + ch, a0,
-                                           std::min(64, R.atom0 + R.natom - a0)));
+                                           std::min(SK_CHUNK, R.atom0 + R.natom - a0)));
             }
             rep_chunk0.push_back(rep_chunk0.back() + nchunk);
         } else {
@@ -883,13 +886,15 @@ int build(mspms_ctx* c) {
         if (P.iSF_type != MSPMS_SF_NONE) c->any_sf = true;
     }
     if (c->any_recip) {
-        // force pass: (atom block of 64) x (row group), ~2048 waves in total
+        // force pass: (atom block of 64) x (row group), ~4096 waves in total
+        const long KF_WAVES = 4096;
         long blocks = 0;
         for (int r = 0; r < c->nrep; r++) blocks += (reps[r].natom + 63) / 64;
         for (int r = 0; r < c->nrep; r++) {
             const RepDev& R = reps[r];
             int nb = (R.natom + 63) / 64;
-            int groups = (int)std::max(1L, std::min((long)R.nkrow, (2048L + blocks - 1) / std::max(blocks, 1L)));
+            int groups = (int)std::max(
+                1L, std::min((long)R.nkrow, (KF_WAVES + blocks - 1) / std::max(blocks, 1L)));
             // split rows into `groups` ranges of about equal k count
             std::vector<int> cut(1, R.krow0);
             double per = (double)R.nk / groups, acc = 0.0;
@@ -1308,13 +1313,24 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         HIPC(hipEventRecord(c->ev_fork, s));
         HIPC(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         LaunchCtx Ls = launch_ctx(c, c->side);
+        if (recip_side) {
+            // the intramolecular corrections need positions only: they run
+            // first on this stream, into the same (3, N) buffer as the
+            // reciprocal forces, beside the list build and the pair kernel
+            launch_zero(Ls, frec, 3 * n);
+            if (c->any_intra) {
+                ProfScope p(c, c->side, PROF_INTRA);
+                launch_intra(Ls, c->d_species.p, c->d_terms.p, x, y, z, frec, frec + n,
+                             frec + 2 * n, c->d_ep_intra.p, nullptr, nullptr, nullptr, nullptr,
+                             nullptr);
+            }
+        }
         {
             ProfScope p(c, c->side, PROF_RECIP_S);
             launch_recip_s(Ls, c->rb, x, y, z);
         }
         if (recip_side) {
             ProfScope p(c, c->side, PROF_RECIP_F);
-            launch_zero(Ls, frec, 3 * n);
             launch_recip_f(Ls, c->rb, x, y, z, frec, frec + n, frec + 2 * n, nullptr);
         }
         HIPC(hipEventRecord(c->ev_join, c->side));
@@ -1338,12 +1354,15 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     const bool fuse = kick && !grad && !c->any_vacuum && !c->any_coul && !c->any_sf &&
                       (c->any_intra || recip_side);
     if (kicked) *kicked = fuse;
-    if (c->any_intra) {
+    if (recip_side) {
+        // reciprocal forces and intramolecular corrections join the long
+        // forces, one thread per atom
+        ProfScope p(c, s, PROF_INTRA);
+        launch_add_frec(L, fx, fy, fz, frec, fuse ? kick : nullptr);
+    } else if (c->any_intra) {
         ProfScope p(c, s, PROF_INTRA);
         launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p, gq,
-                     ge, gs, frec, fuse ? kick : nullptr);
-    } else if (recip_side) {
-        launch_add_frec(L, fx, fy, fz, frec, fuse ? kick : nullptr);
+                     ge, gs, nullptr, fuse ? kick : nullptr);
     }
     if (c->any_recip) {
         if (!recip_side) {

@@ -162,6 +162,125 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
     ep[4] = vir;
 }
 
+// Molecules of up to NA <= TPM sites, TPM lanes per molecule (64 / TPM
+// molecules per single-wave workgroup).  Lane j owns site j (kick, drift)
+// and computes bonded terms j, j + TPM, ... into its own LDS copy of the
+// molecule's forces; a site's force is then the sum of the TPM copies in
+// lane order, so the chain of one inner step is one term long instead of
+// the molecule's whole term list.  With at most one term per lane (water:
+// 2 bonds + 1 angle) the sums are those of the reference's term order.
+template <int NA, int TPM>
+__global__ void __launch_bounds__(64) k_respa_inner_tpm(
+    int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
+    const TermDev* __restrict__ terms, const SiteDev* __restrict__ sites, double* x, double* y,
+    double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
+    const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
+    double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
+    Status* st) {
+#pragma clang fp contract(off)
+    static_assert(NA <= TPM && 64 % TPM == 0, "one site per lane");
+    constexpr int MPW = 64 / TPM;
+    __shared__ double lx[NA * MPW], ly[NA * MPW], lz[NA * MPW];
+    __shared__ double cfx[TPM][NA * MPW], cfy[TPM][NA * MPW], cfz[TPM][NA * MPW];
+    const int l = threadIdx.x, mi = l / TPM, j = l % TPM;
+    const int m = blockIdx.x * MPW + mi;
+    const bool live = m < nmol;
+    int na = 0, a0 = 0, site0 = 0, bt0 = 0, nbt = 0;
+    if (live) {
+        const MolDev M = mols[m];
+        const SpeciesDev S = species[M.species];
+        na = M.natom;
+        a0 = M.atom0;
+        site0 = S.site0;
+        bt0 = S.bterm0;
+        nbt = S.nbterm;
+    }
+    const bool own = live && j < na;  // this lane owns site j of its molecule
+    const int sl = j * MPW + mi;      // its LDS slot
+    double r_mass = 0.0, vxo = 0.0, vyo = 0.0, vzo = 0.0, fxo = 0.0, fyo = 0.0, fzo = 0.0;
+    if (own) {
+        const int i = a0 + j;
+        r_mass = 1.0 / sites[site0 + j].aw;
+        lx[sl] = x[i];
+        ly[sl] = y[i];
+        lz[sl] = z[i];
+        vxo = vx[i] + (deltby2 * fxl[i] * r_mass);
+        vyo = vy[i] + (deltby2 * fyl[i] * r_mass);
+        vzo = vz[i] + (deltby2 * fzl[i] * r_mass);
+        fxo = fxs[i];
+        fyo = fys[i];
+        fzo = fzs[i];
+    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    for (int ll = 0; ll < nstep_inner; ll++) {
+        __syncthreads();  // the previous step's copies have been read
+        if (own) {
+            vxo += (deltsby2 * fxo * r_mass);
+            vyo += (deltsby2 * fyo * r_mass);
+            vzo += (deltsby2 * fzo * r_mass);
+            lx[sl] = lx[sl] + delts * vxo;
+            ly[sl] = ly[sl] + delts * vyo;
+            lz[sl] = lz[sl] + delts * vzo;
+        }
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            cfx[j][k * MPW + mi] = 0.0;
+            cfy[j][k * MPW + mi] = 0.0;
+            cfz[j][k * MPW + mi] = 0.0;
+        }
+        __syncthreads();
+        e[0] = e[1] = e[2] = e[3] = 0.0;
+        vir = 0.0;
+        for (int t = bt0 + j; t < bt0 + nbt; t += TPM) {
+            double u = 0.0, v = 0.0;
+            int sing = 0;
+            const int cls =
+                bonded_term(terms[t], mi, lx, ly, lz, cfx[j], cfy[j], cfz[j], &u, &v, &sing, MPW);
+            if (sing) atomicOr(&st->flags, ST_SINGULAR);
+            e[cls] += u;
+            vir += v;
+        }
+        __syncthreads();
+        if (own) {
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+            for (int c = 0; c < TPM; c++) {
+                gx += cfx[c][sl];
+                gy += cfy[c][sl];
+                gz += cfz[c][sl];
+            }
+            fxo = gx;
+            fyo = gy;
+            fzo = gz;
+            vxo += (deltsby2 * fxo * r_mass);
+            vyo += (deltsby2 * fyo * r_mass);
+            vzo += (deltsby2 * fzo * r_mass);
+        }
+    }
+    if (own) {
+        const int i = a0 + j;
+        x[i] = lx[sl];
+        y[i] = ly[sl];
+        z[i] = lz[sl];
+        vx[i] = vxo;
+        vy[i] = vyo;
+        vz[i] = vzo;
+        fxs[i] = fxo;
+        fys[i] = fyo;
+        fzs[i] = fzo;
+    }
+    // the last inner step's energies, summed over the molecule's lanes
+    double ev[5] = {e[0], e[1], e[2], e[3], vir};
+#pragma unroll
+    for (int o = 1; o < TPM; o <<= 1)
+#pragma unroll
+        for (int k = 0; k < 5; k++) ev[k] += __shfl_xor(ev[k], o, 64);
+    if (live && j == 0) {
+        double* ep = epart + (size_t)m * 5;
+        for (int k = 0; k < 5; k++) ep[k] = ev[k];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __restrict__ atom_site,
                                                      const SiteDev* __restrict__ sites, double* vx,
                                                      double* vy, double* vz,
@@ -195,7 +314,10 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
                        fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart,        \
                        c.status)
     if (c.max_mol_atoms <= 4) {
-        RESPA_LDS(4);
+        hipLaunchKernelGGL((k_respa_inner_tpm<4, 4>), dim3((c.nmol + 15) / 16), dim3(64), 0,
+                           c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz,
+                           fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner,
+                           epart, c.status);
     } else if (c.max_mol_atoms <= 8) {
         RESPA_LDS(8);
     } else if (c.max_mol_atoms <= 16) {

@@ -7,7 +7,7 @@
 // exp(i k.r) = ex^kx ey^ky ez^kz with e = exp(i 2 pi x / L): three sincos
 // per atom, then complex multiplies (|k| <= KMAX steps, error ~KMAX ulp).
 //
-//   k_sk     one workgroup per (replica, 64-atom chunk); a lane owns one
+//   k_sk     one workgroup per (replica, atom chunk of <= 64); a lane owns one
 //            (kx, ky) row and accumulates S for all its kz in registers
 //            over the chunk; per-atom power tables live in LDS.  Partial
 //            S per chunk is written out (no atomics).
