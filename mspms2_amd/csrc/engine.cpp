@@ -165,6 +165,7 @@ struct mspms_ctx {
     DevBuf<double> d_fin_part;
     DevBuf<unsigned long long> d_fin_partu;
     DevBuf<int> d_fin_count;
+    DevBuf<int> d_fin_on;    // gate of the RESPA step's finalize (1: run)
 
     // solid-fluid tables
     SFDev sf{};
@@ -1115,12 +1116,14 @@ int build(mspms_ctx* c) {
         (rc = c->d_kin.ensure((size_t)c->natom)) ||
         (rc = c->d_fin_part.ensure((size_t)c->nrep * FIN_BLOCKS * FIN_ND)) ||
         (rc = c->d_fin_partu.ensure((size_t)c->nrep * FIN_BLOCKS * 4)) ||
-        (rc = c->d_fin_count.ensure((size_t)c->nrep)))
+        (rc = c->d_fin_count.ensure((size_t)c->nrep)) || (rc = c->d_fin_on.ensure(1)))
         return rc;
     HIPC(hipMemsetAsync(c->d_fin_count.p, 0, sizeof(int) * (size_t)c->nrep, s));
+    HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s));
     c->fin.part = c->d_fin_part.p;
     c->fin.partu = c->d_fin_partu.p;
     c->fin.count = c->d_fin_count.p;
+    c->fin.on = nullptr;
     {
         // one partial of each kind per thread in the largest replica
         size_t most = 1;
@@ -1271,14 +1274,19 @@ __global__ void k_reset_status(Status* st) {
 
 // The frclong terms from the partial sums; with sterms, also the frcshort
 // terms of the inner steps (ep_short) and the kinetic energy (kin).
+// gated: the pass runs only while the context's finalize flag is 1 (the
+// device vver run clears it for the steps whose terms nobody reads)
 void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
-                     mspms_short_terms* sterms = nullptr, double* ukin = nullptr) {
+                     mspms_short_terms* sterms = nullptr, double* ukin = nullptr,
+                     bool gated = false) {
+    FinBufs fb = c->fin;
+    fb.on = gated ? c->d_fin_on.p : nullptr;
     launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1, c->d_ep_intra.p,
                          c->any_intra ? 1 : 0, c->rb.uk, c->any_recip ? 1 : 0, c->d_ep_sf.p,
                          c->any_sf ? 1 : 0, c->any_vacuum ? c->d_uvac.p : nullptr,
                          c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate,
                          sterms ? c->d_ep_short.p : nullptr, sterms, sterms ? c->d_kin.p : nullptr,
-                         sterms ? ukin : nullptr, c->fin);
+                         sterms ? ukin : nullptr, fb);
 }
 
 // grad (optional): per-atom parameter derivatives gq | ge | gs, 3 * natom;
@@ -1424,7 +1432,7 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
     {
         // frclong, frcshort (last inner step) and kinetic terms in one pass
         ProfScope p(c, s, PROF_FINAL);
-        launch_finalize(c, L, lterms, sterms, ukin);
+        launch_finalize(c, L, lterms, sterms, ukin, true);
     }
     HIPC(hipGetLastError());
     return 0;
@@ -1675,7 +1683,28 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
         });
         if (rc) return rc;
     }
+    // without histories only the last step's terms are read: the steps
+    // before it skip the finalize pass (its gate is set between launches,
+    // outside any capture)
+    const bool gate = !lhist && !shist && !ukin_hist && nsteps > 1;
+    // the gate is open again when this call returns, whatever the path
+    struct Reopen {
+        int* flag;
+        hipStream_t s;
+        bool armed;
+        ~Reopen() {
+            if (armed) (void)hipMemsetD32Async((hipDeviceptr_t)flag, 1, 1, s);
+        }
+    } reopen{c->d_fin_on.p, s, false};
+    if (gate) {
+        HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 0, 1, s));
+        reopen.armed = true;
+    }
     for (int t = 0; t < nsteps; t++) {
+        if (gate && t == nsteps - 1) {
+            HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s));
+            reopen.armed = false;
+        }
         if (use_graph) {
             HIPC(hipGraphLaunch(c->graph_exec, s));
         } else {

@@ -123,6 +123,9 @@ struct FinBufs {
     unsigned long long* partu; // [nrep][nb][4]
     int* count;                // [nrep]
     int nb;
+    const int* on;             // null, or a device flag: 0 = this pass only
+                               //   clears the rebuild request (a step whose
+                               //   terms nobody reads)
 };
 
 struct Status {

@@ -336,6 +336,11 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     __shared__ double lp[FIN_BLOCKS * FIN_ND];
     __shared__ unsigned long long lq[FIN_BLOCKS * 4];
     __shared__ int last;
+    if (fb.on && !*fb.on) {
+        // the cluster kernels of this call have read the rebuild request
+        if (blockIdx.x == 0 && threadIdx.x == 0 && lstate) lstate[LS_NEED] = 0;
+        return;
+    }
     const int r = blockIdx.x / nb, b = blockIdx.x % nb;
     const RepDev& R = reps[r];
     const int t0 = b * FIN_T + threadIdx.x, nt = nb * FIN_T;
