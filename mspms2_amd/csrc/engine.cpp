@@ -867,13 +867,14 @@ int build(mspms_ctx* c) {
             ntab_max = std::max(ntab_max, P.kmaxx + P.kmaxy + std::max(P.kmaxz, 1) + 3);
             nkz_max = std::max(nkz_max, 2 * P.kmaxz + 1);
             // S pass: rows sorted by length; one workgroup per SK_CHUNK-atom
-            // chunk (the kernel's tables hold up to 64): smaller chunks give
-            // more waves (rows x chunks threads) for more partial sums
+            // chunk (the kernel's tables hold up to 64).  Smaller chunks give
+            // more waves but take more of the GPU from the pair kernel that
+            // runs beside this pass: 64 measured fastest for the RESPA step
             std::vector<KRow> rs(rows.begin() + R.krow0, rows.end());
             std::stable_sort(rs.begin(), rs.end(),
                              [](const KRow& a, const KRow& b) { return a.nk > b.nk; });
             rows_s.insert(rows_s.end(), rs.begin(), rs.end());
-            const int SK_CHUNK = 32;
+            const int SK_CHUNK = 64;
             int nchunk = (R.natom + SK_CHUNK - 1) / SK_CHUNK;
             for (int ch = 0; ch < nchunk; ch++) {
                 int a0 = R.atom0 + SK_CHUNK * ch;

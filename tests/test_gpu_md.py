@@ -295,3 +295,43 @@ def test_siman_replica_batch_matches_restatement():
         assert cyc[r] == c.cycles and tf[r] == c.T
         assert rel(x[r], c.x) <= 1e-9 and rel(v[r], c.v) <= 1e-9
     eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_without_histories_equals_single_steps(precision):
+    """A device run of 6 steps reduces its terms only on the last step; its
+    state and terms equal 6 single steps from the same start (the reciprocal
+    force atomics leave last-bit differences only)."""
+    import torch
+    from mspms2_amd import abi
+    rd, s, g = _load("md_water216_nve", 10)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    out = []
+    for mode in ("run", "steps"):
+        eng = Engine(s, precision=precision)
+        x0 = np.stack([s.xx, s.yy, s.zz])
+        f0, _ = eng.frclong(*x0)
+        fs0, _ = eng.frcshort(*x0)
+        pos, vel, fl, fs = t(x0), t(g["v0"]), t(f0), t(fs0)
+        lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        uk = torch.zeros(1, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        if mode == "run":
+            eng.vver_run_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, 6, lt, st, uk,
+                                graph=True)
+        else:
+            for _ in range(6):
+                eng.vver_step_device(pos, vel, fl, fs, s.delt, rd.run.nstep_inner, lt, st, uk)
+        torch.cuda.synchronize()
+        eng.sync()
+        terms = abi.LongTerms.from_buffer_copy(lt.cpu().numpy().tobytes()).as_dict()
+        out.append((pos.cpu().numpy(), vel.cpu().numpy(), terms, uk.item()))
+        eng.close()
+    (x1, v1, t1, k1), (x2, v2, t2, k2) = out
+    assert rel(x1, x2) <= 1e-12 and rel(v1, v2) <= 1e-12
+    assert abs(k1 - k2) <= 1e-12 * abs(k2)
+    for k in ("uvdw", "ureal", "ufourier", "uinter", "virial_inter"):
+        assert abs(t1[k] - t2[k]) <= 1e-10 * (abs(t2[k]) + abs(t2["ureal"])), k
+    assert t1["n_lj_pairs"] == t2["n_lj_pairs"]
