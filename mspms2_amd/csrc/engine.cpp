@@ -1326,12 +1326,15 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
             // the intramolecular corrections need positions only: they run
             // first on this stream, into the same (3, N) buffer as the
             // reciprocal forces, beside the list build and the pair kernel
-            launch_zero(Ls, frec, 3 * n);
             if (c->any_intra) {
+                // every atom belongs to one molecule thread, which clears its
+                // atoms' entries first: no separate zero fill
                 ProfScope p(c, c->side, PROF_INTRA);
                 launch_intra(Ls, c->d_species.p, c->d_terms.p, x, y, z, frec, frec + n,
                              frec + 2 * n, c->d_ep_intra.p, nullptr, nullptr, nullptr, nullptr,
-                             nullptr);
+                             nullptr, true);
+            } else {
+                launch_zero(Ls, frec, 3 * n);
             }
         }
         {

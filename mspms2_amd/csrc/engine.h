@@ -282,7 +282,9 @@ void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const double* z, double* fx, double* fy, double* fz,
                   double* epart, double* gq = nullptr, double* ge = nullptr,
                   double* gs = nullptr, const double* frec = nullptr,
-                  const Kick* kick = nullptr);
+                  const Kick* kick = nullptr, bool zero_first = false);
+// (zero_first: each molecule's thread sets its atoms' fx..fz to zero before
+// adding, so fx..fz need no clearing pass)
 // fx += frec ((3, N) reciprocal-space forces) and a capturable zero fill
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
                      const double* frec, const struct Kick* kick = nullptr);

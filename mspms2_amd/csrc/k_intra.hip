@@ -159,7 +159,8 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
                                                const double* __restrict__ z, double* fx,
                                                double* fy, double* fz, double* epart,
                                                GradOut G, int natom_total,
-                                               const double* __restrict__ frec, Kick K) {
+                                               const double* __restrict__ frec, Kick K,
+                                               int zero_first) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
@@ -168,6 +169,12 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
     IntraAcc acc = {0.0, 0.0, 0.0, 0.0};
     bool same = true;  // isSameMole carried into the 1-4 section
     const int base = M.atom0;
+    if (zero_first)  // this thread is the only writer of its molecule's atoms
+        for (int k = 0; k < M.natom; k++) {
+            fx[base + k] = 0.0;
+            fy[base + k] = 0.0;
+            fz[base + k] = 0.0;
+        }
     if (frec) {
         // the reciprocal-space forces, computed into their own (3, N) buffer
         // beside the pair kernel, join the long forces here
@@ -732,17 +739,18 @@ void launch_grad_sites(const LaunchCtx& c, const SpeciesDev* species, const int4
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* terms,
                   const double* x, const double* y, const double* z, double* fx, double* fy,
                   double* fz, double* epart, double* gq, double* ge, double* gs,
-                  const double* frec, const Kick* kick) {
+                  const double* frec, const Kick* kick, bool zero_first) {
     int g = (c.nmol + 255) / 256;
     const Kick K = kick ? *kick : Kick{nullptr, nullptr, nullptr, nullptr, 0.0};
     if (gq)
         hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps,
                            c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{gq, ge, gs}, c.natom, frec, K);
+                           GradOut{gq, ge, gs}, c.natom, frec, K, zero_first ? 1 : 0);
     else
         hipLaunchKernelGGL(k_intra<false>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
                            c.reps, c.sites, species, terms, x, y, z, fx, fy, fz, epart,
-                           GradOut{nullptr, nullptr, nullptr}, c.natom, frec, K);
+                           GradOut{nullptr, nullptr, nullptr}, c.natom, frec, K,
+                           zero_first ? 1 : 0);
 }
 
 // fx += frec (3, N): the reciprocal forces where no k_intra runs
