@@ -1279,7 +1279,7 @@ __global__ void k_reset_status(Status* st) {
 // device vver run clears it for the steps whose terms nobody reads)
 void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
                      mspms_short_terms* sterms = nullptr, double* ukin = nullptr,
-                     bool gated = false) {
+                     bool gated = false, const FinAtoms* atoms = nullptr) {
     FinBufs fb = c->fin;
     fb.on = gated ? c->d_fin_on.p : nullptr;
     launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1, c->d_ep_intra.p,
@@ -1287,7 +1287,7 @@ void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
                          c->any_sf ? 1 : 0, c->any_vacuum ? c->d_uvac.p : nullptr,
                          c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate,
                          sterms ? c->d_ep_short.p : nullptr, sterms, sterms ? c->d_kin.p : nullptr,
-                         sterms ? ukin : nullptr, fb);
+                         sterms ? ukin : nullptr, fb, atoms);
 }
 
 // grad (optional): per-atom parameter derivatives gq | ge | gs, 3 * natom;
@@ -1297,10 +1297,13 @@ void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
 // kick (optional): the RESPA step's closing half kick, applied by the last
 // kernel that adds long forces when no kernel adds after it; *kicked tells
 // whether it was (otherwise the caller runs it).
+// frec_deferred (optional): with the reciprocal forces on the side stream,
+// leave their addition (and the kick) to the caller's finalize pass and
+// report it
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
                  double* fy, double* fz, mspms_long_terms* terms, hipStream_t s,
                  double* grad = nullptr, bool finalize = true, const Kick* kick = nullptr,
-                 bool* kicked = nullptr) {
+                 bool* kicked = nullptr, bool* frec_deferred = nullptr) {
     const size_t n = (size_t)c->natom;
     double* gq = grad;
     double* ge = grad ? grad + n : nullptr;
@@ -1366,7 +1369,11 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     const bool fuse = kick && !grad && !c->any_vacuum && !c->any_coul && !c->any_sf &&
                       (c->any_intra || recip_side);
     if (kicked) *kicked = fuse;
-    if (recip_side) {
+    const bool defer = recip_side && frec_deferred;
+    if (frec_deferred) *frec_deferred = defer;
+    if (defer) {
+        if (kicked) *kicked = false;  // the caller's finalize adds frec and kicks
+    } else if (recip_side) {
         // reciprocal forces and intramolecular corrections join the long
         // forces, one thread per atom
         ProfScope p(c, s, PROF_INTRA);
@@ -1426,17 +1433,19 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
                            deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
     }
     const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2};
-    bool kicked = false;
+    bool kicked = false, deferred = false;
     int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
-                          nullptr, false, &kick, &kicked);
+                          nullptr, false, &kick, &kicked, &deferred);
     if (rc) return rc;
-    if (!kicked)
+    if (!kicked && !deferred)
         launch_respa_final(L, vel, vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, deltby2,
                            c->d_kin.p);
     {
-        // frclong, frcshort (last inner step) and kinetic terms in one pass
+        // (the side stream's forces and the closing kick,) then frclong,
+        // frcshort (last inner step) and kinetic terms in one pass
         ProfScope p(c, s, PROF_FINAL);
-        launch_finalize(c, L, lterms, sterms, ukin, true);
+        const FinAtoms A{fl, fl + n, fl + 2 * n, c->d_frec.p, kick, c->natom};
+        launch_finalize(c, L, lterms, sterms, ukin, true, deferred ? &A : nullptr);
     }
     HIPC(hipGetLastError());
     return 0;

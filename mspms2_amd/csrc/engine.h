@@ -277,6 +277,15 @@ struct Kick {
     double *vx, *vy, *vz, *kin;
     double deltby2;
 };
+// A RESPA step's per-atom closing pass run by its finalize: fx += frec (the
+// side stream's reciprocal and intramolecular forces) and the half kick,
+// before the partials are reduced (also on gated-off steps)
+struct FinAtoms {
+    double *fx, *fy, *fz;
+    const double* frec;  // (3, natom)
+    Kick K;
+    int natom;
+};
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const TermDev* terms, const double* x, const double* y,
                   const double* z, double* fx, double* fy, double* fz,
@@ -315,7 +324,8 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0,
                           const double* ep_sf, int have_sf, const double* uvac,
                           const double* ep_coul, void* terms, int* lstate,
                           const double* ep_short, void* sterms, const double* kin,
-                          double* ukin, const FinBufs& fb);
+                          double* ukin, const FinBufs& fb,
+                          const struct FinAtoms* atoms = nullptr);
 // simple Coulomb between molecules (iChargeType 3): COMs into com[nmol],
 // per-molecule energy partials into ep[nmol]; molecules of <= 16 sites
 int coulomb_mol_supported(int max_mol_atoms);

@@ -331,8 +331,9 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     const double* __restrict__ uk, int have_recip, const double* __restrict__ ep_sf, int have_sf,
     const double* __restrict__ uvac, const double* __restrict__ ep_coul,
     mspms_long_terms* __restrict__ out, int* lstate, const double* __restrict__ ep_short,
-    mspms_short_terms* __restrict__ sout, const double* __restrict__ kin,
-    double* __restrict__ ukin, FinBufs fb, int nb) {
+    mspms_short_terms* __restrict__ sout, const double* kin,  // written by the pass below
+    double* __restrict__ ukin, FinBufs fb, int nb, FinAtoms A, const int* __restrict__ atom_site,
+    const SiteDev* __restrict__ sites) {
 #pragma clang fp contract(off)
     constexpr int ND = STEP ? 17 : 11;
     static_assert(ND <= FIN_ND, "finalize partial slots");
@@ -343,6 +344,19 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     __shared__ double lp[FIN_BLOCKS * FIN_ND];
     __shared__ unsigned long long lq[FIN_BLOCKS * 4];
     __shared__ int last;
+    if (A.fx) {
+        // the step's closing pass over this replica's atoms, strided as the
+        // kinetic partials below, so each thread reads back its own kicks
+        const RepDev& Ra = reps[blockIdx.x / nb];
+        const size_t n = (size_t)A.natom;
+        for (int i = (blockIdx.x % nb) * FIN_T + threadIdx.x; i < Ra.natom; i += nb * FIN_T) {
+            const int a = Ra.atom0 + i;
+            A.fx[a] += A.frec[a];
+            A.fy[a] += A.frec[n + a];
+            A.fz[a] += A.frec[2 * n + a];
+            if (A.K.vx) kick_atom(A.K, a, sites[atom_site[a]].aw, A.fx, A.fy, A.fz);
+        }
+    }
     if (fb.on && !*fb.on) {
         // the cluster kernels of this call have read the rebuild request
         if (blockIdx.x == 0 && threadIdx.x == 0 && lstate) lstate[LS_NEED] = 0;
@@ -799,18 +813,23 @@ void launch_finalize_long(const LaunchCtx& c, const int* col_cl0, const double* 
                           int have_recip, const double* ep_sf, int have_sf,
                           const double* uvac, const double* ep_coul, void* terms,
                           int* lstate, const double* ep_short, void* sterms,
-                          const double* kin, double* ukin, const FinBufs& fb) {
+                          const double* kin, double* ukin, const FinBufs& fb,
+                          const FinAtoms* atoms) {
     const dim3 grid(c.nrep * fb.nb);
+    FinAtoms A{};
+    if (atoms) A = *atoms;
     if (ep_short)
         hipLaunchKernelGGL(k_finalize<true>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
                            ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
                            ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
-                           ep_short, (mspms_short_terms*)sterms, kin, ukin, fb, fb.nb);
+                           ep_short, (mspms_short_terms*)sterms, kin, ukin, fb, fb.nb, A,
+                           c.atom_site, c.sites);
     else
         hipLaunchKernelGGL(k_finalize<false>, grid, dim3(FIN_T), 0, c.stream, c.reps, col_cl0,
                            ep_pair, ip_pair, have_pair, ep_intra, have_intra, uk, have_recip,
                            ep_sf, have_sf, uvac, ep_coul, (mspms_long_terms*)terms, lstate,
-                           nullptr, nullptr, nullptr, nullptr, fb, fb.nb);
+                           nullptr, nullptr, nullptr, nullptr, fb, fb.nb, A, c.atom_site,
+                           c.sites);
 }
 
 int coulomb_mol_supported(int max_mol_atoms) { return max_mol_atoms <= 16; }
