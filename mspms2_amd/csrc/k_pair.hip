@@ -493,14 +493,18 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
         sh_f[w][isub][2] = fr[2];
     }
     double er[4] = {e_lj, e_sh, e_re, vir};
-    unsigned long long nr[4] = {c_lj, c_el, h_lj, h_el};
     butterfly<double, 4>(er, 32);
-    butterfly<unsigned long long, 4>(nr, 32);
+    // counts fit 32 bits per wave; the 64-bit hashes only when asked for
+    unsigned int cr[2] = {c_lj, c_el};
+    butterfly<unsigned int, 2>(cr, 32);
+    unsigned long long hr[2] = {h_lj, h_el};
+    if (want_hash) butterfly<unsigned long long, 2>(hr, 32);
     if (lane == 0) {
-        for (int k = 0; k < 4; k++) {
-            sh_e[w][k] = HALF ? 2.0 * er[k] : er[k];
-            sh_n[w][k] = nr[k];
-        }
+        for (int k = 0; k < 4; k++) sh_e[w][k] = HALF ? 2.0 * er[k] : er[k];
+        sh_n[w][0] = cr[0];
+        sh_n[w][1] = cr[1];
+        sh_n[w][2] = want_hash ? hr[0] : 0ull;
+        sh_n[w][3] = want_hash ? hr[1] : 0ull;
     }
     __syncthreads();
     const int t = threadIdx.x;
