@@ -825,7 +825,15 @@ int build(mspms_ctx* c) {
         R.k0 = (int)kvec.size();
         if (P.iChargeType == MSPMS_ELEC_EWALD) {
             c->any_recip = true;
-            for (int kx = -P.kmaxx; kx <= P.kmaxx; kx++)
+            // The reference sums over k and -k (src/frclong.c:988-1048).  For
+            // real charges S(-k) = conj S(k), so -k gives the same energy
+            // and force terms as k: only the half space kx > 0, or kx = 0
+            // and ky > 0, or kx = ky = 0 and kz > 0 is enumerated, each k
+            // with twice its weight (exact in floating point).
+            auto half = [](int kx, int ky, int kz) {
+                return kx > 0 || (kx == 0 && (ky > 0 || (ky == 0 && kz > 0)));
+            };
+            for (int kx = 0; kx <= P.kmaxx; kx++)
                 for (int ky = -P.kmaxy; ky <= P.kmaxy; ky++) {
                     KRow row{};
                     row.kx = kx;
@@ -834,7 +842,7 @@ int build(mspms_ctx* c) {
                     int n = 0;
                     for (int kz = -P.kmaxz; kz <= P.kmaxz; kz++) {
                         int ksq = kx * kx + ky * ky + kz * kz;
-                        bool ok = ksq <= P.ksqmax && ksq != 0;
+                        bool ok = ksq <= P.ksqmax && ksq != 0 && half(kx, ky, kz);
                         if (ok) {
                             if (n == 0) {
                                 row.kz_lo = kz;
@@ -843,7 +851,7 @@ int build(mspms_ctx* c) {
                             double rkx = P.twopi_lx * kx, rky = P.twopi_ly * ky,
                                    rkz = P.twopi_lz * kz;
                             volatile double rksq = rkx * rkx + rky * rky + rkz * rkz;
-                            kvec.push_back(std::exp(-P.bfactor_ewald * rksq) / rksq);
+                            kvec.push_back(2.0 * (std::exp(-P.bfactor_ewald * rksq) / rksq));
                             k_rep.push_back(r);
                             row.kz_hi = kz;
                             n++;
@@ -852,7 +860,7 @@ int build(mspms_ctx* c) {
                         bool next_ok = false;
                         if (kz < P.kmaxz) {
                             int k2 = kx * kx + ky * ky + (kz + 1) * (kz + 1);
-                            next_ok = k2 <= P.ksqmax && k2 != 0;
+                            next_ok = k2 <= P.ksqmax && k2 != 0 && half(kx, ky, kz + 1);
                         }
                         if (n > 0 && !next_ok) {
                             row.nk = n;

@@ -218,7 +218,9 @@ __global__ void __launch_bounds__(256) k_kvec(int nrow, const KRow* __restrict__
     for (int kz = w.kz_lo; kz <= w.kz_hi; kz++) {
         const double rkz = R.twopi_l[2] * kz;
         const double rksq = rkx * rkx + rky * rky + rkz * rkz;
-        kvec[w.k0 + (kz - w.kz_lo)] = exp(-R.bfactor * rksq) / rksq;
+        // twice the weight: each k stands for k and -k (the half-space
+        // enumeration of the reciprocal tables)
+        kvec[w.k0 + (kz - w.kz_lo)] = 2.0 * (exp(-R.bfactor * rksq) / rksq);
     }
 }
 
