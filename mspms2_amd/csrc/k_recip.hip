@@ -276,16 +276,19 @@ int recip_nkz_supported(int kmaxz) { return kmaxz <= 16; }
 
 void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
                     const double* z) {
+    // threads per chunk: one per (kx, ky) row of the half space (about 80
+    // rows for kmax 8 / KSQMAX 64), rows looped past it
+    constexpr int SK_T = 128;
     size_t lds = (size_t)64 * rb.ntab_max * sizeof(double2);
     if (rb.nkz_max <= 9)
-        hipLaunchKernelGGL(k_sk<9>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s, c.reps,
+        hipLaunchKernelGGL(k_sk<9>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s, c.reps,
                            rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part, rb.nk_stride);
     else if (rb.nkz_max <= 17)
-        hipLaunchKernelGGL(k_sk<17>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s,
+        hipLaunchKernelGGL(k_sk<17>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s,
                            c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
                            rb.nk_stride);
     else
-        hipLaunchKernelGGL(k_sk<33>, dim3(rb.nwork_s), dim3(256), lds, c.stream, rb.work_s,
+        hipLaunchKernelGGL(k_sk<33>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s,
                            c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
                            rb.nk_stride);
     hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 7) / 8), dim3(256), 0, c.stream,
