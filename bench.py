@@ -605,7 +605,7 @@ def main():
     wpc = mixed["tune"]["waves_per_cluster"]
     cand = {
         "k_pair": (g.get("pair", 0.0), pair_flop, PEAK_FP32_TFLOPS,
-                   ["k_pair_mx<1, %d, true>" % wpc]),
+                   ["k_pair_mx<1, %d, true, false>" % wpc]),
         "k_sk+k_sred+k_kforce": (g.get("recip_s", 0.0) + g.get("recip_f", 0.0), recip_flop,
                                  PEAK_FP64_TFLOPS, ["k_sk<", "k_sred", "k_kforce<"]),
     }

@@ -252,14 +252,17 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
 // each j are summed with shuffles and added to j's force with an FP64
 // atomic; i's total is added atomically too (k_list_check zeroed both).
 // Energies and virial are then doubled, since the finalize halves them.
-template <int ELEC, int WPC, bool HALF>
+// HASH: the pair hashes are kept (a compile-time switch: their registers
+// would otherwise cost the kernel a wave per SIMD)
+template <int ELEC, int WPC, bool HALF, bool HASH>
 __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const double4* __restrict__ cl_center, const float4* __restrict__ plist,
     const int* __restrict__ plist_n, int cap, const int* __restrict__ slot_orig,
     const float4* __restrict__ slot_off, const float4* __restrict__ slot_par,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash) {
+    double* __restrict__ epart, unsigned long long* __restrict__ ipart) {
+    constexpr bool want_hash = HASH;
     __shared__ double sh_f[WPC][MSPMS_CL][3];
     __shared__ double sh_e[WPC][4];
     __shared__ unsigned long long sh_n[WPC][4];
@@ -550,19 +553,26 @@ void launch_64(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy,
                        b.slot_x64, b.slot_off, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
 }
 
+template <int ELEC, int WPC, bool HALF, bool HASH>
+void launch_mx_k(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
+                 double* epart, unsigned long long* ipart) {
+    hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, HALF, HASH>), dim3((b.maxcl + 7) / 8 * 8),
+                       dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
+                       b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz,
+                       epart, ipart);
+}
+
 template <int ELEC, int WPC>
 void launch_mx(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy, double* fz,
                double* epart, unsigned long long* ipart, int want_hash) {
-    if (b.half)
-        hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, true>), dim3((b.maxcl + 7) / 8 * 8),
-                           dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
-                           b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy,
-                           fz, epart, ipart, want_hash);
+    if (b.half && want_hash)
+        launch_mx_k<ELEC, WPC, true, true>(c, b, fx, fy, fz, epart, ipart);
+    else if (b.half)
+        launch_mx_k<ELEC, WPC, true, false>(c, b, fx, fy, fz, epart, ipart);
+    else if (want_hash)
+        launch_mx_k<ELEC, WPC, false, true>(c, b, fx, fy, fz, epart, ipart);
     else
-        hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, false>), dim3((b.maxcl + 7) / 8 * 8),
-                           dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
-                           b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy,
-                           fz, epart, ipart, want_hash);
+        launch_mx_k<ELEC, WPC, false, false>(c, b, fx, fy, fz, epart, ipart);
 }
 
 template <int ELEC>
