@@ -255,9 +255,10 @@ def run_probe(system, calls=5):
                       "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
 
 
-def run_trace(system, steps, skin):
+def run_trace(system, steps, skin, shape=None):
     """Only the timed region of the headline: `steps` mixed RESPA steps
-    replayed from the captured graph (after the autotuner and 10 eager
+    replayed from the captured graph (after the autotuner, or the launch
+    shape given as (sites per cell, waves per i-cluster), and 10 eager
     steps), for kernel-trace runs; the last `steps` dispatches of each
     kernel are the graph's."""
     import torch
@@ -272,7 +273,11 @@ def run_trace(system, steps, skin):
     st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
     uk = torch.zeros(1, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-    tune = eng.autotune(pos, repeats=2)
+    if shape is None:
+        tune = eng.autotune(pos, repeats=2)
+    else:
+        eng.set_launch_shape(float(shape[0]), int(shape[1]))
+        tune = {"cell_atoms": float(shape[0]), "waves_per_cluster": int(shape[1])}
     eng.set_list_skin(skin)
     for _ in range(10):
         eng.vver_step_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, lt, st, uk)
@@ -559,6 +564,8 @@ def main():
                     help="only the short frclong + reference-copy workload for counter runs")
     ap.add_argument("--trace-steps", type=int, default=0,
                     help="only this many graph-replayed mixed steps, for kernel-trace runs")
+    ap.add_argument("--shape", nargs=2, type=float, metavar=("CELL_ATOMS", "WPC"),
+                    help="with --trace-steps: this launch shape instead of the autotuner's")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-kernel HBM traffic of two --pmc passes over --probe")
     args = ap.parse_args()
@@ -577,7 +584,7 @@ def main():
         run_probe(system)
         return
     if args.trace_steps > 0:
-        run_trace(system, args.trace_steps, args.skin / system.params["sigma_base"])
+        run_trace(system, args.trace_steps, args.skin / system.params["sigma_base"], args.shape)
         return
     skin = args.skin / system.params["sigma_base"]
     mixed = run_gpu(system, "mixed", args.steps, args.warmup, skin=skin)
