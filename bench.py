@@ -45,7 +45,7 @@ PEAK_HBM_GBS = 8000.0
 
 def make_workload():
     from mspms2_amd import configs
-    from tests.util import system_of
+    from mspms2_amd.system import system_of
     rd = configs.water_box(n_side=16, L=49.66, kappa=0.28, kmax=8, ksqmax=64, rc=10.0,
                            seed=2, nstep_inner=NSTEP_INNER, delt_fs=DT_FS)
     return rd, system_of(rd)
@@ -139,7 +139,13 @@ def run_gpu(system, precision, steps, warmup, profile=True, skin=0.0):
     groups = eng.timings() if profile else {}
     eng.set_profiling(False)
     # (2) the same steps replayed from one captured RESPA-step graph: the
-    #     timed region of the headline figure
+    #     timed region of the headline figure.  Every step reduces its
+    #     energy terms, virial and kinetic energy and appends them to
+    #     device histories (the per-step record the reference's main loop
+    #     consumes through collect_aves, src/mspms2.c:101-102)
+    lhist = torch.zeros(steps * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    shist = torch.zeros(steps * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    khist = torch.zeros(steps, dtype=torch.float64, device=dev)
     eng.vver_run_device(pos, vel, fl, fs, delt, NSTEP_INNER, 2, lterms, sterms, ukin, graph=True,
                         stream=stream)
     torch.cuda.synchronize()
@@ -147,7 +153,7 @@ def run_gpu(system, precision, steps, warmup, profile=True, skin=0.0):
     t_wall = time.perf_counter()
     start.record(st)
     eng.vver_run_device(pos, vel, fl, fs, delt, NSTEP_INNER, steps, lterms, sterms, ukin,
-                        graph=True, stream=stream)
+                        lhist=lhist, shist=shist, ukin_hist=khist, graph=True, stream=stream)
     stop.record(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
@@ -156,6 +162,22 @@ def run_gpu(system, precision, steps, warmup, profile=True, skin=0.0):
     builds = eng.list_builds() - builds0
     raw = lterms.cpu().numpy().view(np.uint8)
     lt = abi.LongTerms.from_buffer_copy(raw.tobytes()).as_dict()
+    hl = (abi.LongTerms * steps).from_buffer_copy(lhist.cpu().numpy().tobytes())
+    hist_ok = bool(np.isfinite([hl[k].uinter for k in range(steps)]).all()) and \
+        hl[steps - 1].uinter == lt["uinter"]
+    # the long forces and positions the last timed step left (launch shape
+    # and list state of the timed region), for the accuracy check
+    pos_after = pos.cpu().numpy().copy()
+    f_after = fl.cpu().numpy().copy()
+    # the same steps with only the last step's terms reduced (no consumer
+    # of the others): an extra figure, not the headline
+    start.record(st)
+    eng.vver_run_device(pos, vel, fl, fs, delt, NSTEP_INNER, steps, lterms, sterms, ukin,
+                        graph=True, stream=stream)
+    stop.record(st)
+    torch.cuda.synchronize()
+    eng.sync()
+    ms_last_only = start.elapsed_time(stop) / steps
     # the drop-in path (host buffers: positions up, forces down, every call)
     eng.set_profiling(False)
     hx, hy, hz = (np.ascontiguousarray(v) for v in pos.cpu().numpy())
@@ -165,19 +187,56 @@ def run_gpu(system, precision, steps, warmup, profile=True, skin=0.0):
         eng.frclong(hx, hy, hz)
     host_ms = 1e3 * (time.perf_counter() - t_host) / 5
     out = dict(ms_per_step=ms, wall_ms_per_step=1e3 * wall / steps, groups=groups, terms=lt,
-               finite=bool(torch.isfinite(pos).all().item()), f_initial=f0, t_initial=t0,
-               tune=tune, host_frclong_ms=host_ms, eager_ms_per_step=ms_eager,
-               list_builds_per_step=builds / steps)
+               finite=bool(torch.isfinite(pos).all().item()) and hist_ok, f_initial=f0,
+               t_initial=t0, tune=tune, host_frclong_ms=host_ms, eager_ms_per_step=ms_eager,
+               list_builds_per_step=builds / steps, ms_per_step_last_terms_only=ms_last_only,
+               pos_after=pos_after, f_after=f_after)
     eng.close()
     return out
 
 
-def run_drift(system, precision, steps=1000, skin=0.0):
+def equilibrate(system, steps, skin=0.0, chunk=100):
+    """Equilibration of the jittered lattice: `steps` device RESPA steps
+    (mixed) in chunks of `chunk`, the velocities rescaled to the run's
+    temperature after each chunk (the lattice start releases ~1e6 K of LJ
+    overlap energy, too much for a Nose-Hoover start).  Returns the final
+    (positions, velocities) as host arrays and the temperature before the
+    last rescale."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    dev = torch.device("cuda:0")
+    eng = Engine(system, precision="mixed")
+    eng.set_list_skin(skin)
+    nfree = 3 * system.natom - system.run.nconstraint
+    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
+    vel = torch.tensor(maxwell_velocities(system), device=dev)
+    fl = torch.tensor(eng.frclong()[0], device=dev)
+    fs = torch.tensor(eng.frcshort()[0], device=dev)
+    lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(1, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    tinst = system.treq
+    for done in range(0, steps, chunk):
+        eng.vver_run_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, min(chunk, steps - done),
+                            lt, st, uk, graph=True)
+        torch.cuda.synchronize()
+        eng.sync()
+        tinst = 2.0 * float(uk.item()) / nfree
+        vel.mul_(np.sqrt(system.treq / tinst))
+    torch.cuda.synchronize()
+    out = pos.cpu().numpy(), vel.cpu().numpy(), tinst * system.params["epsilon_base"]
+    eng.close()
+    return out
+
+
+def run_drift(system, precision, steps=1000, skin=0.0, start=None):
     """NVE energy conservation: `steps` RESPA outer steps replayed from the
-    captured graph with per-step terms recorded on the device; returns the
-    least-squares slope of H = ukin + uinter + uintra in K per ns per site
-    (reduced energies are K here: epsilon_base = 1) and the rms fluctuation
-    of H about the fit."""
+    captured graph with per-step terms recorded on the device, from `start`
+    = (positions, velocities) (an equilibrated state) or else the jittered
+    lattice; returns the least-squares slope of H = ukin + uinter + uintra
+    in K per ns per site (reduced energies are K here: epsilon_base = 1)
+    and the rms fluctuation of H about the fit."""
     import torch
     from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
     from mspms2_amd import abi
@@ -185,10 +244,13 @@ def run_drift(system, precision, steps=1000, skin=0.0):
     eng = Engine(system, precision=precision)
     eng.set_list_skin(skin)
     n = system.natom
-    pos = torch.tensor(np.stack([system.xx, system.yy, system.zz]), device=dev)
-    vel = torch.tensor(maxwell_velocities(system), device=dev)
-    f0, _ = eng.frclong()
-    fs0, _ = eng.frcshort()
+    if start is None:
+        start = (np.stack([system.xx, system.yy, system.zz]), maxwell_velocities(system))
+    pos = torch.tensor(start[0], device=dev)
+    vel = torch.tensor(start[1], device=dev)
+    p0 = start[0]
+    f0, _ = eng.frclong(p0[0], p0[1], p0[2])
+    fs0, _ = eng.frcshort(p0[0], p0[1], p0[2])
     fl, fs = torch.tensor(f0, device=dev), torch.tensor(fs0, device=dev)
     lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
     st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
@@ -352,7 +414,7 @@ def make_replicas(nrep=256, seed=2024):
     """Batched-replica workload: n-butane boxes of U{64..512} molecules at
     0.58 g/cm^3, rc = min(14 A, L/2), one seed per box."""
     from mspms2_amd import configs
-    from tests.util import system_of
+    from mspms2_amd.system import system_of
     rng = np.random.default_rng(seed)
     n = rng.integers(64, 513, size=nrep)
     return [system_of(configs.butane_box(nmol=int(k), seed=1000 + r)) for r, k in enumerate(n)]
@@ -427,7 +489,7 @@ def run_lj_nvt(steps=1000, cpu=True):
     import torch
     from mspms2_amd import configs
     from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
-    from tests.util import system_of
+    from mspms2_amd.system import system_of
     rd = configs.lj_fcc(jitter=0.05, seed=7)
     s = system_of(rd)
     dev = torch.device("cuda:0")
@@ -554,6 +616,9 @@ def main():
                     help="boxes in the batched-replica leg (0: skip it)")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
+    ap.add_argument("--equil-steps", type=int, default=2500,
+                    help="velocity-rescaled steps that equilibrate the drift leg's start "
+                         "(0: lattice start)")
     ap.add_argument("--lj-steps", type=int, default=1000,
                     help="steps of the LJ-864 NVT leg (0: skip it)")
     ap.add_argument("--sfgrid", type=float, default=0.1,
@@ -596,8 +661,14 @@ def main():
     t = mixed["terms"]
     n_el, n_lj = t["n_elec_pairs"], t["n_lj_pairs"]
     pairs_per_s = n_el / (ms * 1e-3)
-    # force error of the mixed path against the FP64 path (same positions)
-    fm, ff = mixed["f_initial"], fp64["f_initial"]
+    # force error of the timed mixed path (its launch shape and reused list,
+    # the forces its last timed step computed) against the FP64 engine on
+    # the same positions
+    from mspms2_amd.engine import Engine
+    pa, fm = mixed["pos_after"], mixed["f_after"]
+    e64 = Engine(system, precision="fp64")
+    ff, _ = e64.frclong(pa[0], pa[1], pa[2])
+    e64.close()
     force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
 
     g = mixed["groups"]
@@ -628,12 +699,18 @@ def main():
                 "kernel_ms": kms, "groups_ms": g}
 
     cpu = None
+    force_err_oracle = None
     if not args.no_cpu:
         c = run_reference_cpu(rd)
         cpu_step = c["seconds_per_frclong"]
         cpu = {"value": DT_FS * 1e-6 * 86400.0 / cpu_step, "unit": "ns/day", "cores": 1,
                "kind": c["kind"], "sample": c["sample"],
                "seconds_per_frclong": cpu_step}
+        # the CPU leg's checker: the oracle's forces on the positions the
+        # timed steps ended at, against the timed path's own forces
+        from oracle.oracle import Oracle
+        fo, _ = Oracle(system).frclong(pa[0], pa[1], pa[2])
+        force_err_oracle = float(np.linalg.norm(fm - fo) / np.linalg.norm(fo))
 
     line = {
         "metric": "ns/day + pair-interactions/s, 12k-site water; batched/FP32-acc vs plain FP64",
@@ -665,6 +742,12 @@ def main():
         "fp64_ms_per_step": fp64["ms_per_step"],
         "mixed_over_fp64": fp64["ms_per_step"] / ms,
         "force_rms_rel_err_vs_fp64": force_err,
+        "force_rms_rel_err_vs_oracle": force_err_oracle,
+        "force_err_positions": "after the timed steps, timed launch shape and list state",
+        "per_step_terms": "every timed step reduces and records its terms (device histories)",
+        "ms_per_step_last_step_terms_only": mixed["ms_per_step_last_terms_only"],
+        "ns_per_day_last_step_terms_only":
+            DT_FS * 1e-6 * 86400e3 / mixed["ms_per_step_last_terms_only"],
         "finite": mixed["finite"] and fp64["finite"],
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -672,8 +755,23 @@ def main():
     if cpu:
         line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
     if args.drift_steps > 0:
-        line["energy_drift"] = {p: run_drift(system, p, args.drift_steps, skin)
-                                for p in ("mixed", "fp64")}
+        # both precisions from one equilibrated state (NVT first), so that
+        # the drift shows the integrator and the precision, not relaxation
+        # of the lattice start
+        start = None
+        drift = {}
+        if args.equil_steps > 0:
+            p_eq, v_eq, t_eq = equilibrate(system, args.equil_steps, skin)
+            start = (p_eq, v_eq)
+            drift["start"] = (f"after {args.equil_steps} RESPA steps "
+                              f"({DT_FS * args.equil_steps / 1000:g} ps) with velocities "
+                              f"rescaled to {system.treq:g} K every 100 steps "
+                              f"(T = {t_eq:.1f} K before the last rescale)")
+        else:
+            drift["start"] = "jittered lattice (not equilibrated)"
+        for p in ("mixed", "fp64"):
+            drift[p] = run_drift(system, p, args.drift_steps, skin, start)
+        line["energy_drift"] = drift
     if args.replicas > 0:
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
     if args.lj_steps > 0:

@@ -3,7 +3,7 @@
  *
  * Link libmspms_dropin.a + libmspms_gpu.so into the mspms2 program in place
  * of src/frclong.c and src/frcshort.c.  The two functions keep the
- * reference's prototypes (src/funcs.h:225 `int frclong();`, :233
+ * reference's prototypes (src/funcs.h:18 `int frclong();`, :26
  * `int frcshort();`) and its global-state contract: they read the
  * positions, topology (sample_mole, nspecie, nmole_per_specie) and run
  * constants from the program's globals (src/molevars.h, src/vars.h) and

@@ -373,9 +373,13 @@ int mspms_vver_step_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
  * once as a HIP graph on the context stream and replayed on `stream`; the
  * graph is kept for the same argument set until the system is rebuilt.
  * lhist / shist / ukin_hist (device, optional) receive every step's terms
- * and kinetic energies, nsteps * nreplica entries, step-major.  No
- * counterpart in the reference: its outer loop (src/mspms2.c:99-132) calls
- * the integrator once per step from the host. */
+ * and kinetic energies, nsteps * nreplica entries, step-major: each step's
+ * closing reduction pass appends them itself (no copy per step), which is
+ * the per-step record the reference's collect_aves() accumulates
+ * (src/mspms2.c:101-102, src/averages.c:19-95).  Without any history only
+ * the last step's terms are reduced.  No counterpart in the reference: its
+ * outer loop (src/mspms2.c:99-132) calls the integrator once per step from
+ * the host. */
 int mspms_vver_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
                           double* fs, double delt, int nstep_inner, int nsteps,
                           mspms_long_terms* lterms, mspms_short_terms* sterms,

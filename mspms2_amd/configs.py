@@ -10,6 +10,8 @@ bytes on every machine with this numpy.
                   O-H and H-O-H), Ewald or Wolf
   butane_box      united-atom TraPPE n-butane
   methane_tube    LJ methane inside a smooth cylindrical pore
+  explicit_tube   LJ methane and a ghost-site diatomic around an explicit
+                  carbon nanotube wall (iSF_type 2, ATOMEXPLICIT)
   reference_water the reference's Examples/Water inputs (TR water, Morse)
 """
 from __future__ import annotations
@@ -518,3 +520,91 @@ def tube_cell_fluid(uclx: float, ucly: float, uclz: float, ncell_z: int = 12,
                    iSF_type=abi.SF_MY_INTERP, mi_ucl=(uclx, ucly, uclz),
                    mi_center=(cx, cy, 0.0), mi_files=list(files))
     return RunDir(run=run, sysname="tube-fluid", species=[sp1, sp2], xyz=xyz)
+
+
+# --------------------------------------------------------------------------- #
+def armchair_tube(n: int, nper: int, bond: float = 1.42) -> np.ndarray:
+    """Carbon atoms of an (n, n) armchair nanotube on the z axis, nper axial
+    periods of a = sqrt(3) * bond: the graphene sheet (lattice vectors
+    a1, a2 = a (sqrt(3)/2, +-1/2), two-atom basis) cut to the chiral vector
+    C = n (a1 + a2) and the translation T = a1 - a2, then rolled.  Returns
+    (4 n nper, 3) Angstrom."""
+    a = math.sqrt(3.0) * bond
+    cx, ty = n * a * math.sqrt(3.0), nper * a
+    pts = set()
+    m = 2 * (n + nper) + 4
+    for i in range(-m, m + 1):
+        for j in range(-m, m + 1):
+            X = 0.5 * math.sqrt(3.0) * a * (i + j)
+            Y = 0.5 * a * (i - j)
+            for dx in (0.0, a / math.sqrt(3.0)):
+                u, v = (X + dx) % cx, Y % ty
+                pts.add((round(u, 9) % round(cx, 9), round(v, 9) % round(ty, 9)))
+    uv = np.array(sorted(pts))
+    R = cx / (2.0 * math.pi)
+    th = 2.0 * math.pi * uv[:, 0] / cx
+    out = np.stack([R * np.cos(th), R * np.sin(th), uv[:, 1]], axis=1)
+    assert out.shape[0] == 4 * n * nper, out.shape
+    return out
+
+
+def ghost_diatomic_species(nmole: int) -> SpeciesSpec:
+    """A CH3 bead bonded to a ghost (GHOST_LJ) site: the ghost check of the
+    solid-fluid loops (src/frclong.c:1154-1199) skips its second site."""
+    sites = [Site(type=0, name="CH3", xx=0.0, yy=0.0, zz=0.0, aw=0.015035, epsilon=98.0,
+                  sigma=3.75, charge=0.0, ghost_type=abi.GHOST_NONE, interp_type=1),
+             Site(type=1, name="X", xx=0.0, yy=0.0, zz=1.0, aw=0.001, epsilon=0.0,
+                  sigma=0.0, charge=0.0, ghost_type=abi.GHOST_LJ, interp_type=1)]
+    return SpeciesSpec(name="ch3x", nmole=nmole, sites=sites,
+                       bonds=[(0, 1, abi.BOND_HARMONIC, 2.0e5, 1.0, 0.0)])
+
+
+def explicit_tube(n: int = 10, nper: int = 9, box_xy: float = 30.0, n_in: int = 5,
+                  n_out: int = 60, n_dimer: int = 20, rc: float = 10.0, switch: bool = False,
+                  seed: int = 31, min_dist: float = 3.0) -> RunDir:
+    """Methane (3.73 A, 148 K) and CH3-ghost dimers around an explicit
+    (n, n) carbon nanotube (iSF_type 2): the wall's atoms are the
+    ATOMEXPLICIT section (sigma 3.4 A, epsilon 28 K), the box is periodic
+    with the tube's period along z.  n_in methanes sit on the axis inside
+    the tube, the rest at random outside it."""
+    rng = np.random.default_rng(seed)
+    wall = armchair_tube(n, nper)
+    R = float(np.hypot(wall[0, 0], wall[0, 1]))
+    lz = nper * math.sqrt(3.0) * 1.42
+    L = np.array([box_xy, box_xy, lz])
+
+    def far(c, pts, d):
+        if len(pts) == 0:
+            return True
+        dd = np.asarray(pts) - c
+        dd -= L * np.rint(dd / L)
+        return bool(np.min((dd * dd).sum(axis=1)) > d * d)
+
+    fluid = [np.array([0.0, 0.0, (k + 0.5) * lz / n_in]) for k in range(n_in)]
+    while len(fluid) < n_in + n_out:
+        c = rng.uniform(-0.5, 0.5, 3) * L
+        if np.hypot(c[0], c[1]) > R + min_dist and far(c, wall, min_dist) and \
+                far(c, fluid, min_dist):
+            fluid.append(c)
+    dimers = []
+    while len(dimers) < n_dimer:
+        c = rng.uniform(-0.5, 0.5, 3) * L
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        c2 = c + d
+        if np.hypot(c[0], c[1]) > R + min_dist and far(c, wall, min_dist) and \
+                far(c, fluid, min_dist) and far(c, [x for p in dimers for x in p], min_dist):
+            dimers.append((c, c2))
+    xyz = np.vstack([np.array(fluid)] + [np.array(p) for p in dimers])
+    run = RunInput(title=f"explicit ({n},{n}) tube", sigma_base=1.0, epsilon_base=1.0,
+                   mass_base=1.0, treq=300.0, boxlx=box_xy, boxly=box_xy, boxlz=lz,
+                   rcuton=(0.8 * rc if switch else rc), rcutoff=rc, rcutoffelec=rc,
+                   isLJswitchOn=1 if switch else 0, isLJlrcOn=0, delt=2.0, nstep_inner=4,
+                   iChargeType=abi.ELEC_NONE, iSF_type=abi.SF_ATOM_EXPLICIT,
+                   solid_type=0, solid_sigma=3.4, solid_epsilon=28.0, solid_charge=0.0,
+                   solid_atoms=[("C", *map(float, w)) for w in wall])
+    ch4 = SpeciesSpec(name="methane", nmole=n_in + n_out, sites=[
+        Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.016043, epsilon=148.0,
+             sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
+    return RunDir(run=run, sysname="explicit-tube", species=[ch4, ghost_diatomic_species(n_dimer)],
+                  xyz=xyz)

@@ -166,6 +166,8 @@ struct mspms_ctx {
     DevBuf<unsigned long long> d_fin_partu;
     DevBuf<int> d_fin_count;
     DevBuf<int> d_fin_on;    // gate of the RESPA step's finalize (1: run)
+    DevBuf<HistDev> d_hist;  // per-step term histories of a device run
+    DevBuf<int> d_hstep;     // [nrep] next history slot
 
     // solid-fluid tables
     SFDev sf{};
@@ -1125,14 +1127,19 @@ int build(mspms_ctx* c) {
         (rc = c->d_kin.ensure((size_t)c->natom)) ||
         (rc = c->d_fin_part.ensure((size_t)c->nrep * FIN_BLOCKS * FIN_ND)) ||
         (rc = c->d_fin_partu.ensure((size_t)c->nrep * FIN_BLOCKS * 4)) ||
-        (rc = c->d_fin_count.ensure((size_t)c->nrep)) || (rc = c->d_fin_on.ensure(1)))
+        (rc = c->d_fin_count.ensure((size_t)c->nrep)) || (rc = c->d_fin_on.ensure(1)) ||
+        (rc = c->d_hist.ensure(1)) || (rc = c->d_hstep.ensure((size_t)c->nrep)))
         return rc;
     HIPC(hipMemsetAsync(c->d_fin_count.p, 0, sizeof(int) * (size_t)c->nrep, s));
     HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s));
+    HIPC(hipMemsetAsync(c->d_hist.p, 0, sizeof(HistDev), s));
+    HIPC(hipMemsetAsync(c->d_hstep.p, 0, sizeof(int) * (size_t)c->nrep, s));
     c->fin.part = c->d_fin_part.p;
     c->fin.partu = c->d_fin_partu.p;
     c->fin.count = c->d_fin_count.p;
     c->fin.on = nullptr;
+    c->fin.hist = c->d_hist.p;
+    c->fin.hstep = c->d_hstep.p;
     {
         // one partial of each kind per thread in the largest replica
         size_t most = 1;
@@ -1707,19 +1714,33 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
     // without histories only the last step's terms are read: the steps
     // before it skip the finalize pass (its gate is set between launches,
     // outside any capture)
-    const bool gate = !lhist && !shist && !ukin_hist && nsteps > 1;
-    // the gate is open again when this call returns, whatever the path
+    const bool hist = lhist || shist || ukin_hist;
+    const bool gate = !hist && nsteps > 1;
+    // the gate is open and the histories are off again when this call
+    // returns, whatever the path
     struct Reopen {
-        int* flag;
+        mspms_ctx* c;
         hipStream_t s;
-        bool armed;
+        bool armed, hist_on;
         ~Reopen() {
-            if (armed) (void)hipMemsetD32Async((hipDeviceptr_t)flag, 1, 1, s);
+            if (armed) (void)hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s);
+            if (hist_on) {
+                LaunchCtx L = launch_ctx(c, s);
+                launch_set_hist(L, c->d_hist.p, HistDev{}, c->d_hstep.p);
+            }
         }
-    } reopen{c->d_fin_on.p, s, false};
+    } reopen{c, s, false, false};
     if (gate) {
         HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 0, 1, s));
         reopen.armed = true;
+    }
+    if (hist && nsteps > 0) {
+        // every step's finalize appends its terms to slot t of the histories
+        LaunchCtx L = launch_ctx(c, s);
+        launch_set_hist(L, c->d_hist.p, HistDev{lhist, shist, ukin_hist, nr, nsteps, 1},
+                        c->d_hstep.p);
+        HIPC(hipGetLastError());
+        reopen.hist_on = true;
     }
     for (int t = 0; t < nsteps; t++) {
         if (gate && t == nsteps - 1) {
@@ -1732,15 +1753,6 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
             int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
             if (rc) return rc;
         }
-        if (lhist)
-            HIPC(hipMemcpyAsync(lhist + (size_t)t * nr, lterms, sizeof(mspms_long_terms) * nr,
-                                hipMemcpyDeviceToDevice, s));
-        if (shist)
-            HIPC(hipMemcpyAsync(shist + (size_t)t * nr, sterms, sizeof(mspms_short_terms) * nr,
-                                hipMemcpyDeviceToDevice, s));
-        if (ukin_hist)
-            HIPC(hipMemcpyAsync(ukin_hist + (size_t)t * nr, ukin, sizeof(double) * nr,
-                                hipMemcpyDeviceToDevice, s));
     }
     return 0;
 }
@@ -2260,13 +2272,33 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
     if (!x || !y || !z || repeats < 1) return fail(MSPMS_EINVAL, "bad arguments");
     HIPC(hipSetDevice(c->device));
     const hipStream_t s = c->stream;
-    const bool prof = c->profiling;
+    // the caller's settings come back on every exit path; the shape only
+    // changes when a tuned shape has been built
+    struct Restore {
+        mspms_ctx* c;
+        bool prof, rebuild;
+        double cell;
+        int wpc;
+        bool keep_shape = false;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Restore() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            c->profiling = prof;
+            c->force_rebuild = rebuild;
+            if (!keep_shape && (c->cell_atoms != cell || c->pair_wpc != wpc)) {
+                c->cell_atoms = cell;
+                c->pair_wpc = wpc;
+                (void)build(c);
+            }
+        }
+    } g{c, c->profiling, c->force_rebuild, c->cell_atoms, c->pair_wpc};
     c->profiling = false;
     // every timed call builds its list, as a call after a move would
     c->force_rebuild = true;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventCreate(&g.e0));
+    HIPC(hipEventCreate(&g.e1));
+    hipEvent_t e0 = g.e0, e1 = g.e1;
     const double cells[] = {6.0, 8.0, 12.0, 16.0};
     const int wpcs[] = {1, 2, 4, 8};
     const int nw = 4;
@@ -2316,14 +2348,11 @@ extern "C" int mspms_autotune(mspms_ctx* c, const double* x, const double* y, co
         }
         if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    c->profiling = prof;
-    c->force_rebuild = false;
     if (rc) return rc;
     c->cell_atoms = res.cell_atoms;
     c->pair_wpc = res.waves_per_cluster;
     if ((rc = build(c))) return rc;
+    g.keep_shape = true;
     if (out) *out = res;
     return 0;
 }

@@ -118,6 +118,17 @@ enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_LAYOUT = 3, LS_N = 4 };
 #define FIN_BLOCKS 64
 #define FIN_T 256
 #define FIN_ND 17
+// Per-step term histories of a device run, appended by the finalize pass
+// of every RESPA step (one slot per step and replica); `on` is 0 outside a
+// run that records them.  Lives in device memory at a fixed address, so a
+// captured step graph appends to whatever histories the current run set.
+struct HistDev {
+    mspms_long_terms* lh;   // [cap][nrep] or null
+    mspms_short_terms* sh;  // [cap][nrep] or null
+    double* kh;             // [cap][nrep] or null
+    int nrep, cap, on;
+};
+
 struct FinBufs {
     double* part;              // [nrep][nb][FIN_ND]
     unsigned long long* partu; // [nrep][nb][4]
@@ -126,6 +137,8 @@ struct FinBufs {
     const int* on;             // null, or a device flag: 0 = this pass only
                                //   clears the rebuild request (a step whose
                                //   terms nobody reads)
+    const HistDev* hist;       // null, or the run's history descriptor
+    int* hstep;                // [nrep] next history slot of each replica
 };
 
 struct Status {
@@ -338,6 +351,8 @@ void launch_coulomb_mol(const LaunchCtx& c, const SpeciesDev* species, const dou
 void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* x,
                    const double* y, const double* z, double* fx, double* fy, double* fz,
                    double* uvac, double* gq);
+// sets a device run's history descriptor and clears the per-replica slots
+void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hstep);
 void launch_finalize_short(const LaunchCtx& c, const double* epart,
                            void* terms);
 void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species,

@@ -501,7 +501,26 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
         u.virial_intra = d[15];
         sout[r] = u;
         if (ukin) ukin[r] = d[ND - 1] * 0.5;
+        // this step's terms appended to the run's histories
+        if (fb.hist && fb.hist->on) {
+            const HistDev h = *fb.hist;
+            const int k = fb.hstep[r];
+            if (k < h.cap) {
+                const size_t slot = (size_t)k * h.nrep + r;
+                if (h.lh) h.lh[slot] = t;
+                if (h.sh) h.sh[slot] = u;
+                if (h.kh) h.kh[slot] = ukin ? ukin[r] : 0.0;
+            }
+            fb.hstep[r] = k + 1;
+        }
     }
+}
+
+// the history descriptor of a device run, written in stream order
+__global__ void k_set_hist(HistDev* d, HistDev v, int* hstep, int nrep) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d = v;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x)
+        hstep[r] = 0;
 }
 
 __global__ void __launch_bounds__(1024) k_finalize_short(const RepDev* __restrict__ reps,
@@ -852,6 +871,11 @@ void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* 
                    double* uvac, double* gq) {
     hipLaunchKernelGGL(k_vacuum, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, c.mols, species,
                        c.sites, x, y, z, fx, fy, fz, uvac, gq);
+}
+
+void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hstep) {
+    hipLaunchKernelGGL(k_set_hist, dim3((c.nrep + 255) / 256), dim3(256), 0, c.stream, d, v,
+                       hstep, c.nrep);
 }
 
 void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms) {

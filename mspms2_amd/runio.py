@@ -112,6 +112,15 @@ class RunInput:
     tube_sigma: float = 3.4
     tube_epsilon: float = 28.0
     tubes: List[Tuple[float, float, float]] = field(default_factory=list)
+    # ATOMEXPLICIT (init_sf_atom_explicit, src/sffrc.c:81-147): solid type,
+    # uniform solid sigma (A), epsilon (K) and charge, then one
+    # (name, x, y, z) line per solid atom (A); written only when atoms are
+    # given
+    solid_type: int = 0
+    solid_sigma: float = 3.4
+    solid_epsilon: float = 28.0
+    solid_charge: float = 0.0
+    solid_atoms: List[Tuple[str, float, float, float]] = field(default_factory=list)
     # HMC (src/initvars.c:248-290)
     nstep_md_per_hmc: int = 10
     pdisp: float = 1.0
@@ -181,6 +190,14 @@ def write_in(path: str, r: RunInput) -> None:
     lines.append(f"{_g(r.tube_epsilon):<24s} tube epsilon (K)")
     for (tx, ty, tr) in r.tubes:
         lines.append(f"{_g(tx)} {_g(ty)} {_g(tr)}      x y radius (A)")
+    if r.solid_atoms:
+        lines.append("")
+        lines.append("   AtomExplicit")
+        lines.append(f"{len(r.solid_atoms):<24d} solid_natom")
+        lines.append(f"{r.solid_type} {_g(r.solid_sigma)} {_g(r.solid_epsilon)} "
+                     f"{_g(r.solid_charge)}      type sigma (A) epsilon (K) charge")
+        for (nm, sx, sy, sz) in r.solid_atoms:
+            lines.append(f"{nm} {repr(float(sx))} {repr(float(sy))} {repr(float(sz))}")
     lines.append("")
     lines.append("  HMC")
     for name, kind in [("nstep_md_per_hmc", "d"), ("pdisp", "g"), ("pvolm", "g"),
@@ -240,6 +257,18 @@ def read_in(path: str) -> RunInput:
         r.tube_sigma = float(lines[i + 1].split()[0])
         r.tube_epsilon = float(lines[i + 2].split()[0])
         r.tubes = [tuple(float(v) for v in lines[i + 3 + t].split()[:3]) for t in range(nt)]
+    i = _section(lines, "ATOMEXPLICIT")
+    if i is not None:
+        # src/sffrc.c:101-128: the count; "type sigma epsilon charge" on
+        # one line (read twice there: %d, then %d %lf %lf %lf); then one
+        # "name x y z" per atom
+        n = int(lines[i].split()[0])
+        a = lines[i + 1].split()
+        r.solid_type = int(a[0])
+        r.solid_sigma, r.solid_epsilon, r.solid_charge = float(a[1]), float(a[2]), float(a[3])
+        toks = " ".join(lines[i + 2:]).split()
+        r.solid_atoms = [(toks[4 * k], float(toks[4 * k + 1]), float(toks[4 * k + 2]),
+                          float(toks[4 * k + 3])) for k in range(n)]
     i = _section(lines, "HMC")
     if i is not None:
         vals = [lines[i + k].split()[0] for k in range(10)]

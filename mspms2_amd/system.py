@@ -184,6 +184,14 @@ class System:
                      tube_r=np.array([t[2] / sb for t in r.tubes]),
                      const_3pisq_theta=C3_PI_SQ_THETA * sb * sb,
                      solid_sigma=r.tube_sigma / sb, solid_epsilon=r.tube_epsilon / eb)
+        if r.iSF_type == abi.SF_ATOM_EXPLICIT:
+            # init_sf_atom_explicit (src/sffrc.c:101-128): reduced by division
+            if not r.solid_atoms:
+                raise ValueError("iSF_type 2 needs the ATOMEXPLICIT section")
+            sa = np.array([a[1:] for a in r.solid_atoms], dtype=np.float64)
+            p.update(solid_natom=len(r.solid_atoms),
+                     solid_xx=sa[:, 0] / sb, solid_yy=sa[:, 1] / sb, solid_zz=sa[:, 2] / sb,
+                     solid_sigma=r.solid_sigma / sb, solid_epsilon=r.solid_epsilon / eb)
         ck = rd.checkpoint
         if ck is not None:
             # loadit (src/loadsave.c:60-128): positions, velocities and box of
@@ -285,3 +293,15 @@ class System:
             p.cubes = C.pointer(q)
         p._keep = keep
         return p
+
+
+def system_of(rundir_obj: RunDir) -> "System":
+    """RunDir -> System through a write / read round trip of the run files,
+    so that a generated workload is read by the same text path as an
+    existing run directory (src/readins.c:21-263)."""
+    import tempfile
+
+    from . import runio
+    with tempfile.TemporaryDirectory(prefix="mspms_rd_") as tmp:
+        rundir_obj.write(tmp)
+        return System.from_rundir(runio.RunDir.read(tmp))

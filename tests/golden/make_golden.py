@@ -55,6 +55,9 @@ CASES = {
     "mixture_wolf_switch": lambda: configs.mixture_box(elec="wolf", kappa=0.2, switch=True,
                                                        seed=13),
     "bonded_zoo": lambda: configs.bonded_zoo(),
+    # explicit (10,10) nanotube wall (iSF_type 2), plain LJ and switched LJ
+    "tube_explicit": lambda: configs.explicit_tube(),
+    "tube_explicit_switch": lambda: configs.explicit_tube(switch=True, seed=32),
 }
 
 BIG = {
@@ -207,9 +210,20 @@ def make_file_cases():
     print(rname, rn, d["md"]["ukin"])
 
 
-def main():
+def main(only=()):
+    """Every fixture, or with names given only those single-evaluation
+    cases (CASES)."""
     if not O.reference_available():
         raise SystemExit("oracle/_ref is not built (needs the reference checkout)")
+    if only:
+        for name in only:
+            path = os.path.join(HERE, name)
+            os.makedirs(path, exist_ok=True)
+            CASES[name]().write(path)
+            d = O.run_reference(path)
+            save(path, d)
+            print(name, d["natom"], d["uinter"], d["usflj"], d["uintra"])
+        return
     for name, fn in CASES.items():
         path = os.path.join(HERE, name)
         os.makedirs(path, exist_ok=True)
@@ -249,4 +263,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -116,6 +116,7 @@ def _fd(fun, s, atoms, h):
 @pytest.mark.parametrize("name,which", [
     ("water216_ewald", "long"), ("water216_wolf", "long"), ("mixture_ewald", "long"),
     ("mixture_wolf_switch", "long"), ("tube300_hypergeo", "long"), ("bonded_zoo", "long"),
+    ("tube_explicit", "long"), ("tube_explicit_switch", "long"),
     ("butane64", "short"), ("examples_water", "short"), ("bonded_zoo_nofene", "short"),
 ])
 def test_forces_are_minus_gradient(name, which):
@@ -143,6 +144,54 @@ def test_forces_are_minus_gradient(name, which):
     an = f[:, atoms]
     scale = max(np.abs(an).max(), 1e-300)
     assert np.abs(an - fd).max() <= 2e-5 * scale, (an, fd)
+
+
+@pytest.mark.parametrize("name", ["tube_explicit", "tube_explicit_switch"])
+def test_explicit_wall_bruteforce(name):
+    """cal_sf_atom_explicit (src/sffrc.c:1213-1314): usflj and the wall's
+    forces equal a numpy sum of 4 eps_ij [(s/r)^12 - (s/r)^6] over (non-ghost
+    fluid site, wall atom) pairs within the cutoff, minimum image, Lorentz-
+    Berthelot mixing, times the switch (rc^2 - r^2)^2 (rc^2 + 2 r^2 - 3 ron^2)
+    / (rc^2 - ron^2)^3 beyond ron when it is on; the forces are its exact
+    gradient.  The wall's share of the oracle's forces is the difference to
+    the same system without the wall (iSF_type 0)."""
+    s = _golden_system(name)
+    p = s.params
+    f, t = O.Oracle(s).frclong()
+    p0 = dict(p, iSF_type=0)
+    s0 = System(species=s.species, params=p0, xx=s.xx, yy=s.yy, zz=s.zz, run=s.run)
+    f0, t0 = O.Oracle(s0).frclong()
+    x = np.stack([s.xx, s.yy, s.zz], 1)
+    w = np.stack([p["solid_xx"], p["solid_yy"], p["solid_zz"]], 1)
+    L = np.array([p["boxlx"], p["boxly"], p["boxlz"]])
+    sig = np.concatenate([np.tile(sp.sigma, sp.nmole) for sp in s.species])
+    eps = np.concatenate([np.tile(sp.epsilon, sp.nmole) for sp in s.species])
+    real = np.concatenate([np.tile(sp.ghost_type == abi.GHOST_NONE, sp.nmole)
+                           for sp in s.species])
+    d = x[:, None, :] - w[None, :, :]
+    d -= L * np.rint(d / L)
+    r2 = (d * d).sum(-1)
+    rc2 = p["rcutoff"] ** 2
+    m = (r2 < rc2) & real[:, None]
+    sij = 0.5 * (sig + p["solid_sigma"])[:, None]
+    eij = np.sqrt(eps * p["solid_epsilon"])[:, None]
+    q = np.where(m, (sij * sij) / np.where(m, r2, 1.0), 0.0)
+    q6 = q ** 3
+    u = 4.0 * eij * (q6 * q6 - q6)                   # pair energies
+    du = -4.0 * eij * (12 * q6 * q6 - 6 * q6) / np.where(m, r2, 1.0)  # dU/d(r^2) * 2
+    if p["isLJswitchOn"]:
+        ron2 = p["rcuton"] ** 2
+        den = (rc2 - ron2) ** 3
+        sw = np.where(r2 < ron2, 1.0, (rc2 - r2) ** 2 * (rc2 + 2 * r2 - 3 * ron2) / den)
+        dsw = np.where(r2 < ron2, 0.0, 6.0 * (rc2 - r2) * (ron2 - r2) / den)  # dS/d(r^2)
+        du = du * sw + 2.0 * u * dsw
+        u = u * sw
+    u = np.where(m, u, 0.0)
+    fw = -np.einsum("ij,ijk->ik", np.where(m, du, 0.0), d).T
+    assert t["usflj"] == pytest.approx(u.sum(), rel=1e-12)
+    assert np.linalg.norm((f - f0) - fw) <= 1e-11 * np.linalg.norm(fw)
+    # ghost sites within the cutoff of wall atoms exist, so the skip matters
+    assert t0["usflj"] == 0.0 and ((r2 < rc2) & ~real[:, None]).any()
 
 
 def test_fene_sign_defect():

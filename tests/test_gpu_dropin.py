@@ -24,7 +24,8 @@ HARNESS = os.path.join(os.path.dirname(O.__file__), "_ref", "mspms_dropin_harnes
 
 CASES = ["lj864_jitter", "water216_ewald", "water216_wolf", "butane64", "mixture_ewald",
          "mixture_wolf_switch", "bonded_zoo", "examples_water", "mixture_water4_vacuum",
-         "water216_coulomb", "tube_cell_myinterp"]
+         "water216_coulomb", "tube_cell_myinterp", "tube300_hypergeo", "tube_explicit",
+         "tube_explicit_switch"]
 
 
 def _golden(name):

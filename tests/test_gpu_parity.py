@@ -44,6 +44,8 @@ CASES = {
     "mixture_water4_vacuum": "golden",
     "water216_coulomb": "golden",
     "tube_cell_myinterp": "golden",
+    "tube_explicit": "golden",
+    "tube_explicit_switch": "golden",
 }
 
 _cache = {}

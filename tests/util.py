@@ -1,25 +1,15 @@
 """Shared helpers of the test suite: system builders and comparisons."""
 import os
-import tempfile
 
 import numpy as np
 
-from mspms2_amd import runio
-from mspms2_amd.system import System
+from mspms2_amd.system import system_of  # noqa: F401  (re-exported for the tests)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 ENERGY_KEYS = ("uvdw", "ushift", "ureal", "uexcl", "ufourier", "uself", "uelec", "usflj",
                "uinter", "virial_inter")
 SHORT_KEYS = ("ubond", "uangle", "udih", "uimp", "uintra", "virial_intra")
-
-
-def system_of(rundir_obj):
-    """RunDir -> System through a write/read round trip (the reference's
-    own path reads the same text)."""
-    with tempfile.TemporaryDirectory(prefix="mspms_t_") as tmp:
-        rundir_obj.write(tmp)
-        return System.from_rundir(runio.RunDir.read(tmp))
 
 
 def tube_grid_system(hx=0.5, natom=300):
