@@ -405,6 +405,19 @@ int mspms_vver_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
  * replica's stream (the generator's known-answer test). */
 int mspms_hmc_init(mspms_ctx* ctx, const int* ij, const int* kl, const double* treq,
                    const int* nconstraint, double* d_vel);
+/* Per-chain outer time steps and hmc()'s equilibration-phase delt
+ * adaptation (src/hmc.c:47-58).  delt[r] is replica r's reduced outer step;
+ * adj_cycle[r] = nstep_delt_adj_cycle (0, or adj_cycle null: delt fixed) and
+ * rreq_disp[r] the target acceptance: after every adj_cycle moves of a chain
+ * its delt becomes delt * (1 - rreq_disp + accepted / adj_cycle), the
+ * reference's update while bEquilibrium holds.  From this call on
+ * mspms_hmc_run_device steps each chain with its own delt (its delt argument
+ * is ignored); mspms_hmc_get_delt reads the current steps back.  A later
+ * mspms_set_system / mspms_set_replicas ends it.  No counterpart in the
+ * reference, which keeps one global delt (src/vars.h). */
+int mspms_hmc_set_delt(mspms_ctx* ctx, const double* delt, const int* adj_cycle,
+                       const double* rreq_disp);
+int mspms_hmc_get_delt(mspms_ctx* ctx, double* delt);
 int mspms_hmc_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl,
                          double* fs, double delt, int nstep_inner, int nmd, int nmove,
                          mspms_long_terms* lterms, mspms_short_terms* sterms,

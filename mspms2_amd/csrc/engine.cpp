@@ -211,6 +211,10 @@ struct mspms_ctx {
     DevBuf<HmcRep> d_hmc;
     DevBuf<double> d_aw_atom, d_pos_old;
     bool hmc_ready = false;
+    DevBuf<double> d_hmc_delt;          // [nrep] each chain's outer step
+    bool hmc_rep_delt = false;          // chains step with d_hmc_delt
+    const double* cur_rep_delt = nullptr;  // integrator kernels' per-replica
+                                           // steps while an HMC move is enqueued
 
     // Nose-Hoover NVT (mspms_nvt_set): states, the last solve's sum m v^2,
     // the Newton iterate (3, N)
@@ -285,6 +289,7 @@ LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     L.pair_wpc = c->pair_wpc;
     L.max_mol_atoms = 0;
     for (const SpeciesHost& q : c->sp) L.max_mol_atoms = std::max(L.max_mol_atoms, q.natom);
+    L.rep_delt = c->cur_rep_delt;
     L.sites = c->d_sites.p;
     L.mols = c->d_mols.p;
     L.atom_mol = c->d_atom_mol.p;
@@ -583,6 +588,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_hmc.release();
     c->d_aw_atom.release();
     c->d_pos_old.release();
+    c->d_hmc_delt.release();
     c->d_nvt.release();
     c->d_npt.release();
     c->d_nvt_carry.release();
@@ -1207,6 +1213,7 @@ extern "C" int mspms_set_system(mspms_ctx* c, int nspecie, const mspms_species* 
     for (int k = 0; k < nspecie; k++) c->nmole[k] = sp[k].nmole;
     c->params.assign(1, *p);
     c->hmc_ready = false;  // the chains belong to the previous layout
+    c->hmc_rep_delt = false;
     c->nvt_ready = false;
     c->npt_ready = false;
     return build(c);
@@ -1236,6 +1243,7 @@ extern "C" int mspms_set_replicas(mspms_ctx* c, int nrep, const int* nmole,
     c->nmole.assign(nmole, nmole + (size_t)nrep * ns);
     c->params.assign(params, params + nrep);
     c->hmc_ready = false;
+    c->hmc_rep_delt = false;
     c->nvt_ready = false;
     c->npt_ready = false;
     return build(c);
@@ -1447,7 +1455,7 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
                            vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
                            deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
     }
-    const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2};
+    const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2, L.rep_delt, L.atom_mol, L.mols};
     bool kicked = false, deferred = false;
     int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
                           nullptr, false, &kick, &kicked, &deferred);
@@ -1800,10 +1808,18 @@ int enqueue_hmc_move(mspms_ctx* c, double* pos, double* vel, double* fl, double*
     rc = enqueue_short(c, pos, pos + n, pos + 2 * n, fs, fs + n, fs + 2 * n, sterms, s);
     if (rc) return rc;
     launch_hmc_mark(L, c->d_hmc.p, lterms, sterms);
+    // with per-chain steps the integrator kernels read each replica's delt
+    double* rep_delt = c->hmc_rep_delt ? c->d_hmc_delt.p : nullptr;
+    struct Unset {
+        mspms_ctx* c;
+        ~Unset() { c->cur_rep_delt = nullptr; }
+    } unset{c};
+    c->cur_rep_delt = rep_delt;
     for (int t = 0; t < nmd; t++)
         if ((rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s)))
             return rc;
-    launch_hmc_end(L, c->d_hmc.p, c->d_rng.p, lterms, sterms, ukin, pos, c->d_pos_old.p);
+    launch_hmc_end(L, c->d_hmc.p, c->d_rng.p, lterms, sterms, ukin, pos, c->d_pos_old.p,
+                   rep_delt);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1856,6 +1872,46 @@ extern "C" int mspms_hmc_init(mspms_ctx* c, const int* ij, const int* kl, const 
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s));
     c->hmc_ready = true;
+    c->hmc_rep_delt = false;  // fresh chains step with the run's delt
+    return 0;
+}
+
+extern "C" int mspms_hmc_set_delt(mspms_ctx* c, const double* delt, const int* adj_cycle,
+                                  const double* rreq_disp) {
+    if (!c || !c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first");
+    if (!delt) return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    const int nr = c->nrep;
+    std::vector<HmcRep> h(nr);
+    const hipStream_t s = c->stream;
+    HIPC(hipMemcpyAsync(h.data(), c->d_hmc.p, sizeof(HmcRep) * nr, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    std::vector<double> d(delt, delt + nr);
+    for (int r = 0; r < nr; r++) {
+        if (!(d[r] > 0.0)) return fail(MSPMS_EINVAL, "replica %d: delt must be positive", r);
+        const int cyc = adj_cycle ? adj_cycle[r] : 0;
+        if (cyc < 0) return fail(MSPMS_EINVAL, "replica %d: negative adjustment cycle", r);
+        h[r].adj_cycle = cyc;
+        h[r].rreq = rreq_disp ? rreq_disp[r] : 0.0;
+        h[r].adj_moves = 0;
+        h[r].adj_accepted = 0;
+    }
+    int rc;
+    if ((rc = c->d_hmc.upload(h, s)) || (rc = c->d_hmc_delt.upload(d, s))) return rc;
+    HIPC(hipStreamSynchronize(s));
+    drop_graph(c);
+    c->hmc_rep_delt = true;
+    return 0;
+}
+
+extern "C" int mspms_hmc_get_delt(mspms_ctx* c, double* delt) {
+    if (!c || !c->hmc_ready || !c->hmc_rep_delt)
+        return fail(MSPMS_EINVAL, "mspms_hmc_set_delt first");
+    if (!delt) return fail(MSPMS_EINVAL, "null argument");
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipMemcpyAsync(delt, c->d_hmc_delt.p, sizeof(double) * c->nrep, hipMemcpyDeviceToHost,
+                        c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -168,6 +168,10 @@ struct LaunchCtx {
     const int* atom_site;
     const RepDev* reps;
     Status* status;
+    const double* rep_delt;   // null, or each replica's own outer time step
+                              //   (HMC with delt adaptation): integrator
+                              //   kernels then derive deltby2, delts and
+                              //   deltsby2 from it as the host does
 };
 
 struct RecipBufs {
@@ -222,6 +226,11 @@ struct HmcRep {
     double uinter_old, uintra_old, ukin_old, ushift_old;
     double last_dH;
     long long accepted, moves;
+    // delt adaptation of the equilibration phase (src/hmc.c:47-58):
+    // every adj_cycle moves delt *= 1 - rreq + accepted / adj_cycle
+    // (adj_cycle 0: off); counts since the last adjustment
+    double rreq;
+    int adj_cycle, adj_moves, adj_accepted, pad;
 };
 
 // velinit of every replica (active: only where active[r] != 0; null: all)
@@ -231,7 +240,8 @@ void launch_hmc_begin(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const dou
                       const double* pos, double* pos_old, double* vel);
 void launch_hmc_mark(const LaunchCtx& c, HmcRep* hmc, const void* lterms, const void* sterms);
 void launch_hmc_end(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const void* lterms,
-                    const void* sterms, const double* ukin, double* pos, const double* pos_old);
+                    const void* sterms, const double* ukin, double* pos, const double* pos_old,
+                    double* rep_delt = nullptr);
 void launch_hmc_draw(const LaunchCtx& c, RanmarDev* rng, int rep, int n, double* out);
 
 // NPT state of one replica (k_npt.hip): the thermostat and barostat of
@@ -289,6 +299,11 @@ void launch_pair(const LaunchCtx& c, const ClusterBufs& b, int precision,
 struct Kick {
     double *vx, *vy, *vz, *kin;
     double deltby2;
+    // null, or per-replica time steps: deltby2 = rep_delt[rep] / 2.0 with the
+    // atom's replica from atom_mol / mols
+    const double* rep_delt;
+    const int* atom_mol;
+    const MolDev* mols;
 };
 // A RESPA step's per-atom closing pass run by its finalize: fx += frec (the
 // side stream's reciprocal and intramolecular forces) and the half kick,

@@ -18,7 +18,8 @@
 //   nmd RESPA steps (vver)
 //   k_hmc_end     dH = [(uinter - old) + (uintra - old) + (ukin - old) +
 //                 (ushift - old)] / treq; accept if dH <= 0 or a uniform
-//                 draw < exp(-dH); otherwise restore the positions
+//                 draw < exp(-dH); otherwise restore the positions; with
+//                 per-chain steps, the equilibration-phase delt adjustment
 // The random draws are sequential per replica (one lane); momentum,
 // kinetic energy and the position save / restore use the whole workgroup.
 #include "engine.h"
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(256) k_hmc_end(const RepDev* __restrict__ reps
                                                  const mspms_short_terms* __restrict__ st,
                                                  const double* __restrict__ ukin, double* pos,
                                                  const double* __restrict__ pos_old,
-                                                 int natom_total) {
+                                                 int natom_total, double* rep_delt) {
 #pragma clang fp contract(off)
     __shared__ int accept;
     const int rep = blockIdx.x;
@@ -222,6 +223,19 @@ __global__ void __launch_bounds__(256) k_hmc_end(const RepDev* __restrict__ reps
         }
         H.last_dH = dH;
         accept = acc ? 1 : 0;
+        // hmc()'s delt adjustment (src/hmc.c:47-58): md_move counted the
+        // move (counts[20], counts[21], src/dispmd.c:55,74); once a cycle
+        // is full, delt *= 1 - rreq_disp + accepted / cycle
+        if (rep_delt && H.adj_cycle > 0) {
+            H.adj_moves++;
+            if (acc) H.adj_accepted++;
+            if (H.adj_moves == H.adj_cycle) {
+                const double rinst = H.adj_accepted * 1.0 / H.adj_cycle;
+                rep_delt[rep] = rep_delt[rep] * (1.0 - H.rreq + rinst);
+                H.adj_moves = 0;
+                H.adj_accepted = 0;
+            }
+        }
     }
     __syncthreads();
     if (accept) return;
@@ -261,10 +275,10 @@ void launch_hmc_mark(const LaunchCtx& c, HmcRep* hmc, const void* lterms, const 
 
 void launch_hmc_end(const LaunchCtx& c, HmcRep* hmc, RanmarDev* rng, const void* lterms,
                     const void* sterms, const double* ukin, double* pos,
-                    const double* pos_old) {
+                    const double* pos_old, double* rep_delt) {
     hipLaunchKernelGGL(k_hmc_end, dim3(c.nrep), dim3(256), 0, c.stream, c.reps, hmc, rng,
                        (const mspms_long_terms*)lterms, (const mspms_short_terms*)sterms, ukin,
-                       pos, pos_old, c.natom);
+                       pos, pos_old, c.natom, rep_delt);
 }
 
 void launch_hmc_draw(const LaunchCtx& c, RanmarDev* rng, int rep, int n, double* out) {

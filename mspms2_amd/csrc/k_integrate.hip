@@ -18,18 +18,31 @@
 
 namespace {
 
+// a replica's own outer step (HMC with delt adaptation): the host's
+// derivation, src/initvars.c:605-607
+__device__ inline void rep_steps(const double* rep_delt, int rep, int nstep_inner,
+                                 double& deltby2, double& delts, double& deltsby2) {
+#pragma clang fp contract(off)
+    if (!rep_delt) return;
+    const double d = rep_delt[rep];
+    deltby2 = d / 2.0;
+    delts = d / nstep_inner;
+    deltsby2 = delts / 2.0;
+}
+
 __global__ void __launch_bounds__(256) k_respa_inner(
     int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
     const TermDev* __restrict__ terms, const SiteDev* __restrict__ sites, double* x, double* y,
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st) {
+    Status* st, const double* __restrict__ rep_delt) {
 #pragma clang fp contract(off)
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
     const SpeciesDev S = species[M.species];
+    rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     const int a0 = M.atom0, a1 = M.atom0 + M.natom;
     for (int i = a0; i < a1; i++) {
         double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
@@ -87,7 +100,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st) {
+    Status* st, const double* __restrict__ rep_delt) {
 #pragma clang fp contract(off)
     __shared__ double lx[NA * 64], ly[NA * 64], lz[NA * 64], lvx[NA * 64], lvy[NA * 64],
         lvz[NA * 64], lfx[NA * 64], lfy[NA * 64], lfz[NA * 64], lrm[NA * 64];
@@ -96,6 +109,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
     if (m >= nmol) return;
     const MolDev M = mols[m];
     const SpeciesDev S = species[M.species];
+    rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     const int na = M.natom, a0 = M.atom0;
     for (int k = 0; k < na; k++) {
         const int i = a0 + k, s = k * 64 + l;
@@ -166,9 +180,10 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
 // molecules per single-wave workgroup).  Lane j owns site j (kick, drift)
 // and computes bonded terms j, j + TPM, ... into its own LDS copy of the
 // molecule's forces; a site's force is then the sum of the TPM copies in
-// lane order, so the chain of one inner step is one term long instead of
-// the molecule's whole term list.  With at most one term per lane (water:
-// 2 bonds + 1 angle) the sums are those of the reference's term order.
+// lane order, so the chain of one inner step is ceil(terms / TPM) terms
+// long instead of the molecule's whole term list.  The sums associate the
+// terms' contributions differently from frcshort's running sum in term
+// order (src/frcshort.c:47-79), a difference of rounding only.
 template <int NA, int TPM>
 __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
@@ -176,7 +191,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st) {
+    Status* st, const double* __restrict__ rep_delt) {
 #pragma clang fp contract(off)
     static_assert(NA <= TPM && 64 % TPM == 0, "one site per lane");
     constexpr int MPW = 64 / TPM;
@@ -194,6 +209,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
         site0 = S.site0;
         bt0 = S.bterm0;
         nbt = S.nbterm;
+        rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     }
     const bool own = live && j < na;  // this lane owns site j of its molecule
     const int sl = j * MPW + mi;      // its LDS slot
@@ -287,10 +303,14 @@ __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __res
                                                      const double* __restrict__ fxl,
                                                      const double* __restrict__ fyl,
                                                      const double* __restrict__ fzl,
-                                                     double deltby2, double* kin) {
+                                                     double deltby2, double* kin,
+                                                     const double* __restrict__ rep_delt,
+                                                     const int* __restrict__ atom_mol,
+                                                     const MolDev* __restrict__ mols) {
 #pragma clang fp contract(off)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
+    if (rep_delt) deltby2 = rep_delt[mols[atom_mol[i]].rep] / 2.0;
     double r_mass = 1.0 / sites[atom_site[i]].aw;
     double a = vx[i] + (deltby2 * fxl[i] * r_mass);
     double b = vy[i] + (deltby2 * fyl[i] * r_mass);
@@ -312,12 +332,12 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
     hipLaunchKernelGGL(k_respa_inner_lds<NA>, dim3((c.nmol + 63) / 64), dim3(64), 0, c.stream,   \
                        c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl,   \
                        fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart,        \
-                       c.status)
+                       c.status, c.rep_delt)
     if (c.max_mol_atoms <= 4) {
         hipLaunchKernelGGL((k_respa_inner_tpm<4, 4>), dim3((c.nmol + 15) / 16), dim3(64), 0,
                            c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz,
                            fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner,
-                           epart, c.status);
+                           epart, c.status, c.rep_delt);
     } else if (c.max_mol_atoms <= 8) {
         RESPA_LDS(8);
     } else if (c.max_mol_atoms <= 16) {
@@ -326,7 +346,8 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
         int g = (c.nmol + 255) / 256;
         hipLaunchKernelGGL(k_respa_inner, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
                            species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl, fxs, fys,
-                           fzs, deltby2, delts, deltsby2, nstep_inner, epart, c.status);
+                           fzs, deltby2, delts, deltsby2, nstep_inner, epart, c.status,
+                           c.rep_delt);
     }
 #undef RESPA_LDS
 }
@@ -336,5 +357,6 @@ void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,
                         double deltby2, double* kin) {
     int g = (c.natom + 255) / 256;
     hipLaunchKernelGGL(k_respa_final, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site,
-                       c.sites, vx, vy, vz, fxl, fyl, fzl, deltby2, kin);
+                       c.sites, vx, vy, vz, fxl, fyl, fzl, deltby2, kin, c.rep_delt, c.atom_mol,
+                       c.mols);
 }

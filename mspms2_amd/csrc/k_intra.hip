@@ -139,9 +139,10 @@ __device__ inline void kick_atom(const Kick& K, int i, double aw, const double* 
                                  const double* fy, const double* fz) {
 #pragma clang fp contract(off)
     const double r_mass = 1.0 / aw;
-    const double a = K.vx[i] + (K.deltby2 * fx[i] * r_mass);
-    const double b = K.vy[i] + (K.deltby2 * fy[i] * r_mass);
-    const double c = K.vz[i] + (K.deltby2 * fz[i] * r_mass);
+    const double deltby2 = K.rep_delt ? K.rep_delt[K.mols[K.atom_mol[i]].rep] / 2.0 : K.deltby2;
+    const double a = K.vx[i] + (deltby2 * fx[i] * r_mass);
+    const double b = K.vy[i] + (deltby2 * fy[i] * r_mass);
+    const double c = K.vz[i] + (deltby2 * fz[i] * r_mass);
     K.vx[i] = a;
     K.vy[i] = b;
     K.vz[i] = c;
@@ -774,7 +775,7 @@ void launch_intra(const LaunchCtx& c, const SpeciesDev* species, const TermDev* 
                   double* fz, double* epart, double* gq, double* ge, double* gs,
                   const double* frec, const Kick* kick, bool zero_first) {
     int g = (c.nmol + 255) / 256;
-    const Kick K = kick ? *kick : Kick{nullptr, nullptr, nullptr, nullptr, 0.0};
+    const Kick K = kick ? *kick : Kick{};
     if (gq)
         hipLaunchKernelGGL(k_intra<true>, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols, c.reps,
                            c.sites, species, terms, x, y, z, fx, fy, fz, epart,
@@ -808,7 +809,7 @@ __global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) 
 
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
                      const double* frec, const Kick* kick) {
-    const Kick K = kick ? *kick : Kick{nullptr, nullptr, nullptr, nullptr, 0.0};
+    const Kick K = kick ? *kick : Kick{};
     hipLaunchKernelGGL(k_add_frec, dim3((c.natom + 255) / 256), dim3(256), 0, c.stream, c.natom,
                        fx, fy, fz, frec, c.atom_site, c.sites, K);
 }

@@ -12,20 +12,12 @@
 // at zero here (the generator reads uninitialised locals if the first pair of
 // a node lies in the switching shell).
 #include <hip/hip_runtime.h>
-#include "../../include/mspms_gpu.h"
+#include "sfgrid.h"
 
 namespace {
 
 constexpr int SFG_T = 256;
-constexpr int SFG_MAX_SOLID = 1000;  // solid_atom_max, tools/sfgridgen.c:26
 constexpr double SFG_RGAS = 8.314472;  // tools/sfgridgen.c:27
-
-struct SfgParams {
-    double x0, y0, z0, itvl;
-    double sigmasf, epsilonsf, uclz;
-    double rcutonsq, rcutoffsq, roff2_minus_ron2_cube;
-    int nx, ny, nz, nuc_z, nsolid, lj_switch;
-};
 
 __global__ void __launch_bounds__(SFG_T) k_sfgridgen(SfgParams P, const double* __restrict__ sx,
                                                      const double* __restrict__ sy,
@@ -131,12 +123,11 @@ extern "C" int mspms_sfgridgen_dims(const mspms_sfgridgen_spec* s, int* nx, int*
     return 0;
 }
 
-// the engine's error reporter (engine.cpp)
-int mspms_fail(int code, const char* what);
 
-extern "C" int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* s, double* out) {
-    if (!s || !out || !s->sx || !s->sy || !s->sz)
-        return mspms_fail(MSPMS_EINVAL, "null argument");
+// The generator constants of tools/sfgridgen.c:121-155 from a spec; 0 or an
+// MSPMS_* error (recorded).
+int sfgridgen_setup(const mspms_sfgridgen_spec* s, SfgParams* P, size_t* ntot) {
+    if (!s || !s->sx || !s->sy || !s->sz) return mspms_fail(MSPMS_EINVAL, "null argument");
     if (s->s_natom < 1 || s->s_natom > SFG_MAX_SOLID)
         return mspms_fail(MSPMS_EINVAL, "1..1000 solid atoms (solid_atom_max)");
     if (!(s->rcutoff > 0.0) || !(s->uclz > 0.0))
@@ -144,29 +135,42 @@ extern "C" int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* s, double
     int nx, ny, nz;
     if (mspms_sfgridgen_dims(s, &nx, &ny, &nz))
         return mspms_fail(MSPMS_EINVAL, "grid interval must be positive");
-    SfgParams P;
-    // tools/sfgridgen.c:121-155
-    P.rcutonsq = s->rcuton * s->rcuton;
-    P.rcutoffsq = s->rcutoff * s->rcutoff;
-    P.roff2_minus_ron2_cube = (P.rcutoffsq - P.rcutonsq) * (P.rcutoffsq - P.rcutonsq) *
-                              (P.rcutoffsq - P.rcutonsq);
-    P.sigmasf = (s->f_sigma + s->s_sigma) / 2.0;
-    P.epsilonsf = sqrt(s->f_epsilon * s->s_epsilon) * SFG_RGAS;
+    P->rcutonsq = s->rcuton * s->rcuton;
+    P->rcutoffsq = s->rcutoff * s->rcutoff;
+    P->roff2_minus_ron2_cube = (P->rcutoffsq - P->rcutonsq) * (P->rcutoffsq - P->rcutonsq) *
+                               (P->rcutoffsq - P->rcutonsq);
+    P->sigmasf = (s->f_sigma + s->s_sigma) / 2.0;
+    P->epsilonsf = sqrt(s->f_epsilon * s->s_epsilon) * SFG_RGAS;
     int nuc_z = 1;
     while (nuc_z * s->uclz < s->rcutoff) nuc_z++;
     const double xcenter = s->uclx / 2.0, ycenter = s->ucly / 2.0;
-    P.x0 = xcenter - s->uclx / 2.0;
-    P.y0 = ycenter - s->ucly / 2.0;
-    P.z0 = 0.0;
-    P.itvl = s->grid_itvl;
-    P.uclz = s->uclz;
-    P.nx = nx;
-    P.ny = ny;
-    P.nz = nz;
-    P.nuc_z = nuc_z;
-    P.nsolid = s->s_natom;
-    P.lj_switch = s->isLJswitch ? 1 : 0;
-    const size_t ntot = (size_t)nx * ny * nz;
+    P->x0 = xcenter - s->uclx / 2.0;
+    P->y0 = ycenter - s->ucly / 2.0;
+    P->z0 = 0.0;
+    P->itvl = s->grid_itvl;
+    P->uclz = s->uclz;
+    P->nx = nx;
+    P->ny = ny;
+    P->nz = nz;
+    P->nuc_z = nuc_z;
+    P->nsolid = s->s_natom;
+    P->lj_switch = s->isLJswitch ? 1 : 0;
+    *ntot = (size_t)nx * ny * nz;
+    return 0;
+}
+
+hipError_t sfgridgen_launch(const SfgParams& P, const double* sx, const double* sy,
+                            const double* sz, double* out, size_t ntot, hipStream_t st) {
+    hipLaunchKernelGGL(k_sfgridgen, dim3((unsigned)((ntot + SFG_T - 1) / SFG_T)), dim3(SFG_T), 0,
+                       st, P, sx, sy, sz, out, ntot);
+    return hipGetLastError();
+}
+
+extern "C" int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* s, double* out) {
+    SfgParams P;
+    size_t ntot = 0;
+    if (!out) return mspms_fail(MSPMS_EINVAL, "null argument");
+    if (int e = sfgridgen_setup(s, &P, &ntot)) return e;
     if (hipSetDevice(device) != hipSuccess) return mspms_fail(MSPMS_EIO, "hipSetDevice");
     double *d_s = nullptr, *d_out = nullptr;
     const size_t ns = (size_t)s->s_natom;
@@ -183,9 +187,7 @@ extern "C" int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* s, double
                               st)) {
         rc = mspms_fail(MSPMS_EIO, "upload");
     } else {
-        hipLaunchKernelGGL(k_sfgridgen, dim3((unsigned)((ntot + SFG_T - 1) / SFG_T)),
-                           dim3(SFG_T), 0, st, P, d_s, d_s + ns, d_s + 2 * ns, d_out, ntot);
-        if (hipGetLastError() != hipSuccess ||
+        if (sfgridgen_launch(P, d_s, d_s + ns, d_s + 2 * ns, d_out, ntot, st) != hipSuccess ||
             hipMemcpyAsync(out, d_out, 13 * ntot * sizeof(double), hipMemcpyDeviceToHost, st) ||
             hipStreamSynchronize(st) != hipSuccess)
             rc = mspms_fail(MSPMS_EIO, "grid kernel");
@@ -195,3 +197,4 @@ extern "C" int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* s, double
     (void)hipStreamDestroy(st);
     return rc;
 }
+

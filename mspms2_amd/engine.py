@@ -79,6 +79,8 @@ def lib():
                                  C.POINTER(C.c_int), vp]
     L.mspms_hmc_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int,
                                        vp, vp, vp, C.c_int, vp]
+    L.mspms_hmc_set_delt.argtypes = [vp, dp, C.POINTER(C.c_int), dp]
+    L.mspms_hmc_get_delt.argtypes = [vp, dp]
     L.mspms_hmc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
     L.mspms_hmc_draw.argtypes = [vp, C.c_int, C.c_int, dp]
     L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
@@ -312,6 +314,23 @@ class Engine:
         nc = np.ascontiguousarray(np.broadcast_to(nconstraint, n), dtype=np.int32)
         _check(lib().mspms_hmc_init(self._h, abi.iptr(ij), abi.iptr(kl), abi.dptr(tq),
                                     abi.iptr(nc), self._ptr(vel)), "mspms_hmc_init")
+
+    def hmc_set_delt(self, delt, adj_cycle=0, rreq_disp=0.0):
+        """Per-chain outer steps (reduced) and the equilibration-phase delt
+        adaptation: every adj_cycle moves, delt *= 1 - rreq_disp +
+        accepted / adj_cycle (adj_cycle 0: fixed); scalars or per-replica
+        sequences."""
+        n = self.nrep
+        d = np.ascontiguousarray(np.broadcast_to(delt, n), dtype=np.float64)
+        cy = np.ascontiguousarray(np.broadcast_to(adj_cycle, n), dtype=np.int32)
+        rr = np.ascontiguousarray(np.broadcast_to(rreq_disp, n), dtype=np.float64)
+        _check(lib().mspms_hmc_set_delt(self._h, abi.dptr(d), abi.iptr(cy), abi.dptr(rr)),
+               "mspms_hmc_set_delt")
+
+    def hmc_get_delt(self):
+        out = np.zeros(self.nrep)
+        _check(lib().mspms_hmc_get_delt(self._h, abi.dptr(out)), "mspms_hmc_get_delt")
+        return out
 
     def hmc_run_device(self, pos, vel, fl, fs, delt, nstep_inner, nmd, nmove, lterms, sterms,
                        ukin, graph=True, stream=None):

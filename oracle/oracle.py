@@ -171,7 +171,7 @@ def reference_available() -> bool:
 
 def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
                   md_steps: int = 0, files_out: Optional[str] = None,
-                  siman: bool = False) -> dict:
+                  siman: bool = False, hmc: int = 0) -> dict:
     """Runs the reference's start-up path + frclong + frcshort on a run
     directory (in a scratch copy) and returns its dump; md_steps > 0 then
     runs that many steps of the input's integrator (vver / vver_nh_3 /
@@ -180,7 +180,9 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
     of the final state (trj.mspms, ss.mspms, sav.mspms) are copied there;
     an old.mspms in the run directory is passed along (restart inputs), and
     so are the cube files a MYINTERP section names.
-    siman runs the reference's simulated annealing driver instead."""
+    siman runs the reference's simulated annealing driver instead; hmc > 0
+    runs that many moves of its HMC driver and adds the per-move record
+    (d["hmc"]: accept, dH, delt, uinter, uintra, ukin per move)."""
     if not reference_available():
         raise FileNotFoundError(REF_HARNESS)
     with tempfile.TemporaryDirectory(prefix="mspms_ref_") as tmp:
@@ -200,7 +202,8 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
         out = os.path.join(tmp, "dump.bin")
         cmd = [REF_HARNESS, tmp, out] + (["--repeat", str(repeat)] if repeat else []) + \
             (["--md-steps", str(md_steps)] if md_steps else []) + \
-            (["--write-files"] if files_out else []) + (["--siman"] if siman else [])
+            (["--write-files"] if files_out else []) + (["--siman"] if siman else []) + \
+            (["--hmc", str(hmc)] if hmc else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness failed ({r.returncode}): {r.stderr[-2000:]}")
@@ -210,6 +213,11 @@ def run_reference(rundir: str, repeat: int = 0, timeout: Optional[float] = None,
                 d["frclong_seconds_per_call"] = float(ln.split()[1])
             if ln.startswith("md_seconds_per_step"):
                 d["md_seconds_per_step"] = float(ln.split()[1])
+            if ln.startswith("hmc_move"):
+                t = ln.split()
+                d.setdefault("hmc", []).append(
+                    dict(accept=int(t[2]), dH=float(t[3]), delt=float(t[4]),
+                         uinter=float(t[5]), uintra=float(t[6]), ukin=float(t[7])))
         if files_out:
             os.makedirs(files_out, exist_ok=True)
             for fn in ("trj.mspms", "ss.mspms", "sav.mspms"):

@@ -24,13 +24,22 @@
  * frame, snapshot and checkpoint of that state into RUNDIR (trajectory(),
  * snapshot(), saveit(): trj.mspms, ss.mspms, sav.mspms).
  *
+ * With --hmc N it runs N moves of the reference's hybrid Monte Carlo
+ * driver hmc() (src/hmc.c:23-70: velinit + frclong + frcshort on every move
+ * but the first, md_move over nstep_md_per_hmc vver steps, Metropolis on
+ * dH, the delt adaptation while bEquilibrium) as the main loop calls it
+ * (src/mspms2.c:99-100, istep from nstep_start, within nstep_end), prints
+ * one line per move
+ *   hmc_move K ACCEPT dH delt uinter uintra ukin
+ * and dumps the state after the last move (version 2).
+ *
  * With --siman it runs the reference's simulated annealing driver siman()
  * (src/siman.c:61-182: cycles of velinit + vver_nh_3 steps, cooling
  * between them) instead, and dumps the state it ends in (version 2, no
  * integrator timing).
  *
  * Usage: mspms_ref_harness RUNDIR OUTFILE [--repeat K] [--md-steps N] [--write-files]
- *                          [--siman]
+ *                          [--siman] [--hmc N]
  *   --repeat K times K further frclong() calls and prints seconds/call;
  *   --md-steps N also prints the integrator's seconds per step.
  *
@@ -54,7 +63,7 @@ static double v0[3][NATOM_MAX];
 
 int main(int argc, char** argv)
 {
-    int repeat = 0, nmd = 0, files = 0, sa = 0, k;
+    int repeat = 0, nmd = 0, files = 0, sa = 0, nhmc = 0, k;
     char outpath[4096];
     FILE* fo;
     if (argc < 3) {
@@ -70,6 +79,8 @@ int main(int argc, char** argv)
             files = 1;
         else if (!strcmp(argv[k], "--siman"))
             sa = 1;
+        else if (!strcmp(argv[k], "--hmc") && k + 1 < argc)
+            nhmc = atoi(argv[++k]);
     if (argv[2][0] == '/') {
         snprintf(outpath, sizeof outpath, "%s", argv[2]);
     } else {
@@ -109,6 +120,19 @@ int main(int argc, char** argv)
     }
     if (sa) {
         siman();
+        nmd = 1;  /* version-2 dump of the final state */
+    } else if (nhmc > 0) {
+        if (what_simulation != HYBRID_MONTE_CARLO || nstep_start + nhmc - 1 > nstep_end) {
+            fprintf(stderr, "ref_harness: --hmc needs an HMC input with at least %d steps "
+                            "in its current phase\n", nhmc);
+            return 3;
+        }
+        for (k = 0; k < nhmc; k++) {
+            istep = nstep_start + k;
+            hmc();
+            printf("hmc_move %d %d %.17g %.17g %.17g %.17g %.17g\n", k, isAccept ? 1 : 0, dH,
+                   delt, uinter, uintra, ukin);
+        }
         nmd = 1;  /* version-2 dump of the final state */
     } else if (nmd > 0) {
         struct timespec t0, t1;

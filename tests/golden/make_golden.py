@@ -156,6 +156,44 @@ def make_sfgridgen_case():
     print("sfgridgen_ch3_1A", os.path.getsize(os.path.join(path, "out.sfgridgen")))
 
 
+HMC_CASES = {
+    # 48 butanes, 3 fs, 20 moves of the equilibration phase with the delt
+    # adaptation every 5 moves toward 50 % acceptance (src/hmc.c:47-61)
+    "hmc_butane_eq": (48, 5, 20, 20, 20),
+    # 40 butanes, 3 fs, 12 production moves (nstep_eq 0: delt fixed)
+    "hmc_butane_prod": (40, 6, 0, 12, 12),
+}
+
+
+def hmc_case(name):
+    nmol, seed, neq, nstep, _ = HMC_CASES[name]
+    rd = configs.butane_box(nmol=nmol, seed=seed, delt_fs=3.0, nstep_inner=5)
+    r = rd.run
+    r.nstep_eq, r.nstep = neq, nstep
+    r.nstep_delt_adj_cycle = 5
+    r.rreq_disp = 0.5
+    return rd
+
+
+def make_hmc_cases():
+    """The reference's own hmc() on two small n-butane boxes: per move the
+    accept flag, dH, delt after the move, uinter, uintra and ukin, and the
+    final positions and velocities."""
+    for name, (_, _, _, _, nmove) in HMC_CASES.items():
+        path = os.path.join(HERE, name)
+        os.makedirs(path, exist_ok=True)
+        hmc_case(name).write(path)
+        d = O.run_reference(path, hmc=nmove)
+        h = d["hmc"]
+        np.savez_compressed(os.path.join(path, "hmc.npz"),
+                            accept=np.array([m["accept"] for m in h], dtype=np.int8),
+                            **{k: np.array([m[k] for m in h])
+                               for k in ("dH", "delt", "uinter", "uintra", "ukin")},
+                            x=np.stack([d["xx"], d["yy"], d["zz"]]), v=d["v"], v0=d["v0"],
+                            natom=d["natom"])
+        print(name, d["natom"], "".join(str(m["accept"]) for m in h), h[-1]["delt"])
+
+
 def make_myinterp_case():
     """CH3 sites and CH2 dimers in the (11,11) nanotube cell of
     sfgridgen_ch3_1A (the reference's tools/CH3-SWNT11x11.in), iSF_type 4:
@@ -212,10 +250,13 @@ def make_file_cases():
 
 def main(only=()):
     """Every fixture, or with names given only those single-evaluation
-    cases (CASES)."""
+    cases (CASES), and "hmc" for the HMC chains."""
     if not O.reference_available():
         raise SystemExit("oracle/_ref is not built (needs the reference checkout)")
     if only:
+        if "hmc" in only:
+            make_hmc_cases()
+            only = [n for n in only if n != "hmc"]
         for name in only:
             path = os.path.join(HERE, name)
             os.makedirs(path, exist_ok=True)
@@ -248,6 +289,7 @@ def main(only=()):
             print(name, n, d["md"]["ukin"], d["md"]["ps"])
     make_file_cases()
     make_siman_case()
+    make_hmc_cases()
     make_sfgridgen_case()
     make_myinterp_case()
     for name, fn in BIG.items():

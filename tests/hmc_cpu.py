@@ -1,5 +1,6 @@
 """CPU restatement of one hybrid Monte Carlo chain (test infrastructure):
-hmc() with canonical moves (src/hmc.c:23-41), md_move (src/dispmd.c:12-103),
+hmc() with canonical moves and the equilibration-phase delt adaptation
+(src/hmc.c:23-61), md_move (src/dispmd.c:12-103),
 vver (src/md.c:22-83), velinit (src/initvars.c:151-232) and gaussran
 (src/random.c:113-138) over the oracle's frclong / frcshort and its RANMAR
 stream, expression order as in the reference (numpy loops are vectorised
@@ -53,6 +54,10 @@ class HmcChain:
         self.x = np.stack([system.xx, system.yy, system.zz]).astype(np.float64)
         self.v = self.velinit()
         self.first = True
+        # counts[20] / counts[21] of the reference: canonical moves and
+        # accepted ones since the last delt adjustment
+        self.n_moves = 0
+        self.n_accepted = 0
 
     def velinit(self):
         n = self.s.natom
@@ -110,3 +115,20 @@ class HmcChain:
             self.x = x_old
             self.ukin = ukin_old
         return acc, dh
+
+    def run(self, nmove, delt, nstep_inner, nmd, adapt=None):
+        """nmove moves of hmc(); adapt = (nstep_delt_adj_cycle, rreq_disp)
+        while the chain is in its equilibration phase (bEquilibrium): every
+        cycle of moves, delt *= 1 - rreq_disp + accepted / cycle
+        (src/hmc.c:47-58).  Returns [(accepted, dH, delt after the move)]."""
+        out = []
+        for _ in range(nmove):
+            acc, dh = self.move(delt, nstep_inner, nmd)
+            self.n_moves += 1
+            self.n_accepted += 1 if acc else 0
+            if adapt is not None and self.n_moves == adapt[0]:
+                rinst = self.n_accepted * 1.0 / adapt[0]
+                delt = delt * (1.0 - adapt[1] + rinst)
+                self.n_moves = self.n_accepted = 0
+            out.append((acc, dh, delt))
+        return out

@@ -122,3 +122,81 @@ def test_mixed_chains_run_and_accept():
     assert bool(torch.isfinite(pos).all())
     assert acc.sum() > 0
     eng.close()
+
+
+def _reference_chains():
+    """The two reference HMC chains (tests/golden/hmc_butane_*, the
+    reference's own hmc()) as one replica table."""
+    from tests.test_hmc_golden import CASES, adapt_of, load
+    out = [load(n) for n in CASES]
+    return [s for s, _ in out], [ref for _, ref in out], [adapt_of(s.run) for s, _ in out]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_fp64_chains_match_reference_hmc(graph):
+    """Both reference chains in one table, FP64: velinit's velocities to
+    1e-13; per move the accept decision equal, dH within 1e-7 (relative,
+    floor 1e-3) and each chain's delt (one adapting every 5 moves toward
+    50 % acceptance, one fixed) bit for bit; positions after the last move
+    within 1e-8 of the box."""
+    import torch
+    dev = torch.device("cuda:0")
+    systems, refs, adapts = _reference_chains()
+    eng = Engine.replicas(systems, precision="fp64")
+    pos, vel, fl, fs, lt, st, uk = _device_state(eng, systems, dev)
+    runs = [s.run for s in systems]
+    eng.hmc_init([r.ij for r in runs], [r.jk for r in runs], [s.treq for s in systems],
+                 [r.nconstraint for r in runs], vel)
+    torch.cuda.synchronize()
+    v = vel.cpu().numpy()
+    for k, ref in enumerate(refs):
+        vr = eng.unpack(v)[k]
+        assert np.abs(vr - ref["v0"]).max() <= 1e-13 * np.abs(ref["v0"]).max()
+    eng.hmc_set_delt([s.delt for s in systems], [a[0] if a else 0 for a in adapts],
+                     [a[1] if a else 0.0 for a in adapts])
+    inner, nmd = runs[0].nstep_inner, runs[0].nstep_md_per_hmc
+    nmove = min(len(ref["accept"]) for ref in refs)
+    prev = np.zeros(len(systems), dtype=np.int64)
+    for m in range(nmove):
+        eng.hmc_run_device(pos, vel, fl, fs, systems[0].delt, inner, nmd, 1, lt, st, uk,
+                           graph=graph)
+        torch.cuda.synchronize()
+        acc, mov, dh = eng.hmc_stats()
+        delt = eng.hmc_get_delt()
+        for k, ref in enumerate(refs):
+            a = int(acc[k] - prev[k])
+            print(f"chain {k} move {m}: ref accept={ref['accept'][m]} dH={ref['dH'][m]:.6e} "
+                  f"delt={ref['delt'][m]:.6e} | gpu accept={a} dH={dh[k]:.6e} "
+                  f"delt={delt[k]:.6e}", flush=True)
+            assert a == int(ref["accept"][m])
+            assert abs(dh[k] - ref["dH"][m]) <= 1e-7 * max(abs(ref["dH"][m]), 1e-3)
+            assert delt[k] == ref["delt"][m]
+        prev = acc.copy()
+    eng.sync()
+    if nmove == len(refs[1]["accept"]):
+        x = eng.unpack(pos.cpu().numpy())[1]
+        assert np.abs(x - refs[1]["x"]).max() <= 1e-8 * systems[1].params["boxlx"]
+    eng.close()
+
+
+def test_fp64_chain_reaches_reference_final_state():
+    """The adapting chain alone for all of its reference moves: its final
+    positions within 1e-8 of the box and its final delt bit for bit."""
+    import torch
+    dev = torch.device("cuda:0")
+    systems, refs, adapts = _reference_chains()
+    s, ref, ad = systems[0], refs[0], adapts[0]
+    eng = Engine.replicas([s], precision="fp64")
+    pos, vel, fl, fs, lt, st, uk = _device_state(eng, [s], dev)
+    r = s.run
+    eng.hmc_init(r.ij, r.jk, s.treq, r.nconstraint, vel)
+    eng.hmc_set_delt(s.delt, ad[0], ad[1])
+    eng.hmc_run_device(pos, vel, fl, fs, s.delt, r.nstep_inner, r.nstep_md_per_hmc,
+                       len(ref["accept"]), lt, st, uk, graph=True)
+    torch.cuda.synchronize()
+    eng.sync()
+    acc, mov, _ = eng.hmc_stats()
+    assert int(mov[0]) == len(ref["accept"]) and int(acc[0]) == int(ref["accept"].sum())
+    assert eng.hmc_get_delt()[0] == ref["delt"][-1]
+    assert np.abs(pos.cpu().numpy() - ref["x"]).max() <= 1e-8 * s.params["boxlx"]
+    eng.close()
