@@ -18,8 +18,10 @@ engine is timed beside it.
 Prints one JSON line (the benchmark contract) with `roofline` for the
 dominant kernel and `cpu_baseline`: the reference's own frclong (oracle/_ref,
 built from the reference sources) timed on the host, one thread.  Extra
-legs: NVE energy drift (mixed and FP64), the batched-replica workload
-(256 n-butane boxes, one table against one engine per box).
+legs: NVE energy drift (mixed and FP64) from an equilibrated state, batched
+HMC (256 n-butane boxes, one table against one engine per box), the Wolf
+water fitting set with the parameter adjoint, the LJ-864 NVT workload and
+the grid generators.
 """
 from __future__ import annotations
 
@@ -415,39 +417,37 @@ def make_replicas(nrep=256, seed=2024):
     0.58 g/cm^3, rc = min(14 A, L/2), one seed per box."""
     from mspms2_amd import configs
     from mspms2_amd.system import system_of
-    rng = np.random.default_rng(seed)
-    n = rng.integers(64, 513, size=nrep)
-    return [system_of(configs.butane_box(nmol=int(k), seed=1000 + r)) for r, k in enumerate(n)]
+    return [system_of(rd) for rd in configs.butane_table(nrep, seed)]
 
 
-def _replica_run(systems, steps, warmup, nstep_inner, dev):
-    """RESPA steps of one engine over `systems` (one replica table), replayed
-    from a captured graph; returns (ms per step, i<j pairs in range per step)."""
+def _hmc_run(systems, moves, warmup, nmd, nstep_inner, dev):
+    """HMC moves (nmd RESPA steps each) of one engine over `systems` (one
+    replica table, one chain per replica), replayed from a captured move
+    graph; returns (ms per move, i<j LJ pairs in range per force evaluation,
+    accepted fraction, finite)."""
     import torch
     from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
     from mspms2_amd import abi
     eng = Engine.replicas(systems, precision="mixed")
     nr = len(systems)
     pos = torch.tensor(eng.packed_positions(), device=dev)
-    vel = torch.tensor(np.concatenate([maxwell_velocities(s, seed=11) for s in systems], axis=1),
-                       device=dev)
-    f0, _ = eng.frclong()
-    fs0, _ = eng.frcshort()
-    fl, fs = torch.tensor(f0, device=dev), torch.tensor(fs0, device=dev)
+    vel = torch.zeros_like(pos)
+    fl, fs = torch.zeros_like(pos), torch.zeros_like(pos)
     lt = torch.zeros(nr * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
     stt = torch.zeros(nr * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
     uk = torch.zeros(nr, dtype=torch.float64, device=dev)
+    eng.hmc_init([1000 + r for r in range(nr)], [2000 + r for r in range(nr)],
+                 [s.treq for s in systems], [s.run.nconstraint for s in systems], vel)
     st = torch.cuda.Stream()
-    torch.cuda.synchronize()
     delt = systems[0].delt
-    eng.vver_run_device(pos, vel, fl, fs, delt, nstep_inner, warmup, lt, stt, uk, graph=True,
-                        stream=st.cuda_stream)
+    eng.hmc_run_device(pos, vel, fl, fs, delt, nstep_inner, nmd, warmup, lt, stt, uk,
+                       graph=True, stream=st.cuda_stream)
     torch.cuda.synchronize()
     eng.sync()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(st)
-    eng.vver_run_device(pos, vel, fl, fs, delt, nstep_inner, steps, lt, stt, uk, graph=True,
-                        stream=st.cuda_stream)
+    eng.hmc_run_device(pos, vel, fl, fs, delt, nstep_inner, nmd, moves, lt, stt, uk,
+                       graph=True, stream=st.cuda_stream)
     b.record(st)
     torch.cuda.synchronize()
     eng.sync()
@@ -455,29 +455,178 @@ def _replica_run(systems, steps, warmup, nstep_inner, dev):
     sz = LONG_TERMS_BYTES
     pairs = sum(abi.LongTerms.from_buffer_copy(raw[r * sz:(r + 1) * sz]).n_lj_pairs
                 for r in range(nr))
+    acc, mov, _ = eng.hmc_stats()
     finite = bool(torch.isfinite(pos).all().item())
     eng.close()
-    return a.elapsed_time(b) / steps, pairs, finite
+    return a.elapsed_time(b) / moves, pairs, float(acc.sum()) / max(1, int(mov.sum())), finite
 
 
-def run_replica_batch(systems, steps=20, warmup=3, nstep_inner=5):
-    """All replicas in one table (one launch set per kernel) against the
-    same replicas run one after another, each its own engine and graph."""
+def run_replica_batch(systems, moves=10, warmup=2, nmd=10, nstep_inner=5):
+    """Batched HMC: every box's chain in one replica table, one captured move
+    graph (velinit, forces, nmd RESPA steps of 1 fs with nstep_inner inner
+    steps, Metropolis) against the same boxes run one after another, each
+    its own engine and graph.  A move evaluates frclong nmd + 1 times."""
     import torch
     dev = torch.device("cuda:0")
-    ms_b, pairs, fin = _replica_run(systems, steps, warmup, nstep_inner, dev)
+    ms_b, pairs, acc, fin = _hmc_run(systems, moves, warmup, nmd, nstep_inner, dev)
     ms_seq, fin_s = 0.0, True
     for s in systems:
-        ms1, _, f1 = _replica_run([s], max(2, steps // 4), 1, nstep_inner, dev)
+        ms1, _, _, f1 = _hmc_run([s], 2, 1, nmd, nstep_inner, dev)
         ms_seq += ms1
         fin_s = fin_s and f1
+    evals = nmd + 1
     return {"workload": f"{len(systems)} n-butane boxes, 64-512 molecules each, "
-                        f"{sum(s.natom for s in systems)} sites; RESPA 1 fs / {nstep_inner}",
-            "batched_ms_per_step": ms_b, "sequential_ms_per_step": ms_seq,
-            "pairs_per_step": int(pairs),
-            "batched_pair_interactions_per_s": pairs / (ms_b * 1e-3),
-            "sequential_pair_interactions_per_s": pairs / (ms_seq * 1e-3),
-            "batched_over_sequential": ms_seq / ms_b, "finite": fin and fin_s}
+                        f"{sum(s.natom for s in systems)} sites; HMC moves of {nmd} RESPA "
+                        f"steps (1 fs / {nstep_inner}), one chain per box",
+            "batched_ms_per_move": ms_b, "sequential_ms_per_move": ms_seq,
+            "pairs_per_force_evaluation": int(pairs), "force_evaluations_per_move": evals,
+            "batched_pair_interactions_per_s": evals * pairs / (ms_b * 1e-3),
+            "sequential_pair_interactions_per_s": evals * pairs / (ms_seq * 1e-3),
+            "batched_over_sequential": ms_seq / ms_b, "acceptance": acc,
+            "finite": fin and fin_s}
+
+
+def run_wolf_fit(nconf=64, calls=5, cpu=True):
+    """The force-field fitting workload: nconf configurations of 1024 Wolf
+    waters (kappa 0.2 / A, rc 12 A) in one replica table; one call gives
+    every configuration's energy terms, forces and dE/d(epsilon, sigma,
+    charge) per site in FP64 (mspms_energy_param_grad, host buffers: the
+    positions go up and forces and derivatives come down in every call).
+    CPU: the oracle's FP64 frclong of one configuration on one host core
+    (a derivative by central differences would take two per parameter)."""
+    from mspms2_amd import configs
+    from mspms2_amd.engine import Engine
+    from mspms2_amd.system import system_of
+    systems = [system_of(rd) for rd in configs.wolf_water_set(nconf)]
+    eng = Engine.replicas(systems, precision="fp64")
+    x = eng.packed_positions()
+    _, terms, _ = eng.param_grad(*x)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        _, terms, g = eng.param_grad(*x)
+    dt = (time.perf_counter() - t0) / calls
+    eng.close()
+    pairs = sum(t["n_elec_pairs"] for t in terms)
+    out = {"workload": f"{nconf} x 1024 Wolf waters (kappa 0.2 / A, rc 12 A), energy + forces "
+                       f"+ per-site dE/d(eps, sigma, q), FP64",
+           "ms_per_call": 1e3 * dt, "configurations_per_s": nconf / dt,
+           "pairs_per_configuration": pairs / nconf,
+           "finite": bool(np.isfinite(g["charge"]).all())}
+    if cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(systems[0])
+        o.frclong()
+        t0 = time.perf_counter()
+        o.frclong()
+        tc = time.perf_counter() - t0
+        out["cpu_port"] = {"seconds_per_configuration_forward": tc, "cores": 1,
+                           "sample": "one frclong of configuration 0 (oracle restatement)"}
+        out["speedup_vs_cpu_forward_only"] = tc * nconf / dt
+    return out
+
+
+TUBE_GRID_BYTES_PER_LOOKUP = 256 + 24 + 24   # 64 FP32 node values + position + force
+
+
+def make_tube(nload=16):
+    """The methane-in-tube workload: 16 loadings of 20,000-100,000 LJ
+    methanes in a 100 x 100 x 4000 A box inside a 50 A smooth tube given as
+    a 128 x 128 x 1024 FP32 tricubic Hermite grid (512 MiB) tabulated from
+    the closed-form tube."""
+    from mspms2_amd import configs, sfgrid
+    from mspms2_amd.system import system_of
+    h = 100.0 / 127.0
+    grid = sfgrid.tube_grid([(3.73, 148.0)], R=50.0, x0=-50.0, y0=-50.0, nx=128, ny=128, hx=h,
+                            hy=h, nz=1024, hz=4000.0 / 1023.0, sigma_s=3.37, eps_s=27.68)
+    systems = [system_of(rd) for rd in configs.tube_loadings(nload)]
+    for s in systems:
+        s.grid = grid
+    return systems
+
+
+def run_tube(systems, steps=20, warmup=3, skin_a=1.0, cpu=True):
+    """All loadings in one replica table, mixed, RESPA 2 fs outer / 4 inner
+    steps with the wall's grid lookups in the inner loop (mspms_set_sf_split)
+    and the pairs in the outer step; graph-replayed steps recording every
+    step's terms.  The inner-loop kernel is the grid kernel: its roofline is
+    HBM, 304 B per lookup (256 B of node values gathered, position, force)."""
+    import torch
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from mspms2_amd import abi
+    dev = torch.device("cuda:0")
+    nr = len(systems)
+    ntot = sum(s.natom for s in systems)
+    eng = Engine.replicas(systems, precision="mixed")
+    eng.set_sf_split(True)
+    eng.set_list_skin(skin_a / systems[0].params["sigma_base"])
+    pos = torch.tensor(eng.packed_positions(), device=dev)
+    vel = torch.tensor(np.concatenate([maxwell_velocities(s, seed=13) for s in systems], axis=1),
+                       device=dev)
+    fl = torch.tensor(eng.frclong()[0], device=dev)
+    fs = torch.tensor(eng.frcshort()[0], device=dev)
+    lt = torch.zeros(nr * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    stt = torch.zeros(nr * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    uk = torch.zeros(nr, dtype=torch.float64, device=dev)
+    delt, nin = systems[0].delt, systems[0].run.nstep_inner
+    st = torch.cuda.Stream()
+    eng.vver_run_device(pos, vel, fl, fs, delt, nin, warmup, lt, stt, uk, graph=True,
+                        stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    eng.sync()
+    eng.set_profiling(True)
+    for _ in range(steps):
+        eng.vver_step_device(pos, vel, fl, fs, delt, nin, lt, stt, uk, st.cuda_stream)
+    torch.cuda.synchronize()
+    eng.sync()
+    groups = eng.timings()
+    eng.set_profiling(False)
+    lh = torch.zeros(steps * nr * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    eng.vver_run_device(pos, vel, fl, fs, delt, nin, steps, lt, stt, uk, lhist=lh, graph=True,
+                        stream=st.cuda_stream)
+    b.record(st)
+    torch.cuda.synchronize()
+    eng.sync()
+    ms = a.elapsed_time(b) / steps
+    raw = lt.cpu().numpy().tobytes()
+    sz = LONG_TERMS_BYTES
+    terms = [abi.LongTerms.from_buffer_copy(raw[r * sz:(r + 1) * sz]) for r in range(nr)]
+    pairs = sum(t.n_lj_pairs for t in terms)
+    finite = bool(torch.isfinite(pos).all().item()) and all(np.isfinite(t.uinter) for t in terms)
+    eng.close()
+    kms = groups.get("bonded", 0.0)
+    gbytes = ntot * nin * TUBE_GRID_BYTES_PER_LOOKUP
+    out = {"workload": f"{nr} methane loadings ({systems[0].natom}-{systems[-1].natom} sites, "
+                       f"{ntot} in all) in a 50 A tube, 100 x 100 x 4000 A box, rc 12 A; "
+                       f"128 x 128 x 1024 FP32 Hermite grid; RESPA {DT_FS:g} fs / {nin} with "
+                       f"the grid in the inner loop",
+           "ms_per_step": ms, "ns_per_day": DT_FS * 1e-6 * 86400e3 / ms,
+           "pairs_per_step": int(pairs), "pair_interactions_per_s": pairs / (ms * 1e-3),
+           "grid_lookups_per_step": ntot * nin,
+           "grid_kernel": {"name": "k_respa_inner_tpm<1, 1> (the inner loop with the grid)",
+                           "ms": kms, "bytes_per_launch": gbytes,
+                           "achieved_GBps": gbytes / (kms * 1e-3) / 1e9 if kms > 0 else None,
+                           "frac_of_hbm_peak": (gbytes / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS)
+                           if kms > 0 else None},
+           "groups_ms": groups, "finite": finite}
+    if cpu:
+        from oracle import oracle as O
+        if O.reference_available():
+            # the reference's frclong on the smallest loading with the
+            # closed-form tube (iSF_type 1): the all-pairs loop plus
+            # cal_sf_hypergeo per site, one core
+            from mspms2_amd import configs
+            rd = configs.tube_loading(systems[0].natom, seed=500)
+            rd.run.iSF_type = 1
+            with tempfile.TemporaryDirectory(prefix="mspms_bench_") as tmp:
+                rd.write(tmp)
+                d = O.run_reference(tmp, repeat=1, timeout=600)
+            out["cpu_reference"] = {"seconds_per_frclong": d["frclong_seconds_per_call"],
+                                    "cores": 1,
+                                    "sample": f"one frclong of the {systems[0].natom}-site "
+                                              f"loading, closed-form tube (iSF_type 1)"}
+    return out
 
 
 def run_lj_nvt(steps=1000, cpu=True):
@@ -614,6 +763,12 @@ def main():
                     help="pair-list buffer in Angstrom (lists reused until a site moves half of it)")
     ap.add_argument("--replicas", type=int, default=256,
                     help="boxes in the batched-replica leg (0: skip it)")
+    ap.add_argument("--tube-loadings", type=int, default=16,
+                    help="loadings of the methane-in-tube leg (0: skip it)")
+    ap.add_argument("--tube-only", action="store_true",
+                    help="run only the methane-in-tube leg and print its record")
+    ap.add_argument("--wolf-configs", type=int, default=64,
+                    help="configurations of the Wolf-water fitting leg (0: skip it)")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="NVE steps of the energy-drift leg, per precision (0: skip it)")
     ap.add_argument("--equil-steps", type=int, default=2500,
@@ -643,6 +798,9 @@ def main():
         return
     if args.sfgrid_only:
         print(json.dumps(run_sfgrid(args.sfgrid, cpu=not args.no_cpu)))
+        return
+    if args.tube_only:
+        print(json.dumps(run_tube(make_tube(args.tube_loadings), cpu=not args.no_cpu)))
         return
     rd, system = make_workload()
     if args.probe:
@@ -774,6 +932,10 @@ def main():
         line["energy_drift"] = drift
     if args.replicas > 0:
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
+    if args.wolf_configs > 0:
+        line["wolf_fit"] = run_wolf_fit(args.wolf_configs, cpu=not args.no_cpu)
+    if args.tube_loadings > 0:
+        line["tube"] = run_tube(make_tube(args.tube_loadings), cpu=not args.no_cpu)
     if args.lj_steps > 0:
         line["lj864_nvt"] = run_lj_nvt(args.lj_steps, cpu=not args.no_cpu)
     if args.sfgrid > 0:

@@ -224,9 +224,13 @@ typedef struct mspms_long_terms {
     uint64_t lj_pair_hash, elec_pair_hash;
 } mspms_long_terms;
 
-/* Outputs of frcshort (src/frcshort.c:26-87). */
+/* Outputs of frcshort (src/frcshort.c:26-87).  usflj: the solid-fluid
+ * energy when mspms_set_sf_split has moved the wall into frcshort and
+ * frcshort ran on its own (0 otherwise; a RESPA step reports the wall's
+ * energy in its long terms, as src/frclong.c:1223 does). */
 typedef struct mspms_short_terms {
     double ubond, uangle, udih, uimp, uintra, virial_intra;
+    double usflj;
 } mspms_short_terms;
 
 /* Parameter gradients of the nonbonded energy uinter (LJ + Ewald or Wolf,
@@ -558,8 +562,17 @@ int mspms_npt_run_device(mspms_ctx* ctx, double* pos, double* vel, double* fl, d
  * device-buffer evaluations since the previous mspms_sync (list overflow,
  * non-finite input, grid out of range, singular improper), which the
  * host-buffer calls check by themselves; the status is cleared.  On a
- * pair-list overflow the lists are regrown before MSPMS_ENOMEM is returned,
- * so that the caller can repeat the steps since the previous sync. */
+ * pair-list overflow the lists are regrown before MSPMS_ENOMEM is returned.
+ * If the overflow happened in NVE RESPA steps (mspms_vver_step_device /
+ * mspms_vver_run_device, and the steps of HMC moves), the positions and
+ * velocities of that run are put back to the start of the first step that
+ * overflowed, i.e. the end of the last good step (every step saves its start
+ * while no overflow is flagged; the message names the step); the caller
+ * recomputes the forces and terms (mspms_frclong_device /
+ * mspms_frcshort_device) and repeats the steps from there.  For the other
+ * device calls (NVT, NPT, annealing, plain evaluations) nothing is put back:
+ * keep a copy of the state from the previous sync to repeat them.  HMC chain
+ * statistics and random streams are not put back. */
 int mspms_sync(mspms_ctx* ctx);
 
 /* The context's own HIP stream (hipStream_t). */
@@ -600,6 +613,20 @@ int mspms_set_launch_shape(mspms_ctx* ctx, double cell_atoms, int waves_per_clus
  * the positions, default 0: a list is reused only for unchanged
  * positions).  mspms_list_builds reports the number of builds since the
  * last change of system, parameters, launch shape or skin. */
+/* RESPA split of the solid-fluid wall (on != 0): the wall's forces move
+ * from frclong to frcshort, so a device RESPA step evaluates them in every
+ * inner step with the bonded terms (the fast forces) and frclong keeps the
+ * pairs and the electrostatics.  A RESPA step still reports the wall's
+ * energy in the long terms (usflj, in uinter); a standalone frcshort reports
+ * it in mspms_short_terms.usflj and a standalone frclong leaves it out.
+ * Supported by the NVE RESPA step (mspms_vver_*) and HMC; the NVT / NPT
+ * integrators refuse it.  No counterpart in the reference, which evaluates
+ * the wall in frclong (src/frclong.c:1145-1209). */
+int mspms_set_sf_split(mspms_ctx* ctx, int on);
+/* Upper bound on the pair-list capacity per cluster row (entries; 0: none,
+ * the default).  Rows that need more overflow (MSPMS_ENOMEM, see mspms_sync,
+ * which lifts the bound).  No counterpart in the reference. */
+int mspms_set_list_capacity(mspms_ctx* ctx, int max_entries);
 int mspms_set_list_skin(mspms_ctx* ctx, double skin);
 int mspms_list_builds(mspms_ctx* ctx, long long* builds);
 

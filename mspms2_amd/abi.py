@@ -110,7 +110,7 @@ class LongTerms(C.Structure):
 
 class ShortTerms(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
-        "ubond", "uangle", "udih", "uimp", "uintra", "virial_intra")]
+        "ubond", "uangle", "udih", "uimp", "uintra", "virial_intra", "usflj")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

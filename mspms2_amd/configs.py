@@ -8,6 +8,11 @@ bytes on every machine with this numpy.
   lj_fcc          LJ-864 argon on an FCC lattice, rho*=0.8442, rc=2.5 sigma
   water_box       flexible 3-site water (SPC/Fw-like charges/LJ, harmonic
                   O-H and H-O-H), Ewald or Wolf
+  wolf_water_set  seeded configurations of 1024 waters with Wolf sums (the
+                  force-field fitting workload)
+  butane_table    256 n-butane boxes of 64-512 molecules (replica table)
+  tube_loading    LJ methane filling a 50 A smooth tube, 100 x 100 x 4000 A box
+  tube_loadings   its 16 loadings from 20,000 to 100,000 sites
   butane_box      united-atom TraPPE n-butane
   methane_tube    LJ methane inside a smooth cylindrical pore
   explicit_tube   LJ methane and a ghost-site diatomic around an explicit
@@ -203,14 +208,19 @@ def water_box(n_side: int = 16, L: float = 49.66, seed: int = 2,
               elec: str = "ewald", kappa: float = 0.28, kmax: int = 8,
               ksqmax: int = 64, rc: float = 10.0, jitter: float = 0.3,
               nstep_inner: int = 8, delt_fs: float = 2.0, min_dist: float = 1.6,
-              ensemble: int = 0) -> RunDir:
+              ensemble: int = 0, nmol: Optional[int] = None) -> RunDir:
     """Flexible three-site water on a jittered cubic lattice with uniform
-    random orientations (n_side=16: 12,288 sites; Ewald or Wolf)."""
+    random orientations (n_side=16: 12,288 sites; Ewald or Wolf).  With nmol,
+    nmol lattice sites of the n_side^3 lattice are taken at random."""
     rng = np.random.default_rng(seed)
-    nmol = n_side ** 3
     a = L / n_side
     idx = np.array([[i, j, k] for i in range(n_side) for j in range(n_side)
                     for k in range(n_side)], dtype=np.float64)
+    if nmol is not None:
+        if nmol > len(idx):
+            raise ValueError("nmol exceeds the lattice")
+        idx = idx[np.sort(rng.permutation(len(idx))[:nmol])]
+    nmol = len(idx)
     centers = (idx + 0.5) * a
     sp = water_species(nmol)
     tmpl = np.array([[s.xx, s.yy, s.zz] for s in sp.sites])
@@ -608,3 +618,70 @@ def explicit_tube(n: int = 10, nper: int = 9, box_xy: float = 30.0, n_in: int = 
              sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
     return RunDir(run=run, sysname="explicit-tube", species=[ch4, ghost_diatomic_species(n_dimer)],
                   xyz=xyz)
+
+
+# --------------------------------------------------------------------------- #
+def wolf_water_set(nconf: int = 64, nmol: int = 1024, L: float = 31.284, kappa: float = 0.2,
+                   rc: float = 12.0, seed0: int = 400) -> list:
+    """nconf seeded configurations of nmol flexible waters (water_box's
+    model) at the same density as the 12,288-site box, Wolf damping kappa
+    (1/A) with cutoff rc (A): random occupations of the 11^3 lattice."""
+    n_side = int(math.ceil(nmol ** (1.0 / 3.0) - 1e-9))
+    return [water_box(n_side=n_side, L=L, seed=seed0 + k, elec="wolf", kappa=kappa, rc=rc,
+                      nmol=nmol) for k in range(nconf)]
+
+
+def butane_table(nrep: int = 256, seed: int = 2024) -> list:
+    """The batched-replica workload: nrep n-butane boxes of U{64..512}
+    molecules (one seeded draw), box r seeded 1000 + r, at 0.58 g/cm^3 with
+    rc = min(14 A, L/2)."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(64, 513, size=nrep)
+    return [butane_box(nmol=int(k), seed=1000 + r) for r, k in enumerate(n)]
+
+
+# --------------------------------------------------------------------------- #
+def tube_loading(natom: int, seed: int = 0, R: float = 50.0, box_xy: float = 100.0,
+                 length: float = 4000.0, rc: float = 12.0, wall_gap: float = 4.0,
+                 min_dist: float = 3.0, delt_fs: float = 2.0, nstep_inner: int = 4) -> RunDir:
+    """natom LJ methanes (3.73 A, 148 K) inside a smooth tube of radius R on
+    the z axis (iSF_type 3: the tube as a tricubic Hermite grid), periodic
+    box box_xy^2 x length (A), cutoff rc.  Sites are a random choice of the
+    points of a cubic lattice inside r < R - wall_gap, spaced so that about
+    1.2 natom points fit, each jittered by up to (spacing - min_dist) / 2 per
+    axis; vectorized, so that 100,000 sites take a fraction of a second."""
+    rng = np.random.default_rng(seed)
+    rf = R - wall_gap
+    a = (math.pi * rf * rf * length / (1.2 * natom)) ** (1.0 / 3.0)
+    while True:
+        if a < min_dist:
+            raise ValueError("the tube cannot hold that many sites")
+        nxy = int(2 * rf / a) + 1
+        nz = int(length / a)
+        g = np.arange(nxy) * a - (nxy - 1) * a / 2.0
+        X, Y = np.meshgrid(g, g, indexing="ij")
+        keep = X * X + Y * Y < (rf - 0.5 * (a - min_dist)) ** 2
+        if keep.sum() * nz >= natom:
+            break
+        a *= 0.98
+    xy = np.stack([X[keep], Y[keep]], axis=1)
+    zs = (np.arange(nz) + 0.5) * (length / nz)
+    pts = np.concatenate([np.column_stack([xy, np.full(len(xy), z)]) for z in zs])
+    pick = np.sort(rng.permutation(len(pts))[:natom])
+    j = 0.5 * (min(a, length / nz) - min_dist)
+    xyz = pts[pick] + rng.uniform(-j, j, size=(natom, 3))
+    run = RunInput(title=f"methane tube loading {natom}", sigma_base=1.0, epsilon_base=1.0,
+                   mass_base=1.0, treq=300.0, boxlx=box_xy, boxly=box_xy, boxlz=length,
+                   rcuton=rc, rcutoff=rc, rcutoffelec=rc, isLJlrcOn=0, delt=delt_fs,
+                   nstep_inner=nstep_inner, iChargeType=abi.ELEC_NONE, iSF_type=abi.SF_TASOS,
+                   tube_sigma=3.37, tube_epsilon=27.68, tubes=[(0.0, 0.0, R)])
+    sp = SpeciesSpec(name="methane", nmole=natom, sites=[
+        Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.016043, epsilon=148.0,
+             sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
+    return RunDir(run=run, sysname="methane-tube", species=[sp], xyz=xyz)
+
+
+def tube_loadings(n: int = 16, nmin: int = 20000, nmax: int = 100000, seed0: int = 500):
+    """n loadings N_i = nmin + i (nmax - nmin) / (n - 1), i = 0..n-1."""
+    return [tube_loading(int(round(nmin + i * (nmax - nmin) / (n - 1))), seed=seed0 + i)
+            for i in range(n)]

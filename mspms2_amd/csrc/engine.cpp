@@ -117,6 +117,7 @@ struct mspms_ctx {
     int natom = 0, nmol = 0, nsite = 0;
     int charge_union = MSPMS_ELEC_NONE;  // charge type shared by the replicas
     bool any_intra = false, any_bonded = false, any_recip = false, any_sf = false;
+    bool sf_split = false;  // the solid-fluid wall in frcshort / the inner loop
     bool any_vacuum = false, any_coul = false;
     int want_hash = -1;                  // -1: FP64 only
     // launch shape, set by mspms_autotune: sites per binning cell and waves
@@ -146,9 +147,10 @@ struct mspms_ctx {
     DevBuf<int2> d_slot_info;
     DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center, d_atom_x0;
     DevBuf<int> d_lstate;
-    DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext;
+    DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext, d_cl_box;
     DevBuf<float4> d_plist;
     int cap_hint = 0;
+    int cap_limit = 0;  // caller's bound on the row capacity (0: none)
 
     // reciprocal
     RecipBufs rb{};
@@ -166,6 +168,11 @@ struct mspms_ctx {
     DevBuf<unsigned long long> d_fin_partu;
     DevBuf<int> d_fin_count;
     DevBuf<int> d_fin_on;    // gate of the RESPA step's finalize (1: run)
+    DevBuf<double> d_snap;   // (6, natom) start of the last step begun
+                             //   without a pair-list overflow (RESPA runs)
+    DevBuf<int> d_snap_count;  // steps that wrote it since the last sync
+    double* snap_pos = nullptr;  // the buffers of the run it belongs to
+    double* snap_vel = nullptr;
     DevBuf<HistDev> d_hist;  // per-step term histories of a device run
     DevBuf<int> d_hstep;     // [nrep] next history slot
 
@@ -290,6 +297,8 @@ LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     L.max_mol_atoms = 0;
     for (const SpeciesHost& q : c->sp) L.max_mol_atoms = std::max(L.max_mol_atoms, q.natom);
     L.rep_delt = c->cur_rep_delt;
+    L.snap = nullptr;
+    L.snap_count = nullptr;
     L.sites = c->d_sites.p;
     L.mols = c->d_mols.p;
     L.atom_mol = c->d_atom_mol.p;
@@ -551,6 +560,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_slot_off.release();
     c->d_slot_par.release();
     c->d_cl_ext.release();
+    c->d_cl_box.release();
     c->d_plist.release();
     c->d_rows.release();
     c->d_rows_s.release();
@@ -944,6 +954,7 @@ int build(mspms_ctx* c) {
     int ncell = cell0, ncol = col0;
     size_t maxcl = (size_t)(c->natom / MSPMS_CL + ncol + 1);
     int cap = (int)std::min<double>((double)max_rep_cl, std::max(max_est, (double)c->cap_hint));
+    if (c->cap_limit > 0) cap = std::min(cap, c->cap_limit);
     cap = std::max(cap, 16);
     if ((rc = c->d_atom_cell.ensure(c->natom)) || (rc = c->d_atom_slot.ensure(c->natom)) ||
         (rc = c->d_atom_wz.ensure(c->natom)) || (rc = c->d_cell_count.ensure(ncell)) ||
@@ -957,7 +968,7 @@ int build(mspms_ctx* c) {
         (rc = c->d_lstate.ensure(LS_N)) ||
         (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_par.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
-        (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
+        (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_cl_box.ensure(2 * maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
         (rc = c->d_plist_n.ensure(maxcl)))
         return rc;
     HIPC(hipMemsetAsync(c->d_cell_count.p, 0, sizeof(int) * (size_t)ncell, s));
@@ -983,6 +994,7 @@ int build(mspms_ctx* c) {
     b.slot_par = c->d_slot_par.p;
     b.cl_center = c->d_cl_center.p;
     b.cl_ext = c->d_cl_ext.p;
+    b.cl_box = c->d_cl_box.p;
     b.plist = c->d_plist.p;
     b.plist_n = c->d_plist_n.p;
     b.maxcl = (int)maxcl;
@@ -1093,11 +1105,20 @@ int build(mspms_ctx* c) {
         } else {
             const mspms_sf_grid& G = *P.grid;
             size_t nn = (size_t)G.nx * G.ny * G.nz, tot = nn * 8 * (size_t)G.ntype;
-            // the node table is large (up to GiB): upload once per source
+            // the node table is large (up to GiB): uploaded once per source,
+            // through a staging buffer, and turned into node-major records
             if (c->sf_nodes_src != G.nodes || c->sf_nodes_uploaded != tot) {
-                if ((rc = c->d_sf_nodes.ensure(tot))) return rc;
-                HIPC(hipMemcpyAsync(c->d_sf_nodes.p, G.nodes, tot * sizeof(float),
-                                    hipMemcpyHostToDevice, s));
+                DevBuf<float> stage;
+                struct Release {
+                    DevBuf<float>& b;
+                    ~Release() { b.release(); }
+                } release_stage{stage};
+                if ((rc = c->d_sf_nodes.ensure(tot)) || (rc = stage.ensure(tot))) return rc;
+                HIPC(hipMemcpyAsync(stage.p, G.nodes, tot * sizeof(float), hipMemcpyHostToDevice,
+                                    s));
+                launch_grid_node_major(s, stage.p, c->d_sf_nodes.p, nn, G.ntype);
+                HIPC(hipGetLastError());
+                HIPC(hipStreamSynchronize(s));
                 c->sf_nodes_src = G.nodes;
                 c->sf_nodes_uploaded = tot;
             }
@@ -1134,8 +1155,11 @@ int build(mspms_ctx* c) {
         (rc = c->d_fin_part.ensure((size_t)c->nrep * FIN_BLOCKS * FIN_ND)) ||
         (rc = c->d_fin_partu.ensure((size_t)c->nrep * FIN_BLOCKS * 4)) ||
         (rc = c->d_fin_count.ensure((size_t)c->nrep)) || (rc = c->d_fin_on.ensure(1)) ||
-        (rc = c->d_hist.ensure(1)) || (rc = c->d_hstep.ensure((size_t)c->nrep)))
+        (rc = c->d_hist.ensure(1)) || (rc = c->d_hstep.ensure((size_t)c->nrep)) ||
+        (rc = c->d_snap.ensure((size_t)6 * c->natom)) || (rc = c->d_snap_count.ensure(1)))
         return rc;
+    HIPC(hipMemsetAsync(c->d_snap_count.p, 0, sizeof(int), s));
+    c->snap_pos = c->snap_vel = nullptr;
     HIPC(hipMemsetAsync(c->d_fin_count.p, 0, sizeof(int) * (size_t)c->nrep, s));
     HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s));
     HIPC(hipMemsetAsync(c->d_hist.p, 0, sizeof(HistDev), s));
@@ -1305,9 +1329,13 @@ void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
                      bool gated = false, const FinAtoms* atoms = nullptr) {
     FinBufs fb = c->fin;
     fb.on = gated ? c->d_fin_on.p : nullptr;
+    // with the wall split into the inner loop, a RESPA step (sterms) still
+    // reports its energy, which the last inner step left in ep_sf; a
+    // standalone frclong leaves it out
+    const bool have_sf = c->any_sf && (!c->sf_split || sterms);
     launch_finalize_long(L, c->cb.col_cl0, c->d_ep_pair.p, c->d_ip_pair.p, 1, c->d_ep_intra.p,
                          c->any_intra ? 1 : 0, c->rb.uk, c->any_recip ? 1 : 0, c->d_ep_sf.p,
-                         c->any_sf ? 1 : 0, c->any_vacuum ? c->d_uvac.p : nullptr,
+                         have_sf ? 1 : 0, c->any_vacuum ? c->d_uvac.p : nullptr,
                          c->any_coul ? c->d_ep_coul.p : nullptr, terms, c->cb.lstate,
                          sterms ? c->d_ep_short.p : nullptr, sterms, sterms ? c->d_kin.p : nullptr,
                          sterms ? ukin : nullptr, fb, atoms);
@@ -1389,7 +1417,8 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     }
     if (c->any_recip) HIPC(hipStreamWaitEvent(s, c->ev_join, 0));  // join
     // nothing adds long forces after k_intra / k_add_frec
-    const bool fuse = kick && !grad && !c->any_vacuum && !c->any_coul && !c->any_sf &&
+    const bool sf_long = c->any_sf && !c->sf_split;
+    const bool fuse = kick && !grad && !c->any_vacuum && !c->any_coul && !sf_long &&
                       (c->any_intra || recip_side);
     if (kicked) *kicked = fuse;
     const bool defer = recip_side && frec_deferred;
@@ -1418,7 +1447,7 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         ProfScope p(c, s, PROF_INTRA);
         launch_coulomb_mol(L, c->d_species.p, x, y, z, c->d_mcom.p, fx, fy, fz, c->d_ep_coul.p);
     }
-    if (c->any_sf) {
+    if (sf_long) {
         ProfScope p(c, s, PROF_SF);
         launch_sf(L, c->sf, x, y, z, fx, fy, fz, c->d_ep_sf.p);
     }
@@ -1435,7 +1464,9 @@ int enqueue_short(mspms_ctx* c, const double* x, const double* y, const double* 
     LaunchCtx L = launch_ctx(c, s);
     ProfScope p(c, s, PROF_BONDED);
     launch_bonded(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_short.p);
-    launch_finalize_short(L, c->d_ep_short.p, terms);
+    const bool sf_short = c->any_sf && c->sf_split;
+    if (sf_short) launch_sf(L, c->sf, x, y, z, fx, fy, fz, c->d_ep_sf.p);
+    launch_finalize_short(L, c->d_ep_short.p, terms, sf_short ? c->d_ep_sf.p : nullptr);
     HIPC(hipGetLastError());
     return 0;
 }
@@ -1449,11 +1480,19 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
     // src/initvars.c:605-607
     const double deltby2 = delt / 2.0, delts = delt / nstep_inner, deltsby2 = delts / 2.0;
     LaunchCtx L = launch_ctx(c, s);
+    // each step saves its start while no overflow is flagged (mspms_sync
+    // puts it back after one)
+    L.snap = c->d_snap.p;
+    L.snap_count = c->d_snap_count.p;
+    c->snap_pos = pos;
+    c->snap_vel = vel;
     {
         ProfScope p(c, s, PROF_BONDED);
+        const bool sf_inner = c->any_sf && c->sf_split;
         launch_respa_inner(L, c->d_species.p, c->d_terms.p, pos, pos + n, pos + 2 * n, vel,
                            vel + n, vel + 2 * n, fl, fl + n, fl + 2 * n, fs, fs + n, fs + 2 * n,
-                           deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p);
+                           deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p,
+                           sf_inner ? &c->sf : nullptr, c->d_ep_sf.p);
     }
     const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2, L.rep_delt, L.atom_mol, L.mols};
     bool kicked = false, deferred = false;
@@ -1742,6 +1781,8 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
         HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 0, 1, s));
         reopen.armed = true;
     }
+    // the overflow snapshot counts this run's steps
+    HIPC(hipMemsetAsync(c->d_snap_count.p, 0, sizeof(int), s));
     if (hist && nsteps > 0) {
         // every step's finalize appends its terms to slot t of the histories
         LaunchCtx L = launch_ctx(c, s);
@@ -2027,6 +2068,8 @@ extern "C" int mspms_siman_run_device(mspms_ctx* c, double* pos, double* vel, do
                                       double* t_final, mspms_long_terms* lterms,
                                       mspms_short_terms* sterms, double* ukin, void* stream) {
     if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (c->any_sf && c->sf_split)
+        return fail(MSPMS_EINVAL, "the solid-fluid RESPA split is for the NVE step and HMC");
     if (!c->hmc_ready) return fail(MSPMS_EINVAL, "mspms_hmc_init first (velocity streams)");
     if (!c->nvt_ready) return fail(MSPMS_EINVAL, "mspms_nvt_set first");
     if (!pos || !vel || !fl || !fs || !t_initial || !t_min || !mu_t || !lterms || !sterms ||
@@ -2099,6 +2142,8 @@ extern "C" int mspms_nvt_run_device(mspms_ctx* c, double* pos, double* vel, doub
                                     mspms_long_terms* lterms, mspms_short_terms* sterms,
                                     double* ukin, int use_graph, void* stream) {
     if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (c->any_sf && c->sf_split)
+        return fail(MSPMS_EINVAL, "the solid-fluid RESPA split is for the NVE step and HMC");
     if (!c->nvt_ready) return fail(MSPMS_EINVAL, "mspms_nvt_set first");
     if (nstep_inner < 1 || nsteps < 0) return fail(MSPMS_EINVAL, "bad step counts");
     if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
@@ -2262,6 +2307,8 @@ extern "C" int mspms_npt_run_device(mspms_ctx* c, double* pos, double* vel, doub
                                     mspms_long_terms* lterms, mspms_short_terms* sterms,
                                     double* ukin, int use_graph, void* stream) {
     if (!c || c->sp.empty()) return fail(MSPMS_EINVAL, "context has no system");
+    if (c->any_sf && c->sf_split)
+        return fail(MSPMS_EINVAL, "the solid-fluid RESPA split is for the NVE step and HMC");
     if (!c->npt_ready) return fail(MSPMS_EINVAL, "mspms_npt_set first");
     if (nstep_inner < 1 || nsteps < 0) return fail(MSPMS_EINVAL, "bad step counts");
     if (!pos || !vel || !fl || !fs || !lterms || !sterms || !ukin)
@@ -2425,6 +2472,20 @@ extern "C" int mspms_set_launch_shape(mspms_ctx* c, double cell_atoms, int waves
     return c->sp.empty() ? 0 : build(c);
 }
 
+extern "C" int mspms_set_list_capacity(mspms_ctx* c, int max_entries) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    if (max_entries < 0) return fail(MSPMS_EINVAL, "negative capacity");
+    c->cap_limit = max_entries;
+    return c->sp.empty() ? 0 : build(c);
+}
+
+extern "C" int mspms_set_sf_split(mspms_ctx* c, int on) {
+    if (!c) return fail(MSPMS_EINVAL, "null ctx");
+    if (c->sf_split != (on != 0)) drop_graph(c);  // captured steps change
+    c->sf_split = on != 0;
+    return 0;
+}
+
 extern "C" int mspms_set_list_skin(mspms_ctx* c, double skin) {
     if (!c) return fail(MSPMS_EINVAL, "null ctx");
     if (!(skin >= 0.0 && skin <= 8.0)) return fail(MSPMS_EINVAL, "skin must be within 0..8");
@@ -2481,13 +2542,36 @@ extern "C" int mspms_sync(mspms_ctx* c) {
     accumulate_profile(c);
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, c->stream, c->d_status.p);
     HIPC(hipStreamSynchronize(c->stream));
+    int steps_ok = 0;
+    HIPC(hipMemcpy(&steps_ok, c->d_snap_count.p, sizeof(int), hipMemcpyDeviceToHost));
+    HIPC(hipMemsetAsync(c->d_snap_count.p, 0, sizeof(int), c->stream));
     if (c->h_status->flags & ST_OVERFLOW) {
-        // the evaluations since the last sync dropped pairs; size the lists
-        // for the largest row seen so that a repeat of them fits
+        // the evaluations since the last sync dropped pairs.  A RESPA run
+        // gets back the positions and velocities it had at the start of the
+        // first step that overflowed (the end of the last good step); then
+        // the lists are sized for the largest row seen, so that a repeat of
+        // the steps fits
+        bool restored = false;
+        if (c->snap_pos && c->snap_vel && steps_ok > 0) {
+            const size_t n3 = (size_t)3 * c->natom;
+            HIPC(hipMemcpy(c->snap_pos, c->d_snap.p, n3 * sizeof(double),
+                           hipMemcpyDeviceToDevice));
+            HIPC(hipMemcpy(c->snap_vel, c->d_snap.p + n3, n3 * sizeof(double),
+                           hipMemcpyDeviceToDevice));
+            restored = true;
+        }
+        c->snap_pos = c->snap_vel = nullptr;
         c->cap_hint = std::max(c->cap_hint * 2, c->h_status->max_list + 64);
+        c->cap_limit = 0;
         drop_graph(c);
         if ((rc = build(c))) return rc;
-        return fail(MSPMS_ENOMEM, "pair list overflow (lists regrown; repeat the steps)");
+        if (restored)
+            return fail(MSPMS_ENOMEM,
+                        "pair list overflow: positions and velocities restored to the start of "
+                        "the RESPA step that overflowed (step %d since the last sync or run "
+                        "start); forces and terms are stale; lists regrown",
+                        steps_ok);
+        return fail(MSPMS_ENOMEM, "pair list overflow (lists regrown; repeat the calls)");
     }
     if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
     if (c->h_status->flags & ST_GRID_EDOM)

@@ -98,6 +98,10 @@ struct ClusterBufs {
                          //   molecule id and flags as int bits (SLOT_*)
     double4* cl_center;  // [maxcl]
     float4* cl_ext;      // [maxcl]
+    float4* cl_box;      // [2 maxcl] the cluster's current box relative to its
+                         //   build-time centre: (mid, -) then (half extent, -);
+                         //   written by the build and refreshed by
+                         //   k_list_check on every reuse
     float4* plist;       // [maxcl*cap] entries: FP32 centre separation
                          //   Ci - (Cj + image shift) (Ci - Cj for an
                          //   ambiguous image) and the entry bits in .w
@@ -168,6 +172,10 @@ struct LaunchCtx {
     const int* atom_site;
     const RepDev* reps;
     Status* status;
+    double* snap;             // null, or (6, natom): positions and velocities
+                              //   at the start of the last RESPA step that
+                              //   began without a pair-list overflow
+    int* snap_count;          // steps that wrote the snapshot
     const double* rep_delt;   // null, or each replica's own outer time step
                               //   (HMC with delt adaptation): integrator
                               //   kernels then derive deltby2, delts and
@@ -201,7 +209,8 @@ struct SFDev {
     int nx, ny, nz, pad;     // Hermite grid (Angstrom, K)
     double x0, y0, z0, hx, hy, hz;
     double sigma_base, epsilon_base;
-    const float* nodes;      // [ntype][8][nz][ny][nx]
+    const float* nodes;      // [ntype][nz][ny][nx][8]: a node's F, Fx, Fy, Fxy,
+                             //   Fz, Fxz, Fyz, Fxyz (derivative-bit order)
     const double* min_pe;    // [ntype] min node F (mylibtasos/grid.f90:102)
     // SF_MY_INTERP: nx, ny, nz = cubes per axis, x0, y0 = xmin, ymin
     double zmax;
@@ -278,6 +287,9 @@ void launch_cube_major(hipStream_t s, const double* src, double* dst, int ntype,
 void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x,
                const double* y, const double* z, double* fx, double* fy,
                double* fz, double* usf);
+// the Hermite grid's C-ABI layout [type][field][node] into the device's
+// node-major [type][node][8] records (derivative-bit order)
+void launch_grid_node_major(hipStream_t s, const float* src, float* dst, size_t nn, int ntype);
 
 // Buffered (Verlet) cluster list: the list is built with rc_list = cutoff +
 // skin and reused while every atom stays within skin/2 of its position at
@@ -368,14 +380,17 @@ void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* 
                    double* uvac, double* gq);
 // sets a device run's history descriptor and clears the per-replica slots
 void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hstep);
-void launch_finalize_short(const LaunchCtx& c, const double* epart,
-                           void* terms);
+void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms,
+                           const double* ep_sf = nullptr);
 void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species,
                         const TermDev* terms, double* x, double* y, double* z,
                         double* vx, double* vy, double* vz, const double* fxl,
                         const double* fyl, const double* fzl, double* fxs,
                         double* fys, double* fzs, double deltby2, double delts,
-                        double deltsby2, int nstep_inner, double* epart);
+                        double deltsby2, int nstep_inner, double* epart,
+                        const SFDev* sf = nullptr, double* usf = nullptr);
+// (sf: the solid-fluid wall added to the short-range forces in every inner
+// step, the last step's per-site energies in usf)
 // closing half kick; kin[i] = v.v / r_mass per atom (the step's finalize
 // kernel reduces it into ukin)
 void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,

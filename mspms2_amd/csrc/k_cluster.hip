@@ -42,8 +42,8 @@ __global__ void __launch_bounds__(256) k_list_check(
     double* __restrict__ fz, const double4* __restrict__ atom_x0,
     const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ slot_orig,
     const double4* __restrict__ slot_w64, const double4* __restrict__ cl_center,
-    double4* __restrict__ slot_x64, float4* __restrict__ slot_off, int* lstate,
-    double half_skin2, int force, int half) {
+    double4* __restrict__ slot_x64, float4* __restrict__ slot_off, float4* __restrict__ cl_box,
+    int* lstate, double half_skin2, int force, int half) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = lstate[LS_VALID] != 0 && !force && lstate[LS_LAYOUT] == half;
     bool need = !valid;
@@ -63,7 +63,11 @@ __global__ void __launch_bounds__(256) k_list_check(
             need = !(dx * dx + dy * dy + dz * dz <= half_skin2);  // NaN: rebuild
         }
     }
-    if (valid && t < nslot && t < MSPMS_CL * min(*ncl_total, maxcl)) {
+    // the cluster's current box over its valid slots (8 adjacent lanes)
+    const float BIG = 3.0e38f;
+    float lo[3] = {BIG, BIG, BIG}, hi[3] = {-BIG, -BIG, -BIG};
+    const bool live_cl = valid && t < nslot && t < MSPMS_CL * min(*ncl_total, maxcl);
+    if (live_cl) {
         const int o = slot_orig[t];
         if (o >= 0) {
             const double4 p0 = atom_x0[o];
@@ -71,9 +75,32 @@ __global__ void __launch_bounds__(256) k_list_check(
             const double4 w = slot_w64[t];
             const double4 C = cl_center[t / MSPMS_CL];
             slot_x64[t] = make_double4(xi, yi, zi, 0.0);
-            slot_off[t] = make_float4((float)((w.x - C.x) + (xi - p0.x)),
-                                      (float)((w.y - C.y) + (yi - p0.y)),
-                                      (float)((w.z - C.z) + (zi - p0.z)), (float)w.w);
+            const float4 off = make_float4((float)((w.x - C.x) + (xi - p0.x)),
+                                           (float)((w.y - C.y) + (yi - p0.y)),
+                                           (float)((w.z - C.z) + (zi - p0.z)), (float)w.w);
+            slot_off[t] = off;
+            lo[0] = hi[0] = off.x;
+            lo[1] = hi[1] = off.y;
+            lo[2] = hi[2] = off.z;
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < MSPMS_CL; m <<= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fminf(lo[d], __shfl_xor(lo[d], m, 64));
+            hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], m, 64));
+        }
+    if (live_cl && (t & (MSPMS_CL - 1)) == 0) {
+        const int c = t / MSPMS_CL;
+        if (lo[0] <= hi[0]) {
+            cl_box[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]),
+                                        0.5f * (lo[2] + hi[2]), 0.f);
+            cl_box[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]),
+                                            0.5f * (hi[2] - lo[2]), 0.f);
+        } else {  // no valid slot: a point box
+            cl_box[2 * c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            cl_box[2 * c + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
     if (__ballot(need) && (threadIdx.x & 63) == 0) atomicOr(&lstate[LS_NEED], 1);
@@ -267,7 +294,7 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
                        int2* __restrict__ slot_info, const double4* __restrict__ slot_w64,
                        float4* __restrict__ slot_off, float4* __restrict__ slot_par,
                        double4* __restrict__ cl_center,
-                       float4* __restrict__ cl_ext, int* lstate) {
+                       float4* __restrict__ cl_ext, float4* __restrict__ cl_box, int* lstate) {
     if (!lstate[LS_NEED]) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int c = t >> 3, k = t & 7;
@@ -305,9 +332,12 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
     double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
     if (k == 0) {
         cl_center[c] = make_double4(cx, cy, cz, (double)col_rep[col]);
-        cl_ext[c] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.000001f + 1e-6f,
-                                (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
-                                (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
+        const float4 ext = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.000001f + 1e-6f,
+                                       (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
+                                       (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
+        cl_ext[c] = ext;
+        cl_box[2 * c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        cl_box[2 * c + 1] = ext;
     }
     if (valid) {
         int s = slot_info[slot].y;
@@ -479,7 +509,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     hipLaunchKernelGGL(k_list_check, dim3((std::max(c.natom, nslot) + T - 1) / T), dim3(T), 0,
                        c.stream, c.natom, nslot, x, y, z, c.atom_site, c.sites, fx, fy, fz,
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
-                       b.slot_x64, b.slot_off, b.lstate, half_skin2, force, b.half);
+                       b.slot_x64, b.slot_off, b.cl_box, b.lstate, half_skin2, force, b.half);
     hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
                        b.cell_count, fx, fy, fz, c.status, b.lstate);
@@ -496,7 +526,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,
                        b.slot_info, b.slot_w64, b.slot_off, b.slot_par, b.cl_center, b.cl_ext,
-                       b.lstate);
+                       b.cl_box, b.lstate);
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,

@@ -184,7 +184,8 @@ __global__ void k_hmc_mark(int nrep, HmcRep* __restrict__ hmc,
     const int rep = blockIdx.x * blockDim.x + threadIdx.x;
     if (rep >= nrep) return;
     HmcRep& H = hmc[rep];
-    H.uinter_old = lt[rep].uinter;
+    // with the wall split into frcshort, its energy belongs to uinter
+    H.uinter_old = lt[rep].uinter + st[rep].usflj;
     H.uintra_old = st[rep].uintra;
     H.ushift_old = lt[rep].ushift;
     H.ukin_old = H.ukin;

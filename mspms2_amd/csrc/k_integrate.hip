@@ -14,9 +14,59 @@
 //                   kinetic energy partial (src/md.c:67-80)
 // Per-atom arithmetic follows md.c expression for expression.
 #include "engine.h"
+#include "sf_eval.h"
 #include "terms.h"
 
 namespace {
+
+// The solid-fluid wall in the inner loop (the RESPA split): each inner step
+// adds the wall's force on a GHOST_NONE site to its short-range force; the
+// last inner step's energies go to usf[atom] (0 for other sites).
+struct SfInner {
+    SFDev sf;
+    const RepDev* reps;
+    double* usf;
+    int on;
+};
+
+__device__ inline double sf_inner_site(const SfInner& W, const SiteDev& S, int rep, double x,
+                                       double y, double z, double& fx, double& fy, double& fz,
+                                       Status* st) {
+    if (S.ghost != MSPMS_GHOST_NONE) return 0.0;
+    int edom = 0;
+    const double u = sfeval::sf_site(W.sf, W.reps[rep], S, x, y, z, fx, fy, fz, &edom);
+    if (edom) atomicOr(&st->flags, ST_GRID_EDOM);
+    return u;
+}
+
+// The start-of-step snapshot: while no pair-list overflow has been
+// flagged, each step saves its starting positions and velocities, so that
+// after an overflow the state at the end of the last good step can be put
+// back.
+struct SnapOut {
+    double* p;   // (6, natom) or null
+    int* count;
+    int natom;
+};
+
+__device__ inline bool snap_wanted(const SnapOut& SN, const Status* st) {
+    return SN.p && !(st->flags & ST_OVERFLOW);
+}
+
+__device__ inline void snap_atom(const SnapOut& SN, int i, double x, double y, double z,
+                                 double vx, double vy, double vz) {
+    const size_t n = (size_t)SN.natom;
+    SN.p[i] = x;
+    SN.p[n + i] = y;
+    SN.p[2 * n + i] = z;
+    SN.p[3 * n + i] = vx;
+    SN.p[4 * n + i] = vy;
+    SN.p[5 * n + i] = vz;
+}
+
+__device__ inline void snap_count(const SnapOut& SN) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(SN.count, 1);
+}
 
 // a replica's own outer step (HMC with delt adaptation): the host's
 // derivation, src/initvars.c:605-607
@@ -36,7 +86,7 @@ __global__ void __launch_bounds__(256) k_respa_inner(
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st, const double* __restrict__ rep_delt) {
+    Status* st, const double* __restrict__ rep_delt, SfInner W, SnapOut SN) {
 #pragma clang fp contract(off)
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
@@ -44,6 +94,10 @@ __global__ void __launch_bounds__(256) k_respa_inner(
     const SpeciesDev S = species[M.species];
     rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     const int a0 = M.atom0, a1 = M.atom0 + M.natom;
+    if (snap_wanted(SN, st)) {
+        for (int i = a0; i < a1; i++) snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
+        if (m == 0) snap_count(SN);
+    }
     for (int i = a0; i < a1; i++) {
         double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
         vx[i] += (deltby2 * fxl[i] * r_mass);
@@ -73,6 +127,16 @@ __global__ void __launch_bounds__(256) k_respa_inner(
             e[cls] += u;
             vir += v;
         }
+        if (W.on)
+            for (int i = a0; i < a1; i++) {
+                double gx = fxs[i], gy = fys[i], gz = fzs[i];
+                const double u = sf_inner_site(W, sites[S.site0 + (i - a0)], M.rep, x[i], y[i],
+                                               z[i], gx, gy, gz, st);
+                fxs[i] = gx;
+                fys[i] = gy;
+                fzs[i] = gz;
+                if (ll == nstep_inner - 1) W.usf[i] = u;
+            }
         for (int i = a0; i < a1; i++) {
             double r_mass = 1.0 / sites[S.site0 + (i - a0)].aw;
             vx[i] += (deltsby2 * fxs[i] * r_mass);
@@ -100,7 +164,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st, const double* __restrict__ rep_delt) {
+    Status* st, const double* __restrict__ rep_delt, SfInner W, SnapOut SN) {
 #pragma clang fp contract(off)
     __shared__ double lx[NA * 64], ly[NA * 64], lz[NA * 64], lvx[NA * 64], lvy[NA * 64],
         lvz[NA * 64], lfx[NA * 64], lfy[NA * 64], lfz[NA * 64], lrm[NA * 64];
@@ -111,6 +175,13 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
     const SpeciesDev S = species[M.species];
     rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     const int na = M.natom, a0 = M.atom0;
+    if (snap_wanted(SN, st)) {
+        for (int k = 0; k < na; k++) {
+            const int i = a0 + k;
+            snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
+        }
+        if (m == 0) snap_count(SN);
+    }
     for (int k = 0; k < na; k++) {
         const int i = a0 + k, s = k * 64 + l;
         const double r_mass = 1.0 / sites[S.site0 + k].aw;
@@ -148,6 +219,17 @@ __global__ void __launch_bounds__(64) k_respa_inner_lds(
             e[cls] += u;
             vir += v;
         }
+        if (W.on)
+            for (int k = 0; k < na; k++) {
+                const int s = k * 64 + l;
+                double gx = lfx[s], gy = lfy[s], gz = lfz[s];
+                const double u = sf_inner_site(W, sites[S.site0 + k], M.rep, lx[s], ly[s], lz[s],
+                                               gx, gy, gz, st);
+                lfx[s] = gx;
+                lfy[s] = gy;
+                lfz[s] = gz;
+                if (ll == nstep_inner - 1) W.usf[a0 + k] = u;
+            }
         for (int k = 0; k < na; k++) {
             const int s = k * 64 + l;
             const double r_mass = lrm[s];
@@ -191,7 +273,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
     const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
     double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
-    Status* st, const double* __restrict__ rep_delt) {
+    Status* st, const double* __restrict__ rep_delt, SfInner W, SnapOut SN) {
 #pragma clang fp contract(off)
     static_assert(NA <= TPM && 64 % TPM == 0, "one site per lane");
     constexpr int MPW = 64 / TPM;
@@ -200,7 +282,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     const int l = threadIdx.x, mi = l / TPM, j = l % TPM;
     const int m = blockIdx.x * MPW + mi;
     const bool live = m < nmol;
-    int na = 0, a0 = 0, site0 = 0, bt0 = 0, nbt = 0;
+    int na = 0, a0 = 0, site0 = 0, bt0 = 0, nbt = 0, rep = 0;
     if (live) {
         const MolDev M = mols[m];
         const SpeciesDev S = species[M.species];
@@ -209,11 +291,18 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
         site0 = S.site0;
         bt0 = S.bterm0;
         nbt = S.nbterm;
+        rep = M.rep;
         rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     }
+    double usf_last = 0.0;  // the wall's energy of this lane's site (last step)
     const bool own = live && j < na;  // this lane owns site j of its molecule
     const int sl = j * MPW + mi;      // its LDS slot
     double r_mass = 0.0, vxo = 0.0, vyo = 0.0, vzo = 0.0, fxo = 0.0, fyo = 0.0, fzo = 0.0;
+    if (own && snap_wanted(SN, st)) {
+        const int i = a0 + j;
+        snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
+    }
+    if (SN.p && m == 0 && j == 0 && snap_wanted(SN, st)) snap_count(SN);
     if (own) {
         const int i = a0 + j;
         r_mass = 1.0 / sites[site0 + j].aw;
@@ -268,6 +357,9 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
             fxo = gx;
             fyo = gy;
             fzo = gz;
+            if (W.on)
+                usf_last = sf_inner_site(W, sites[site0 + j], rep, lx[sl], ly[sl], lz[sl], fxo, fyo,
+                                         fzo, st);
             vxo += (deltsby2 * fxo * r_mass);
             vyo += (deltsby2 * fyo * r_mass);
             vzo += (deltsby2 * fzo * r_mass);
@@ -284,6 +376,7 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
         fxs[i] = fxo;
         fys[i] = fyo;
         fzs[i] = fzo;
+        if (W.on) W.usf[i] = usf_last;
     }
     // the last inner step's energies, summed over the molecule's lanes
     double ev[5] = {e[0], e[1], e[2], e[3], vir};
@@ -327,17 +420,30 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
                         double* x, double* y, double* z, double* vx, double* vy, double* vz,
                         const double* fxl, const double* fyl, const double* fzl, double* fxs,
                         double* fys, double* fzs, double deltby2, double delts,
-                        double deltsby2, int nstep_inner, double* epart) {
+                        double deltsby2, int nstep_inner, double* epart, const SFDev* sf,
+                        double* usf) {
+    SnapOut SN{c.snap, c.snap_count, c.natom};
+    SfInner W{};
+    if (sf) {
+        W.sf = *sf;
+        W.reps = c.reps;
+        W.usf = usf;
+        W.on = 1;
+    }
 #define RESPA_LDS(NA)                                                                        \
     hipLaunchKernelGGL(k_respa_inner_lds<NA>, dim3((c.nmol + 63) / 64), dim3(64), 0, c.stream,   \
                        c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl,   \
                        fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart,        \
-                       c.status, c.rep_delt)
-    if (c.max_mol_atoms <= 4) {
-        hipLaunchKernelGGL((k_respa_inner_tpm<4, 4>), dim3((c.nmol + 15) / 16), dim3(64), 0,
-                           c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz,
-                           fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner,
-                           epart, c.status, c.rep_delt);
+                       c.status, c.rep_delt, W, SN)
+#define RESPA_TPM(NA, TPM)                                                                   \
+    hipLaunchKernelGGL((k_respa_inner_tpm<NA, TPM>), dim3((c.nmol + 64 / TPM - 1) / (64 / TPM)), \
+                       dim3(64), 0, c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z,  \
+                       vx, vy, vz, fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2,       \
+                       nstep_inner, epart, c.status, c.rep_delt, W, SN)
+    if (c.max_mol_atoms <= 1) {
+        RESPA_TPM(1, 1);  // one lane per single-site molecule
+    } else if (c.max_mol_atoms <= 4) {
+        RESPA_TPM(4, 4);
     } else if (c.max_mol_atoms <= 8) {
         RESPA_LDS(8);
     } else if (c.max_mol_atoms <= 16) {
@@ -347,9 +453,10 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
         hipLaunchKernelGGL(k_respa_inner, dim3(g), dim3(256), 0, c.stream, c.nmol, c.mols,
                            species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl, fxs, fys,
                            fzs, deltby2, delts, deltsby2, nstep_inner, epart, c.status,
-                           c.rep_delt);
+                           c.rep_delt, W, SN);
     }
 #undef RESPA_LDS
+#undef RESPA_TPM
 }
 
 void launch_respa_final(const LaunchCtx& c, double* vx, double* vy, double* vz,

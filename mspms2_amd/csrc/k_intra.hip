@@ -500,6 +500,7 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
         u.uimp = d[14];
         u.uintra = u.ubond + u.uangle + u.udih + u.uimp;  // src/frcshort.c:87
         u.virial_intra = d[15];
+        u.usflj = 0.0;  // a step reports the wall in its long terms
         sout[r] = u;
         if (ukin) ukin[r] = d[ND - 1] * 0.5;
         // this step's terms appended to the run's histories
@@ -526,13 +527,16 @@ __global__ void k_set_hist(HistDev* d, HistDev v, int* hstep, int nrep) {
 
 __global__ void __launch_bounds__(1024) k_finalize_short(const RepDev* __restrict__ reps,
                                                         const double* __restrict__ ep,
+                                                        const double* __restrict__ ep_sf,
                                                         mspms_short_terms* __restrict__ out) {
-    __shared__ double sh[5 * 16];
+    __shared__ double sh[6 * 16];
     const RepDev& R = reps[blockIdx.x];
-    double p[5] = {0, 0, 0, 0, 0};
+    double p[6] = {0, 0, 0, 0, 0, 0};
     for (int m = R.mol0 + threadIdx.x; m < R.mol0 + R.nmol; m += blockDim.x)
         for (int k = 0; k < 5; k++) p[k] += ep[(size_t)m * 5 + k];
-    block_sum<5>(p, sh);
+    if (ep_sf)
+        for (int a = R.atom0 + threadIdx.x; a < R.atom0 + R.natom; a += blockDim.x) p[5] += ep_sf[a];
+    block_sum<6>(p, sh);
     if (threadIdx.x != 0) return;
     mspms_short_terms t;
     t.ubond = p[0];
@@ -541,6 +545,7 @@ __global__ void __launch_bounds__(1024) k_finalize_short(const RepDev* __restric
     t.uimp = p[3];
     t.uintra = t.ubond + t.uangle + t.udih + t.uimp;  // src/frcshort.c:87
     t.virial_intra = p[4];
+    t.usflj = p[5];
     out[blockIdx.x] = t;
 }
 
@@ -879,7 +884,8 @@ void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hste
                        hstep, c.nrep);
 }
 
-void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms) {
+void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms,
+                           const double* ep_sf) {
     hipLaunchKernelGGL(k_finalize_short, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, epart,
-                       (mspms_short_terms*)terms);
+                       ep_sf, (mspms_short_terms*)terms);
 }

@@ -257,11 +257,11 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
 template <int ELEC, int WPC, bool HALF, bool HASH>
 __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
-    const double4* __restrict__ cl_center, const float4* __restrict__ plist,
-    const int* __restrict__ plist_n, int cap, const int* __restrict__ slot_orig,
-    const float4* __restrict__ slot_off, const float4* __restrict__ slot_par,
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    double* __restrict__ epart, unsigned long long* __restrict__ ipart) {
+    const double4* __restrict__ cl_center, const float4* __restrict__ cl_box,
+    const float4* __restrict__ plist, const int* __restrict__ plist_n, int cap,
+    const int* __restrict__ slot_orig, const float4* __restrict__ slot_off,
+    const float4* __restrict__ slot_par, double* __restrict__ fx, double* __restrict__ fy,
+    double* __restrict__ fz, double* __restrict__ epart, unsigned long long* __restrict__ ipart) {
     constexpr bool want_hash = HASH;
     __shared__ double sh_f[WPC][MSPMS_CL][3];
     __shared__ double sh_e[WPC][4];
@@ -294,6 +294,13 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     // wait for the prefetched loads
     const bool sw_on = R.switch_on != 0;
     const int atom0 = R.atom0;
+    // pruning of the row: an entry whose current cluster boxes are farther
+    // apart than the cutoff (a margin of 1e-3 of it above FP32 rounding)
+    // holds no pair in range and is skipped; the boxes are the clusters'
+    // current extents about their build-time centres
+    const float rprune = (float)fmax(R.rc_lj, R.rc_el) * 1.001f;
+    const float rprune2 = rprune * rprune;
+    const float4 bim = cl_box[2 * ci], bih = cl_box[2 * ci + 1];
 
     // FP64 accumulators, fed from FP32 partials that cover at most FLUSH row
     // entries of this wave (FLUSH pair terms per lane)
@@ -325,9 +332,32 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const int nmine = nrow > w ? (nrow - w + WPC - 1) / WPC : 0;
     int it = 0;
     for (int base = 0; base < nmine; base += 64) {
-        const int cnt = min(64, nmine - base);
         const float4 my_ent = base + lane < nmine ? row[w + (base + lane) * WPC]
                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+        bool keep = base + lane < nmine;
+        {
+            const uint32_t ent = __float_as_uint(my_ent.w);
+            const int cj = (int)(ent >> 7);
+            if (keep && cj != ci && !(ent & 64u)) {
+                const float4 bjm = cl_box[2 * cj], bjh = cl_box[2 * cj + 1];
+                const float gx = fmaxf(fabsf(my_ent.x + (bim.x - bjm.x)) - (bih.x + bjh.x), 0.f);
+                const float gy = fmaxf(fabsf(my_ent.y + (bim.y - bjm.y)) - (bih.y + bjh.y), 0.f);
+                const float gz = fmaxf(fabsf(my_ent.z + (bim.z - bjm.z)) - (bih.z + bjh.z), 0.f);
+                keep = gx * gx + gy * gy + gz * gz < rprune2;
+            }
+        }
+        // the kept entries of this chunk, visited in row order
+        uint64_t kept = __ballot(keep);
+        const int cnt = __popcll(kept);
+        if (cnt == 0) continue;
+        int last = 0;
+        auto next_lane = [&]() {
+            if (kept) {
+                last = __builtin_ctzll(kept);
+                kept &= kept - 1;
+            }
+            return last;
+        };
         auto entry_at = [&](int k) {
             return make_float4(
                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ent.x), k)),
@@ -339,8 +369,8 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             float4 sep, pj, qj;
             int oj;
         };
-        auto fetch = [&](int k, JSet& J) {
-            J.sep = entry_at(min(k, cnt - 1));
+        auto fetch = [&](JSet& J) {
+            J.sep = entry_at(next_lane());
             const int cn = (int)(__float_as_uint(J.sep.w) >> 7);
             J.pj = slot_off[cn * MSPMS_CL + jsub];
             J.qj = slot_par[cn * MSPMS_CL + jsub];
@@ -465,12 +495,12 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             }
         };
         JSet A, B;
-        fetch(0, A);
+        fetch(A);
         for (int tt = 0; tt < cnt; tt += 2) {
-            fetch(tt + 1, B);
+            fetch(B);
             process(A);
             if (tt + 1 < cnt) {
-                fetch(tt + 2, A);
+                fetch(A);
                 process(B);
             }
         }
@@ -558,8 +588,8 @@ void launch_mx_k(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* f
                  double* epart, unsigned long long* ipart) {
     hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, HALF, HASH>), dim3((b.maxcl + 7) / 8 * 8),
                        dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
-                       b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par, fx, fy, fz,
-                       epart, ipart);
+                       b.cl_box, b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par,
+                       fx, fy, fz, epart, ipart);
 }
 
 template <int ELEC, int WPC>

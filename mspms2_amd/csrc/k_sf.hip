@@ -1,59 +1,15 @@
-// k_sf.hip -- solid-fluid terms of frclong on gfx950 (src/frclong.c:1145-1209).
-// One thread per site; only GHOST_NONE sites interact with the solid.
-//   SF_HYPERGEO       smooth-cylinder LJ tube, hypergeometric series
-//                     (cal_sf_hypergeo, src/sffrc.c:1092-1209), FP64
-//   SF_ATOM_EXPLICIT  LJ against explicit wall atoms
-//                     (cal_sf_atom_explicit, src/sffrc.c:1213-1314), FP64
-//   SF_TASOS          tricubic Hermite interpolation of an FP32 node grid
-//                     (call_tasos_forces, src/sffrc.c:1322-1337 ->
-//                     CForce_atom, mylibtasos/forces.f90:6-173): a gather of
-//                     8 corners x 8 fields = 256 B per site, FP64 arithmetic
-//   SF_MY_INTERP      32-monomial polynomials per cube (get_values_from_grid,
-//                     src/sffrc.c:951-1089): one contiguous KiB per site
-//                     (cube-major tables), FP64, the reference's order
-// Forces are added to the long-range forces (the site's own thread is the
-// only writer); the energy goes to a per-site partial for the finalize.
+// k_sf.hip -- solid-fluid terms on gfx950 (src/frclong.c:1145-1209): one
+// thread per site; only GHOST_NONE sites interact with the solid.  The
+// per-site models (hypergeometric tube, explicit wall atoms, tricubic
+// Hermite grid, myinterp cube polynomials) are sfeval::sf_site (sf_eval.h).
+// Forces are added to the caller's force arrays (the site's own thread is
+// the only writer); the energy goes to a per-site partial for the finalize.
+// k_grid_node_major turns the Hermite grid of the C ABI into the device's
+// node-major records.
 #include "engine.h"
-#include "physics.h"
+#include "sf_eval.h"
 
 namespace {
-
-__device__ inline double mic(double d, double L) {
-#pragma clang fp contract(off)
-    return d - L * rint(d / L);
-}
-
-// hypergeo (src/sffrc.c:1092-1116)
-__device__ inline void hypergeo(double a, double b, double c, double z, double rr, double* sumV,
-                                double* sumF) {
-#pragma clang fp contract(off)
-    double zn = 1.0, an1 = 1.0;
-    double sv = 1.0, sf = 0.0;
-    for (int i = 0; i <= 10; i++) {
-        double an = (a + i) * (b + i) / ((c + i) * (1 + i));
-        an1 = an1 * an;
-        zn *= z;
-        sv = sv + an1 * zn;
-        sf = sf - 2.0 * (i + 1.0) * an1 * zn / rr;
-    }
-    *sumV = sv;
-    *sumF = sf;
-}
-
-// 1-D cubic Hermite basis on t in [0,1], spacing h: value weights of the two
-// nodes, slope weights (times h), and their t-derivatives divided by h
-// (mylibtasos/cubicauxiliary.f90:1-60: g, -f, h w(t), -h w(1-t)).
-__device__ inline void hermite_1d(double t, double h, double b[4], double db[4]) {
-    double t2 = t * t, t3 = t2 * t;
-    b[0] = 2.0 * t3 - 3.0 * t2 + 1.0;
-    b[1] = 3.0 * t2 - 2.0 * t3;
-    b[2] = h * (t3 - 2.0 * t2 + t);
-    b[3] = h * (t3 - t2);
-    db[0] = (6.0 * t2 - 6.0 * t) / h;
-    db[1] = (6.0 * t - 6.0 * t2) / h;
-    db[2] = 3.0 * t2 - 4.0 * t + 1.0;
-    db[3] = 3.0 * t2 - 2.0 * t;
-}
 
 __global__ void __launch_bounds__(256) k_sf(int natom, const int* __restrict__ atom_site,
                                             const int* __restrict__ atom_mol,
@@ -65,7 +21,6 @@ __global__ void __launch_bounds__(256) k_sf(int natom, const int* __restrict__ a
                                             const double* __restrict__ z, double* fx,
                                             double* fy, double* fz, double* __restrict__ usf,
                                             Status* st) {
-#pragma clang fp contract(off)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
     const SiteDev S = sites[atom_site[i]];
@@ -74,189 +29,33 @@ __global__ void __launch_bounds__(256) k_sf(int natom, const int* __restrict__ a
         return;
     }
     const RepDev& R = reps[mols[atom_mol[i]].rep];
-    double u = 0.0;
-    if (sf.type == MSPMS_SF_HYPERGEO) {
-        double sij = 0.5 * (S.sigma + sf.solid_sigma);
-        double eij = sqrt(S.epsilon * sf.solid_epsilon);
-        double fxi = fx[i], fyi = fy[i];
-        for (int t = 0; t < sf.ntube; t++) {
-            double rx = mic(x[i] - sf.tube[3 * t], R.L[0]);
-            double ry = mic(y[i] - sf.tube[3 * t + 1], R.L[1]);
-            double rsq = rx * rx + ry * ry, r = sqrt(rsq);
-            double TR = sf.tube[3 * t + 2], TRsq = TR * TR;
-            double ljc, k, zf, hr, hfr, ha, hfa, k2, k4, k10, r1, r2, a1, a2, fij;
-            if (r < TR) {
-                ljc = sf.c3pisq * eij * sij * sij;
-                k = sij * TR / (TRsq - rsq);
-                zf = rsq / TRsq;
-                hypergeo(-4.5, -4.5, 1.0, zf, r, &hr, &hfr);
-                k2 = k * k;
-                k4 = k2 * k2;
-                k10 = k4 * k4 * k2;
-                r1 = 0.65625 * k10;
-                r2 = -hfr + hr * (20.0 * r / (TRsq - rsq));
-                hypergeo(-1.5, -1.5, 1.0, zf, r, &ha, &hfa);
-                a1 = -k4;
-                a2 = -hfa + ha * (8.0 * r / (TRsq - rsq));
-                u = ljc * (r1 * hr + a1 * ha);
-                fij = -ljc * (r1 * r2 + a1 * a2);
-            } else {
-                ljc = sf.c3pisq * eij * sij * sij * TR;
-                k = sij * r / (rsq - TRsq);
-                zf = TRsq / rsq;
-                hypergeo(-4.5, -4.5, 1.0, zf, r, &hr, &hfr);
-                k2 = k * k;
-                k4 = k2 * k2;
-                k10 = k4 * k4 * k2;
-                r1 = 0.65625 * k10;
-                r2 = hfr / r + hr * (-20.0 / (rsq - TRsq) + 9.0 / rsq);
-                hypergeo(-1.5, -1.5, 1.0, zf, r, &ha, &hfa);
-                a1 = -k4;
-                a2 = hfa / r + ha * (-8.0 / (rsq - TRsq) + 3.0 / rsq);
-                u = ljc * (r1 * hr + a1 * ha) / r;
-                fij = -ljc * (r1 * r2 + a1 * a2);
-            }
-            fxi += fij * rx / r;
-            fyi += fij * ry / r;
-        }
-        fx[i] = fxi;
-        fy[i] = fyi;
-    } else if (sf.type == MSPMS_SF_ATOM_EXPLICIT) {
-        double sij = 0.5 * (S.sigma + sf.solid_sigma);
-        double eij = sqrt(S.epsilon * sf.solid_epsilon);
-        double fxi = fx[i], fyi = fy[i], fzi = fz[i], us = 0.0;
-        for (int j = 0; j < sf.nsolid; j++) {
-            double rx = mic(x[i] - sf.solid[3 * j], R.L[0]);
-            double ry = mic(y[i] - sf.solid[3 * j + 1], R.L[1]);
-            double rz = mic(z[i] - sf.solid[3 * j + 2], R.L[2]);
-            double rsq = rx * rx + ry * ry + rz * rz;
-            if (!(rsq < R.rc2_lj)) continue;
-            double sw = 0.0;
-            if (R.switch_on) {
-                if (rsq < R.rcon2)
-                    sw = 1.0;
-                else
-                    sw = (R.rc2_lj - rsq) * (R.rc2_lj - rsq) * (R.rc2_lj + 2.0 * rsq - 3.0 * R.rcon2) /
-                         R.roff2_ron2_cube;
-            }
-            double rr = sij * sij / rsq, r6 = rr * rr * rr, r12 = r6 * r6, d = r12 - r6;
-            double ut = eij * d, fij;
-            us += R.switch_on ? ut * sw : ut;
-            if (R.switch_on) {
-                if (rsq < R.rcon2)
-                    fij = 24.0 * eij * (d + r12) / rsq;
-                else
-                    fij = 24.0 * eij * (d + r12) / rsq * sw -
-                          4.0 * ut * 12.0 * (R.rc2_lj - rsq) * (R.rcon2 - rsq) / R.roff2_ron2_cube;
-            } else {
-                fij = 24.0 * eij * (d + r12) / rsq;
-            }
-            fxi += fij * rx;
-            fyi += fij * ry;
-            fzi += fij * rz;
-        }
-        fx[i] = fxi;
-        fy[i] = fyi;
-        fz[i] = fzi;
-        u = us * 4.0;
-    } else if (sf.type == MSPMS_SF_TASOS) {
-        // call_tasos_forces: Angstrom in, K and K/A out, reduced back
-        const int ty = S.interp - 1;
-        if (ty < 0 || ty >= sf.ntype) {
-            atomicOr(&st->flags, ST_GRID_EDOM);
-            usf[i] = 0.0;
-            return;
-        }
-        double xp = x[i] * sf.sigma_base - sf.x0, yp = y[i] * sf.sigma_base - sf.y0,
-               zp = z[i] * sf.sigma_base - sf.z0;
-        const double pz = (sf.nz - 1) * sf.hz;
-        zp = zp - floor(zp / pz) * pz;  // mylibtasos/pbc.f90:82-86
-        int ix = (int)floor(xp / sf.hx), iy = (int)floor(yp / sf.hy), iz = (int)floor(zp / sf.hz);
-        if (!(ix >= 0 && iy >= 0 && iz >= 0 && ix <= sf.nx - 2 && iy <= sf.ny - 2 &&
-              iz <= sf.nz - 2)) {
-            atomicOr(&st->flags, ST_GRID_EDOM);
-            usf[i] = 0.0;
-            return;
-        }
-        double bx[4], by[4], bz[4], dbx[4], dby[4], dbz[4];
-        hermite_1d((xp - ix * sf.hx) / sf.hx, sf.hx, bx, dbx);
-        hermite_1d((yp - iy * sf.hy) / sf.hy, sf.hy, by, dby);
-        hermite_1d((zp - iz * sf.hz) / sf.hz, sf.hz, bz, dbz);
-        const size_t nn = (size_t)sf.nx * sf.ny * sf.nz;
-        const float* base = sf.nodes + (size_t)ty * 8 * nn;
-        double fval = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-        // field order F, Fx, Fy, Fz, Fxy, Fxz, Fyz, Fxyz; derivative bits (x,y,z)
-        const int field_of[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+    double fxi = fx[i], fyi = fy[i], fzi = fz[i];
+    int edom = 0;
+    usf[i] = sfeval::sf_site(sf, R, S, x[i], y[i], z[i], fxi, fyi, fzi, &edom);
+    if (edom) atomicOr(&st->flags, ST_GRID_EDOM);
+    fx[i] = fxi;
+    fy[i] = fyi;
+    fz[i] = fzi;
+}
+
+// The Hermite grid from the C-ABI layout [type][field][node] (fields F, Fx,
+// Fy, Fz, Fxy, Fxz, Fyz, Fxyz) to the device layout [type][node][o], o the
+// derivative bits (x, y, z): F, Fx, Fy, Fxy, Fz, Fxz, Fyz, Fxyz, one 32-B
+// record per node, so that a lookup reads 8 records instead of 64 scattered
+// floats.
+__global__ void __launch_bounds__(256) k_grid_node_major(const float* __restrict__ src,
+                                                         float* __restrict__ dst, size_t nn,
+                                                         int ntype) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn * (size_t)ntype) return;
+    const size_t t = k / nn, node = k % nn;
+    const int field_of[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+    float v[8];
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const size_t node = ((size_t)(iz + cz) * sf.ny + (iy + cy)) * sf.nx + (ix + cx);
-#pragma unroll
-            for (int o = 0; o < 8; o++) {
-                const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
-                const double v = (double)base[(size_t)field_of[o] * nn + node];
-                const double wx = bx[cx + 2 * ox], wy = by[cy + 2 * oy], wz = bz[cz + 2 * oz];
-                fval += v * wx * wy * wz;
-                gx += v * dbx[cx + 2 * ox] * wy * wz;
-                gy += v * wx * dby[cy + 2 * oy] * wz;
-                gz += v * wx * wy * dbz[cz + 2 * oz];
-            }
-        }
-        double pe = (fval < 1.5 * sf.min_pe[ty]) ? -fval : fval;  // forces.f90:166-171
-        const double fconv = sf.sigma_base / sf.epsilon_base;
-        u = pe / sf.epsilon_base;
-        fx[i] += -gx * fconv;
-        fy[i] += -gy * fconv;
-        fz[i] += -gz * fconv;
-    } else if (sf.type == MSPMS_SF_MY_INTERP) {
-        // get_values_from_grid (src/sffrc.c:951-1089) and its caller
-        // (src/frclong.c:1193-1207): the same expressions in the same order
-        const int ty = S.interp - 1;  // init_my_interp decrements interp_type
-        if (ty < 0 || ty >= sf.ntype) {
-            atomicOr(&st->flags, ST_GRID_EDOM);
-            usf[i] = 0.0;
-            return;
-        }
-        const double fxx = x[i] * sf.sigma_base, fyy = y[i] * sf.sigma_base;
-        double fzz = z[i] * sf.sigma_base;
-        const double tx = (fxx - sf.x0) / sf.hx, tyy = (fyy - sf.y0) / sf.hy;
-        fzz = fzz - sf.zmax * rint(fzz / sf.zmax);  // periodic in z
-        if (fzz < 0) fzz = sf.zmax + fzz;
-        const double tz = (fzz - 0.0) / sf.hz;      // zmin = 0
-        // a cube index outside the tables (the reference reads past them)
-        if (!(tx > -1.0 && tx < sf.nx && tyy > -1.0 && tyy < sf.ny && tz > -1.0 && tz < sf.nz)) {
-            atomicOr(&st->flags, ST_GRID_EDOM);
-            usf[i] = 0.0;
-            return;
-        }
-        const int iAx = (int)tx, iAy = (int)tyy, iAz = (int)tz;
-        const int cubeidx = iAx * sf.ny * sf.nz + iAy * sf.nz + iAz;
-        const double* q =
-            sf.cubes + ((size_t)ty * ((size_t)sf.nx * sf.ny * sf.nz) + (size_t)cubeidx) * 128;
-        // Amatrix_ele_assign_value (src/sffrc.c:750-803)
-        const double X = fxx, Y = fyy, Z = fzz;
-        const double x2 = X * X, x3 = x2 * X, y2 = Y * Y, y3 = y2 * Y, z2 = Z * Z, z3 = z2 * Z;
-        const double v[32] = {1.0,        X,          Y,          Z,      x2,     X * Y,
-                              X * Z,      y2,         Y * Z,      z2,     x3,     x2 * Y,
-                              X * y2,     x2 * Z,     X * z2,     y3,     y2 * Z, Y * z2,
-                              z3,         X * Y * Z,  x3 * Y,     x2 * y2, X * y3, x3 * Z,
-                              x2 * z2,    X * z3,     y3 * Z,     y2 * z2, Y * z3, x2 * Y * Z,
-                              X * y2 * Z, X * Y * z2};
-        double r[4];
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-            double a = q[f * 32] * v[0];
-#pragma unroll
-            for (int m = 1; m < 32; m++) a = a + q[f * 32 + m] * v[m];
-            r[f] = a;
-        }
-        const double R_RGAS = 0.120272219;  // src/defs.h:96
-        u = r[0] * R_RGAS / sf.epsilon_base;
-        fx[i] += r[1] * R_RGAS * sf.sigma_base / sf.epsilon_base;
-        fy[i] += r[2] * R_RGAS * sf.sigma_base / sf.epsilon_base;
-        fz[i] += r[3] * R_RGAS * sf.sigma_base / sf.epsilon_base;
-    }
-    usf[i] = u;
+    for (int o = 0; o < 8; o++) v[o] = src[(t * 8 + field_of[o]) * nn + node];
+    float4* d = reinterpret_cast<float4*>(dst) + 2 * k;
+    d[0] = make_float4(v[0], v[1], v[2], v[3]);
+    d[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
 }  // namespace
@@ -266,4 +65,10 @@ void launch_sf(const LaunchCtx& c, const SFDev& sf, const double* x, const doubl
     int g = (c.natom + 255) / 256;
     hipLaunchKernelGGL(k_sf, dim3(g), dim3(256), 0, c.stream, c.natom, c.atom_site, c.atom_mol,
                        c.mols, c.sites, c.reps, sf, x, y, z, fx, fy, fz, usf, c.status);
+}
+
+void launch_grid_node_major(hipStream_t s, const float* src, float* dst, size_t nn, int ntype) {
+    const size_t n = nn * (size_t)ntype;
+    hipLaunchKernelGGL(k_grid_node_major, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src,
+                       dst, nn, ntype);
 }

@@ -86,6 +86,8 @@ def lib():
     L.mspms_autotune.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(abi.TuneResult)]
     L.mspms_set_launch_shape.argtypes = [vp, C.c_double, C.c_int]
     L.mspms_set_list_skin.argtypes = [vp, C.c_double]
+    L.mspms_set_sf_split.argtypes = [vp, C.c_int]
+    L.mspms_set_list_capacity.argtypes = [vp, C.c_int]
     L.mspms_nvt_set.argtypes = [vp, C.POINTER(abi.NvtState)]
     L.mspms_nvt_get.argtypes = [vp, C.POINTER(abi.NvtState)]
     L.mspms_nvt_run_device.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp,
@@ -459,6 +461,16 @@ class Engine:
         """Pair-list buffer: lists are built with the cutoff + skin and reused
         while every site stays within skin / 2 of its position at the build."""
         _check(lib().mspms_set_list_skin(self._h, C.c_double(float(skin))), "mspms_set_list_skin")
+
+    def set_list_capacity(self, max_entries: int):
+        """Bound on the pair-list row capacity (0: none)."""
+        _check(lib().mspms_set_list_capacity(self._h, int(max_entries)),
+               "mspms_set_list_capacity")
+
+    def set_sf_split(self, on: bool = True):
+        """RESPA split of the solid-fluid wall: its forces move from frclong
+        to frcshort (the inner loop of a device RESPA step)."""
+        _check(lib().mspms_set_sf_split(self._h, 1 if on else 0), "mspms_set_sf_split")
 
     def list_builds(self) -> int:
         n = C.c_longlong(0)
