@@ -483,6 +483,17 @@ typedef struct mspms_sfgridgen_spec {
 } mspms_sfgridgen_spec;
 int mspms_sfgridgen_dims(const mspms_sfgridgen_spec* spec, int* nx, int* ny, int* nz);
 int mspms_sfgridgen(int device, const mspms_sfgridgen_spec* spec, double* out);
+/* mspms_sfgridgen without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture preparation, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * solid atoms are staged in a fixed per-GPU device array of 3 x 1000 FP64
+ * (24,000 bytes, held until the process ends); more than 1000 solid atoms is
+ * refused with MSPMS_EINVAL.  The 13 x nx*ny*nz tables (mspms_sfgridgen_dims)
+ * are written into the caller's device buffer d_out on `stream` (null: the
+ * null stream); the call returns when they are written.  Calls that overlap
+ * on one GPU share the staging array and race. */
+int mspms_sfgridgen_noalloc(int device, const mspms_sfgridgen_spec* spec, double* d_out,
+                            void* stream);
 
 /* Generator of the myinterp cube polynomials (the reference reads them,
  * src/sffrc.c:228-748, but ships no program that writes them).

@@ -21,7 +21,7 @@ DROPIN = os.path.join(HERE, "libmspms_dropin.a")
 ARCH = os.environ.get("MSPMS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_cluster.hip", "k_pair.hip", "k_intra.hip", "k_recip.hip", "k_integrate.hip",
-               "k_sf.hip", "k_hmc.hip", "k_nvt.hip", "k_npt.hip", "k_sfgrid.hip", "k_myinterp.hip",
+               "k_sf.hip", "k_hmc.hip", "k_nvt.hip", "k_npt.hip", "k_sfgrid.hip", "k_sfgrid_noalloc.hip", "k_myinterp.hip",
                "engine.cpp", "terms_host.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
