@@ -45,6 +45,11 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X published FP64
 PEAK_HBM_GBS = 8000.0
 
 
+def progress(what):
+    """A progress line on stderr (stdout carries only the JSON record)."""
+    print(f"bench: {what}", file=sys.stderr, flush=True)
+
+
 def make_workload():
     from mspms2_amd import configs
     from mspms2_amd.system import system_of
@@ -810,7 +815,9 @@ def main():
         run_trace(system, args.trace_steps, args.skin / system.params["sigma_base"], args.shape)
         return
     skin = args.skin / system.params["sigma_base"]
+    progress("12,288-site water, mixed")
     mixed = run_gpu(system, "mixed", args.steps, args.warmup, skin=skin)
+    progress("12,288-site water, FP64")
     fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False,
                    skin=skin)
 
@@ -859,6 +866,7 @@ def main():
     cpu = None
     force_err_oracle = None
     if not args.no_cpu:
+        progress("reference frclong on the host")
         c = run_reference_cpu(rd)
         cpu_step = c["seconds_per_frclong"]
         cpu = {"value": DT_FS * 1e-6 * 86400.0 / cpu_step, "unit": "ns/day", "cores": 1,
@@ -918,6 +926,7 @@ def main():
         # of the lattice start
         start = None
         drift = {}
+        progress("energy drift")
         if args.equil_steps > 0:
             p_eq, v_eq, t_eq = equilibrate(system, args.equil_steps, skin)
             start = (p_eq, v_eq)
@@ -931,14 +940,19 @@ def main():
             drift[p] = run_drift(system, p, args.drift_steps, skin, start)
         line["energy_drift"] = drift
     if args.replicas > 0:
+        progress("batched HMC replicas")
         line["replicas"] = run_replica_batch(make_replicas(args.replicas))
     if args.wolf_configs > 0:
+        progress("Wolf fitting set")
         line["wolf_fit"] = run_wolf_fit(args.wolf_configs, cpu=not args.no_cpu)
     if args.tube_loadings > 0:
+        progress("methane-in-tube loadings")
         line["tube"] = run_tube(make_tube(args.tube_loadings), cpu=not args.no_cpu)
     if args.lj_steps > 0:
+        progress("LJ-864 NVT")
         line["lj864_nvt"] = run_lj_nvt(args.lj_steps, cpu=not args.no_cpu)
     if args.sfgrid > 0:
+        progress("grid generators")
         line["sfgrid"] = run_sfgrid(args.sfgrid, cpu=not args.no_cpu)
     print(json.dumps(line))
 

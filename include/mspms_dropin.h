@@ -43,6 +43,23 @@ int bndfrc(int iSpecie, int iBond, int iabs, double* uij, double* virial_ij);  /
 int aglfrc(int iSpecie, int iAngle, int iabs, double* uij, double* virial_ij); /* src/aglfrc.c:17 */
 int dihfrc(int iSpecie, int iDih, int iabs, double* uij, double* virial_ij);   /* src/dihfrc.c:17 */
 int impfrc(int iSpecie, int iImp, int iabs, double* uij, double* virial_ij);   /* src/impfrc.c:17 */
+int xatom_coulomb_frc(double rijsq, double chargeij, double* uij, double* fij); /* src/elecfrc.c:159 */
+/* molecule-molecule Coulomb, cut off on the centre-of-mass distance; adds to
+ * uelec, virial_inter and fxl/fyl/fzl as the reference does */
+int xmole_coulomb_frc(int iSpecie, int iMole, int iabs, int iAtom, int jSpecie, int jMole,
+                      int jabs, int jAtom);                         /* src/elecfrc.c:67 */
+/* solid-fluid wall of one site: uij, and the site's force added to fxl.. */
+int cal_sf_hypergeo(int ii, int iSpecie, int iAtom, double* uij, double* fij); /* src/sffrc.c:1120 */
+int cal_sf_atom_explicit(int ii, int iSpecie, int iAtom, double* uij,
+                         double* fij);                              /* src/sffrc.c:1213 */
+/* cubic-interpolation table lookup (the MYINTERP tables read at start-up) */
+int get_values_from_grid(double fxx, double fyy, double fzz, int type, double* usf,
+                         double* fsf);                              /* src/sffrc.c:951 */
+/* The Hermite grid of the Fortran library is not reachable from here: this
+ * symbol prints a message and exits; give the grid to the context API
+ * (mspms_sf_grid in mspms_gpu.h) instead. */
+int call_tasos_forces(int itype, double fxx, double fyy, double fzz, double* usflj_tasos,
+                      double* tasos_force);                         /* src/sffrc.c:1322 */
 
 /* Releases the device context (optional; the OS reclaims it at exit). */
 void mspms_dropin_release(void);

@@ -329,6 +329,8 @@ int mspms_frcshort_device(mspms_ctx* ctx, const double* d_xx,
  *                          (src/elecfrc.c:35-46)
  *   MSPMS_TERM_WOLF_REAL   wolf_real_frc(rijsq, chargeij=a, &u, &f)
  *                          (src/elecfrc.c:48-60)
+ *   MSPMS_TERM_COULOMB     xatom_coulomb_frc(rijsq, chargeij=a, &u, &f)
+ *                          (src/elecfrc.c:159-170)
  * mspms_term_bonded replaces bndfrc / aglfrc / dihfrc / impfrc(iSpecie,
  * index, iabs, &u, &virial) (src/bndfrc.c:17, src/aglfrc.c:17,
  * src/dihfrc.c:17, src/impfrc.c:17) for the molecule whose sites are at
@@ -337,7 +339,7 @@ int mspms_frcshort_device(mspms_ctx* ctx, const double* d_xx,
  * improper returns MSPMS_EINVAL where impfrc exits. */
 enum {
     MSPMS_TERM_LJ = 0, MSPMS_TERM_EWALD_REAL = 1, MSPMS_TERM_EWALD_EXCL = 2,
-    MSPMS_TERM_WOLF_REAL = 3
+    MSPMS_TERM_WOLF_REAL = 3, MSPMS_TERM_COULOMB = 4
 };
 enum { MSPMS_TERM_BOND = 0, MSPMS_TERM_ANGLE = 1, MSPMS_TERM_DIH = 2, MSPMS_TERM_IMP = 3 };
 int mspms_term_pair(const mspms_params* p, int which, double rijsq, double a,
@@ -346,6 +348,22 @@ int mspms_term_bonded(const mspms_species* sp, int kind, int index,
                       const double* xx, const double* yy, const double* zz,
                       double* fxs, double* fys, double* fzs, double* u,
                       double* vir);
+/* The solid-fluid term of one site, evaluated on the host by the code the
+ * kernels run: cal_sf_hypergeo (iSF_type 1, src/sffrc.c:1118-1209) and
+ * cal_sf_atom_explicit (iSF_type 2, src/sffrc.c:1213-1314) for site `site`
+ * of *sp at reduced (x, y, z); the site's force is added to (*fx, *fy, *fz)
+ * as the reference adds it to fxl, the energy goes to *u and the scalar
+ * force of the last tube / wall atom in range to *fij (left unchanged when
+ * none is).  Other iSF_type values: MSPMS_EINVAL. */
+int mspms_term_sf(const mspms_params* p, const mspms_species* sp, int site, double x,
+                  double y, double z, double* fx, double* fy, double* fz, double* u,
+                  double* fij);
+/* get_values_from_grid (src/sffrc.c:951-1089): the cube polynomials of table
+ * `type` (the reference's decremented interp type, 0-based) of p->cubes at
+ * reduced (x, y, z); *usf in J/mol and fsf[3] in J/mol/Angstrom, the
+ * reference's raw units.  A cube outside the tables: MSPMS_EDOM. */
+int mspms_term_myinterp(const mspms_params* p, double x, double y, double z, int type,
+                        double* usf, double* fsf);
 
 /* Adjoint of the nonbonded energy with respect to the per-site force-field
  * parameters, in FP64 whatever the context precision.  Host buffers; grad

@@ -53,9 +53,10 @@ def build(verbose: bool = False) -> str:
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     # drop-in objects: plain C over the reference's globals
-    # (dropin_terms.c: the term-level functions, an archive member of its own)
+    # (dropin_globals.c: the globals read into the C ABI's tables;
+    # dropin_terms.c: the term-level functions, an archive member of its own)
     dobjs = []
-    for name in ("dropin.c", "dropin_terms.c"):
+    for name in ("dropin.c", "dropin_globals.c", "dropin_terms.c"):
         dobj = os.path.join(BUILD, os.path.splitext(name)[0] + ".o")
         _run(["gcc", "-std=c99", "-O2", "-fPIC", "-fcommon", "-Wall", "-c",
               os.path.join(CSRC, name), "-o", dobj])

@@ -1519,6 +1519,18 @@ int read_status(mspms_ctx* c, hipStream_t s) {
     return 0;
 }
 
+// Headroom of the pair-list rows: when the longest row seen reaches 3/4 of
+// the capacity, the lists are resized to 1.5 times that row, so that later
+// builds of a moving system (local density fluctuations) stay within it.
+// Called after a status read, outside any device run.
+int keep_list_headroom(mspms_ctx* c) {
+    const int seen = c->h_status->max_list;
+    if (c->cap_limit > 0 || seen * 4 < c->cb.cap * 3) return 0;
+    c->cap_hint = std::max(c->cap_hint, seen + seen / 2 + 16);
+    drop_graph(c);
+    return build(c);
+}
+
 void accumulate_profile(mspms_ctx* c) {
     if (!c->profiling || c->pending.empty()) return;
     (void)hipEventSynchronize(c->ev[2 * c->pending.size() - 1]);
@@ -1592,7 +1604,7 @@ extern "C" int mspms_frclong_host(mspms_ctx* c, const double* xx, const double* 
     if (c->h_status->flags & ST_NONFINITE) return fail(MSPMS_ENOTFINITE, "non-finite position");
     if (c->h_status->flags & ST_GRID_EDOM)
         return fail(MSPMS_EDOM, "site outside the solid-fluid grid or without a grid type");
-    return 0;
+    return keep_list_headroom(c);
 }
 
 extern "C" int mspms_frcshort_host(mspms_ctx* c, const double* xx, const double* yy,
@@ -2577,7 +2589,7 @@ extern "C" int mspms_sync(mspms_ctx* c) {
     if (c->h_status->flags & ST_GRID_EDOM)
         return fail(MSPMS_EDOM, "site outside the solid-fluid grid or without a grid type");
     if (c->h_status->flags & ST_SINGULAR) return fail(MSPMS_EINVAL, "singular improper");
-    return 0;
+    return keep_list_headroom(c);
 }
 
 extern "C" void* mspms_stream(const mspms_ctx* c) { return c ? (void*)c->stream : nullptr; }

@@ -90,6 +90,16 @@ __host__ __device__ inline void ewald_excl64(double rijsq, double qq, double C, 
     *fij = -*uij / rijsq;
 }
 
+// xatom_coulomb_frc (src/elecfrc.c:159-170): bare Coulomb, u = C qq / r,
+// F_i = (u / r^2) r_ij
+__host__ __device__ inline void coulomb64(double rijsq, double qq, double C, double* uij,
+                                          double* fij) {
+#pragma clang fp contract(off)
+    double rij = sqrt(rijsq);
+    *uij = C * qq / rij;
+    *fij = *uij / rijsq;
+}
+
 // wolf_real_frc (src/elecfrc.c:48-60)
 __host__ __device__ inline void wolf_real64(double rijsq, double qq, double C, double kappa, double rc,
                                    double v1, double v2, double f1, double f2, double* uij,

@@ -59,9 +59,55 @@ __host__ __device__ inline void hermite_1d(double t, double h, double b[4], doub
 // units); its force is added to (fxi, fyi, fzi) as the reference adds it to
 // fxl.  A site outside a grid's tables sets *edom and adds nothing.  Host
 // callers pass tables in host memory.
+// The 32 monomials of Amatrix_ele_assign_value (src/sffrc.c:750-803) at
+// absolute (X, Y, Z) and the four cube polynomials (E, Fx, Fy, Fz), in the
+// reference's order; q(f, m) is coefficient m of polynomial f.
+template <class Coef>
+__host__ __device__ inline void mi_poly(const Coef& q, double X, double Y, double Z,
+                                        double r[4]) {
+#pragma clang fp contract(off)
+    const double x2 = X * X, x3 = x2 * X, y2 = Y * Y, y3 = y2 * Y, z2 = Z * Z, z3 = z2 * Z;
+    const double v[32] = {1.0,        X,          Y,          Z,      x2,     X * Y,
+                          X * Z,      y2,         Y * Z,      z2,     x3,     x2 * Y,
+                          X * y2,     x2 * Z,     X * z2,     y3,     y2 * Z, Y * z2,
+                          z3,         X * Y * Z,  x3 * Y,     x2 * y2, X * y3, x3 * Z,
+                          x2 * z2,    X * z3,     y3 * Z,     y2 * z2, Y * z3, x2 * Y * Z,
+                          X * y2 * Z, X * Y * z2};
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        double a = q(f, 0) * v[0];
+#pragma unroll
+        for (int m = 1; m < 32; m++) a = a + q(f, m) * v[m];
+        r[f] = a;
+    }
+}
+
+// get_values_from_grid's cube location (src/sffrc.c:968-987) for reduced
+// (x, y, z): the absolute coordinates (z folded into [0, zmax)) and the cube
+// index, or -1 outside the tables (the reference reads past them)
+__host__ __device__ inline int mi_cube(double xi, double yi, double zi, double sigma_base,
+                                       double x0, double y0, double zmax, double hx, double hy,
+                                       double hz, int nx, int ny, int nz, double* X, double* Y,
+                                       double* Z) {
+#pragma clang fp contract(off)
+    const double fxx = xi * sigma_base, fyy = yi * sigma_base;
+    double fzz = zi * sigma_base;
+    const double tx = (fxx - x0) / hx, tyy = (fyy - y0) / hy;
+    fzz = fzz - zmax * rint(fzz / zmax);  // periodic in z
+    if (fzz < 0) fzz = zmax + fzz;
+    const double tz = (fzz - 0.0) / hz;   // zmin = 0
+    *X = fxx;
+    *Y = fyy;
+    *Z = fzz;
+    if (!(tx > -1.0 && tx < nx && tyy > -1.0 && tyy < ny && tz > -1.0 && tz < nz)) return -1;
+    const int iAx = (int)tx, iAy = (int)tyy, iAz = (int)tz;
+    return iAx * ny * nz + iAy * nz + iAz;
+}
+
 __host__ __device__ inline double sf_site(const SFDev& sf, const RepDev& R, const SiteDev& S,
                                           double xi, double yi, double zi, double& fxi,
-                                          double& fyi, double& fzi, int* edom) {
+                                          double& fyi, double& fzi, int* edom,
+                                          double* fij_last = nullptr) {
 #pragma clang fp contract(off)
     double u = 0.0;
     if (sf.type == MSPMS_SF_HYPERGEO) {
@@ -107,6 +153,7 @@ __host__ __device__ inline double sf_site(const SFDev& sf, const RepDev& R, cons
             }
             hfx += fij * rx / r;
             hfy += fij * ry / r;
+            if (fij_last) *fij_last = fij;
         }
         fxi = hfx;
         fyi = hfy;
@@ -143,6 +190,7 @@ __host__ __device__ inline double sf_site(const SFDev& sf, const RepDev& R, cons
             efx += fij * rx;
             efy += fij * ry;
             efz += fij * rz;
+            if (fij_last) *fij_last = fij;
         }
         fxi = efx;
         fyi = efy;
@@ -205,38 +253,17 @@ __host__ __device__ inline double sf_site(const SFDev& sf, const RepDev& R, cons
             *edom = 1;
             return 0.0;
         }
-        const double fxx = xi * sf.sigma_base, fyy = yi * sf.sigma_base;
-        double fzz = zi * sf.sigma_base;
-        const double tx = (fxx - sf.x0) / sf.hx, tyy = (fyy - sf.y0) / sf.hy;
-        fzz = fzz - sf.zmax * rint(fzz / sf.zmax);  // periodic in z
-        if (fzz < 0) fzz = sf.zmax + fzz;
-        const double tz = (fzz - 0.0) / sf.hz;      // zmin = 0
-        // a cube index outside the tables (the reference reads past them)
-        if (!(tx > -1.0 && tx < sf.nx && tyy > -1.0 && tyy < sf.ny && tz > -1.0 && tz < sf.nz)) {
+        double X, Y, Z;
+        const int cubeidx = mi_cube(xi, yi, zi, sf.sigma_base, sf.x0, sf.y0, sf.zmax, sf.hx,
+                                    sf.hy, sf.hz, sf.nx, sf.ny, sf.nz, &X, &Y, &Z);
+        if (cubeidx < 0) {
             *edom = 1;
             return 0.0;
         }
-        const int iAx = (int)tx, iAy = (int)tyy, iAz = (int)tz;
-        const int cubeidx = iAx * sf.ny * sf.nz + iAy * sf.nz + iAz;
         const double* q =
             sf.cubes + ((size_t)ty * ((size_t)sf.nx * sf.ny * sf.nz) + (size_t)cubeidx) * 128;
-        // Amatrix_ele_assign_value (src/sffrc.c:750-803)
-        const double X = fxx, Y = fyy, Z = fzz;
-        const double x2 = X * X, x3 = x2 * X, y2 = Y * Y, y3 = y2 * Y, z2 = Z * Z, z3 = z2 * Z;
-        const double v[32] = {1.0,        X,          Y,          Z,      x2,     X * Y,
-                              X * Z,      y2,         Y * Z,      z2,     x3,     x2 * Y,
-                              X * y2,     x2 * Z,     X * z2,     y3,     y2 * Z, Y * z2,
-                              z3,         X * Y * Z,  x3 * Y,     x2 * y2, X * y3, x3 * Z,
-                              x2 * z2,    X * z3,     y3 * Z,     y2 * z2, Y * z3, x2 * Y * Z,
-                              X * y2 * Z, X * Y * z2};
         double r[4];
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-            double a = q[f * 32] * v[0];
-#pragma unroll
-            for (int m = 1; m < 32; m++) a = a + q[f * 32 + m] * v[m];
-            r[f] = a;
-        }
+        mi_poly([q](int f, int m) { return q[f * 32 + m]; }, X, Y, Z, r);
         const double R_RGAS = 0.120272219;  // src/defs.h:96
         u = r[0] * R_RGAS / sf.epsilon_base;
         fxi += r[1] * R_RGAS * sf.sigma_base / sf.epsilon_base;

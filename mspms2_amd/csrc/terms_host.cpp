@@ -6,10 +6,16 @@
 //   mspms_term_bonded  bndfrc / aglfrc / dihfrc / impfrc (src/bndfrc.c:17-78,
 //                      src/aglfrc.c:17-190, src/dihfrc.c:17-284,
 //                      src/impfrc.c:17-191) on one molecule's coordinates
+//   mspms_term_sf      cal_sf_hypergeo / cal_sf_atom_explicit for one site
+//                      (src/sffrc.c:1118-1314)
+//   mspms_term_myinterp  get_values_from_grid (src/sffrc.c:951-1089)
 // They exist for term-level parity tests and for callers that evaluate
 // single terms; the force evaluation itself runs on the GPU only.
+#include <vector>
+
 #include "engine.h"
 #include "physics.h"
+#include "sf_eval.h"
 #include "terms.h"
 #include "../../include/mspms_gpu.h"
 
@@ -33,6 +39,9 @@ extern "C" int mspms_term_pair(const mspms_params* p, int which, double rijsq, d
     case MSPMS_TERM_WOLF_REAL:
         wolf_real64(rijsq, a, p->coulomb_prefactor, p->kappa, p->rcutoffelec, p->wolfvcon1,
                     p->wolfvcon2, p->wolffcon1, p->wolffcon2, uij, fij);
+        return 0;
+    case MSPMS_TERM_COULOMB:
+        coulomb64(rijsq, a, p->coulomb_prefactor, uij, fij);
         return 0;
     default:
         return mspms_fail(MSPMS_EINVAL, "unknown pair term");
@@ -83,5 +92,77 @@ extern "C" int mspms_term_bonded(const mspms_species* q, int kind, int k, const 
     bonded_term(T, 0, xx, yy, zz, fxs, fys, fzs, u, vir, &singular);
     if (singular)
         return mspms_fail(MSPMS_EINVAL, "improper at a singular geometry (src/impfrc.c:96-134)");
+    return 0;
+}
+
+extern "C" int mspms_term_sf(const mspms_params* p, const mspms_species* sp, int site, double x,
+                             double y, double z, double* fx, double* fy, double* fz, double* u,
+                             double* fij) {
+    if (!p || !sp || !fx || !fy || !fz || !u || !fij)
+        return mspms_fail(MSPMS_EINVAL, "null argument");
+    if (site < 0 || site >= sp->natom) return mspms_fail(MSPMS_EINVAL, "site index");
+    SFDev F{};
+    F.type = p->iSF_type;
+    std::vector<double> tab;
+    if (p->iSF_type == MSPMS_SF_HYPERGEO) {
+        F.ntube = p->ntube;
+        for (int t = 0; t < p->ntube; t++) {
+            tab.push_back(p->tube_x[t]);
+            tab.push_back(p->tube_y[t]);
+            tab.push_back(p->tube_r[t]);
+        }
+        F.tube = tab.data();
+        F.c3pisq = p->const_3pisq_theta;
+    } else if (p->iSF_type == MSPMS_SF_ATOM_EXPLICIT) {
+        F.nsolid = p->solid_natom;
+        for (int k = 0; k < p->solid_natom; k++) {
+            tab.push_back(p->solid_xx[k]);
+            tab.push_back(p->solid_yy[k]);
+            tab.push_back(p->solid_zz[k]);
+        }
+        F.solid = tab.data();
+    } else {
+        return mspms_fail(MSPMS_EINVAL, "mspms_term_sf: iSF_type 1 or 2");
+    }
+    F.solid_sigma = p->solid_sigma;
+    F.solid_epsilon = p->solid_epsilon;
+    RepDev R{};
+    R.L[0] = p->boxlx;
+    R.L[1] = p->boxly;
+    R.L[2] = p->boxlz;
+    R.rc2_lj = p->rcutoffsq;
+    R.rcon2 = p->rcutonsq;
+    R.roff2_ron2_cube = p->roff2_minus_ron2_cube;
+    R.switch_on = p->isLJswitchOn;
+    SiteDev S{};
+    S.sigma = sp->sigma[site];
+    S.epsilon = sp->epsilon[site];
+    S.ghost = sp->ghost_type[site];
+    int edom = 0;
+    double gx = *fx, gy = *fy, gz = *fz;
+    *u = sfeval::sf_site(F, R, S, x, y, z, gx, gy, gz, &edom, fij);
+    *fx = gx;
+    *fy = gy;
+    *fz = gz;
+    return 0;
+}
+
+extern "C" int mspms_term_myinterp(const mspms_params* p, double x, double y, double z, int type,
+                                   double* usf, double* fsf) {
+    if (!p || !p->cubes || !usf || !fsf) return mspms_fail(MSPMS_EINVAL, "null argument");
+    const mspms_sf_cubes& Q = *p->cubes;
+    if (type < 0 || type >= Q.ntype) return mspms_fail(MSPMS_EINVAL, "myinterp table type");
+    double X, Y, Z;
+    const int cube = sfeval::mi_cube(x, y, z, p->sigma_base, Q.xmin, Q.ymin, Q.zmax, Q.hx, Q.hy,
+                                     Q.hz, Q.ncx, Q.ncy, Q.ncz, &X, &Y, &Z);
+    if (cube < 0) return mspms_fail(MSPMS_EDOM, "position outside the myinterp tables");
+    // the file's order: table (type * 128 + polynomial * 32 + coefficient)[cube]
+    const double* const* T = Q.tables + (size_t)type * 128;
+    double r[4];
+    sfeval::mi_poly([T, cube](int f, int m) { return T[f * 32 + m][cube]; }, X, Y, Z, r);
+    *usf = r[0];
+    fsf[0] = r[1];
+    fsf[1] = r[2];
+    fsf[2] = r[3];
     return 0;
 }

@@ -57,6 +57,22 @@ CFLAGS="-std=c99 -O2 -fcommon -w"
     # engine's frclong/frcshort); frclong.o / frcshort.o are left out.
     ar rcs "$OUT/libmspms_ref_callers.a" $(ls *.o | grep -v -e '^harness.o$' -e '^frclong.o$' -e '^frcshort.o$')
     cp harness.o "$OUT/ref_harness.o"
+    # Term harness: the reference's own frclong / frcshort and callers with
+    # the engine's term-level functions (mspms2_amd/csrc/dropin_terms.c over
+    # libmspms_gpu.so's host term code) in place of ljfrc.c, elecfrc.c and
+    # the bonded-term files; sffrc.c stays for its start-up readers, and its
+    # cal_sf_* / get_values_from_grid yield to the engine's (linked first).
+    PKG="$HERE/../mspms2_amd"
+    if [ -f "$PKG/_build/dropin_terms.o" ] && [ -f "$PKG/libmspms_gpu.so" ]; then
+        gcc -fcommon -o "$OUT/mspms_terms_harness" harness.o "$PKG/_build/dropin_terms.o" \
+            "$PKG/_build/dropin_globals.o" \
+            $(ls *.o | grep -v -e '^harness.o$' -e '^ljfrc.o$' -e '^elecfrc.o$' \
+                  -e '^bndfrc.o$' -e '^aglfrc.o$' -e '^dihfrc.o$' -e '^impfrc.o$') \
+            -L"$PKG" -lmspms_gpu -lm -Wl,--allow-multiple-definition \
+            -Wl,-rpath,'$ORIGIN/../../mspms2_amd' -Wl,-rpath-link,/opt/rocm/lib \
+            -Wl,--unresolved-symbols=ignore-in-object-files
+        echo "build_ref.sh: built $OUT/mspms_terms_harness"
+    fi
     # layout probe of SAMPLE_MOLECULE for the drop-in ABI check
     cat > layout.c <<'EOF'
 #include <stdio.h>

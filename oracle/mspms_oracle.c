@@ -1515,6 +1515,10 @@ void oracle_term_pair(const mspms_params* p, int which, double rijsq, double a,
     case 0: lj_term(p, rijsq, a, b, u, f, ushift); break; /* ljfrc */
     case 1: ewald_real(p, rijsq, a, u, f); break;         /* ewald_real_frc */
     case 2: ewald_excl(p, rijsq, a, u, f); break;         /* ewald_excl_frc */
+    case 4:                                               /* xatom_coulomb_frc */
+        *u = p->coulomb_prefactor * a / sqrt(rijsq);      /* src/elecfrc.c:164-167 */
+        *f = *u / rijsq;
+        break;
     default: wolf_real(p, rijsq, a, u, f); break;         /* wolf_real_frc */
     }
 }

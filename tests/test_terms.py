@@ -61,7 +61,8 @@ def test_pair_terms_bit_exact(name):
     for rsq in rng.uniform(0.3 * rc2 / 36, rc2, 400):
         a, b = rng.uniform(0.5, 1.5), rng.uniform(0.1, 3.0)
         qq = rng.uniform(-1.0, 1.0)
-        for which, (x, y) in ((0, (a, b)), (1, (qq, 0.0)), (2, (qq, 0.0)), (3, (qq, 0.0))):
+        for which, (x, y) in ((0, (a, b)), (1, (qq, 0.0)), (2, (qq, 0.0)), (3, (qq, 0.0)),
+                               (4, (qq, 0.0))):
             for o in out:
                 o.value = np.nan
             assert L.mspms_term_pair(C.byref(p), which, rsq, x, y, C.byref(out[0]),
@@ -72,7 +73,7 @@ def test_pair_terms_bit_exact(name):
             ref = [o.value for o in out[3:]]
             assert got == ref, (name, which, rsq, got, ref)
             n += 1
-    assert n == 1600
+    assert n == 2000
 
 
 @pytest.mark.parametrize("name", ["bonded_zoo", "butane", "examples_water"])
