@@ -352,10 +352,15 @@ def run_trace(system, steps, skin, shape=None):
         eng.vver_step_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, lt, st, uk)
     torch.cuda.synchronize()
     b0 = eng.list_builds()
-    eng.vver_run_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, steps, lt, st, uk, graph=True)
+    # per-step terms recorded, as in the timed region
+    lh = torch.zeros(steps * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    sh = torch.zeros(steps * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+    kh = torch.zeros(steps, dtype=torch.float64, device=dev)
+    eng.vver_run_device(pos, vel, fl, fs, system.delt, NSTEP_INNER, steps, lt, st, uk, lhist=lh,
+                        shist=sh, ukin_hist=kh, graph=True)
     torch.cuda.synchronize()
     eng.sync()
-    print(json.dumps({"trace": "%d graph-replayed mixed RESPA steps" % steps,
+    print(json.dumps({"trace": "%d graph-replayed mixed RESPA steps, per-step terms" % steps,
                       "list_builds": eng.list_builds() - b0,
                       "launch_shape": {k: tune[k] for k in ("cell_atoms", "waves_per_cluster")}}))
     eng.close()
@@ -556,7 +561,7 @@ def run_tube(systems, steps=20, warmup=3, skin_a=1.0, cpu=True):
     step's terms.  The inner-loop kernel is the grid kernel: its roofline is
     HBM, 304 B per lookup (256 B of node values gathered, position, force)."""
     import torch
-    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine
+    from mspms2_amd.engine import LONG_TERMS_BYTES, SHORT_TERMS_BYTES, Engine, EngineError
     from mspms2_amd import abi
     dev = torch.device("cuda:0")
     nr = len(systems)
@@ -574,25 +579,51 @@ def run_tube(systems, steps=20, warmup=3, skin_a=1.0, cpu=True):
     uk = torch.zeros(nr, dtype=torch.float64, device=dev)
     delt, nin = systems[0].delt, systems[0].run.nstep_inner
     st = torch.cuda.Stream()
-    eng.vver_run_device(pos, vel, fl, fs, delt, nin, warmup, lt, stt, uk, graph=True,
-                        stream=st.cuda_stream)
-    torch.cuda.synchronize()
-    eng.sync()
-    eng.set_profiling(True)
-    for _ in range(steps):
-        eng.vver_step_device(pos, vel, fl, fs, delt, nin, lt, stt, uk, st.cuda_stream)
-    torch.cuda.synchronize()
-    eng.sync()
+
+    def phase(run):
+        """Runs a phase of device steps; after a pair-list overflow
+        (MSPMS_ENOMEM at the sync: the lists are widened and the positions
+        and velocities are back at the start of the step that overflowed)
+        the forces and terms are recomputed there and the phase runs again,
+        as the library's overflow contract asks.  The loadings start from
+        lattices and adsorb onto the wall within a few steps, so the longest
+        pair-list row grows fast at first."""
+        for attempt in range(4):
+            run()
+            torch.cuda.synchronize()
+            try:
+                eng.sync()
+                return attempt
+            except EngineError as e:
+                if "ENOMEM" not in str(e) or attempt == 3:
+                    raise
+                progress(f"  pair-list overflow, lists widened; phase repeated ({e})")
+                eng.frclong_device(pos, fl, lt)
+                eng.frcshort_device(pos, fs, stt)
+                torch.cuda.synchronize()
+                eng.sync()
+
+    phase(lambda: eng.vver_run_device(pos, vel, fl, fs, delt, nin, warmup, lt, stt, uk,
+                                      graph=True, stream=st.cuda_stream))
+
+    def eager():
+        eng.set_profiling(True)
+        for _ in range(steps):
+            eng.vver_step_device(pos, vel, fl, fs, delt, nin, lt, stt, uk, st.cuda_stream)
+
+    phase(eager)
     groups = eng.timings()
     eng.set_profiling(False)
     lh = torch.zeros(steps * nr * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(st)
-    eng.vver_run_device(pos, vel, fl, fs, delt, nin, steps, lt, stt, uk, lhist=lh, graph=True,
-                        stream=st.cuda_stream)
-    b.record(st)
-    torch.cuda.synchronize()
-    eng.sync()
+
+    def timed():
+        a.record(st)
+        eng.vver_run_device(pos, vel, fl, fs, delt, nin, steps, lt, stt, uk, lhist=lh,
+                            graph=True, stream=st.cuda_stream)
+        b.record(st)
+
+    repeats = phase(timed)
     ms = a.elapsed_time(b) / steps
     raw = lt.cpu().numpy().tobytes()
     sz = LONG_TERMS_BYTES
@@ -614,7 +645,8 @@ def run_tube(systems, steps=20, warmup=3, skin_a=1.0, cpu=True):
                            "achieved_GBps": gbytes / (kms * 1e-3) / 1e9 if kms > 0 else None,
                            "frac_of_hbm_peak": (gbytes / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS)
                            if kms > 0 else None},
-           "groups_ms": groups, "finite": finite}
+           "groups_ms": groups, "finite": finite,
+           "timed_region_repeats_after_overflow": repeats}
     if cpu:
         from oracle import oracle as O
         if O.reference_available():
@@ -921,40 +953,56 @@ def main():
     if cpu:
         line["speedup_vs_cpu_reference"] = ns_day / cpu["value"]
     if args.drift_steps > 0:
-        # both precisions from one equilibrated state (NVT first), so that
-        # the drift shows the integrator and the precision, not relaxation
-        # of the lattice start
-        start = None
-        drift = {}
-        progress("energy drift")
-        if args.equil_steps > 0:
-            p_eq, v_eq, t_eq = equilibrate(system, args.equil_steps, skin)
-            start = (p_eq, v_eq)
-            drift["start"] = (f"after {args.equil_steps} RESPA steps "
-                              f"({DT_FS * args.equil_steps / 1000:g} ps) with velocities "
-                              f"rescaled to {system.treq:g} K every 100 steps "
-                              f"(T = {t_eq:.1f} K before the last rescale)")
-        else:
-            drift["start"] = "jittered lattice (not equilibrated)"
-        for p in ("mixed", "fp64"):
-            drift[p] = run_drift(system, p, args.drift_steps, skin, start)
-        line["energy_drift"] = drift
+        side_leg(line, "energy_drift", "energy drift",
+                 lambda: drift_leg(system, args.equil_steps, args.drift_steps, skin))
     if args.replicas > 0:
-        progress("batched HMC replicas")
-        line["replicas"] = run_replica_batch(make_replicas(args.replicas))
+        side_leg(line, "replicas", "batched HMC replicas",
+                 lambda: run_replica_batch(make_replicas(args.replicas)))
     if args.wolf_configs > 0:
-        progress("Wolf fitting set")
-        line["wolf_fit"] = run_wolf_fit(args.wolf_configs, cpu=not args.no_cpu)
+        side_leg(line, "wolf_fit", "Wolf fitting set",
+                 lambda: run_wolf_fit(args.wolf_configs, cpu=not args.no_cpu))
     if args.tube_loadings > 0:
-        progress("methane-in-tube loadings")
-        line["tube"] = run_tube(make_tube(args.tube_loadings), cpu=not args.no_cpu)
+        side_leg(line, "tube", "methane-in-tube loadings",
+                 lambda: run_tube(make_tube(args.tube_loadings), cpu=not args.no_cpu))
     if args.lj_steps > 0:
-        progress("LJ-864 NVT")
-        line["lj864_nvt"] = run_lj_nvt(args.lj_steps, cpu=not args.no_cpu)
+        side_leg(line, "lj864_nvt", "LJ-864 NVT",
+                 lambda: run_lj_nvt(args.lj_steps, cpu=not args.no_cpu))
     if args.sfgrid > 0:
-        progress("grid generators")
-        line["sfgrid"] = run_sfgrid(args.sfgrid, cpu=not args.no_cpu)
+        side_leg(line, "sfgrid", "grid generators",
+                 lambda: run_sfgrid(args.sfgrid, cpu=not args.no_cpu))
     print(json.dumps(line))
+
+
+def drift_leg(system, equil_steps, drift_steps, skin):
+    """Both precisions from one equilibrated state (velocity-rescaled RESPA
+    steps first), so that the drift shows the integrator and the precision,
+    not relaxation of the lattice start."""
+    start = None
+    drift = {}
+    if equil_steps > 0:
+        p_eq, v_eq, t_eq = equilibrate(system, equil_steps, skin)
+        start = (p_eq, v_eq)
+        drift["start"] = (f"after {equil_steps} RESPA steps "
+                          f"({DT_FS * equil_steps / 1000:g} ps) with velocities "
+                          f"rescaled to {system.treq:g} K every 100 steps "
+                          f"(T = {t_eq:.1f} K before the last rescale)")
+    else:
+        drift["start"] = "jittered lattice (not equilibrated)"
+    for p in ("mixed", "fp64"):
+        drift[p] = run_drift(system, p, drift_steps, skin, start)
+    return drift
+
+
+def side_leg(line, key, what, fn):
+    """A measurement reported beside the headline: an error in it is
+    recorded under its key (and on stderr) instead of ending the run, so the
+    headline line is still printed."""
+    progress(what)
+    try:
+        line[key] = fn()
+    except Exception as e:  # noqa: BLE001 -- any failure of a side leg is reported
+        line[key] = {"error": f"{type(e).__name__}: {e}"}
+        progress(f"  {what} failed: {type(e).__name__}: {e}")
 
 
 if __name__ == "__main__":

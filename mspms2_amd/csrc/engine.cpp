@@ -149,7 +149,10 @@ struct mspms_ctx {
     DevBuf<int> d_lstate;
     DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext, d_cl_box;
     DevBuf<float4> d_plist;
+    DevBuf<int> d_atom_gslot;
+    DevBuf<double> d_fslot;
     int cap_hint = 0;
+    int cap_max = 1;  // no row holds more entries than a replica has clusters
     int cap_limit = 0;  // caller's bound on the row capacity (0: none)
 
     // reciprocal
@@ -562,6 +565,8 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_cl_ext.release();
     c->d_cl_box.release();
     c->d_plist.release();
+    c->d_atom_gslot.release();
+    c->d_fslot.release();
     c->d_rows.release();
     c->d_rows_s.release();
     c->d_kvec.release();
@@ -953,6 +958,7 @@ int build(mspms_ctx* c) {
     ClusterBufs& b = c->cb;
     int ncell = cell0, ncol = col0;
     size_t maxcl = (size_t)(c->natom / MSPMS_CL + ncol + 1);
+    c->cap_max = (int)max_rep_cl;
     int cap = (int)std::min<double>((double)max_rep_cl, std::max(max_est, (double)c->cap_hint));
     if (c->cap_limit > 0) cap = std::min(cap, c->cap_limit);
     cap = std::max(cap, 16);
@@ -969,7 +975,8 @@ int build(mspms_ctx* c) {
         (rc = c->d_slot_off.ensure(maxcl * MSPMS_CL)) ||
         (rc = c->d_slot_par.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
         (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_cl_box.ensure(2 * maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
-        (rc = c->d_plist_n.ensure(maxcl)))
+        (rc = c->d_plist_n.ensure(maxcl)) || (rc = c->d_atom_gslot.ensure(c->natom)) ||
+        (rc = c->d_fslot.ensure(3 * maxcl * MSPMS_CL)))
         return rc;
     HIPC(hipMemsetAsync(c->d_cell_count.p, 0, sizeof(int) * (size_t)ncell, s));
     HIPC(hipMemsetAsync(c->d_lstate.p, 0, sizeof(int) * LS_N, s));  // no list yet
@@ -997,6 +1004,8 @@ int build(mspms_ctx* c) {
     b.cl_box = c->d_cl_box.p;
     b.plist = c->d_plist.p;
     b.plist_n = c->d_plist_n.p;
+    b.atom_gslot = c->d_atom_gslot.p;
+    b.fslot = c->d_fslot.p;
     b.maxcl = (int)maxcl;
     b.cap = cap;
     b.ncell = ncell;
@@ -1351,6 +1360,12 @@ void launch_finalize(mspms_ctx* c, const LaunchCtx& L, mspms_long_terms* terms,
 // frec_deferred (optional): with the reciprocal forces on the side stream,
 // leave their addition (and the kick) to the caller's finalize pass and
 // report it
+// the slot-ordered forces of the half-list (mixed) pair kernel, or none
+SlotForces slot_forces(const mspms_ctx* c) {
+    if (!c->cb.half) return SlotForces{};
+    return SlotForces{c->cb.fslot, c->cb.atom_gslot, (size_t)c->cb.maxcl * MSPMS_CL};
+}
+
 int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z, double* fx,
                  double* fy, double* fz, mspms_long_terms* terms, hipStream_t s,
                  double* grad = nullptr, bool finalize = true, const Kick* kick = nullptr,
@@ -1415,6 +1430,9 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
         launch_pair(L, c->cb, grad ? MSPMS_PREC_FP64 : c->precision, c->charge_union, fx, fy,
                     fz, c->d_ep_pair.p, c->d_ip_pair.p, grad ? 1 : hash, gq, ge, gs);
     }
+    // the half-list kernel's slot-ordered forces join the atoms' forces in
+    // the first per-atom pass after it
+    const SlotForces slots = slot_forces(c);
     if (c->any_recip) HIPC(hipStreamWaitEvent(s, c->ev_join, 0));  // join
     // nothing adds long forces after k_intra / k_add_frec
     const bool sf_long = c->any_sf && !c->sf_split;
@@ -1424,16 +1442,20 @@ int enqueue_long(mspms_ctx* c, const double* x, const double* y, const double* z
     const bool defer = recip_side && frec_deferred;
     if (frec_deferred) *frec_deferred = defer;
     if (defer) {
-        if (kicked) *kicked = false;  // the caller's finalize adds frec and kicks
+        // the caller's finalize folds the slot forces, adds frec and kicks
+        if (kicked) *kicked = false;
     } else if (recip_side) {
         // reciprocal forces and intramolecular corrections join the long
         // forces, one thread per atom
         ProfScope p(c, s, PROF_INTRA);
-        launch_add_frec(L, fx, fy, fz, frec, fuse ? kick : nullptr);
-    } else if (c->any_intra) {
-        ProfScope p(c, s, PROF_INTRA);
-        launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p, gq,
-                     ge, gs, nullptr, fuse ? kick : nullptr);
+        launch_add_frec(L, fx, fy, fz, frec, fuse ? kick : nullptr, &slots);
+    } else {
+        if (slots.f) launch_fold_slots(L, slots, fx, fy, fz);
+        if (c->any_intra) {
+            ProfScope p(c, s, PROF_INTRA);
+            launch_intra(L, c->d_species.p, c->d_terms.p, x, y, z, fx, fy, fz, c->d_ep_intra.p,
+                         gq, ge, gs, nullptr, fuse ? kick : nullptr);
+        }
     }
     if (c->any_recip) {
         if (!recip_side) {
@@ -1506,7 +1528,7 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
         // (the side stream's forces and the closing kick,) then frclong,
         // frcshort (last inner step) and kinetic terms in one pass
         ProfScope p(c, s, PROF_FINAL);
-        const FinAtoms A{fl, fl + n, fl + 2 * n, c->d_frec.p, kick, c->natom};
+        const FinAtoms A{fl, fl + n, fl + 2 * n, c->d_frec.p, kick, c->natom, slot_forces(c)};
         launch_finalize(c, L, lterms, sterms, ukin, true, deferred ? &A : nullptr);
     }
     HIPC(hipGetLastError());
@@ -1520,15 +1542,34 @@ int read_status(mspms_ctx* c, hipStream_t s) {
 }
 
 // Headroom of the pair-list rows: when the longest row seen reaches 3/4 of
-// the capacity, the lists are resized to 1.5 times that row, so that later
-// builds of a moving system (local density fluctuations) stay within it.
-// Called after a status read, outside any device run.
+// the capacity, the rows are widened to twice that row, so that later
+// builds of a moving system stay within it (in inhomogeneous systems, such
+// as fluid adsorbing on a wall, the longest row can grow by a third within
+// a few steps).
+// The rows are copied into the wider buffer, so a valid list stays valid and
+// the build count goes on; only the captured graph (which holds the old
+// buffer) is dropped.  Called after a status read, outside any device run.
 int keep_list_headroom(mspms_ctx* c) {
+    ClusterBufs& b = c->cb;
     const int seen = c->h_status->max_list;
-    if (c->cap_limit > 0 || seen * 4 < c->cb.cap * 3) return 0;
-    c->cap_hint = std::max(c->cap_hint, seen + seen / 2 + 16);
+    if (c->cap_limit > 0 || seen * 4 < b.cap * 3) return 0;
+    const int cap = std::min(c->cap_max, 2 * seen + 16);
+    c->cap_hint = std::max(c->cap_hint, cap);
+    if (cap <= b.cap) return 0;
+    DevBuf<float4> wide;
+    int rc = wide.ensure((size_t)b.maxcl * cap);
+    if (rc) return rc;
+    HIPC(hipMemcpy2DAsync(wide.p, (size_t)cap * sizeof(float4), b.plist,
+                          (size_t)b.cap * sizeof(float4), (size_t)b.cap * sizeof(float4),
+                          (size_t)b.maxcl, hipMemcpyDeviceToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    std::swap(c->d_plist.p, wide.p);
+    std::swap(c->d_plist.n, wide.n);
+    wide.release();
     drop_graph(c);
-    return build(c);
+    b.plist = c->d_plist.p;
+    b.cap = cap;
+    return 0;
 }
 
 void accumulate_profile(mspms_ctx* c) {

@@ -106,6 +106,13 @@ struct ClusterBufs {
                          //   Ci - (Cj + image shift) (Ci - Cj for an
                          //   ambiguous image) and the entry bits in .w
     int* plist_n;        // [maxcl]
+    int* atom_gslot;     // [natom] slot of each atom at the last build (-1: not
+                         //   binned, a full ghost)
+    double* fslot;       // [3 * 8*maxcl] pair forces of the half-list kernel in
+                         //   slot order (x, y, z planes): its atomics hit one
+                         //   64-B line per j-cluster and component; zeroed by
+                         //   k_list_check, folded into the atoms' forces
+                         //   after the pair kernel (SlotForces)
     int maxcl, cap, ncell, ncol;
     int half;            // layout this call wants: 1 = each cluster pair in one
                          //   row (mixed pair kernel, j forces by atomics),
@@ -320,12 +327,22 @@ struct Kick {
 // A RESPA step's per-atom closing pass run by its finalize: fx += frec (the
 // side stream's reciprocal and intramolecular forces) and the half kick,
 // before the partials are reduced (also on gated-off steps)
+// The half-list pair forces in slot order, added to the atoms' forces by the
+// first per-atom pass after the pair kernel (fx[a] += f[gslot[a]] per plane).
+struct SlotForces {
+    const double* f;     // null: nothing to fold
+    const int* gslot;
+    size_t nslot;        // plane stride
+};
 struct FinAtoms {
     double *fx, *fy, *fz;
     const double* frec;  // (3, natom)
     Kick K;
     int natom;
+    SlotForces S;
 };
+void launch_fold_slots(const LaunchCtx& c, const SlotForces& S, double* fx, double* fy,
+                       double* fz);
 void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const TermDev* terms, const double* x, const double* y,
                   const double* z, double* fx, double* fy, double* fz,
@@ -334,9 +351,11 @@ void launch_intra(const LaunchCtx& c, const SpeciesDev* species,
                   const Kick* kick = nullptr, bool zero_first = false);
 // (zero_first: each molecule's thread sets its atoms' fx..fz to zero before
 // adding, so fx..fz need no clearing pass)
-// fx += frec ((3, N) reciprocal-space forces) and a capturable zero fill
+// fx += (slot forces, when given) + frec ((3, N) reciprocal-space forces),
+// and a capturable zero fill
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
-                     const double* frec, const struct Kick* kick = nullptr);
+                     const double* frec, const struct Kick* kick = nullptr,
+                     const SlotForces* slots = nullptr);
 void launch_zero(const LaunchCtx& c, double* p, size_t n);
 void launch_bonded(const LaunchCtx& c, const SpeciesDev* species,
                    const TermDev* terms, const double* x, const double* y,

@@ -43,10 +43,16 @@ __global__ void __launch_bounds__(256) k_list_check(
     const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ slot_orig,
     const double4* __restrict__ slot_w64, const double4* __restrict__ cl_center,
     double4* __restrict__ slot_x64, float4* __restrict__ slot_off, float4* __restrict__ cl_box,
-    int* lstate, double half_skin2, int force, int half) {
+    double* __restrict__ fslot, int* lstate, double half_skin2, int force, int half) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = lstate[LS_VALID] != 0 && !force && lstate[LS_LAYOUT] == half;
     bool need = !valid;
+    if (half && t < nslot) {
+        // the half-list kernel's slot-ordered forces start from zero
+        fslot[t] = 0.0;
+        fslot[(size_t)nslot + t] = 0.0;
+        fslot[2 * (size_t)nslot + t] = 0.0;
+    }
     if (t < natom) {
         const bool full_ghost = sites[atom_site[t]].ghost == MSPMS_GHOST_FULL;
         // a full ghost is not binned: its force starts from zero
@@ -122,7 +128,7 @@ __global__ void k_bin(int natom, const double* __restrict__ x, const double* __r
                       int* __restrict__ atom_cell, int* __restrict__ atom_slot,
                       double* __restrict__ atom_wz, int* __restrict__ cell_count,
                       double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-                      Status* st, const int* lstate) {
+                      int* __restrict__ atom_gslot, Status* st, const int* lstate) {
     if (!lstate[LS_NEED]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
@@ -131,6 +137,7 @@ __global__ void k_bin(int natom, const double* __restrict__ x, const double* __r
         // skipped by the pair loops (src/frclong.c:94,107); its force is
         // built from the reciprocal and other per-atom terms only
         atom_cell[i] = -1;
+        atom_gslot[i] = -1;
         fx[i] = 0.0;
         fy[i] = 0.0;
         fz[i] = 0.0;
@@ -256,7 +263,8 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
                         const int* __restrict__ col_cl0, int* __restrict__ slot_orig,
                         int2* __restrict__ slot_info, double4* __restrict__ slot_x64,
                         double4* __restrict__ slot_w64, double4* __restrict__ atom_x0,
-                        const SiteDev* __restrict__ sites, int maxcl, const int* lstate) {
+                        int* __restrict__ atom_gslot, const SiteDev* __restrict__ sites,
+                        int maxcl, const int* lstate) {
     if (!lstate[LS_NEED]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natom) return;
@@ -275,7 +283,11 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
     int local = c - R.cell0;
     int col = R.col0 + local / R.ncz;
     int slot = MSPMS_CL * col_cl0[col] + (a - cell_start[col_cell0[col]]) + rank;
-    if (slot >= MSPMS_CL * maxcl) return;  // overflow flagged by k_scan
+    if (slot >= MSPMS_CL * maxcl) {  // overflow flagged by k_scan
+        atom_gslot[i] = -1;
+        return;
+    }
+    atom_gslot[i] = slot;
     double xi = x[i], yi = y[i], zi = z[i];
     if (!(isfinite(xi) && isfinite(yi) && isfinite(zi))) xi = yi = zi = 0.0;
     slot_orig[slot] = i;
@@ -509,10 +521,11 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     hipLaunchKernelGGL(k_list_check, dim3((std::max(c.natom, nslot) + T - 1) / T), dim3(T), 0,
                        c.stream, c.natom, nslot, x, y, z, c.atom_site, c.sites, fx, fy, fz,
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
-                       b.slot_x64, b.slot_off, b.cl_box, b.lstate, half_skin2, force, b.half);
+                       b.slot_x64, b.slot_off, b.cl_box, b.fslot, b.lstate, half_skin2, force,
+                       b.half);
     hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
-                       b.cell_count, fx, fy, fz, c.status, b.lstate);
+                       b.cell_count, fx, fy, fz, b.atom_gslot, c.status, b.lstate);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, b.ncell, b.ncol, b.cell_count,
                        b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total, b.maxcl,
                        c.status, b.lstate);
@@ -521,7 +534,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
     hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,
                        b.atom_wz, b.cell_start, b.cell_members, c.atom_mol, c.atom_site, c.mols,
                        c.reps, b.col_cell0, b.col_cl0, b.slot_orig, b.slot_info, b.slot_x64,
-                       b.slot_w64, b.atom_x0, c.sites, b.maxcl, b.lstate);
+                       b.slot_w64, b.atom_x0, b.atom_gslot, c.sites, b.maxcl, b.lstate);
     int gB = (b.maxcl * MSPMS_CL + T - 1) / T;
     hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,

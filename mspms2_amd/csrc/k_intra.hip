@@ -134,6 +134,17 @@ __device__ inline bool is_long(const RepDev& R, double rx, double ry, double rz)
     return fabs(rx) > R.L[0] * 0.5 || fabs(ry) > R.L[1] * 0.5 || fabs(rz) > R.L[2] * 0.5;
 }
 
+// an atom's share of the slot-ordered pair forces
+__device__ inline void fold_slot(const SlotForces& S, int i, double& fx, double& fy,
+                                 double& fz) {
+    if (!S.f) return;
+    const int g = S.gslot[i];
+    if (g < 0) return;
+    fx += S.f[g];
+    fy += S.f[S.nslot + g];
+    fz += S.f[2 * S.nslot + g];
+}
+
 // closing half kick of atom i (k_respa_final's arithmetic)
 __device__ inline void kick_atom(const Kick& K, int i, double aw, const double* fx,
                                  const double* fy, const double* fz) {
@@ -352,9 +363,11 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
         const size_t n = (size_t)A.natom;
         for (int i = (blockIdx.x % nb) * FIN_T + threadIdx.x; i < Ra.natom; i += nb * FIN_T) {
             const int a = Ra.atom0 + i;
-            A.fx[a] += A.frec[a];
-            A.fy[a] += A.frec[n + a];
-            A.fz[a] += A.frec[2 * n + a];
+            double ax = A.fx[a], ay = A.fy[a], az = A.fz[a];
+            fold_slot(A.S, a, ax, ay, az);
+            A.fx[a] = ax + A.frec[a];
+            A.fy[a] = ay + A.frec[n + a];
+            A.fz[a] = az + A.frec[2 * n + a];
             if (A.K.vx) kick_atom(A.K, a, sites[atom_site[a]].aw, A.fx, A.fy, A.fz);
         }
     }
@@ -798,13 +811,28 @@ __global__ void __launch_bounds__(256) k_add_frec(int n, double* __restrict__ fx
                                                   double* __restrict__ fz,
                                                   const double* __restrict__ frec,
                                                   const int* __restrict__ atom_site,
-                                                  const SiteDev* __restrict__ sites, Kick K) {
+                                                  const SiteDev* __restrict__ sites, Kick K,
+                                                  SlotForces S) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    fx[i] += frec[i];
-    fy[i] += frec[(size_t)n + i];
-    fz[i] += frec[2 * (size_t)n + i];
+    double ax = fx[i], ay = fy[i], az = fz[i];
+    fold_slot(S, i, ax, ay, az);
+    fx[i] = ax + frec[i];
+    fy[i] = ay + frec[(size_t)n + i];
+    fz[i] = az + frec[2 * (size_t)n + i];
     if (K.vx) kick_atom(K, i, sites[atom_site[i]].aw, fx, fy, fz);
+}
+
+__global__ void __launch_bounds__(256) k_fold_slots(int n, double* __restrict__ fx,
+                                                    double* __restrict__ fy,
+                                                    double* __restrict__ fz, SlotForces S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ax = fx[i], ay = fy[i], az = fz[i];
+    fold_slot(S, i, ax, ay, az);
+    fx[i] = ax;
+    fy[i] = ay;
+    fz[i] = az;
 }
 
 __global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) {
@@ -813,10 +841,17 @@ __global__ void __launch_bounds__(256) k_zero(size_t n, double* __restrict__ p) 
 }
 
 void launch_add_frec(const LaunchCtx& c, double* fx, double* fy, double* fz,
-                     const double* frec, const Kick* kick) {
+                     const double* frec, const Kick* kick, const SlotForces* slots) {
     const Kick K = kick ? *kick : Kick{};
+    const SlotForces S = slots ? *slots : SlotForces{};
     hipLaunchKernelGGL(k_add_frec, dim3((c.natom + 255) / 256), dim3(256), 0, c.stream, c.natom,
-                       fx, fy, fz, frec, c.atom_site, c.sites, K);
+                       fx, fy, fz, frec, c.atom_site, c.sites, K, S);
+}
+
+void launch_fold_slots(const LaunchCtx& c, const SlotForces& S, double* fx, double* fy,
+                       double* fz) {
+    hipLaunchKernelGGL(k_fold_slots, dim3((c.natom + 255) / 256), dim3(256), 0, c.stream,
+                       c.natom, fx, fy, fz, S);
 }
 
 // zero fill by a kernel (capturable; no memset nodes in the step graph)

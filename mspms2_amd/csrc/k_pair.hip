@@ -249,9 +249,12 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
 // HALF: each cluster pair sits in one row only (the self pair in its own,
 // where lanes with j > i take each site pair once).  A lane's pair force
 // goes to i as before and, negated, to j: per entry the eight i-lanes of
-// each j are summed with shuffles and added to j's force with an FP64
-// atomic; i's total is added atomically too (k_list_check zeroed both).
-// Energies and virial are then doubled, since the finalize halves them.
+// each j are summed with shuffles and added with an FP64 atomic to j's slot
+// in the slot-ordered force planes (one 64-B line per j-cluster and
+// component); i's total is added atomically too.  k_list_check zeroed the
+// planes; the first per-atom pass after this kernel folds them into the
+// atoms' forces.  Energies and virial are doubled, since the finalize
+// halves them.
 // HASH: the pair hashes are kept (a compile-time switch: their registers
 // would otherwise cost the kernel a wave per SIMD)
 template <int ELEC, int WPC, bool HALF, bool HASH>
@@ -261,7 +264,8 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     const float4* __restrict__ plist, const int* __restrict__ plist_n, int cap,
     const int* __restrict__ slot_orig, const float4* __restrict__ slot_off,
     const float4* __restrict__ slot_par, double* __restrict__ fx, double* __restrict__ fy,
-    double* __restrict__ fz, double* __restrict__ epart, unsigned long long* __restrict__ ipart) {
+    double* __restrict__ fz, double* __restrict__ fslot, int nslot,
+    double* __restrict__ epart, unsigned long long* __restrict__ ipart) {
     constexpr bool want_hash = HASH;
     __shared__ double sh_f[WPC][MSPMS_CL][3];
     __shared__ double sh_e[WPC][4];
@@ -374,7 +378,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             const int cn = (int)(__float_as_uint(J.sep.w) >> 7);
             J.pj = slot_off[cn * MSPMS_CL + jsub];
             J.qj = slot_par[cn * MSPMS_CL + jsub];
-            J.oj = HALF ? slot_orig[cn * MSPMS_CL + jsub] : 0;
+            J.oj = HALF && HASH ? slot_orig[cn * MSPMS_CL + jsub] : 0;
         };
         auto process = [&](const JSet& J) {
             if (it > 0 && it % FLUSH == 0) flush();  // uniform over the wave
@@ -382,7 +386,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             const float4 sep = J.sep;
             const uint32_t ent = __float_as_uint(sep.w);
             const float4 pj4 = J.pj, qj4 = J.qj;
-            const int ojs = J.oj;  // j's atom (HALF)
+            const int ojs = J.oj;  // j's atom (HALF and HASH)
             const int cj = (int)(ent >> 7);
             const int jslot = cj * MSPMS_CL + jsub;
             const int flj = __float_as_int(qj4.w);
@@ -487,10 +491,12 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                     v[k] += dpp_f32<0x4E>(v[k]);   // quad_perm [2,3,0,1]
                     v[k] += dpp_f32<0x141>(v[k]);  // row_half_mirror
                 }
+                // into the slot-ordered forces: the eight j-lanes of a
+                // component write one 64-B line
                 if (isub == 0 && (flj & SLOT_VALID) && (v[0] != 0.f || v[1] != 0.f || v[2] != 0.f)) {
-                    atomicAdd(&fx[ojs], -(double)v[0]);
-                    atomicAdd(&fy[ojs], -(double)v[1]);
-                    atomicAdd(&fz[ojs], -(double)v[2]);
+                    atomicAdd(&fslot[jslot], -(double)v[0]);
+                    atomicAdd(&fslot[(size_t)nslot + jslot], -(double)v[1]);
+                    atomicAdd(&fslot[2 * (size_t)nslot + jslot], -(double)v[2]);
                 }
             }
         };
@@ -547,12 +553,12 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             double g[3] = {0.0, 0.0, 0.0};
             for (int ww = 0; ww < WPC; ww++)
                 for (int d = 0; d < 3; d++) g[d] += sh_f[ww][t][d];
-            const int o = slot_orig[sl];
             if (HALF) {
-                atomicAdd(&fx[o], g[0]);
-                atomicAdd(&fy[o], g[1]);
-                atomicAdd(&fz[o], g[2]);
+                atomicAdd(&fslot[sl], g[0]);
+                atomicAdd(&fslot[(size_t)nslot + sl], g[1]);
+                atomicAdd(&fslot[2 * (size_t)nslot + sl], g[2]);
             } else {
+                const int o = slot_orig[sl];
                 fx[o] = g[0];
                 fy[o] = g[1];
                 fz[o] = g[2];
@@ -589,7 +595,7 @@ void launch_mx_k(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* f
     hipLaunchKernelGGL((k_pair_mx<ELEC, WPC, HALF, HASH>), dim3((b.maxcl + 7) / 8 * 8),
                        dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, b.cl_center,
                        b.cl_box, b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_off, b.slot_par,
-                       fx, fy, fz, epart, ipart);
+                       fx, fy, fz, b.fslot, b.maxcl * MSPMS_CL, epart, ipart);
 }
 
 template <int ELEC, int WPC>
