@@ -150,6 +150,7 @@ struct mspms_ctx {
     DevBuf<float4> d_slot_off, d_slot_par, d_cl_ext, d_cl_box;
     DevBuf<float4> d_plist;
     DevBuf<int> d_atom_gslot;
+    DevBuf<double2> d_eatom;
     DevBuf<double> d_fslot;
     int cap_hint = 0;
     int cap_max = 1;  // no row holds more entries than a replica has clusters
@@ -566,6 +567,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_cl_box.release();
     c->d_plist.release();
     c->d_atom_gslot.release();
+    c->d_eatom.release();
     c->d_fslot.release();
     c->d_rows.release();
     c->d_rows_s.release();
@@ -745,7 +747,7 @@ int build(mspms_ctx* c) {
     std::vector<double> kvec;
     std::vector<int> k_rep, rep_chunk0(1, 0);
     std::vector<int4> work_s, work_f;
-    int ntab_max = 1, nkz_max = 1, nk_stride = 1;
+    int ntab_max = 1, nkz_max = 1, nk_stride = 1, nkrow_max = 1;
     size_t max_rep_cl = 1;
     double max_est = 0.0;
     for (int r = 0; r < c->nrep; r++) {
@@ -897,6 +899,7 @@ int build(mspms_ctx* c) {
             nk_stride = std::max(nk_stride, R.nk);
             ntab_max = std::max(ntab_max, P.kmaxx + P.kmaxy + std::max(P.kmaxz, 1) + 3);
             nkz_max = std::max(nkz_max, 2 * P.kmaxz + 1);
+            nkrow_max = std::max(nkrow_max, R.nkrow);
             // S pass: rows sorted by length; one workgroup per SK_CHUNK-atom
             // chunk (the kernel's tables hold up to 64).  Smaller chunks give
             // more waves but take more of the GPU from the pair kernel that
@@ -1021,7 +1024,8 @@ int build(mspms_ctx* c) {
             (rc = c->d_work_s.upload(work_s, s)) || (rc = c->d_work_f.upload(work_f, s)) ||
             (rc = c->d_rep_chunk0.upload(rep_chunk0, s)) ||
             (rc = c->d_s_part.ensure((size_t)nchunk * nk_stride)) ||
-            (rc = c->d_ak.ensure(kvec.size())) || (rc = c->d_uk.ensure(kvec.size())))
+            (rc = c->d_ak.ensure(kvec.size())) || (rc = c->d_uk.ensure(kvec.size())) ||
+            (rc = c->d_eatom.ensure(3 * (size_t)c->natom)))
             return rc;
         R.rows = c->d_rows.p;
         R.nrow = (int)rows.size();
@@ -1040,6 +1044,8 @@ int build(mspms_ctx* c) {
         R.nk_total = (int)kvec.size();
         R.ntab_max = ntab_max;
         R.nkz_max = nkz_max;
+        R.nkrow_max = nkrow_max;
+        R.eatom = c->d_eatom.p;
     }
     // ---- solid-fluid tables (replica 0's model, shared) ----
     memset(&c->sf, 0, sizeof c->sf);

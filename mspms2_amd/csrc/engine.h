@@ -205,6 +205,9 @@ struct RecipBufs {
     double* uk;          // [nk_total] kvec |S|^2
     int nk_total, ntab_max, nkz_max;
     int nrow;            // rows of all replicas
+    int nkrow_max;       // most rows of one replica
+    double2* eatom;      // [3][natom] exp(i 2 pi x_d / L_d) per atom and axis,
+                         //   written by k_sk, read by k_kforce
 };
 
 // solid-fluid model (shared by all replicas), passed by value to k_sf

@@ -7,10 +7,11 @@
 // exp(i k.r) = ex^kx ey^ky ez^kz with e = exp(i 2 pi x / L): three sincos
 // per atom, then complex multiplies (|k| <= KMAX steps, error ~KMAX ulp).
 //
-//   k_sk     one workgroup per (replica, atom chunk of <= 64); a lane owns one
-//            (kx, ky) row and accumulates S for all its kz in registers
-//            over the chunk; per-atom power tables live in LDS.  Partial
-//            S per chunk is written out (no atomics).
+//   k_sk     one workgroup per (replica, atom chunk of <= 64); a lane owns a
+//            segment of SEG consecutive kz of one (kx, ky) row and
+//            accumulates their S in registers over the chunk; per-atom
+//            power tables live in LDS.  Partial S per chunk is written out
+//            (no atomics).
 //   k_sred   per k: deterministic sum over chunks, U_k, and the force
 //            coefficients A_k, B_k = 2 Vf C kvec (S_r, S_i).
 //   k_kforce one lane per atom over a range of rows; exp(i k.r) by
@@ -42,9 +43,15 @@ __device__ inline cplx cpow_signed(cplx b, int n) {
     return n < 0 ? conjc(r) : r;
 }
 
-// tab layout: [atom][n], n over x powers 0..kmx, y 0..kmy, z 0..kmz
-template <int NKZ>
-__global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
+// tab layout: [atom][n], n over x powers 0..kmx, y 0..kmy, z 0..kmz.
+// A lane takes item (row r, segment g) with r fastest, so that a wave's
+// lanes share a segment index over rows sorted by length; the segment
+// starts at kz0 = kz_lo + g SEG, whose power comes from the z table.  A
+// lane's serial chain is (SEG + 2 complex products) per atom, against
+// (row length + 2) with one lane per row: there are only ~100 rows at
+// kmax 8, so segments are what fills the waves.
+template <int SEG>
+__global__ void __launch_bounds__(1024) k_sk(const int4* __restrict__ work,
                                             const RepDev* __restrict__ reps,
                                             const KRow* __restrict__ rows_s,
                                             const double* __restrict__ x,
@@ -52,7 +59,8 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
                                             const double* __restrict__ z,
                                             const int* __restrict__ atom_site,
                                             const SiteDev* __restrict__ sites,
-                                            double2* __restrict__ s_part, int nk_stride) {
+                                            double2* __restrict__ s_part, int nk_stride,
+                                            double2* __restrict__ eatom, int natom) {
     extern __shared__ double2 tab[];
     __shared__ double qsh[64];
     const int4 w = work[blockIdx.x];
@@ -70,6 +78,7 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
             const double* p = d == 0 ? x : (d == 1 ? y : z);
             double s, c;
             sincos(R.twopi_l[d] * p[a0 + a], &s, &c);
+            eatom[(size_t)d * natom + a0 + a] = make_double2(c, s);  // for k_kforce
             cplx e = {c, s}, v = {1.0, 0.0};
             for (int n = 0; n <= km; n++) {
                 row[n] = make_double2(v.re, v.im);
@@ -82,14 +91,18 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
         }
     }
     __syncthreads();
-    for (int r = threadIdx.x; r < R.nkrow; r += blockDim.x) {
+    const int nseg = (2 * kmz + 1 + SEG - 1) / SEG;  // a row holds at most 2 kmz + 1 kz
+    for (int it = threadIdx.x; it < R.nkrow * nseg; it += blockDim.x) {
+        const int r = it % R.nkrow, g = it / R.nkrow;
         const KRow K = rows_s[R.krow0 + r];
-        const int nk = K.nk;
-        double sr[NKZ], si[NKZ];
+        const int nk = min(SEG, K.nk - g * SEG);
+        if (nk <= 0) continue;
+        const int kz0 = K.kz_lo + g * SEG;
+        double sr[SEG], si[SEG];
 #pragma unroll
-        for (int t = 0; t < NKZ; t++) sr[t] = si[t] = 0.0;
+        for (int t = 0; t < SEG; t++) sr[t] = si[t] = 0.0;
         const int ax = K.kx < 0 ? -K.kx : K.kx, ay = K.ky < 0 ? -K.ky : K.ky;
-        const int alo = K.kz_lo < 0 ? -K.kz_lo : K.kz_lo;
+        const int alo = kz0 < 0 ? -kz0 : kz0;
         int a = 0;
         // four atoms at a time: four independent kz recurrences in flight
         for (; a + 4 <= na; a += 4) {
@@ -102,13 +115,13 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
                               t1 = ta[kmx + kmy + 2 + 1];
                 const cplx cx = {tx.x, K.kx < 0 ? -tx.y : tx.y};
                 const cplx cy = {ty.x, K.ky < 0 ? -ty.y : ty.y};
-                const cplx cz = {tz.x, K.kz_lo < 0 ? -tz.y : tz.y};
+                const cplx cz = {tz.x, kz0 < 0 ? -tz.y : tz.y};
                 s4[u] = {t1.x, t1.y};
                 c4[u] = cmul(cmul(cx, cy), cz);
                 q4[u] = qsh[a + u];
             }
 #pragma unroll
-            for (int t = 0; t < NKZ; t++) {
+            for (int t = 0; t < SEG; t++) {
                 if (t < nk) {
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
@@ -125,12 +138,12 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
                     t1 = ta[kmx + kmy + 2 + 1];
             cplx cx = {tx.x, K.kx < 0 ? -tx.y : tx.y};
             cplx cy = {ty.x, K.ky < 0 ? -ty.y : ty.y};
-            cplx cz = {tz.x, K.kz_lo < 0 ? -tz.y : tz.y};
+            cplx cz = {tz.x, kz0 < 0 ? -tz.y : tz.y};
             const cplx step = {t1.x, t1.y};
             cplx c = cmul(cmul(cx, cy), cz);
             const double q = qsh[a];
 #pragma unroll
-            for (int t = 0; t < NKZ; t++) {
+            for (int t = 0; t < SEG; t++) {
                 if (t < nk) {
                     sr[t] += q * c.re;
                     si[t] += q * c.im;
@@ -138,9 +151,9 @@ __global__ void __launch_bounds__(256) k_sk(const int4* __restrict__ work,
                 }
             }
         }
-        double2* out = s_part + (size_t)chunk * nk_stride + (K.k0 - R.k0);
+        double2* out = s_part + (size_t)chunk * nk_stride + (K.k0 - R.k0) + g * SEG;
 #pragma unroll
-        for (int t = 0; t < NKZ; t++)
+        for (int t = 0; t < SEG; t++)
             if (t < nk) out[t] = make_double2(sr[t], si[t]);
     }
 }
@@ -195,21 +208,20 @@ __global__ void __launch_bounds__(64) k_kforce(const int4* __restrict__ work,
                                                const double* __restrict__ z,
                                                const int* __restrict__ atom_site,
                                                const SiteDev* __restrict__ sites, double* fx,
-                                               double* fy, double* fz, double* gq) {
+                                               double* fy, double* fz, double* gq,
+                                               const double2* __restrict__ eatom, int natom) {
     const int4 w = work[blockIdx.x];
     const RepDev& R = reps[w.x];
     const int a = w.y + threadIdx.x;
     const bool live = a < R.atom0 + R.natom;
     const int ai = live ? a : R.atom0;
+    // exp(i 2 pi x / L) per axis, computed once per atom by k_sk (the same
+    // sincos), instead of once per (atom, row group)
     cplx e[3];
-    {
-        double s, c;
-        sincos(R.twopi_l[0] * x[ai], &s, &c);
-        e[0] = {c, s};
-        sincos(R.twopi_l[1] * y[ai], &s, &c);
-        e[1] = {c, s};
-        sincos(R.twopi_l[2] * z[ai], &s, &c);
-        e[2] = {c, s};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double2 v = eatom[(size_t)d * natom + ai];
+        e[d] = {v.x, v.y};
     }
     cplx zp[NZ];
     zp[0] = {1.0, 0.0};
@@ -276,21 +288,16 @@ int recip_nkz_supported(int kmaxz) { return kmaxz <= 16; }
 
 void launch_recip_s(const LaunchCtx& c, const RecipBufs& rb, const double* x, const double* y,
                     const double* z) {
-    // threads per chunk: one per (kx, ky) row of the half space (about 80
-    // rows for kmax 8 / KSQMAX 64), rows looped past it
-    constexpr int SK_T = 128;
+    // threads per chunk: one per (row, kz segment) item of the replica with
+    // the most (99 rows x 4 segments of 5 at kmax 8 / KSQMAX 64), items
+    // looped past 1024
+    constexpr int SEG = 5;
+    const int items = rb.nkrow_max * ((rb.nkz_max + SEG - 1) / SEG);
+    const int sk_t = std::min(1024, std::max(64, (items + 63) / 64 * 64));
     size_t lds = (size_t)64 * rb.ntab_max * sizeof(double2);
-    if (rb.nkz_max <= 9)
-        hipLaunchKernelGGL(k_sk<9>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s, c.reps,
-                           rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part, rb.nk_stride);
-    else if (rb.nkz_max <= 17)
-        hipLaunchKernelGGL(k_sk<17>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s,
-                           c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
-                           rb.nk_stride);
-    else
-        hipLaunchKernelGGL(k_sk<33>, dim3(rb.nwork_s), dim3(SK_T), lds, c.stream, rb.work_s,
-                           c.reps, rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part,
-                           rb.nk_stride);
+    hipLaunchKernelGGL(k_sk<SEG>, dim3(rb.nwork_s), dim3(sk_t), lds, c.stream, rb.work_s, c.reps,
+                       rb.rows_s, x, y, z, c.atom_site, c.sites, rb.s_part, rb.nk_stride,
+                       rb.eatom, c.natom);
     hipLaunchKernelGGL(k_sred, dim3((rb.nk_total + 7) / 8), dim3(256), 0, c.stream,
                        rb.nk_total, rb.k_rep, c.reps, rb.rep_chunk0, rb.s_part, rb.nk_stride,
                        rb.kvec, rb.ak, rb.uk);
@@ -301,7 +308,8 @@ void launch_recip_f(const LaunchCtx& c, const RecipBufs& rb, const double* x, co
     // z powers kept per lane: |kz| <= kmaxz, nkz_max = 2 kmaxz + 1
 #define KFORCE(G, NZ)                                                                          \
     hipLaunchKernelGGL((k_kforce<G, NZ>), dim3(rb.nwork_f), dim3(64), 0, c.stream, rb.work_f,   \
-                       c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz, gq)
+                       c.reps, rb.rows, rb.ak, x, y, z, c.atom_site, c.sites, fx, fy, fz, gq,  \
+                       rb.eatom, c.natom)
     if (gq) {
         if (rb.nkz_max <= 9)
             KFORCE(true, 5);
