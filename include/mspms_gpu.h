@@ -134,13 +134,19 @@ typedef struct mspms_species {
  * F, dF/dx, dF/dy, dF/dz, d2F/dxdy, d2F/dxdz, d2F/dydz, d3F/dxdydz, in
  * Kelvin and Angstrom (the Fortran's units; the engine converts as
  * src/sffrc.c:1322-1337 does).  Grid spans [x0, x0+(nx-1)*hx] etc.; z is
- * periodic with period (nz-1)*hz, x and y are not (out of range = EDOM). */
+ * periodic with period zperiod (0: (nz-1)*hz; at most (nz-1)*hz), x and y
+ * are not (out of range = EDOM).  A grid read from the Fortran library's
+ * files has nx = ngx+2, ny = ngy+2, nz = ngz+2 nodes from the origin and
+ * zperiod = the unit cell's c = ngz*hz (mylibtasos/initialize.f90:96-101,
+ * pbc.f90:82-86); the minimum node F, which sets the PE sign flip of
+ * forces.f90:166-171, is taken over all nodes as grid.f90:102 does. */
 typedef struct mspms_sf_grid {
     int ntype;                 /* number of interp types                  */
     int nx, ny, nz;
     double x0, y0, z0;         /* Angstrom                                */
     double hx, hy, hz;         /* Angstrom                                */
     const float* nodes;        /* [ntype][8][nz][ny][nx] FP32             */
+    double zperiod;            /* Angstrom; 0: (nz-1)*hz                  */
 } mspms_sf_grid;
 
 /* Cube polynomials of the "myinterp" model, the data init_my_interp reads
@@ -362,6 +368,15 @@ int mspms_term_sf(const mspms_params* p, const mspms_species* sp, int site, doub
  * `type` (the reference's decremented interp type, 0-based) of p->cubes at
  * reduced (x, y, z); *usf in J/mol and fsf[3] in J/mol/Angstrom, the
  * reference's raw units.  A cube outside the tables: MSPMS_EDOM. */
+/* call_tasos_forces (src/sffrc.c:1322-1337) over CForce_atom
+ * (mylibtasos/forces.f90:6-173): the tricubic Hermite grid p->grid of table
+ * `type` (the site's interp_type, 1-based) at reduced (x, y, z); *u and
+ * force3[3] in reduced units, as call_tasos_forces returns them (PE with the
+ * sign flip below 1.5 times the minimum node F).  A position outside the
+ * grid or a type without a table: MSPMS_EDOM.  The node table is converted
+ * once per grid and kept. */
+int mspms_term_hermite(const mspms_params* p, int type, double x, double y, double z,
+                       double* u, double* force3);
 int mspms_term_myinterp(const mspms_params* p, double x, double y, double z, int type,
                         double* usf, double* fsf);
 

@@ -54,7 +54,7 @@ class SFGrid(C.Structure):
         ("ntype", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
         ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
-        ("nodes", _fp),
+        ("nodes", _fp), ("zperiod", C.c_double),
     ]
 
 

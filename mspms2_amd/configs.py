@@ -685,3 +685,49 @@ def tube_loadings(n: int = 16, nmin: int = 20000, nmax: int = 100000, seed0: int
     """n loadings N_i = nmin + i (nmax - nmin) / (n - 1), i = 0..n-1."""
     return [tube_loading(int(round(nmin + i * (nmax - nmin) / (n - 1))), seed=seed0 + i)
             for i in range(n)]
+
+
+def tasos_column(nsite: int = 8, auc: float = 4.0, cuc: float = 1.2, length: float = 48.0,
+                 box_xy: float = 30.0, rc: float = 10.0, n_dimer: int = 3, amp: float = 20.0,
+                 seed: int = 41) -> RunDir:
+    """Methanes (3.73 A, 148 K) and CH3-ghost dimers in a column along z,
+    with the solid-fluid field given as the Fortran library's Hermite grid
+    files (iSF_type 3, the TASOS section): a square unit cell of side auc
+    and height cuc, sampled by InitPotentialGrid's 0.2 A grid
+    (mylibtasos/initialize.f90:96-101), whose field is the smooth tube of
+    radius 6 A around the column's axis plus amp * cos(2 pi z / cuc), so
+    that the lookup's z wrap and z derivatives take part.  The sites stay
+    within 1.2 A of the axis, inside the grid's x-y range."""
+    from . import sfgrid
+    rng = np.random.default_rng(seed)
+    (ngx, ngy, ngz), (hx, hy, hz) = sfgrid.tasos_dims(auc, auc, cuc)
+    nx, ny, nz = ngx + 2, ngy + 2, ngz + 2
+    centre = 0.5 * auc
+    g = sfgrid.tube_grid([(3.73, 148.0)], R=6.0, x0=0.0, y0=0.0, nx=nx, ny=ny, hx=hx, hy=hy,
+                         nz=nz, hz=hz, sigma_s=3.37, eps_s=27.68, center=(centre, centre))
+    tab = np.asarray(g["nodes"][0], dtype=np.float64)
+    zs = hz * np.arange(nz)
+    w = 2.0 * np.pi / cuc
+    tab[0] += amp * np.cos(w * zs)[:, None, None]
+    tab[3] += -amp * w * np.sin(w * zs)[:, None, None]
+    ntot = nsite + n_dimer
+    zk = (np.arange(ntot) + 0.5) * (length / ntot)
+    xy = centre + rng.uniform(-1.2, 1.2, size=(ntot, 2))
+    pts = np.column_stack([xy, zk])
+    order = rng.permutation(ntot)
+    meth = pts[order[:nsite]]
+    dim = pts[order[nsite:]]
+    ghost = dim + np.array([0.0, 0.0, 1.0])
+    xyz = np.concatenate([meth, np.stack([dim, ghost], axis=1).reshape(-1, 3)])
+    run = RunInput(title="column on a Hermite grid (TASOS files)", sigma_base=1.0,
+                   epsilon_base=1.0, mass_base=1.0, treq=148.0, boxlx=box_xy, boxly=box_xy,
+                   boxlz=length, rcuton=rc, rcutoff=rc, rcutoffelec=rc, isLJlrcOn=0, delt=2.0,
+                   nstep_inner=4, iChargeType=abi.ELEC_NONE, iSF_type=abi.SF_TASOS,
+                   tasos_uc=(auc, auc, cuc), tasos_radius=6.0,
+                   tasos_ncell=(1, 1, int(round(length / cuc))), tasos_ngrid=1,
+                   tasos_files=["ch4.grid"])
+    sp = SpeciesSpec(name="methane", nmole=nsite, sites=[
+        Site(type=0, name="C", xx=0.0, yy=0.0, zz=0.0, aw=0.016043, epsilon=148.0,
+             sigma=3.73, charge=0.0, ghost_type=0, interp_type=1)])
+    return RunDir(run=run, sysname="tasos-column", species=[sp, ghost_diatomic_species(n_dimer)],
+                  xyz=xyz, grid_files={"ch4.grid": tab})

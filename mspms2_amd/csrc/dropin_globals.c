@@ -10,6 +10,8 @@
  * limits :18-23) is restated in ref_sample_molecule; a test compares its
  * field offsets with the reference's compiled layout.
  */
+#include <ctype.h>
+#include <math.h>
 #include <stdbool.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -67,6 +69,116 @@ static mspms_sf_cubes g_cubes;
  * reference's copy in place (src/sffrc.c:249-257) */
 static int g_interp[MAXSP][REF_SAMPLE_NATOM_MAX];
 
+/* ---- the Fortran library's Hermite grids (iSF_type 3) ---------------------
+ * init_tasos_grid (src/sffrc.c:149-221) hands the TASOS section of in.mspms
+ * to mylibtasos, which sizes the grid (InitPotentialGrid,
+ * mylibtasos/initialize.f90:96-101: USESYMMETRY off, its compiled default,
+ * global.f90:235; grid_spacing 0.2 A, global.f90:214) and reads one file of
+ * list-directed records per grid (Read_Grids, grid.f90:54-129).  That state
+ * lives in Fortran module variables C cannot reach, so the drop-in reads the
+ * same section and files itself, once, from the run directory. */
+static mspms_sf_grid g_grid;
+static float* g_grid_nodes;
+
+static int fortran_anint(double v) { return (int)(floor(fabs(v) + 0.5) * (v >= 0 ? 1 : -1)); }
+
+static void grid_fail(const char* what, const char* name)
+{
+    fprintf(stderr, "mspms gpu: TASOS grid: %s%s%s\n", what, name ? " " : "", name ? name : "");
+    exit(1);
+}
+
+/* the whole file, with Fortran D exponents and comma separators made
+ * readable for strtod */
+static char* slurp(const char* path)
+{
+    FILE* f = fopen(path, "rb");
+    long n;
+    char* buf;
+    size_t k;
+    if (!f) grid_fail("cannot open", path);
+    if (fseek(f, 0, SEEK_END) != 0 || (n = ftell(f)) < 0 || fseek(f, 0, SEEK_SET) != 0)
+        grid_fail("cannot size", path);
+    buf = (char*)malloc((size_t)n + 1);
+    if (!buf || fread(buf, 1, (size_t)n, f) != (size_t)n) grid_fail("cannot read", path);
+    fclose(f);
+    buf[n] = 0;
+    for (k = 0; k < (size_t)n; k++) {
+        if (buf[k] == 'D' || buf[k] == 'd') buf[k] = 'E';
+        else if (buf[k] == ',') buf[k] = ' ';
+    }
+    return buf;
+}
+
+static void read_tasos_grids(void)
+{
+    char line[4096], key[128], names[12][200];
+    double auc, buc, cuc, radius, h[3];
+    int na, nb, nc, nspecies, tnuatoms, ng[3], t, k, found = 0;
+    size_t nn;
+    FILE* f = fopen("in.mspms", "r"); /* INPUT, src/defs.h:4 */
+    if (!f) grid_fail("cannot open", "in.mspms");
+    while (fgets(line, sizeof line, f)) {
+        if (sscanf(line, "%127s", key) != 1) continue;
+        for (k = 0; key[k]; k++) key[k] = (char)toupper((unsigned char)key[k]);
+        if (!strcmp(key, "TASOS")) {
+            found = 1;
+            break;
+        }
+    }
+    if (!found) grid_fail("no TASOS section in", "in.mspms");
+    if (!fgets(line, sizeof line, f) ||
+        sscanf(line, "%lf %lf %lf %lf", &auc, &buc, &cuc, &radius) != 4 ||
+        !fgets(line, sizeof line, f) || sscanf(line, "%d %d %d", &na, &nb, &nc) != 3 ||
+        !fgets(line, sizeof line, f) || sscanf(line, "%d", &nspecies) != 1 ||
+        !fgets(line, sizeof line, f) || sscanf(line, "%d", &tnuatoms) != 1 || tnuatoms < 1 ||
+        tnuatoms > 12 || nspecies < 1 || nspecies > tnuatoms)
+        grid_fail("malformed TASOS section in", "in.mspms");
+    for (t = 0; t < tnuatoms; t++)
+        if (!fgets(line, sizeof line, f) || sscanf(line, "%199s", names[t]) != 1)
+            grid_fail("missing grid file name in", "in.mspms");
+    fclose(f);
+    (void)radius;
+    (void)na;
+    (void)nb;
+    (void)nc;
+    ng[0] = fortran_anint(auc / 0.2);
+    ng[1] = fortran_anint(buc / 0.2);
+    ng[2] = fortran_anint(cuc / 0.2);
+    if (ng[0] < 1 || ng[1] < 1 || ng[2] < 1) grid_fail("unit cell smaller than one grid step", 0);
+    h[0] = auc / ng[0];
+    h[1] = buc / ng[1];
+    h[2] = cuc / ng[2];
+    nn = (size_t)(ng[0] + 2) * (ng[1] + 2) * (ng[2] + 2);
+    g_grid_nodes = (float*)malloc(sizeof(float) * 8 * nn * (size_t)nspecies);
+    if (!g_grid_nodes) grid_fail("out of memory", 0);
+    for (t = 0; t < nspecies; t++) {
+        char* text = slurp(names[t]);
+        char* s = text;
+        size_t node;
+        /* one record of 8 values per node, x fastest (grid.f90:76-99) */
+        for (node = 0; node < nn; node++)
+            for (k = 0; k < 8; k++) {
+                char* e;
+                double v = strtod(s, &e);
+                if (e == s) grid_fail("too few node records in", names[t]);
+                s = e;
+                g_grid_nodes[((size_t)t * 8 + k) * nn + node] = (float)v;
+            }
+        free(text);
+    }
+    g_grid.ntype = nspecies;
+    g_grid.nx = ng[0] + 2;
+    g_grid.ny = ng[1] + 2;
+    g_grid.nz = ng[2] + 2;
+    g_grid.x0 = g_grid.y0 = g_grid.z0 = 0.0;
+    g_grid.hx = h[0];
+    g_grid.hy = h[1];
+    g_grid.hz = h[2];
+    g_grid.nodes = g_grid_nodes;
+    g_grid.zperiod = cuc; /* PBCType 2 wraps z by the cell's c (pbc.f90:82-86) */
+}
+
 void mspms_dropin_read_params(mspms_params* p)
 {
     memset(p, 0, sizeof *p);
@@ -122,6 +234,9 @@ void mspms_dropin_read_params(mspms_params* p)
         p->solid_zz = solid_zz;
         p->solid_sigma = solid_sigma ? *solid_sigma : 0.0;
         p->solid_epsilon = solid_epsilon ? *solid_epsilon : 0.0;
+    } else if (iSF_type == MSPMS_SF_TASOS) {
+        if (!g_grid_nodes) read_tasos_grids();
+        p->grid = &g_grid;
     } else if (iSF_type == MSPMS_SF_MY_INTERP) {
         /* init_my_interp filled ene0..fza31[t] for its nunique_atom files */
         int t, n = 0;

@@ -221,16 +221,15 @@ int get_values_from_grid(double fxx, double fyy, double fzz, int type, double* u
     return 0;
 }
 
+/* the grid the drop-in reads from the TASOS section's files (the Fortran
+ * library's copy is out of reach); reduced position in, reduced energy and
+ * force out, as the reference's wrapper returns them */
 int call_tasos_forces(int itype, double fxx, double fyy, double fzz, double* usflj_tasos,
                       double* tasos_force)
 {
-    (void)itype;
-    (void)fxx;
-    (void)fyy;
-    (void)fzz;
-    (void)usflj_tasos;
-    (void)tasos_force;
-    fprintf(stderr, "mspms gpu call_tasos_forces: the Hermite grid is held by the Fortran "
-                    "library; pass it to the context API as mspms_sf_grid\n");
-    exit(1);
+    mspms_params p;
+    mspms_dropin_read_params(&p);
+    check("call_tasos_forces",
+          mspms_term_hermite(&p, itype, fxx, fyy, fzz, usflj_tasos, tasos_force));
+    return 0;
 }

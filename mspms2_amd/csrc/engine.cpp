@@ -433,7 +433,8 @@ int validate_params(const mspms_params& p) {
         return fail(MSPMS_EINVAL, "explicit solid needs solid_xx/yy/zz");
     if (p.iSF_type == MSPMS_SF_TASOS &&
         (!p.grid || !p.grid->nodes || p.grid->ntype < 1 || p.grid->nx < 2 || p.grid->ny < 2 ||
-         p.grid->nz < 2 || !(p.grid->hx > 0) || !(p.grid->hy > 0) || !(p.grid->hz > 0)))
+         p.grid->nz < 2 || !(p.grid->hx > 0) || !(p.grid->hy > 0) || !(p.grid->hz > 0) ||
+         !(p.grid->zperiod >= 0.0) || p.grid->zperiod > (p.grid->nz - 1) * p.grid->hz))
         return fail(MSPMS_EINVAL, "SF grid missing or malformed");
     if (p.iSF_type == MSPMS_SF_MY_INTERP) {
         const mspms_sf_cubes* q = p.cubes;
@@ -1152,6 +1153,7 @@ int build(mspms_ctx* c) {
             F.hx = G.hx;
             F.hy = G.hy;
             F.hz = G.hz;
+            F.zperiod = G.zperiod > 0.0 ? G.zperiod : (G.nz - 1) * G.hz;
             F.nodes = c->d_sf_nodes.p;
             F.min_pe = c->d_sf_minpe.p;
         }

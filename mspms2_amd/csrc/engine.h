@@ -218,6 +218,7 @@ struct SFDev {
     double solid_sigma, solid_epsilon, c3pisq;
     int nx, ny, nz, pad;     // Hermite grid (Angstrom, K)
     double x0, y0, z0, hx, hy, hz;
+    double zperiod;          // Hermite grid's z period (Angstrom)
     double sigma_base, epsilon_base;
     const float* nodes;      // [ntype][nz][ny][nx][8]: a node's F, Fx, Fy, Fxy,
                              //   Fz, Fxz, Fyz, Fxyz (derivative-bit order)

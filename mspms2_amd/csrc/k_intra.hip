@@ -349,8 +349,10 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
 #pragma clang fp contract(off)
     constexpr int ND = STEP ? 17 : 11;
     static_assert(ND <= FIN_ND, "finalize partial slots");
-    __shared__ double sh[ND * (FIN_T / 64)];
-    __shared__ unsigned long long shu[4 * (FIN_T / 64)];
+    __shared__ double rows[ND * FIN_T];
+    __shared__ unsigned long long rowsu[4 * FIN_T];
+    __shared__ double bsum[ND];
+    __shared__ unsigned long long bsumu[4];
     __shared__ double tot[ND];
     __shared__ unsigned long long totu[4];
     __shared__ double lp[FIN_BLOCKS * FIN_ND];
@@ -424,13 +426,13 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
             d[ND - 1] += vt;
         }
     }
-    block_sum<ND>(d, sh);
-    block_sum_u64<4>(q, shu);
+    block_sum_rows<double, ND, FIN_T>(d, rows, bsum);
+    block_sum_rows<unsigned long long, 4, FIN_T>(q, rowsu, bsumu);
     if (threadIdx.x == 0) {
         double* P = fb.part + ((size_t)r * nb + b) * FIN_ND;
         unsigned long long* Q = fb.partu + ((size_t)r * nb + b) * 4;
-        for (int k = 0; k < ND; k++) P[k] = d[k];
-        for (int k = 0; k < 4; k++) Q[k] = q[k];
+        for (int k = 0; k < ND; k++) P[k] = bsum[k];
+        for (int k = 0; k < 4; k++) Q[k] = bsumu[k];
         __threadfence();
         last = atomicAdd(&fb.count[r], 1) == nb - 1;
     }

@@ -205,7 +205,7 @@ __host__ __device__ inline double sf_site(const SFDev& sf, const RepDev& R, cons
         }
         double xp = xi * sf.sigma_base - sf.x0, yp = yi * sf.sigma_base - sf.y0,
                zp = zi * sf.sigma_base - sf.z0;
-        const double pz = (sf.nz - 1) * sf.hz;
+        const double pz = sf.zperiod;
         zp = zp - floor(zp / pz) * pz;  // mylibtasos/pbc.f90:82-86
         int ix = (int)floor(xp / sf.hx), iy = (int)floor(yp / sf.hy), iz = (int)floor(zp / sf.hz);
         if (!(ix >= 0 && iy >= 0 && iz >= 0 && ix <= sf.nx - 2 && iy <= sf.ny - 2 &&

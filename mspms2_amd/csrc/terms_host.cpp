@@ -9,8 +9,12 @@
 //   mspms_term_sf      cal_sf_hypergeo / cal_sf_atom_explicit for one site
 //                      (src/sffrc.c:1118-1314)
 //   mspms_term_myinterp  get_values_from_grid (src/sffrc.c:951-1089)
+//   mspms_term_hermite   call_tasos_forces (src/sffrc.c:1322-1337) over the
+//                      tricubic Hermite grid (mylibtasos/forces.f90:6-173)
 // They exist for term-level parity tests and for callers that evaluate
 // single terms; the force evaluation itself runs on the GPU only.
+#include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "engine.h"
@@ -164,5 +168,76 @@ extern "C" int mspms_term_myinterp(const mspms_params* p, double x, double y, do
     fsf[0] = r[1];
     fsf[1] = r[2];
     fsf[2] = r[3];
+    return 0;
+}
+
+// The Hermite grid of p->grid as the device holds it (node-major 32-B
+// records, minimum node F per type), converted once per grid and kept for
+// later calls on the same node table.
+namespace {
+struct HostGrid {
+    const float* src = nullptr;
+    size_t count = 0;
+    std::vector<float> nodes;
+    std::vector<double> min_pe;
+};
+std::mutex g_host_grid_mu;
+HostGrid g_host_grid;
+}  // namespace
+
+extern "C" int mspms_term_hermite(const mspms_params* p, int type, double x, double y, double z,
+                                  double* u, double* force3) {
+    if (!p || !u || !force3) return mspms_fail(MSPMS_EINVAL, "null argument");
+    const mspms_sf_grid* G = p->grid;
+    if (!G || !G->nodes || G->ntype < 1 || G->nx < 2 || G->ny < 2 || G->nz < 2 ||
+        !(G->hx > 0) || !(G->hy > 0) || !(G->hz > 0) || !(G->zperiod >= 0.0) ||
+        G->zperiod > (G->nz - 1) * G->hz)
+        return mspms_fail(MSPMS_EINVAL, "mspms_term_hermite: grid missing or malformed");
+    const size_t nn = (size_t)G->nx * G->ny * G->nz, tot = nn * 8 * (size_t)G->ntype;
+    std::lock_guard<std::mutex> lock(g_host_grid_mu);
+    HostGrid& H = g_host_grid;
+    if (H.src != G->nodes || H.count != tot) {
+        static const int field_of[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+        H.nodes.assign(tot, 0.f);
+        H.min_pe.assign(G->ntype, 10e10);  // grid.f90:61
+        for (int t = 0; t < G->ntype; t++)
+            for (size_t k = 0; k < nn; k++) {
+                for (int o = 0; o < 8; o++)
+                    H.nodes[((size_t)t * nn + k) * 8 + o] =
+                        G->nodes[((size_t)t * 8 + field_of[o]) * nn + k];
+                H.min_pe[t] = std::min(H.min_pe[t], (double)G->nodes[(size_t)t * 8 * nn + k]);
+            }
+        H.src = G->nodes;
+        H.count = tot;
+    }
+    SFDev F{};
+    F.type = MSPMS_SF_TASOS;
+    F.ntype = G->ntype;
+    F.nx = G->nx;
+    F.ny = G->ny;
+    F.nz = G->nz;
+    F.x0 = G->x0;
+    F.y0 = G->y0;
+    F.z0 = G->z0;
+    F.hx = G->hx;
+    F.hy = G->hy;
+    F.hz = G->hz;
+    F.zperiod = G->zperiod > 0.0 ? G->zperiod : (G->nz - 1) * G->hz;
+    F.sigma_base = p->sigma_base;
+    F.epsilon_base = p->epsilon_base;
+    F.nodes = H.nodes.data();
+    F.min_pe = H.min_pe.data();
+    RepDev R{};
+    SiteDev S{};
+    S.ghost = MSPMS_GHOST_NONE;
+    S.interp = type;
+    int edom = 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const double pe = sfeval::sf_site(F, R, S, x, y, z, fx, fy, fz, &edom);
+    if (edom) return mspms_fail(MSPMS_EDOM, "site outside the Hermite grid or without a grid type");
+    *u = pe;
+    force3[0] = fx;
+    force3[1] = fy;
+    force3[2] = fz;
     return 0;
 }

@@ -145,6 +145,14 @@ class RunInput:
     mi_ucl: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     mi_center: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     mi_files: List[str] = field(default_factory=list)
+    # TASOS (init_tasos_grid, src/sffrc.c:149-221): unit cell a b c and the
+    # tube radius (Angstrom), cell counts, the number of grids read, then
+    # one grid file per unique atom; written only when files are named
+    tasos_uc: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    tasos_radius: float = 0.0
+    tasos_ncell: Tuple[int, int, int] = (1, 1, 1)
+    tasos_ngrid: int = 0
+    tasos_files: List[str] = field(default_factory=list)
 
 
 _FIXED = [  # (attribute, format) in src/readins.c:31-82 order
@@ -217,6 +225,15 @@ def write_in(path: str, r: RunInput) -> None:
                      "      xcenter ycenter zcenter (A)")
         lines.append(f"{len(r.mi_files):<24d} nunique_atom")
         lines += list(r.mi_files)
+    if r.tasos_files:
+        lines.append("")
+        lines.append("   TASOS")
+        lines.append(" ".join(repr(float(v)) for v in r.tasos_uc) +
+                     f" {repr(float(r.tasos_radius))}      auc buc cuc radius (A)")
+        lines.append(" ".join(str(int(v)) for v in r.tasos_ncell) + "      na nb nc")
+        lines.append(f"{r.tasos_ngrid:<24d} nspecies_yang")
+        lines.append(f"{len(r.tasos_files):<24d} tnuatoms")
+        lines += list(r.tasos_files)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -286,6 +303,15 @@ def read_in(path: str) -> RunInput:
         r.mi_center = tuple(float(v) for v in lines[i + 1].split()[:3])
         n = int(lines[i + 2].split()[0])
         r.mi_files = [lines[i + 3 + k].split()[0] for k in range(n)]
+    i = _section(lines, "TASOS")
+    if i is not None:
+        a = lines[i].split()
+        r.tasos_uc = (float(a[0]), float(a[1]), float(a[2]))
+        r.tasos_radius = float(a[3])
+        r.tasos_ncell = tuple(int(v) for v in lines[i + 1].split()[:3])
+        r.tasos_ngrid = int(lines[i + 2].split()[0])
+        n = int(lines[i + 3].split()[0])
+        r.tasos_files = [lines[i + 4 + k].split()[0] for k in range(n)]
     return r
 
 
@@ -409,6 +435,9 @@ class RunDir:
     checkpoint: Optional[object] = None
     # myinterp cube files (file name of the MYINTERP section -> CubeFile)
     cube_files: Optional[dict] = None
+    # Hermite grid files of the TASOS section (file name -> [8][nz][ny][nx]
+    # node array, sfgrid.read_tasos_table), the first nspecies_yang of them
+    grid_files: Optional[dict] = None
 
     @property
     def natom(self) -> int:
@@ -433,6 +462,10 @@ class RunDir:
             from .myinterp import write_cubes
             for name, cf in self.cube_files.items():
                 write_cubes(os.path.join(path, name), cf)
+        if self.grid_files:
+            from .sfgrid import write_tasos_grid
+            for name, tab in self.grid_files.items():
+                write_tasos_grid(os.path.join(path, name), tab, *self.run.tasos_uc)
         return path
 
     @staticmethod
@@ -450,5 +483,11 @@ class RunDir:
         if run.iSF_type == 4 and run.mi_files:
             from .myinterp import read_cubes
             cubes = {n: read_cubes(os.path.join(path, n)) for n in run.mi_files}
+        grids = None
+        if run.iSF_type == 3 and run.tasos_files:
+            # Read_Grids reads nspecies_yang grids (mylibtasos/grid.f90:69)
+            from .sfgrid import read_tasos_table
+            grids = {n: read_tasos_table(os.path.join(path, n), *run.tasos_uc)
+                     for n in run.tasos_files[:run.tasos_ngrid]}
         return RunDir(run=run, sysname=sysname, species=species, xyz=xyz, checkpoint=ck,
-                      cube_files=cubes)
+                      cube_files=cubes, grid_files=grids)

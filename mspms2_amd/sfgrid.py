@@ -100,3 +100,88 @@ def tube_grid(site_params, R, x0, y0, nx, ny, hx, hy, nz, hz, sigma_s, eps_s,
             nodes[t, k, :, :, :] = np.clip(pl, -CLAMP, CLAMP)[None, :, :]
     return dict(ntype=len(site_params), nx=nx, ny=ny, nz=nz, x0=x0, y0=y0, z0=0.0, hx=hx,
                 hy=hy, hz=hz, nodes=nodes)
+
+
+# ---- the Fortran library's grid files ------------------------------------
+# init_tasos_grid (src/sffrc.c:149-221) reads the TASOS section of in.mspms:
+# the unit cell a, b, c and the tube radius (Angstrom), the cell counts na,
+# nb, nc, the number of grids, the number of unique atoms and one grid file
+# name per unique atom.  InitPotentialGrid (mylibtasos/initialize.f90:41-155,
+# USESYMMETRY off, its compiled default: global.f90:235) sizes the grid
+# from the unit cell and grid_spacing = 0.2 A (global.f90:214):
+# ng = ANINT(a / 0.2), h = a / ng, ng + 2 nodes per axis from the origin;
+# Read_Grids (grid.f90:54-129) reads, per grid, one list-directed record of
+# 8 values (F, dF/dx, dF/dy, dF/dz, d2F/dxdy, d2F/dxdz, d2F/dydz,
+# d3F/dxdydz) per node, x fastest, then y, then z.  Positions wrap in z by c
+# only (PBCType 2, pbc.f90:82-86).
+
+TASOS_GRID_SPACING = 0.2
+
+
+def _anint(v):
+    """Fortran ANINT: nearest integer, halves away from zero."""
+    return int(np.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def tasos_dims(auc, buc, cuc, spacing=TASOS_GRID_SPACING):
+    """(ngx, ngy, ngz), (hx, hy, hz) of InitPotentialGrid without symmetry
+    (mylibtasos/initialize.f90:96-101)."""
+    ng = [_anint(a / spacing) for a in (auc, buc, cuc)]
+    h = [a / n for a, n in zip((auc, buc, cuc), ng)]
+    return tuple(ng), tuple(h)
+
+
+def write_tasos_grid(path, nodes, auc, buc, cuc):
+    """One grid file as Read_Grids reads it: nodes is [8][nz][ny][nx] with
+    (nx, ny, nz) = (ngx + 2, ngy + 2, ngz + 2) of the unit cell; one record
+    of 8 values per node, x fastest.  Values are written with 17
+    significant digits, so FP64 grids read back exactly."""
+    (ngx, ngy, ngz), _ = tasos_dims(auc, buc, cuc)
+    nodes = np.asarray(nodes, dtype=np.float64)
+    if nodes.shape != (8, ngz + 2, ngy + 2, ngx + 2):
+        raise ValueError(f"grid of shape {nodes.shape}; the unit cell needs "
+                         f"(8, {ngz + 2}, {ngy + 2}, {ngx + 2})")
+    flat = nodes.reshape(8, -1).T
+    np.savetxt(path, flat, fmt="%.17g")
+
+
+def _read_records(path, count):
+    with open(path) as f:
+        text = f.read()
+    # list-directed input: blanks, commas or newlines separate values;
+    # Fortran writes D exponents
+    vals = np.array(text.replace(",", " ").replace("D", "E").replace("d", "e").split(),
+                    dtype=np.float64)
+    if vals.size < 8 * count:
+        raise ValueError(f"{path}: {vals.size // 8} node records, {count} needed")
+    return vals[:8 * count].reshape(count, 8)
+
+
+def read_tasos_table(path, auc, buc, cuc):
+    """One grid file (Read_Grids' records for one grid) as a
+    [8][ngz+2][ngy+2][ngx+2] FP64 array."""
+    (ngx, ngy, ngz), _ = tasos_dims(auc, buc, cuc)
+    nx, ny, nz = ngx + 2, ngy + 2, ngz + 2
+    return np.ascontiguousarray(_read_records(path, nx * ny * nz).T.reshape(8, nz, ny, nx))
+
+
+def tasos_grid(tables, auc, buc, cuc):
+    """The mspms_sf_grid fields of the Fortran library's grids: one table
+    type per grid in the TASOS section's order (the site interp_type selects
+    one, 1-based), FP32 node values, nodes from the origin, z period c."""
+    (ngx, ngy, ngz), (hx, hy, hz) = tasos_dims(auc, buc, cuc)
+    nx, ny, nz = ngx + 2, ngy + 2, ngz + 2
+    nodes = np.zeros((len(tables), 8, nz, ny, nx), dtype=np.float32)
+    for t, tab in enumerate(tables):
+        if np.shape(tab) != (8, nz, ny, nx):
+            raise ValueError(f"grid {t} of shape {np.shape(tab)}, the unit cell needs "
+                             f"(8, {nz}, {ny}, {nx})")
+        nodes[t] = np.asarray(tab, dtype=np.float32)
+    return dict(ntype=len(tables), nx=nx, ny=ny, nz=nz, x0=0.0, y0=0.0, z0=0.0, hx=hx, hy=hy,
+                hz=hz, zperiod=float(cuc), nodes=nodes)
+
+
+def read_tasos_grids(files, auc, buc, cuc):
+    """The grids of the named files (Read_Grids) as the mspms_sf_grid
+    fields."""
+    return tasos_grid([read_tasos_table(f, auc, buc, cuc) for f in files], auc, buc, cuc)

@@ -211,11 +211,17 @@ class System:
                 raise ValueError("iSF_type 4 needs the MYINTERP section and its cube files")
             cubes = engine_tables(r.mi_ucl, r.mi_center,
                                   [rd.cube_files[n] for n in r.mi_files])
+        grid = None
+        if r.iSF_type == abi.SF_TASOS and rd.grid_files:
+            from .sfgrid import tasos_grid
+            grid = tasos_grid([rd.grid_files[n] for n in r.tasos_files[:r.tasos_ngrid]],
+                              *r.tasos_uc)
         pressure_base = BOLTZMAN_CONSTANT * eb * 1.0e25 / (sb * sb * sb)
         s = System(species=species, params=p, xx=xx, yy=yy, zz=zz, run=r,
                    time_base=time_base, delt=r.delt / time_base, treq=r.treq / eb,
                    preq=r.preq / pressure_base,
-                   v0=None if ck is None else np.array(ck.v), checkpoint=ck, cubes=cubes)
+                   v0=None if ck is None else np.array(ck.v), checkpoint=ck, cubes=cubes,
+                   grid=grid)
         return s
 
     # ------------------------------------------------------------------ #
@@ -281,6 +287,7 @@ class System:
             g = abi.SFGrid()
             for k in ("ntype", "nx", "ny", "nz", "x0", "y0", "z0", "hx", "hy", "hz"):
                 setattr(g, k, self.grid[k])
+            g.zperiod = float(self.grid.get("zperiod", 0.0))
             nodes = np.ascontiguousarray(self.grid["nodes"], dtype=np.float32)
             keep.append(nodes)
             g.nodes = abi.fptr(nodes)

@@ -640,7 +640,7 @@ int oracle_hermite_site(const mspms_sf_grid* g, int itype, double min_pe,
                         double* force3)
 {
     double xp = x - g->x0, yp = y - g->y0, zp = z - g->z0;
-    double pz = (g->nz - 1) * g->hz;
+    double pz = g->zperiod > 0.0 ? g->zperiod : (g->nz - 1) * g->hz;
     double tx, ty, tz, bx[4], by[4], bz[4], dbx[4], dby[4], dbz[4];
     double fval = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
     int ix, iy, iz, cx, cy, cz;

@@ -47,6 +47,11 @@
  * Fortran grid library, iSF_type 3; the GSL metal-cluster force field,
  * iExternal_FF_type != 0) are refused before init_vars.  iSF_type 4
  * (myinterp, src/sffrc.c:228-1089) is plain C and runs.
+ * --tasos-linked: iSF_type 3 where the harness is linked with the engine's
+ * frclong or call_tasos_forces, which read the TASOS section's grid files
+ * themselves: init_vars runs with iSF_type 0, so that init_tasos_grid (which
+ * calls into the absent library) is skipped, and iSF_type 3 is restored
+ * before frclong.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdio.h>
@@ -63,7 +68,7 @@ static double v0[3][NATOM_MAX];
 
 int main(int argc, char** argv)
 {
-    int repeat = 0, nmd = 0, files = 0, sa = 0, nhmc = 0, k;
+    int repeat = 0, nmd = 0, files = 0, sa = 0, nhmc = 0, tasos = 0, k;
     char outpath[4096];
     FILE* fo;
     if (argc < 3) {
@@ -81,6 +86,8 @@ int main(int argc, char** argv)
             sa = 1;
         else if (!strcmp(argv[k], "--hmc") && k + 1 < argc)
             nhmc = atoi(argv[++k]);
+        else if (!strcmp(argv[k], "--tasos-linked"))
+            tasos = 1;
     if (argv[2][0] == '/') {
         snprintf(outpath, sizeof outpath, "%s", argv[2]);
     } else {
@@ -94,14 +101,20 @@ int main(int argc, char** argv)
     }
     opening();
     readins();
-    if (iSF_type == SF_NANOTUBE_TASOS || iExternal_FF_type != FF_NONE) {
+    if ((iSF_type == SF_NANOTUBE_TASOS && !tasos) || iExternal_FF_type != FF_NONE) {
         fprintf(stderr, "ref_harness: iSF_type=%d / iExternal_FF_type=%d need "
                         "libraries absent from this build\n",
                 iSF_type, iExternal_FF_type);
         return 3;
     }
     fnValidateInput();
-    init_vars();
+    if (tasos && iSF_type == SF_NANOTUBE_TASOS) {
+        iSF_type = SF_NONE;
+        init_vars();
+        iSF_type = SF_NANOTUBE_TASOS;
+    } else {
+        init_vars();
+    }
     fnValidateInit();
     frclong();
     frcshort();

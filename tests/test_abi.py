@@ -60,7 +60,7 @@ int main(void) {
   O(mspms_params, kappa) O(mspms_params, coulomb_prefactor) O(mspms_params, wolffcon2)
   O(mspms_params, ntube) O(mspms_params, solid_natom) O(mspms_params, grid)
   O(mspms_long_terms, virial_inter) O(mspms_long_terms, elec_pair_hash)
-  O(mspms_sf_grid, nodes)
+  O(mspms_sf_grid, nodes) O(mspms_sf_grid, zperiod)
   S(mspms_sfgridgen_spec) O(mspms_sfgridgen_spec, isLJswitch) O(mspms_sfgridgen_spec, uclx)
   O(mspms_sfgridgen_spec, s_epsilon) O(mspms_sfgridgen_spec, sz)
   return 0;

@@ -62,3 +62,33 @@ def test_dropin(name, precision):
     # frcshort forces: relative to the nonbonded force scale as well, since
     # rigid-template inputs put every bonded term at its minimum
     assert np.linalg.norm(d["fs"] - fs) <= 1e-10 * max(np.linalg.norm(fs), fscale)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dropin_tasos_grid_files(precision):
+    """iSF_type 3 through the drop-in: the drop-in reads the TASOS section
+    and its grid files itself (the Fortran library's copy is out of reach;
+    the harness skips init_tasos_grid with --tasos-linked).  Against the
+    oracle on the grid the Python reader loads: FP64 1e-10, mixed 1e-5."""
+    from mspms2_amd import configs, runio
+    from mspms2_amd.system import System
+    if not os.path.exists(HARNESS):
+        pytest.fail(f"{HARNESS} missing: run __graft_entry__.build() where the reference is")
+    print(f"dropin tasos_column {precision}", flush=True)
+    rd = configs.tasos_column()
+    with tempfile.TemporaryDirectory(prefix="mspms_dropin_") as tmp:
+        rd.write(tmp)
+        s = System.from_rundir(runio.RunDir.read(tmp))
+        fl, tl = O.Oracle(s).frclong()
+        env = dict(os.environ, MSPMS_GPU_PRECISION=precision)
+        out = os.path.join(tmp, "dump.bin")
+        r = subprocess.run([HARNESS, tmp, out, "--tasos-linked"], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        d = O.read_ref_dump(out)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert d["usflj"] == pytest.approx(tl["usflj"], rel=1e-10)
+    assert_long_close(d, tl, tol, "tasos_column")
+    err = np.linalg.norm(d["fl"] - fl) / np.linalg.norm(fl)
+    print(f"  force rel {err:.2e}")
+    assert err <= tol

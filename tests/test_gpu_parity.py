@@ -40,6 +40,7 @@ CASES = {
     "tube300_hypergeo": lambda: configs.methane_tube(natom=300, length=60.0, box_xy=40.0,
                                                      tube_r=15.0, rc=10.0),
     "tube300_grid": "grid",
+    "tasos_column": lambda: configs.tasos_column(),
     "examples_water": "golden",
     "mixture_water4_vacuum": "golden",
     "water216_coulomb": "golden",
