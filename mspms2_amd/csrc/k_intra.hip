@@ -355,8 +355,6 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     __shared__ unsigned long long bsumu[4];
     __shared__ double tot[ND];
     __shared__ unsigned long long totu[4];
-    __shared__ double lp[FIN_BLOCKS * FIN_ND];
-    __shared__ unsigned long long lq[FIN_BLOCKS * 4];
     __shared__ int last;
     if (A.fx) {
         // the step's closing pass over this replica's atoms, strided as the
@@ -439,25 +437,31 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     __syncthreads();
     if (!last) return;
     // the replica's last workgroup: the nb sums, read past the
-    // (non-coherent) vector L1 by all threads at once into LDS, then added
-    // in block order
+    // (non-coherent) vector L1; wave w takes values w, w + 4, ...: lane bb
+    // reads workgroup bb's sum (nb <= 64) and the wave adds them in a fixed
+    // order, all loads of the workgroup in flight at once
     __threadfence();
-    for (int t = threadIdx.x; t < nb * FIN_ND; t += FIN_T)
-        lp[t] = __hip_atomic_load(fb.part + (size_t)r * nb * FIN_ND + t, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-    for (int t = threadIdx.x; t < nb * 4; t += FIN_T)
-        lq[t] = __hip_atomic_load(fb.partu + (size_t)r * nb * 4 + t, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (threadIdx.x < ND) {
-        double s = 0.0;
-        for (int bb = 0; bb < nb; bb++) s += lp[bb * FIN_ND + threadIdx.x];
-        tot[threadIdx.x] = s;
-    } else if (threadIdx.x >= 64 && threadIdx.x < 68) {
-        const int k = threadIdx.x - 64;
-        unsigned long long s = 0;
-        for (int bb = 0; bb < nb; bb++) s += lq[bb * 4 + k];
-        totu[k] = s;
+    static_assert(FIN_BLOCKS <= 64, "one lane per workgroup sum");
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        for (int k = w; k < ND + 4; k += FIN_T / 64) {
+            if (k < ND) {
+                const double v =
+                    lane < nb ? __hip_atomic_load(fb.part + ((size_t)r * nb + lane) * FIN_ND + k,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : 0.0;
+                const double s = wave_sum64(v);
+                if (lane == 0) tot[k] = s;
+            } else {
+                const unsigned long long v =
+                    lane < nb
+                        ? __hip_atomic_load(fb.partu + ((size_t)r * nb + lane) * 4 + (k - ND),
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0ull;
+                const unsigned long long s = wave_sum64(v);
+                if (lane == 0) totu[k - ND] = s;
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x != 0) return;

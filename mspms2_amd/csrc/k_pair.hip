@@ -270,6 +270,10 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
     __shared__ double sh_f[WPC][MSPMS_CL][3];
     __shared__ double sh_e[WPC][4];
     __shared__ unsigned long long sh_n[WPC][4];
+    // HALF: the j forces of a chunk's entries, added to the slot-ordered
+    // forces after the chunk (one value per entry, j atom and component)
+    __shared__ float sh_jf[HALF ? WPC : 1][64][3][MSPMS_CL];
+    __shared__ int sh_jc[HALF ? WPC : 1][64];
     const int ci = xcd_block(blockIdx.x, gridDim.x);
     const int ncl = min(*ncl_total, maxcl);
     if (ci >= ncl) return;  // whole workgroup
@@ -380,6 +384,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             J.qj = slot_par[cn * MSPMS_CL + jsub];
             J.oj = HALF && HASH ? slot_orig[cn * MSPMS_CL + jsub] : 0;
         };
+        int kk = 0;  // entries of this chunk processed so far
         auto process = [&](const JSet& J) {
             if (it > 0 && it % FLUSH == 0) flush();  // uniform over the wave
             ++it;
@@ -481,7 +486,7 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                     pfz += gz;
                 }
             }
-            if (HALF && __ballot(hit)) {
+            if (HALF) {
                 // j's share of this entry: the sum over its eight adjacent i-lanes
                 // (lane ^ 1, lane ^ 2, then the mirror within the 8-lane half row)
                 float v[3] = {gx, gy, gz};
@@ -491,14 +496,17 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                     v[k] += dpp_f32<0x4E>(v[k]);   // quad_perm [2,3,0,1]
                     v[k] += dpp_f32<0x141>(v[k]);  // row_half_mirror
                 }
-                // into the slot-ordered forces: the eight j-lanes of a
-                // component write one 64-B line
-                if (isub == 0 && (flj & SLOT_VALID) && (v[0] != 0.f || v[1] != 0.f || v[2] != 0.f)) {
-                    atomicAdd(&fslot[jslot], -(double)v[0]);
-                    atomicAdd(&fslot[(size_t)nslot + jslot], -(double)v[1]);
-                    atomicAdd(&fslot[2 * (size_t)nslot + jslot], -(double)v[2]);
+                // staged in LDS: the eight lanes of a j hold the same sum.
+                // Atomics issued here, between one entry's j loads and the
+                // next's, would sit under the wait for those loads.
+                if (isub == 0) {
+                    sh_jf[w][kk][0][jsub] = -v[0];
+                    sh_jf[w][kk][1][jsub] = -v[1];
+                    sh_jf[w][kk][2][jsub] = -v[2];
                 }
+                if (lane == 0) sh_jc[w][kk] = cj;
             }
+            ++kk;
         };
         JSet A, B;
         fetch(A);
@@ -509,6 +517,22 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                 fetch(A);
                 process(B);
             }
+        }
+        if (HALF) {
+            // the chunk's j forces into the slot-ordered forces: eight
+            // entries per instruction, one 64-B line each (LDS written and
+            // read by this wave only, in order)
+            __builtin_amdgcn_wave_barrier();
+            for (int e0 = 0; e0 < cnt; e0 += 8) {
+                const int e = e0 + (lane >> 3), a = lane & 7;
+                if (e < cnt) {
+                    const int sl = sh_jc[w][e] * MSPMS_CL + a;
+                    atomicAdd(&fslot[sl], (double)sh_jf[w][e][0][a]);
+                    atomicAdd(&fslot[(size_t)nslot + sl], (double)sh_jf[w][e][1][a]);
+                    atomicAdd(&fslot[2 * (size_t)nslot + sl], (double)sh_jf[w][e][2][a]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     flush();
