@@ -525,11 +525,15 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
             __builtin_amdgcn_wave_barrier();
             for (int e0 = 0; e0 < cnt; e0 += 8) {
                 const int e = e0 + (lane >> 3), a = lane & 7;
-                if (e < cnt) {
+                const float f0 = e < cnt ? sh_jf[w][e][0][a] : 0.f;
+                const float f1 = e < cnt ? sh_jf[w][e][1][a] : 0.f;
+                const float f2 = e < cnt ? sh_jf[w][e][2][a] : 0.f;
+                // j atoms without a pair in range (and padding slots) add nothing
+                if (f0 != 0.f || f1 != 0.f || f2 != 0.f) {
                     const int sl = sh_jc[w][e] * MSPMS_CL + a;
-                    atomicAdd(&fslot[sl], (double)sh_jf[w][e][0][a]);
-                    atomicAdd(&fslot[(size_t)nslot + sl], (double)sh_jf[w][e][1][a]);
-                    atomicAdd(&fslot[2 * (size_t)nslot + sl], (double)sh_jf[w][e][2][a]);
+                    atomicAdd(&fslot[sl], (double)f0);
+                    atomicAdd(&fslot[(size_t)nslot + sl], (double)f1);
+                    atomicAdd(&fslot[2 * (size_t)nslot + sl], (double)f2);
                 }
             }
             __builtin_amdgcn_wave_barrier();
