@@ -117,6 +117,7 @@ struct mspms_ctx {
     int natom = 0, nmol = 0, nsite = 0;
     int charge_union = MSPMS_ELEC_NONE;  // charge type shared by the replicas
     bool any_intra = false, any_bonded = false, any_recip = false, any_sf = false;
+    int max_mol_terms = 0;  // most bonded terms of one molecule (any species)
     bool sf_split = false;  // the solid-fluid wall in frcshort / the inner loop
     bool any_vacuum = false, any_coul = false;
     int want_hash = -1;                  // -1: FP64 only
@@ -178,6 +179,9 @@ struct mspms_ctx {
     double* snap_pos = nullptr;  // the buffers of the run it belongs to
     double* snap_vel = nullptr;
     DevBuf<HistDev> d_hist;  // per-step term histories of a device run
+    DevBuf<double> d_hpart;  // their workgroup sums, [steps][nrep][nb][FIN_ND]
+    DevBuf<unsigned long long> d_hpartu;
+    DevBuf<double> d_huvac;  // each step's vacuum energies, [steps][nrep]
     DevBuf<int> d_hstep;     // [nrep] next history slot
 
     // solid-fluid tables
@@ -300,6 +304,7 @@ LaunchCtx launch_ctx(mspms_ctx* c, hipStream_t s) {
     L.pair_wpc = c->pair_wpc;
     L.max_mol_atoms = 0;
     for (const SpeciesHost& q : c->sp) L.max_mol_atoms = std::max(L.max_mol_atoms, q.natom);
+    L.max_mol_terms = c->max_mol_terms;
     L.rep_delt = c->cur_rep_delt;
     L.snap = nullptr;
     L.snap_count = nullptr;
@@ -590,6 +595,9 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_fin_part.release();
     c->d_fin_partu.release();
     c->d_fin_count.release();
+    c->d_hpart.release();
+    c->d_hpartu.release();
+    c->d_huvac.release();
     c->d_ip_pair.release();
     c->d_long.release();
     c->d_short.release();
@@ -635,6 +643,7 @@ int build(mspms_ctx* c) {
     std::vector<SpeciesDev> spd;
     std::vector<TermDev> terms;
     c->any_intra = c->any_bonded = false;
+    c->max_mol_terms = 0;
     for (int k = 0; k < ns; k++) {
         const SpeciesHost& q = c->sp[k];
         SpeciesDev d{};
@@ -721,6 +730,7 @@ int build(mspms_ctx* c) {
             terms.push_back(t);
         }
         d.nbterm = (int)terms.size() - d.bterm0;
+        c->max_mol_terms = std::max(c->max_mol_terms, d.nbterm);
         if (d.nintra > 0) c->any_intra = true;
         if (d.nbterm > 0) c->any_bonded = true;
         spd.push_back(d);
@@ -1010,6 +1020,7 @@ int build(mspms_ctx* c) {
     b.plist_n = c->d_plist_n.p;
     b.atom_gslot = c->d_atom_gslot.p;
     b.fslot = c->d_fslot.p;
+    b.hist = nullptr;  // set with the finalize buffers below
     b.maxcl = (int)maxcl;
     b.cap = cap;
     b.ncell = ncell;
@@ -1186,6 +1197,7 @@ int build(mspms_ctx* c) {
     c->fin.count = c->d_fin_count.p;
     c->fin.on = nullptr;
     c->fin.hist = c->d_hist.p;
+    c->cb.hist = c->d_hist.p;
     c->fin.hstep = c->d_hstep.p;
     {
         // one partial of each kind per thread in the largest replica
@@ -1844,11 +1856,19 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
     }
     // the overflow snapshot counts this run's steps
     HIPC(hipMemsetAsync(c->d_snap_count.p, 0, sizeof(int), s));
+    HistDev hd{};
     if (hist && nsteps > 0) {
-        // every step's finalize appends its terms to slot t of the histories
+        // every step's finalize stores its workgroup sums in slot t; the
+        // terms of all steps are formed after the last one
+        const size_t nsum = (size_t)nsteps * nr * c->fin.nb;
+        int rc = 0;
+        if ((rc = c->d_hpart.ensure(nsum * FIN_ND)) || (rc = c->d_hpartu.ensure(nsum * 4)) ||
+            (rc = c->d_huvac.ensure((size_t)nsteps * nr)))
+            return rc;
         LaunchCtx L = launch_ctx(c, s);
-        launch_set_hist(L, c->d_hist.p, HistDev{lhist, shist, ukin_hist, nr, nsteps, 1},
-                        c->d_hstep.p);
+        hd = HistDev{lhist, shist, ukin_hist, nr, nsteps, 1, c->d_hpart.p, c->d_hpartu.p,
+                     c->fin.nb, c->d_hstep.p, c->any_vacuum ? c->d_huvac.p : nullptr};
+        launch_set_hist(L, c->d_hist.p, hd, c->d_hstep.p);
         HIPC(hipGetLastError());
         reopen.hist_on = true;
     }
@@ -1863,6 +1883,11 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
             int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
             if (rc) return rc;
         }
+    }
+    if (hd.on) {
+        LaunchCtx L = launch_ctx(c, s);
+        launch_hist_terms(L, hd, nsteps, lterms, sterms, ukin);
+        HIPC(hipGetLastError());
     }
     return 0;
 }

@@ -74,6 +74,8 @@ struct KRow {
 };
 
 // the three groups of scratch used by one frclong evaluation
+struct HistDev;
+
 struct ClusterBufs {
     int* atom_cell;      // [natom]
     int* atom_slot;      // [natom]
@@ -113,6 +115,7 @@ struct ClusterBufs {
                          //   64-B line per j-cluster and component; zeroed by
                          //   k_list_check, folded into the atoms' forces
                          //   after the pair kernel (SlotForces)
+    HistDev* hist;       // the run's history descriptor (k_list_check ticks it)
     int maxcl, cap, ncell, ncol;
     int half;            // layout this call wants: 1 = each cluster pair in one
                          //   row (mixed pair kernel, j forces by atomics),
@@ -133,11 +136,20 @@ enum { LS_VALID = 0, LS_NEED = 1, LS_BUILDS = 2, LS_LAYOUT = 3, LS_N = 4 };
 // of every RESPA step (one slot per step and replica); `on` is 0 outside a
 // run that records them.  Lives in device memory at a fixed address, so a
 // captured step graph appends to whatever histories the current run set.
+// Histories of a device run: every step's finalize stores its workgroup
+// sums in the step's slot of part / partu (the step's k_list_check advances
+// *tick first), and k_hist_terms forms all steps' terms after the run, in
+// the order the step's own last workgroup would.
 struct HistDev {
     mspms_long_terms* lh;   // [cap][nrep] or null
     mspms_short_terms* sh;  // [cap][nrep] or null
     double* kh;             // [cap][nrep] or null
     int nrep, cap, on;
+    double* part;               // [cap][nrep][nb][FIN_ND]
+    unsigned long long* partu;  // [cap][nrep][nb][4]
+    int nb;
+    int* tick;                  // steps begun in the run
+    double* uvac;               // [cap][nrep] each step's vacuum energy, or null
 };
 
 struct FinBufs {
@@ -172,6 +184,7 @@ struct LaunchCtx {
     hipStream_t stream;
     int natom, nmol, nsite, nrep;
     int max_mol_atoms;        // largest molecule (atoms) of any species
+    int max_mol_terms;        // most bonded terms of one molecule of any species
     int pair_wpc;             // waves per i-cluster of the mixed pair kernel
     const SiteDev* sites;
     const MolDev* mols;
@@ -402,6 +415,8 @@ void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* 
                    const double* y, const double* z, double* fx, double* fy, double* fz,
                    double* uvac, double* gq);
 // sets a device run's history descriptor and clears the per-replica slots
+void launch_hist_terms(const LaunchCtx& c, const HistDev& h, int nsteps, void* lterms,
+                       void* sterms, double* ukin);
 void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hstep);
 void launch_finalize_short(const LaunchCtx& c, const double* epart, void* terms,
                            const double* ep_sf = nullptr);

@@ -43,8 +43,11 @@ __global__ void __launch_bounds__(256) k_list_check(
     const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ slot_orig,
     const double4* __restrict__ slot_w64, const double4* __restrict__ cl_center,
     double4* __restrict__ slot_x64, float4* __restrict__ slot_off, float4* __restrict__ cl_box,
-    double* __restrict__ fslot, int* lstate, double half_skin2, int force, int half) {
+    double* __restrict__ fslot, int* lstate, double half_skin2, int force, int half,
+    const HistDev* hist) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // a device run recording histories: this call begins the next step
+    if (t == 0 && hist && hist->on) *hist->tick += 1;
     const bool valid = lstate[LS_VALID] != 0 && !force && lstate[LS_LAYOUT] == half;
     bool need = !valid;
     if (half && t < nslot) {
@@ -522,7 +525,7 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                        c.stream, c.natom, nslot, x, y, z, c.atom_site, c.sites, fx, fy, fz,
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
                        b.slot_x64, b.slot_off, b.cl_box, b.fslot, b.lstate, half_skin2, force,
-                       b.half);
+                       b.half, b.hist);
     hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
                        b.cell_count, fx, fy, fz, b.atom_gslot, c.status, b.lstate);

@@ -16,6 +16,7 @@
 #include "engine.h"
 #include "sf_eval.h"
 #include "terms.h"
+#include "wave.h"
 
 namespace {
 
@@ -390,6 +391,295 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     }
 }
 
+// The inner loop with one bonded term per wave: a workgroup of NW waves
+// takes 64 molecules (lane = molecule); wave w owns site w of each molecule
+// (kicks, drift) and computes its term w into its own LDS copy of the
+// molecule's forces, so that a wave runs one kind of term and no wave
+// steps through the others' code (with one lane per site and term in a
+// wave, bonds and angles alternate within the wave and each waits for the
+// other).  A site's force is the sum of the NW copies in wave (= term)
+// order, as k_respa_inner_tpm sums its lane copies.  Molecules of up to NW
+// sites and NW terms.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
+    int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
+    const TermDev* __restrict__ terms, const SiteDev* __restrict__ sites, double* x, double* y,
+    double* z, double* vx, double* vy, double* vz, const double* __restrict__ fxl,
+    const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
+    double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
+    Status* st, const double* __restrict__ rep_delt, SfInner W, SnapOut SN) {
+#pragma clang fp contract(off)
+    __shared__ double lx[NW * 64], ly[NW * 64], lz[NW * 64];
+    __shared__ double cfx[NW][NW * 64], cfy[NW][NW * 64], cfz[NW][NW * 64];
+    __shared__ double le[NW][5][64];
+    const int w = threadIdx.x >> 6, mi = threadIdx.x & 63;
+    const int m = blockIdx.x * 64 + mi;
+    const bool live = m < nmol;
+    int na = 0, a0 = 0, site0 = 0, nbt = 0, rep = 0;
+    TermDev T{};
+    if (live) {
+        const MolDev M = mols[m];
+        const SpeciesDev S = species[M.species];
+        na = M.natom;
+        a0 = M.atom0;
+        site0 = S.site0;
+        nbt = S.nbterm;
+        rep = M.rep;
+        if (w < nbt) T = terms[S.bterm0 + w];  // this wave's term, for every substep
+        rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
+    }
+    const bool own = live && w < na;      // site w of molecule m
+    const bool term = live && w < nbt;    // term w of molecule m
+    const int sl = w * 64 + mi;
+    double usf_last = 0.0;
+    double r_mass = 0.0, vxo = 0.0, vyo = 0.0, vzo = 0.0, fxo = 0.0, fyo = 0.0, fzo = 0.0;
+    if (own && snap_wanted(SN, st)) {
+        const int i = a0 + w;
+        snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
+    }
+    if (SN.p && m == 0 && w == 0 && snap_wanted(SN, st)) snap_count(SN);
+    if (own) {
+        const int i = a0 + w;
+        r_mass = 1.0 / sites[site0 + w].aw;
+        lx[sl] = x[i];
+        ly[sl] = y[i];
+        lz[sl] = z[i];
+        vxo = vx[i] + (deltby2 * fxl[i] * r_mass);
+        vyo = vy[i] + (deltby2 * fyl[i] * r_mass);
+        vzo = vz[i] + (deltby2 * fzl[i] * r_mass);
+        fxo = fxs[i];
+        fyo = fys[i];
+        fzo = fzs[i];
+    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    for (int ll = 0; ll < nstep_inner; ll++) {
+        __syncthreads();  // the previous step's copies have been read
+        if (own) {
+            vxo += (deltsby2 * fxo * r_mass);
+            vyo += (deltsby2 * fyo * r_mass);
+            vzo += (deltsby2 * fzo * r_mass);
+            lx[sl] = lx[sl] + delts * vxo;
+            ly[sl] = ly[sl] + delts * vyo;
+            lz[sl] = lz[sl] + delts * vzo;
+        }
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            cfx[w][k * 64 + mi] = 0.0;
+            cfy[w][k * 64 + mi] = 0.0;
+            cfz[w][k * 64 + mi] = 0.0;
+        }
+        __syncthreads();
+        e[0] = e[1] = e[2] = e[3] = 0.0;
+        vir = 0.0;
+        if (term) {
+            double u = 0.0, v = 0.0;
+            int sing = 0;
+            const int cls = bonded_term(T, mi, lx, ly, lz, cfx[w], cfy[w], cfz[w], &u, &v, &sing, 64);
+            if (sing) atomicOr(&st->flags, ST_SINGULAR);
+            e[cls] += u;
+            vir += v;
+        }
+        __syncthreads();
+        if (own) {
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+            for (int c = 0; c < NW; c++) {
+                gx += cfx[c][sl];
+                gy += cfy[c][sl];
+                gz += cfz[c][sl];
+            }
+            fxo = gx;
+            fyo = gy;
+            fzo = gz;
+            if (W.on)
+                usf_last = sf_inner_site(W, sites[site0 + w], rep, lx[sl], ly[sl], lz[sl], fxo, fyo,
+                                         fzo, st);
+            vxo += (deltsby2 * fxo * r_mass);
+            vyo += (deltsby2 * fyo * r_mass);
+            vzo += (deltsby2 * fzo * r_mass);
+        }
+    }
+    if (own) {
+        const int i = a0 + w;
+        x[i] = lx[sl];
+        y[i] = ly[sl];
+        z[i] = lz[sl];
+        vx[i] = vxo;
+        vy[i] = vyo;
+        vz[i] = vzo;
+        fxs[i] = fxo;
+        fys[i] = fyo;
+        fzs[i] = fzo;
+        if (W.on) W.usf[i] = usf_last;
+    }
+    // the last inner step's energies, summed over the molecule's terms in
+    // term order
+    le[w][0][mi] = e[0];
+    le[w][1][mi] = e[1];
+    le[w][2][mi] = e[2];
+    le[w][3][mi] = e[3];
+    le[w][4][mi] = vir;
+    __syncthreads();
+    if (live && w == 0) {
+        double* ep = epart + (size_t)m * 5;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < NW; c++) s += le[c][k][mi];
+            ep[k] = s;
+        }
+    }
+}
+
+// The tricubic Hermite wall of one site by four lanes (q = lane & 3 of a
+// quad): a lookup reads 4 contiguous 64-B pieces (the x-neighbours' 32-B
+// records of the four (y, z) corner pairs); lane q loads the q-th 16 B of
+// each, so one load instruction of the wave fetches 16 whole pieces where
+// one lane per site fetches 64 scattered quarters.  Lane q's values are
+// those of corner x offset q >> 1 and derivative bits 4 (q & 1) + k; its
+// partial sums are added over the quad (DPP, every lane gets them).  The
+// arithmetic is the lookup of sf_site (mylibtasos/forces.f90:6-173,
+// cubicinterpolation.f90:9-116) with its terms summed in another order.
+// All four lanes must be active.
+__device__ inline double sf_tasos_quad(const SFDev& sf, const SiteDev& S, double xi, double yi,
+                                       double zi, double& fxi, double& fyi, double& fzi,
+                                       int* edom) {
+    const int q = threadIdx.x & 3;
+    const int ty = S.interp - 1;
+    double xp = xi * sf.sigma_base - sf.x0, yp = yi * sf.sigma_base - sf.y0,
+           zp = zi * sf.sigma_base - sf.z0;
+    const double pz = sf.zperiod;
+    zp = zp - floor(zp / pz) * pz;  // mylibtasos/pbc.f90:82-86
+    const int ix = (int)floor(xp / sf.hx), iy = (int)floor(yp / sf.hy), iz = (int)floor(zp / sf.hz);
+    const bool ok = ty >= 0 && ty < sf.ntype && ix >= 0 && iy >= 0 && iz >= 0 &&
+                    ix <= sf.nx - 2 && iy <= sf.ny - 2 && iz <= sf.nz - 2;
+    double fval = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    if (ok) {
+        double bx[4], by[4], bz[4], dbx[4], dby[4], dbz[4];
+        sfeval::hermite_1d((xp - ix * sf.hx) / sf.hx, sf.hx, bx, dbx);
+        sfeval::hermite_1d((yp - iy * sf.hy) / sf.hy, sf.hy, by, dby);
+        sfeval::hermite_1d((zp - iz * sf.hz) / sf.hz, sf.hz, bz, dbz);
+        const size_t nn = (size_t)sf.nx * sf.ny * sf.nz;
+        const float4* base = reinterpret_cast<const float4*>(sf.nodes) + (size_t)ty * 2 * nn;
+        const int cx = q >> 1, half = q & 1;
+        float4 rec[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {  // (cy, cz) = (c & 1, c >> 1)
+            const size_t node =
+                ((size_t)(iz + (c >> 1)) * sf.ny + (iy + (c & 1))) * sf.nx + (ix + cx);
+            rec[c] = base[2 * node + half];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int cy = c & 1, cz = c >> 1;
+            const float vals[4] = {rec[c].x, rec[c].y, rec[c].z, rec[c].w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int ox = k & 1, oy = k >> 1, oz = half;
+                const double v = (double)vals[k];
+                const double wx = bx[cx + 2 * ox], wy = by[cy + 2 * oy], wz = bz[cz + 2 * oz];
+                fval += v * wx * wy * wz;
+                gx += v * dbx[cx + 2 * ox] * wy * wz;
+                gy += v * wx * dby[cy + 2 * oy] * wz;
+                gz += v * wx * wy * dbz[cz + 2 * oz];
+            }
+        }
+    }
+    // the quad's sums (lanes q ^ 1, then q ^ 2)
+    fval += dpp64<0xB1>(fval);
+    gx += dpp64<0xB1>(gx);
+    gy += dpp64<0xB1>(gy);
+    gz += dpp64<0xB1>(gz);
+    fval += dpp64<0x4E>(fval);
+    gx += dpp64<0x4E>(gx);
+    gy += dpp64<0x4E>(gy);
+    gz += dpp64<0x4E>(gz);
+    if (!ok) {
+        *edom = 1;
+        return 0.0;
+    }
+    const double pe = (fval < 1.5 * sf.min_pe[ty]) ? -fval : fval;  // forces.f90:166-171
+    const double fconv = sf.sigma_base / sf.epsilon_base;
+    fxi += -gx * fconv;
+    fyi += -gy * fconv;
+    fzi += -gz * fconv;
+    return pe / sf.epsilon_base;
+}
+
+// The inner loop of single-site molecules with the tricubic Hermite wall in
+// it (the RESPA split): four lanes per site, 16 sites per wave.  All four
+// lanes carry the site's kicks and drifts (the same operations on the same
+// values, so the same results) and share the wall's lookup; lane 0 writes.
+// Single-site molecules have no bonded terms.  Lanes past the last molecule
+// work on the last one and write nothing, so every quad stays whole.
+__global__ void __launch_bounds__(64) k_respa_inner_grid4(
+    int nmol, const MolDev* __restrict__ mols, const SpeciesDev* __restrict__ species,
+    const SiteDev* __restrict__ sites, double* x, double* y, double* z, double* vx, double* vy,
+    double* vz, const double* __restrict__ fxl,
+    const double* __restrict__ fyl, const double* __restrict__ fzl, double* fxs, double* fys,
+    double* fzs, double deltby2, double delts, double deltsby2, int nstep_inner, double* epart,
+    Status* st, const double* __restrict__ rep_delt, SfInner W, SnapOut SN) {
+#pragma clang fp contract(off)
+    const int q = threadIdx.x & 3;
+    const int m0 = blockIdx.x * 16 + (threadIdx.x >> 2);
+    const bool live = m0 < nmol;
+    const int m = live ? m0 : nmol - 1;
+    const MolDev M = mols[m];
+    const int i = M.atom0;
+    const SiteDev S = sites[species[M.species].site0];
+    rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
+    const bool writer = live && q == 0;
+    if (writer && snap_wanted(SN, st)) snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
+    if (SN.p && m0 == 0 && q == 0 && snap_wanted(SN, st)) snap_count(SN);
+    const double r_mass = 1.0 / S.aw;
+    double xi = x[i], yi = y[i], zi = z[i];
+    double vxo = vx[i] + (deltby2 * fxl[i] * r_mass);
+    double vyo = vy[i] + (deltby2 * fyl[i] * r_mass);
+    double vzo = vz[i] + (deltby2 * fzl[i] * r_mass);
+    double fxo = fxs[i], fyo = fys[i], fzo = fzs[i];
+    const bool wall = S.ghost == MSPMS_GHOST_NONE;
+    double usf_last = 0.0;
+    int edom = 0;
+    for (int ll = 0; ll < nstep_inner; ll++) {
+        vxo += (deltsby2 * fxo * r_mass);
+        vyo += (deltsby2 * fyo * r_mass);
+        vzo += (deltsby2 * fzo * r_mass);
+        xi = xi + delts * vxo;
+        yi = yi + delts * vyo;
+        zi = zi + delts * vzo;
+        // frcshort of a single-site molecule: no bonded term, the wall only
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const double u = sf_tasos_quad(W.sf, S, xi, yi, zi, gx, gy, gz, &edom);
+        if (wall) {
+            fxo = gx;
+            fyo = gy;
+            fzo = gz;
+            usf_last = u;
+        } else {
+            fxo = fyo = fzo = 0.0;
+            usf_last = 0.0;
+        }
+        vxo += (deltsby2 * fxo * r_mass);
+        vyo += (deltsby2 * fyo * r_mass);
+        vzo += (deltsby2 * fzo * r_mass);
+    }
+    if (edom && wall && writer) atomicOr(&st->flags, ST_GRID_EDOM);
+    if (!writer) return;
+    x[i] = xi;
+    y[i] = yi;
+    z[i] = zi;
+    vx[i] = vxo;
+    vy[i] = vyo;
+    vz[i] = vzo;
+    fxs[i] = fxo;
+    fys[i] = fyo;
+    fzs[i] = fzo;
+    W.usf[i] = usf_last;
+    double* ep = epart + (size_t)m * 5;
+    for (int k = 0; k < 5; k++) ep[k] = 0.0;
+}
+
 __global__ void __launch_bounds__(256) k_respa_final(int natom, const int* __restrict__ atom_site,
                                                      const SiteDev* __restrict__ sites, double* vx,
                                                      double* vy, double* vz,
@@ -435,13 +725,28 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
                        c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz, fxl, fyl,   \
                        fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart,        \
                        c.status, c.rep_delt, W, SN)
+#define RESPA_TW(NW)                                                                         \
+    hipLaunchKernelGGL(k_respa_inner_tw<NW>, dim3((c.nmol + 63) / 64), dim3(64 * NW), 0,          \
+                       c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z, vx, vy, vz,    \
+                       fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner,       \
+                       epart, c.status, c.rep_delt, W, SN)
 #define RESPA_TPM(NA, TPM)                                                                   \
     hipLaunchKernelGGL((k_respa_inner_tpm<NA, TPM>), dim3((c.nmol + 64 / TPM - 1) / (64 / TPM)), \
                        dim3(64), 0, c.stream, c.nmol, c.mols, species, terms, c.sites, x, y, z,  \
                        vx, vy, vz, fxl, fyl, fzl, fxs, fys, fzs, deltby2, delts, deltsby2,       \
                        nstep_inner, epart, c.status, c.rep_delt, W, SN)
-    if (c.max_mol_atoms <= 1) {
+    if (c.max_mol_atoms <= 1 && sf && sf->type == MSPMS_SF_TASOS) {
+        // single-site molecules with the Hermite wall: four lanes per site
+        hipLaunchKernelGGL(k_respa_inner_grid4, dim3((c.nmol + 15) / 16), dim3(64), 0, c.stream,
+                           c.nmol, c.mols, species, c.sites, x, y, z, vx, vy, vz, fxl, fyl, fzl,
+                           fxs, fys, fzs, deltby2, delts, deltsby2, nstep_inner, epart, c.status,
+                           c.rep_delt, W, SN);
+    } else if (c.max_mol_atoms <= 1) {
         RESPA_TPM(1, 1);  // one lane per single-site molecule
+    } else if (c.max_mol_atoms <= 3 && c.max_mol_terms <= 3) {
+        RESPA_TW(3);  // e.g. flexible three-site water: one term per wave
+    } else if (c.max_mol_atoms <= 4 && c.max_mol_terms <= 4) {
+        RESPA_TW(4);
     } else if (c.max_mol_atoms <= 4) {
         RESPA_TPM(4, 4);
     } else if (c.max_mol_atoms <= 8) {
@@ -456,6 +761,7 @@ void launch_respa_inner(const LaunchCtx& c, const SpeciesDev* species, const Ter
                            c.rep_delt, W, SN);
     }
 #undef RESPA_LDS
+#undef RESPA_TW
 #undef RESPA_TPM
 }
 

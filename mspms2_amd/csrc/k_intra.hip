@@ -324,6 +324,64 @@ __device__ void block_sum_u64(unsigned long long (&v)[NV], unsigned long long* s
     block_sum_t<unsigned long long, NV>(v, sh);
 }
 
+// The terms of one replica from its reduced partials d (ND slots, see
+// k_finalize) and counts q; uvac: the vacuum energies, or null.
+template <bool STEP>
+__device__ inline void form_terms(const RepDev& R, int r, const double* d,
+                                  const unsigned long long* q, const double* uvac,
+                                  mspms_long_terms& t, mspms_short_terms& u, double& ukin2) {
+#pragma clang fp contract(off)
+    constexpr int ND = STEP ? 17 : 11;
+    // full lists visit every inter-molecular pair twice
+    t.uvdw = 0.5 * d[0] + d[4];
+    t.ushift = (0.5 * d[1]) * R.shift4;   // src/frclong.c:985
+    t.ureal = 0.5 * d[2] + d[5];
+    t.uexcl = d[6];
+    double vir = 0.5 * d[3] + d[7];
+    t.ufourier = 0.0;
+    t.uself = 0.0;
+    t.uvacuum = 0.0;
+    t.uelec = 0.0;
+    if (R.charge_type == MSPMS_ELEC_EWALD) {
+        t.ufourier = d[8] * (R.vfactor * R.C);                        // :1048
+        t.uself = R.self_sum_q2 * (R.C * sqrt(R.kappa * R.kappa / MSPMS_PI));  // :1058
+        t.uelec = t.ureal + t.ufourier - t.uself - t.uexcl;           // :1061
+        if (uvac && R.ewald_bc == MSPMS_EWALD_BC_VACUUM) {
+            t.uvacuum = uvac[r];
+            t.uelec += t.uvacuum;                                     // :1125
+        }
+        vir += t.uelec;                                               // :1128
+    } else if (R.charge_type == MSPMS_ELEC_WOLF) {
+        t.uself = R.self_sum_q2 * (R.C * R.wvc3);                     // :1140
+        t.uelec = t.ureal - t.uself - t.uexcl;                        // :1142
+    } else if (R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB) {
+        // every molecule pair twice; the Coulomb virial is its energy
+        // (src/elecfrc.c:134-135)
+        t.uelec = 0.5 * d[10];
+        vir += t.uelec;
+    }
+    t.usflj = d[9];
+    t.uotherff = 0.0;
+    t.uinter = t.uvdw + t.uelec + t.usflj + t.uotherff;               // :1223
+    t.virial_inter = vir;
+    t.n_lj_pairs = (int64_t)q[0];
+    t.n_elec_pairs = (int64_t)q[1];
+    t.lj_pair_hash = q[2];
+    t.elec_pair_hash = q[3];
+    u = mspms_short_terms{};
+    ukin2 = 0.0;
+    if (STEP) {
+        u.ubond = d[11];
+        u.uangle = d[12];
+        u.udih = d[13];
+        u.uimp = d[14];
+        u.uintra = u.ubond + u.uangle + u.udih + u.uimp;  // src/frcshort.c:87
+        u.virial_intra = d[15];
+        u.usflj = 0.0;  // a step reports the wall in its long terms
+        ukin2 = d[ND - 1] * 0.5;
+    }
+}
+
 // Every partial of frclong into its terms, nb workgroups per replica (one
 // partial of each kind per thread where the replica fits): each reduces a
 // strided share of the replica's partials, every load of a pass issued
@@ -426,6 +484,25 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     }
     block_sum_rows<double, ND, FIN_T>(d, rows, bsum);
     block_sum_rows<unsigned long long, 4, FIN_T>(q, rowsu, bsumu);
+    if (STEP && fb.hist && fb.hist->on) {
+        // a run recording every step's terms: the workgroup sums go to this
+        // step's slot of the history of sums; k_hist_terms forms the terms
+        // after the run (no fence or arrival count per step)
+        if (threadIdx.x == 0) {
+            const HistDev* h = fb.hist;
+            const int t = *h->tick - 1;
+            if (t >= 0 && t < h->cap) {
+                double* P = h->part + (((size_t)t * h->nrep + r) * nb + b) * FIN_ND;
+                unsigned long long* Q = h->partu + (((size_t)t * h->nrep + r) * nb + b) * 4;
+                for (int k = 0; k < ND; k++) P[k] = bsum[k];
+                for (int k = 0; k < 4; k++) Q[k] = bsumu[k];
+                if (b == 0 && h->uvac && uvac) h->uvac[(size_t)t * h->nrep + r] = uvac[r];
+            }
+            // the cluster kernels of this call have read the rebuild request
+            if (blockIdx.x == 0 && lstate) lstate[LS_NEED] = 0;
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         double* P = fb.part + ((size_t)r * nb + b) * FIN_ND;
         unsigned long long* Q = fb.partu + ((size_t)r * nb + b) * 4;
@@ -474,66 +551,51 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
     for (int k = 0; k < 4; k++) q[k] = totu[k];
 
     mspms_long_terms t;
-    // full lists visit every inter-molecular pair twice
-    t.uvdw = 0.5 * d[0] + d[4];
-    t.ushift = (0.5 * d[1]) * R.shift4;   // src/frclong.c:985
-    t.ureal = 0.5 * d[2] + d[5];
-    t.uexcl = d[6];
-    double vir = 0.5 * d[3] + d[7];
-    t.ufourier = 0.0;
-    t.uself = 0.0;
-    t.uvacuum = 0.0;
-    t.uelec = 0.0;
-    if (R.charge_type == MSPMS_ELEC_EWALD) {
-        t.ufourier = d[8] * (R.vfactor * R.C);                        // :1048
-        t.uself = R.self_sum_q2 * (R.C * sqrt(R.kappa * R.kappa / MSPMS_PI));  // :1058
-        t.uelec = t.ureal + t.ufourier - t.uself - t.uexcl;           // :1061
-        if (uvac && R.ewald_bc == MSPMS_EWALD_BC_VACUUM) {
-            t.uvacuum = uvac[r];
-            t.uelec += t.uvacuum;                                     // :1125
-        }
-        vir += t.uelec;                                               // :1128
-    } else if (R.charge_type == MSPMS_ELEC_WOLF) {
-        t.uself = R.self_sum_q2 * (R.C * R.wvc3);                     // :1140
-        t.uelec = t.ureal - t.uself - t.uexcl;                        // :1142
-    } else if (R.charge_type == MSPMS_ELEC_SIMPLE_COULOMB) {
-        // every molecule pair twice; the Coulomb virial is its energy
-        // (src/elecfrc.c:134-135)
-        t.uelec = 0.5 * d[10];
-        vir += t.uelec;
-    }
-    t.usflj = d[9];
-    t.uotherff = 0.0;
-    t.uinter = t.uvdw + t.uelec + t.usflj + t.uotherff;               // :1223
-    t.virial_inter = vir;
-    t.n_lj_pairs = (int64_t)q[0];
-    t.n_elec_pairs = (int64_t)q[1];
-    t.lj_pair_hash = q[2];
-    t.elec_pair_hash = q[3];
+    mspms_short_terms u;
+    double uk2;
+    form_terms<STEP>(R, r, d, q, uvac, t, u, uk2);
     out[r] = t;
     if (STEP) {
-        mspms_short_terms u;
-        u.ubond = d[11];
-        u.uangle = d[12];
-        u.udih = d[13];
-        u.uimp = d[14];
-        u.uintra = u.ubond + u.uangle + u.udih + u.uimp;  // src/frcshort.c:87
-        u.virial_intra = d[15];
-        u.usflj = 0.0;  // a step reports the wall in its long terms
         sout[r] = u;
-        if (ukin) ukin[r] = d[ND - 1] * 0.5;
-        // this step's terms appended to the run's histories
-        if (fb.hist && fb.hist->on) {
-            const HistDev h = *fb.hist;
-            const int k = fb.hstep[r];
-            if (k < h.cap) {
-                const size_t slot = (size_t)k * h.nrep + r;
-                if (h.lh) h.lh[slot] = t;
-                if (h.sh) h.sh[slot] = u;
-                if (h.kh) h.kh[slot] = ukin ? ukin[r] : 0.0;
-            }
-            fb.hstep[r] = k + 1;
-        }
+        if (ukin) ukin[r] = uk2;
+    }
+}
+
+// The terms of every step of a run that recorded histories (after it):
+// one wave per (step, replica) adds the step's nb workgroup sums as the
+// step's last workgroup would have (lane bb takes workgroup bb, wave sum),
+// forms the terms and writes the histories; the last step's terms also go
+// to the run's outputs.
+__global__ void __launch_bounds__(64) k_hist_terms(const RepDev* __restrict__ reps, HistDev h,
+                                                   int nb, int nsteps,
+                                                   mspms_long_terms* __restrict__ lterms,
+                                                   mspms_short_terms* __restrict__ sterms,
+                                                   double* __restrict__ ukin) {
+    constexpr int ND = 17;
+    const int t = blockIdx.x / h.nrep, r = blockIdx.x % h.nrep, lane = threadIdx.x;
+    double d[ND];
+    unsigned long long q[4];
+    const double* P = h.part + ((size_t)t * h.nrep + r) * nb * FIN_ND;
+    const unsigned long long* Q = h.partu + ((size_t)t * h.nrep + r) * nb * 4;
+#pragma unroll
+    for (int k = 0; k < ND; k++) d[k] = wave_sum64(lane < nb ? P[lane * FIN_ND + k] : 0.0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = wave_sum64(lane < nb ? Q[lane * 4 + k] : 0ull);
+    if (lane != 0) return;
+    mspms_long_terms lt;
+    mspms_short_terms st;
+    double uk;
+    // the step's vacuum energy is in slot r of its row of h.uvac
+    form_terms<true>(reps[r], r, d, q, h.uvac ? h.uvac + (size_t)t * h.nrep : nullptr, lt, st,
+                     uk);
+    const size_t slot = (size_t)t * h.nrep + r;
+    if (h.lh) h.lh[slot] = lt;
+    if (h.sh) h.sh[slot] = st;
+    if (h.kh) h.kh[slot] = uk;
+    if (t == nsteps - 1) {
+        lterms[r] = lt;
+        sterms[r] = st;
+        if (ukin) ukin[r] = uk;
     }
 }
 
@@ -918,6 +980,12 @@ void launch_vacuum(const LaunchCtx& c, const SpeciesDev* species, const double* 
                    double* uvac, double* gq) {
     hipLaunchKernelGGL(k_vacuum, dim3(c.nrep), dim3(1024), 0, c.stream, c.reps, c.mols, species,
                        c.sites, x, y, z, fx, fy, fz, uvac, gq);
+}
+
+void launch_hist_terms(const LaunchCtx& c, const HistDev& h, int nsteps, void* lterms,
+                       void* sterms, double* ukin) {
+    hipLaunchKernelGGL(k_hist_terms, dim3(nsteps * h.nrep), dim3(64), 0, c.stream, c.reps, h, h.nb,
+                       nsteps, (mspms_long_terms*)lterms, (mspms_short_terms*)sterms, ukin);
 }
 
 void launch_set_hist(const LaunchCtx& c, HistDev* d, const HistDev& v, int* hstep) {

@@ -335,3 +335,83 @@ def test_run_without_histories_equals_single_steps(precision):
     for k in ("uvdw", "ureal", "ufourier", "uinter", "virial_inter"):
         assert abs(t1[k] - t2[k]) <= 1e-10 * (abs(t2[k]) + abs(t2["ureal"])), k
     assert t1["n_lj_pairs"] == t2["n_lj_pairs"]
+
+
+def _histories_vs_steps(s, nin, v0, precision, nsteps=6):
+    """A device run of nsteps recording histories against nsteps single
+    steps from the same start, each step's terms read after it."""
+    import torch
+    from mspms2_amd import abi
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    res = {}
+    for mode in ("run", "steps"):
+        eng = Engine(s, precision=precision)
+        x0 = np.stack([s.xx, s.yy, s.zz])
+        f0, _ = eng.frclong(*x0)
+        fs0, _ = eng.frcshort(*x0)
+        pos, vel, fl, fs = t(x0), t(v0), t(f0), t(fs0)
+        lt = torch.zeros(LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        st = torch.zeros(SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+        uk = torch.zeros(1, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        if mode == "run":
+            lh = torch.zeros(nsteps * LONG_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+            sh = torch.zeros(nsteps * SHORT_TERMS_BYTES // 8, dtype=torch.float64, device=dev)
+            kh = torch.zeros(nsteps, dtype=torch.float64, device=dev)
+            eng.vver_run_device(pos, vel, fl, fs, s.delt, nin, nsteps, lt, st, uk, lhist=lh,
+                                shist=sh, ukin_hist=kh, graph=True)
+            torch.cuda.synchronize()
+            eng.sync()
+            L = (abi.LongTerms * nsteps).from_buffer_copy(lh.cpu().numpy().tobytes())
+            S = (abi.ShortTerms * nsteps).from_buffer_copy(sh.cpu().numpy().tobytes())
+            hist = [(L[k].as_dict(), S[k].as_dict(), float(kh[k].item())) for k in range(nsteps)]
+        else:
+            hist = []
+            for _ in range(nsteps):
+                eng.vver_step_device(pos, vel, fl, fs, s.delt, nin, lt, st, uk)
+                torch.cuda.synchronize()
+                hist.append((abi.LongTerms.from_buffer_copy(lt.cpu().numpy().tobytes()).as_dict(),
+                             abi.ShortTerms.from_buffer_copy(st.cpu().numpy().tobytes()).as_dict(),
+                             float(uk.item())))
+            eng.sync()
+        last = (abi.LongTerms.from_buffer_copy(lt.cpu().numpy().tobytes()).as_dict(),
+                abi.ShortTerms.from_buffer_copy(st.cpu().numpy().tobytes()).as_dict(),
+                float(uk.item()))
+        res[mode] = (hist, last, pos.cpu().numpy())
+        eng.close()
+    (h1, last1, x1), (h2, last2, x2) = res["run"], res["steps"]
+    assert rel(x1, x2) <= 1e-12
+    for k in range(nsteps):
+        (a, b, ka), (c, d, kc) = h1[k], h2[k]
+        scale = abs(c["ureal"]) + abs(c["uvdw"]) + abs(c["uinter"])
+        for key in ("uvdw", "ureal", "ufourier", "uvacuum", "uelec", "uinter", "virial_inter"):
+            assert abs(a[key] - c[key]) <= 1e-10 * scale, (k, key, a[key], c[key])
+        for key in ("n_lj_pairs", "n_elec_pairs"):
+            assert a[key] == c[key], (k, key)
+        for key in ("ubond", "uangle", "uintra", "virial_intra"):
+            assert abs(b[key] - d[key]) <= 1e-10 * (abs(d[key]) + scale), (k, key)
+        assert abs(ka - kc) <= 1e-12 * abs(kc)
+    # the run's outputs are its last step's terms
+    assert last1[0] == h1[-1][0] and last1[1] == h1[-1][1] and last1[2] == h1[-1][2]
+    return h1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_histories_equal_single_steps(precision):
+    """Every step's terms recorded by a device run (each step stores its
+    workgroup sums; the terms are formed after the run) equal those of
+    single steps from the same start, step by step."""
+    rd, s, g = _load("md_water216_nve", 10)
+    _histories_vs_steps(s, rd.run.nstep_inner, g["v0"], precision)
+
+
+def test_run_histories_vacuum_boundary():
+    """The vacuum-boundary energy of every step is recorded with it (the
+    dipole term changes each step): the flexible water of md_water216_nve
+    with the Ewald sum's vacuum boundary (src/frclong.c:1065-1126)."""
+    rd, s, g = _load("md_water216_nve", 10)
+    s.params["fEwald_BC"] = 2
+    h = _histories_vs_steps(s, rd.run.nstep_inner, g["v0"], "fp64")
+    vac = [h[k][0]["uvacuum"] for k in range(len(h))]
+    assert len(set(vac)) > 1 and all(v != 0.0 for v in vac)
