@@ -81,3 +81,10 @@ __device__ void block_sum_rows(const T (&v)[NV], T* rows, T* tot) {
     }
     __syncthreads();
 }
+
+// A 32-bit float moved between lanes by a DPP pattern (all lanes active;
+// row and bank masks full).
+template <int CTRL>
+__device__ inline float dpp32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
