@@ -796,7 +796,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--fp64-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--skin", type=float, default=2.5,
+    ap.add_argument("--skin", type=float, default=5.0,
                     help="pair-list buffer in Angstrom (lists reused until a site moves half of it)")
     ap.add_argument("--replicas", type=int, default=256,
                     help="boxes in the batched-replica leg (0: skip it)")

@@ -93,8 +93,9 @@ struct ClusterBufs {
     double4* slot_x64;   // [8*maxcl] FP64 unwrapped (x,y,z,-), current call
     double4* slot_w64;   // [8*maxcl] FP64 wrapped at the last build (x,y,z,charge)
     double4* atom_x0;    // [natom] unwrapped position at the last build
-    int* lstate;         // [4] list state: LS_VALID, LS_NEED (rebuild wanted
-                         //   by this call), LS_BUILDS (builds so far)
+    int* lstate;         // [LS_N] list state: LS_VALID, LS_NEED (rebuild
+                         //   wanted by this call), LS_BUILDS (builds so far),
+                         //   LS_LAYOUT
     float4* slot_off;    // [8*maxcl] FP32 offset from centre + charge
     float4* slot_par;    // [8*maxcl] FP32 sigma, sqrt(eps) (0: no LJ),
                          //   molecule id and flags as int bits (SLOT_*)

@@ -31,6 +31,7 @@
 // offset becomes (wrapped - centre) + (x - x_build), continuous when the
 // atom crosses a box face.
 #include "engine.h"
+#include "wave.h"
 #include "../../include/mspms_gpu.h"
 
 namespace {
@@ -48,16 +49,39 @@ __global__ void __launch_bounds__(256) k_list_check(
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     // a device run recording histories: this call begins the next step
     if (t == 0 && hist && hist->on) *hist->tick += 1;
-    const bool valid = lstate[LS_VALID] != 0 && !force && lstate[LS_LAYOUT] == half;
+    // The kernel is a short chain of dependent loads on a few waves per CU:
+    // every load that depends on no other (list state, this thread's atom
+    // and slot) is issued before any is used, and only the slot's atom
+    // position waits for the slot's atom index.  Slot data read here before
+    // the first build are never used (the list is not valid then).
+    const bool in_atom = t < natom, in_slot = t < nslot;
+    const int ls_valid = lstate[LS_VALID], ls_layout = lstate[LS_LAYOUT];
+    const int ncl = min(*ncl_total, maxcl);
+    const int site = in_atom ? atom_site[t] : 0;
+    const double4 p0t = in_atom ? atom_x0[t] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double xt = in_atom ? x[t] : 0.0, yt = in_atom ? y[t] : 0.0, zt = in_atom ? z[t] : 0.0;
+    const int o = in_slot ? slot_orig[t] : -1;
+    const double4 w = in_slot ? slot_w64[t] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 C = in_slot ? cl_center[t / MSPMS_CL] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const bool valid = ls_valid != 0 && !force && ls_layout == half;
+    const bool live_cl = valid && in_slot && t < MSPMS_CL * ncl && o >= 0;
+    double4 p0o = make_double4(0.0, 0.0, 0.0, 0.0);
+    double xo = 0.0, yo = 0.0, zo = 0.0;
+    if (live_cl) {
+        p0o = atom_x0[o];
+        xo = x[o];
+        yo = y[o];
+        zo = z[o];
+    }
     bool need = !valid;
-    if (half && t < nslot) {
+    if (half && in_slot) {
         // the half-list kernel's slot-ordered forces start from zero
         fslot[t] = 0.0;
         fslot[(size_t)nslot + t] = 0.0;
         fslot[2 * (size_t)nslot + t] = 0.0;
     }
-    if (t < natom) {
-        const bool full_ghost = sites[atom_site[t]].ghost == MSPMS_GHOST_FULL;
+    if (in_atom) {
+        const bool full_ghost = sites[site].ghost == MSPMS_GHOST_FULL;
         // a full ghost is not binned: its force starts from zero
         // (src/frclong.c:94,107); with one row per cluster pair every pair
         // force arrives by atomic addition, onto zero
@@ -67,40 +91,36 @@ __global__ void __launch_bounds__(256) k_list_check(
             fz[t] = 0.0;
         }
         if (!full_ghost && valid) {
-            const double4 p0 = atom_x0[t];
-            const double dx = x[t] - p0.x, dy = y[t] - p0.y, dz = z[t] - p0.z;
+            const double dx = xt - p0t.x, dy = yt - p0t.y, dz = zt - p0t.z;
             need = !(dx * dx + dy * dy + dz * dz <= half_skin2);  // NaN: rebuild
         }
     }
     // the cluster's current box over its valid slots (8 adjacent lanes)
     const float BIG = 3.0e38f;
     float lo[3] = {BIG, BIG, BIG}, hi[3] = {-BIG, -BIG, -BIG};
-    const bool live_cl = valid && t < nslot && t < MSPMS_CL * min(*ncl_total, maxcl);
     if (live_cl) {
-        const int o = slot_orig[t];
-        if (o >= 0) {
-            const double4 p0 = atom_x0[o];
-            const double xi = x[o], yi = y[o], zi = z[o];
-            const double4 w = slot_w64[t];
-            const double4 C = cl_center[t / MSPMS_CL];
-            slot_x64[t] = make_double4(xi, yi, zi, 0.0);
-            const float4 off = make_float4((float)((w.x - C.x) + (xi - p0.x)),
-                                           (float)((w.y - C.y) + (yi - p0.y)),
-                                           (float)((w.z - C.z) + (zi - p0.z)), (float)w.w);
-            slot_off[t] = off;
-            lo[0] = hi[0] = off.x;
-            lo[1] = hi[1] = off.y;
-            lo[2] = hi[2] = off.z;
-        }
+        slot_x64[t] = make_double4(xo, yo, zo, 0.0);
+        const float4 off = make_float4((float)((w.x - C.x) + (xo - p0o.x)),
+                                       (float)((w.y - C.y) + (yo - p0o.y)),
+                                       (float)((w.z - C.z) + (zo - p0o.z)), (float)w.w);
+        slot_off[t] = off;
+        lo[0] = hi[0] = off.x;
+        lo[1] = hi[1] = off.y;
+        lo[2] = hi[2] = off.z;
     }
+    // over the cluster's 8 adjacent lanes: lane ^ 1, lane ^ 2, then the
+    // mirror within the 8-lane half row (DPP moves, no LDS crossbar)
 #pragma unroll
-    for (int m = 1; m < MSPMS_CL; m <<= 1)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fminf(lo[d], __shfl_xor(lo[d], m, 64));
-            hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], m, 64));
-        }
-    if (live_cl && (t & (MSPMS_CL - 1)) == 0) {
+    for (int d = 0; d < 3; d++) {
+        lo[d] = fminf(lo[d], dpp32<0xB1>(lo[d]));
+        hi[d] = fmaxf(hi[d], dpp32<0xB1>(hi[d]));
+        lo[d] = fminf(lo[d], dpp32<0x4E>(lo[d]));
+        hi[d] = fmaxf(hi[d], dpp32<0x4E>(hi[d]));
+        lo[d] = fminf(lo[d], dpp32<0x141>(lo[d]));
+        hi[d] = fmaxf(hi[d], dpp32<0x141>(hi[d]));
+    }
+    // the cluster's first lane writes its box
+    if (valid && in_slot && t < MSPMS_CL * ncl && (t & (MSPMS_CL - 1)) == 0) {
         const int c = t / MSPMS_CL;
         if (lo[0] <= hi[0]) {
             cl_box[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]),
@@ -205,14 +225,26 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
     if (!lstate[LS_NEED]) return;
     __shared__ int sh[32];
     int carry = 0;
-    for (int base = 0; base < ncell; base += 1024) {
-        int c = base + threadIdx.x;
-        int v = (c < ncell) ? cell_count[c] : 0;
+    // a thread scans 16 consecutive cells per pass: one workgroup scan per
+    // 16,384 cells (a tube of 16 long boxes has ~50,000)
+    constexpr int PER = 16;
+    for (int base = 0; base < ncell; base += 1024 * PER) {
+        const int c0 = base + threadIdx.x * PER;
+        int v[PER], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            v[k] = c0 + k < ncell ? cell_count[c0 + k] : 0;
+            sum += v[k];
+        }
         int tot;
-        int ex = block_exscan<1024>(v, sh, tot);
-        if (c < ncell) {
-            cell_start[c] = carry + ex;
-            cell_count[c] = 0;
+        int run = carry + block_exscan<1024>(sum, sh, tot);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            if (c0 + k < ncell) {
+                cell_start[c0 + k] = run;
+                cell_count[c0 + k] = 0;
+            }
+            run += v[k];
         }
         carry += tot;
     }
@@ -343,13 +375,14 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
         }
     }
+    // the half extents, rounded up (every lane of the cluster holds them)
+    const float4 ext = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.000001f + 1e-6f,
+                                   (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
+                                   (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
     if (!live) return;
     double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
     if (k == 0) {
         cl_center[c] = make_double4(cx, cy, cz, (double)col_rep[col]);
-        const float4 ext = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.000001f + 1e-6f,
-                                       (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
-                                       (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
         cl_ext[c] = ext;
         cl_box[2 * c] = make_float4(0.f, 0.f, 0.f, 0.f);
         cl_box[2 * c + 1] = ext;
@@ -372,8 +405,18 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
 }
 
 // One wave per i-cluster.  Candidate columns cover the x/y range of the
-// i-box widened by rc_list (periodic, each wrapped column at most once);
-// every cluster of a candidate column is tested box-to-box.  Entry layout:
+// i-box widened by rc_list (periodic, each wrapped column at most once).
+// A column's clusters are in wrapped-z order (cells by cz, atoms in a cell
+// by wrapped z), so the j-boxes that can come within rc_list of the i-box
+// lie in the cells that meet [Ci.z - Ei.z - rl, Ci.z + Ei.z + rl] (modulo
+// Lz, with a small margin): their clusters, plus the one that straddles the
+// window if it holds no atom, are the candidates (one or two
+// index ranges per column, visited in ascending order).  A cluster whose z
+// image is ambiguous (Ei.z + Ej.z + rl >= Lz/2) spans at least Lz minus the
+// window, so its atoms meet the window too.  A window as long as the box
+// (Ei.z + rl >= Lz/2) takes whole columns.  Each candidate is tested
+// box-to-box; the window only drops clusters that fail that test, so the
+// row is the same as with whole columns.  Entry layout:
 //   bits 0-1 x image + 1, 2-3 y image + 1, 4-5 z image + 1,
 //   bit 6 "ambiguous image: apply the minimum image per pair",
 //   bits 7-31 j-cluster index,
@@ -382,6 +425,8 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                                                const int* __restrict__ cl_col,
                                                const int* __restrict__ col_cl0,
                                                const int* __restrict__ col_rep,
+                                               const int* __restrict__ col_cell0,
+                                               const int* __restrict__ cell_start,
                                                const RepDev* __restrict__ reps,
                                                const double4* __restrict__ cl_center,
                                                const float4* __restrict__ cl_ext,
@@ -416,6 +461,36 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
         amby = true;
     }
     const double rl2 = rl * rl;
+    // the z window in cells: [p1, q1], and [p2, ncz - 1] when it wraps.  An
+    // atom's cell is floor(wz / h) (k_bin), monotone in wz, so the cells of
+    // the window's ends bound the cells of the atoms inside it; the margin mz
+    // covers the FP32 widening of the cluster extents and the rounding of
+    // the window's ends.
+    const int ncz = R.ncz;
+    const double Lz = R.L[2], ihz = R.inv_h[2], mz = 1e-5 * (Lz + rl + 1.0);
+    const double zlo = Ci.z - (double)Ei.z - rl - mz, zhi = Ci.z + (double)Ei.z + rl + mz;
+    bool zwin = (double)Ei.z + rl + mz < 0.5 * Lz;
+    int p1 = 0, q1 = ncz - 1, p2 = ncz;
+    if (zwin) {
+        if (zlo < 0.0) {  // [0, zhi] and [zlo + Lz, Lz)
+            q1 = (int)floor(zhi * ihz);
+            p2 = (int)floor((zlo + Lz) * ihz);
+        } else if (zhi >= Lz) {  // [0, zhi - Lz] and [zlo, Lz)
+            q1 = (int)floor((zhi - Lz) * ihz);
+            p2 = (int)floor(zlo * ihz);
+        } else {
+            p1 = clampi((int)floor(zlo * ihz), 0, ncz - 1);
+            q1 = (int)floor(zhi * ihz);
+        }
+        q1 = clampi(q1, p1, ncz - 1);
+        p2 = clampi(p2, 0, ncz);
+        if (p2 <= q1 + 1) zwin = false;  // the two ranges meet: whole column
+    }
+    if (!zwin) {
+        p1 = 0;
+        q1 = ncz - 1;
+        p2 = ncz;
+    }
     int n = 0;
     float4* row = plist + (size_t)ci * cap;
     // candidate columns in (y, x) order, 64 per batch; their clusters are
@@ -424,7 +499,8 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
     const int ncx_c = colx_hi - colx_lo + 1, ncand = ncx_c * (coly_hi - coly_lo + 1);
     for (int cb = 0; cb < ncand; cb += 64) {
         const int nb = min(64, ncand - cb);
-        int a = 0, cnt = 0, sx = 0, sy = 0;
+        // this lane's column: clusters [a, a + n1) and [a2, a2 + cnt - n1)
+        int a = 0, a2 = 0, n1 = 0, cnt = 0, sx = 0, sy = 0;
         if (lane < nb) {
             const int q = cb + lane;
             const int cxu = colx_lo + q % ncx_c, cyu = coly_lo + q / ncx_c;
@@ -433,8 +509,27 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
             sx = (cxu - cxw) / R.ncx;
             sy = (cyu - cyw) / R.ncy;
             const int col = R.col0 + cyw * R.ncx + cxw;
-            a = col_cl0[col];
-            cnt = col_cl0[col + 1] - a;
+            // clusters past maxcl (an overflowed build) do not exist
+            const int k0 = min(col_cl0[col], ncl), k1 = min(col_cl0[col + 1], ncl);
+            if (zwin) {
+                // cluster of the atom at column position m: k0 + m / 8
+                const int c0 = col_cell0[col], m0 = cell_start[c0];
+                a = min(col_cl0[col] + ((cell_start[c0 + p1] - m0) >> 3), k1);
+                const int e1 = min(col_cl0[col] + ((cell_start[c0 + q1 + 1] - m0 + 7) >> 3), k1);
+                n1 = max(e1 - a, 0);
+                int e2 = e1;
+                a2 = e1;
+                if (p2 < ncz) {
+                    a2 = max(min(col_cl0[col] + ((cell_start[c0 + p2] - m0) >> 3), k1), e1);
+                    e2 = max(k1, a2);
+                }
+                cnt = n1 + (e2 - a2);
+            } else {
+                a = k0;
+                n1 = max(k1 - k0, 0);
+                a2 = k0 + n1;
+                cnt = n1;
+            }
         }
         // inclusive prefix of the column sizes over the lanes
         int incl = cnt;
@@ -455,13 +550,15 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
                 if (probe < nb && v <= f) qc += step;
             }
             const int qa = __shfl(a, qc & 63, 64), qcnt = __shfl(cnt, qc & 63, 64);
+            const int qa2 = __shfl(a2, qc & 63, 64), qn1 = __shfl(n1, qc & 63, 64);
             const int qx = __shfl(sx, qc & 63, 64), qy = __shfl(sy, qc & 63, 64);
             const int qincl = __shfl(incl, qc & 63, 64);
             bool hit = false;
             uint32_t entry = 0;
             float4 sep = make_float4(0.f, 0.f, 0.f, 0.f);
             if (f < total) {
-                const int cj = qa + (f - (qincl - qcnt));
+                const int o = f - (qincl - qcnt);
+                const int cj = o < qn1 ? qa + o : qa2 + (o - qn1);
                 double4 Cj = cl_center[cj];
                 float4 Ej = cl_ext[cj];
                 double dxc = Ci.x - (Cj.x + qx * R.L[0]);
@@ -545,6 +642,6 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                        b.cl_box, b.lstate);
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
-                       b.col_cl0, b.col_rep, c.reps, b.cl_center, b.cl_ext, b.plist, b.plist_n,
-                       b.cap, c.status, b.lstate, b.half);
+                       b.col_cl0, b.col_rep, b.col_cell0, b.cell_start, c.reps, b.cl_center,
+                       b.cl_ext, b.plist, b.plist_n, b.cap, c.status, b.lstate, b.half);
 }
