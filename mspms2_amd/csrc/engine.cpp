@@ -244,9 +244,11 @@ struct mspms_ctx {
         void *pos, *vel, *fl, *fs, *lt, *st, *uk;
         double delt;
         int nstep_inner;
+        int per = 1;  // steps per graph
         bool operator==(const GraphKey& o) const {
             return pos == o.pos && vel == o.vel && fl == o.fl && fs == o.fs && lt == o.lt &&
-                   st == o.st && uk == o.uk && delt == o.delt && nstep_inner == o.nstep_inner;
+                   st == o.st && uk == o.uk && delt == o.delt && nstep_inner == o.nstep_inner &&
+                   per == o.per;
         }
     } gkey{};
 };
@@ -1807,10 +1809,16 @@ extern "C" int mspms_vver_step_device(mspms_ctx* c, double* pos, double* vel, do
     return enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
 }
 
-// nsteps RESPA outer steps; with use_graph the step is captured once as a
-// HIP graph (on the context stream) and replayed on `stream`.  Optional
-// histories receive each step's terms (nsteps * nrep entries) and kinetic
-// energies (nsteps * nrep doubles).
+// Steps per captured graph of a device run: four steps in one graph drop
+// three of the four graph boundaries per four steps (0.6 us per step at the
+// 12,288-site water step of ~97 us, A/B on one box).
+constexpr int VVER_GRAPH_STEPS = 4;
+
+// nsteps RESPA outer steps; with use_graph VVER_GRAPH_STEPS steps (one step
+// for shorter runs) are captured once as a HIP graph (on the context stream)
+// and replayed on `stream`, the last nsteps % VVER_GRAPH_STEPS steps
+// enqueued directly.  Optional histories receive each step's terms (nsteps
+// * nrep entries) and kinetic energies (nsteps * nrep doubles).
 extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, double* fl,
                                      double* fs, double delt, int nstep_inner, int nsteps,
                                      mspms_long_terms* lterms, mspms_short_terms* sterms,
@@ -1824,10 +1832,16 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
     HIPC(hipSetDevice(c->device));
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const int nr = c->nrep;
+    const int per = nsteps >= VVER_GRAPH_STEPS ? VVER_GRAPH_STEPS : 1;
     if (use_graph) {
-        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner};
+        mspms_ctx::GraphKey k{pos, vel, fl, fs, lterms, sterms, ukin, delt, nstep_inner, per};
         int rc = ensure_graph(c, s, GRAPH_VVER, k, [&](hipStream_t cs) {
-            return enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, cs);
+            for (int q = 0; q < per; q++) {
+                int rq = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms,
+                                      ukin, cs);
+                if (rq) return rq;
+            }
+            return 0;
         });
         if (rc) return rc;
     }
@@ -1872,12 +1886,18 @@ extern "C" int mspms_vver_run_device(mspms_ctx* c, double* pos, double* vel, dou
         HIPC(hipGetLastError());
         reopen.hist_on = true;
     }
-    for (int t = 0; t < nsteps; t++) {
-        if (gate && t == nsteps - 1) {
+    for (int t = 0; t < nsteps;) {
+        // a graph of `per` steps while that many remain, then single steps
+        const bool grp = use_graph && nsteps - t >= per;
+        const int adv = grp ? per : 1;
+        if (gate && t + adv >= nsteps) {
+            // the launch that holds the last step reduces its terms (the
+            // other steps of a graph reduce theirs too: the last one wins)
             HIPC(hipMemsetD32Async((hipDeviceptr_t)c->d_fin_on.p, 1, 1, s));
             reopen.armed = false;
         }
-        if (use_graph) {
+        t += adv;
+        if (grp) {
             HIPC(hipGraphLaunch(c->graph_exec, s));
         } else {
             int rc = enqueue_vver(c, pos, vel, fl, fs, delt, nstep_inner, lterms, sterms, ukin, s);
