@@ -391,6 +391,83 @@ __global__ void __launch_bounds__(64) k_respa_inner_tpm(
     }
 }
 
+// 1 / sqrt(x) within an ulp or two: the hardware estimate (v_rsq_f64) and
+// two Newton steps, a shorter dependency chain than sqrt and a division.
+__device__ inline double rsqrt_nr(double x) {
+#pragma clang fp contract(off)
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+
+// bonded_term for the inner loop's latency chain: the harmonic bond and the
+// harmonic angle with reciprocal square roots in place of sqrt and division
+// (the same expressions otherwise; results differ from bonded_term's in the
+// last bits); every other term is bonded_term's.
+__device__ inline int bonded_term_inner(const TermDev& T, int base, const double* xx,
+                                        const double* yy, const double* zz, double* fxs,
+                                        double* fys, double* fzs, double* u, double* vir,
+                                        int* singular, int stride) {
+#pragma clang fp contract(off)
+    const double* pr = T.para;
+    if (T.kind == TERM_BOND && T.type == MSPMS_BOND_HARMONIC) {
+        const int i1 = base + T.idx[0] * stride, i2 = base + T.idx[1] * stride;
+        const double rx = xx[i1] - xx[i2], ry = yy[i1] - yy[i2], rz = zz[i1] - zz[i2];
+        const double rsq = rx * rx + ry * ry + rz * rz, ri = rsqrt_nr(rsq), r = rsq * ri;
+        const double dr = r - pr[1];
+        *u = 0.5 * pr[0] * dr * dr;
+        const double fij = -pr[0] * dr * ri;
+        *vir = fij * rsq;
+        fxs[i1] += fij * rx;
+        fys[i1] += fij * ry;
+        fzs[i1] += fij * rz;
+        fxs[i2] -= fij * rx;
+        fys[i2] -= fij * ry;
+        fzs[i2] -= fij * rz;
+        return 0;
+    }
+    if (T.kind == TERM_ANGLE && T.type == MSPMS_ANGLE_HARMONIC) {
+        const int ia = base + T.idx[0] * stride, ib = base + T.idx[1] * stride,
+                  ic = base + T.idx[2] * stride;
+        double xab = xx[ia] - xx[ib], yab = yy[ia] - yy[ib], zab = zz[ia] - zz[ib];
+        double xbc = xx[ic] - xx[ib], ybc = yy[ic] - yy[ib], zbc = zz[ic] - zz[ib];
+        const double rrab = rsqrt_nr(xab * xab + yab * yab + zab * zab);
+        const double rrbc = rsqrt_nr(xbc * xbc + ybc * ybc + zbc * zbc);
+        xab = xab * rrab;
+        yab = yab * rrab;
+        zab = zab * rrab;
+        xbc = xbc * rrbc;
+        ybc = ybc * rrbc;
+        zbc = zbc * rrbc;
+        const double cost = xab * xbc + yab * ybc + zab * zbc;
+        const double theta = acos(cost);
+        // 1 / max(1e-8, sin theta)
+        const double s2 = 1.0 - cost * cost;
+        const double rsint = s2 > 1.0e-16 ? rsqrt_nr(s2) : 1.0e8;
+        const double dth = theta - pr[1];
+        *u = 0.5 * pr[0] * dth * dth;
+        const double gam = pr[0] * dth * rsint;
+        const double fxa = gam * (xbc - xab * cost) * rrab, fya = gam * (ybc - yab * cost) * rrab,
+                     fza = gam * (zbc - zab * cost) * rrab;
+        const double fxc = gam * (xab - xbc * cost) * rrbc, fyc = gam * (yab - ybc * cost) * rrbc,
+                     fzc = gam * (zab - zbc * cost) * rrbc;
+        *vir = 0.0;
+        fxs[ia] += fxa;
+        fys[ia] += fya;
+        fzs[ia] += fza;
+        fxs[ib] = fxs[ib] - fxa - fxc;
+        fys[ib] = fys[ib] - fya - fyc;
+        fzs[ib] = fzs[ib] - fza - fzc;
+        fxs[ic] += fxc;
+        fys[ic] += fyc;
+        fzs[ic] += fzc;
+        return 1;
+    }
+    return bonded_term(T, base, xx, yy, zz, fxs, fys, fzs, u, vir, singular, stride);
+}
+
 // The inner loop with one bonded term per wave: a workgroup of NW waves
 // takes 64 molecules (lane = molecule); wave w owns site w of each molecule
 // (kicks, drift) and computes its term w into its own LDS copy of the
@@ -474,7 +551,8 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
         if (term) {
             double u = 0.0, v = 0.0;
             int sing = 0;
-            const int cls = bonded_term(T, mi, lx, ly, lz, cfx[w], cfy[w], cfz[w], &u, &v, &sing, 64);
+            const int cls =
+                bonded_term_inner(T, mi, lx, ly, lz, cfx[w], cfy[w], cfz[w], &u, &v, &sing, 64);
             if (sing) atomicOr(&st->flags, ST_SINGULAR);
             e[cls] += u;
             vir += v;
