@@ -143,7 +143,8 @@ struct mspms_ctx {
     // clustering
     ClusterBufs cb{};
     DevBuf<int> d_atom_cell, d_atom_slot, d_cell_count, d_cell_start, d_cell_members,
-        d_col_cell0, d_col_rep, d_col_cl0, d_cl_col, d_ncl_total, d_slot_orig, d_plist_n;
+        d_col_cell0, d_col_rep, d_col_cl0, d_cl_col, d_ncl_total, d_slot_orig, d_plist_n,
+        d_bin_ctr;
     DevBuf<double> d_atom_wz;
     DevBuf<int2> d_slot_info;
     DevBuf<double4> d_slot_x64, d_slot_w64, d_cl_center, d_atom_x0;
@@ -562,6 +563,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_ncl_total.release();
     c->d_slot_orig.release();
     c->d_plist_n.release();
+    c->d_bin_ctr.release();
     c->d_atom_wz.release();
     c->d_slot_info.release();
     c->d_slot_x64.release();
@@ -992,8 +994,9 @@ int build(mspms_ctx* c) {
         (rc = c->d_slot_par.ensure(maxcl * MSPMS_CL)) || (rc = c->d_cl_center.ensure(maxcl)) ||
         (rc = c->d_cl_ext.ensure(maxcl)) || (rc = c->d_cl_box.ensure(2 * maxcl)) || (rc = c->d_plist.ensure(maxcl * (size_t)cap)) ||
         (rc = c->d_plist_n.ensure(maxcl)) || (rc = c->d_atom_gslot.ensure(c->natom)) ||
-        (rc = c->d_fslot.ensure(3 * maxcl * MSPMS_CL)))
+        (rc = c->d_fslot.ensure(3 * maxcl * MSPMS_CL)) || (rc = c->d_bin_ctr.ensure(1)))
         return rc;
+    HIPC(hipMemsetAsync(c->d_bin_ctr.p, 0, sizeof(int), s));
     HIPC(hipMemsetAsync(c->d_cell_count.p, 0, sizeof(int) * (size_t)ncell, s));
     HIPC(hipMemsetAsync(c->d_lstate.p, 0, sizeof(int) * LS_N, s));  // no list yet
     b.atom_cell = c->d_atom_cell.p;
@@ -1023,6 +1026,7 @@ int build(mspms_ctx* c) {
     b.atom_gslot = c->d_atom_gslot.p;
     b.fslot = c->d_fslot.p;
     b.hist = nullptr;  // set with the finalize buffers below
+    b.bin_ctr = c->d_bin_ctr.p;
     b.maxcl = (int)maxcl;
     b.cap = cap;
     b.ncell = ncell;

@@ -117,6 +117,7 @@ struct ClusterBufs {
                          //   k_list_check, folded into the atoms' forces
                          //   after the pair kernel (SlotForces)
     HistDev* hist;       // the run's history descriptor (k_list_check ticks it)
+    int* bin_ctr;        // [1] k_bin_scan arrival count (zero between builds)
     int maxcl, cap, ncell, ncol;
     int half;            // layout this call wants: 1 = each cluster pair in one
                          //   row (mixed pair kernel, j forces by atomics),

@@ -10,8 +10,8 @@
 //              the skin -> rebuild request; per slot: refresh of the slot
 //              positions for a reused list (overwritten when it rebuilds)
 // and, only when a rebuild is requested (the others return at once):
-//   k_bin      per atom: wrap into the box, integer cell (cx,cy,cz), count
-//   k_scan     one workgroup: exclusive scans over cells and columns
+//   k_bin_scan per atom: wrap into the box, integer cell (cx,cy,cz), count;
+//              the last workgroup: exclusive scans over cells and columns
 //              (cells of a column are contiguous, ordered by cz)
 //   k_scatter  per atom: cell member lists
 //   k_place    per atom: exact rank inside its cell by (wrapped z, index)
@@ -144,17 +144,14 @@ __device__ inline double wrap(double x, double L, double invL) {
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__global__ void k_bin(int natom, const double* __restrict__ x, const double* __restrict__ y,
+__device__ inline void bin_atom(int i, const double* __restrict__ x, const double* __restrict__ y,
                       const double* __restrict__ z, const int* __restrict__ atom_site,
                       const int* __restrict__ atom_mol, const SiteDev* __restrict__ sites,
                       const MolDev* __restrict__ mols, const RepDev* __restrict__ reps,
                       int* __restrict__ atom_cell, int* __restrict__ atom_slot,
                       double* __restrict__ atom_wz, int* __restrict__ cell_count,
                       double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-                      int* __restrict__ atom_gslot, Status* st, const int* lstate) {
-    if (!lstate[LS_NEED]) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natom) return;
+                      int* __restrict__ atom_gslot, Status* st) {
     int s = atom_site[i];
     if (sites[s].ghost == MSPMS_GHOST_FULL) {
         // skipped by the pair loops (src/frclong.c:94,107); its force is
@@ -214,21 +211,19 @@ __device__ int block_exscan(int v, int* sh, int& total) {
     return ex;
 }
 
-// One workgroup: cell_start = exscan(cell_count), cell_count := 0;
-// per column the atom count, cluster count, cluster start and cl_col.
-__global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restrict__ cell_count,
-                                               int* __restrict__ cell_start,
-                                               const int* __restrict__ col_cell0,
-                                               int* __restrict__ col_cl0, int* __restrict__ cl_col,
-                                               int* __restrict__ ncl_total, int maxcl,
-                                               Status* st, const int* lstate) {
-    if (!lstate[LS_NEED]) return;
-    __shared__ int sh[32];
+// By one workgroup of NT threads: cell_start = exscan(cell_count),
+// cell_count := 0; per column the atom count, cluster count, cluster start
+// and cl_col.
+template <int NT>
+__device__ void scan_cells(int ncell, int ncol, int* __restrict__ cell_count,
+                           int* __restrict__ cell_start, const int* __restrict__ col_cell0,
+                           int* __restrict__ col_cl0, int* __restrict__ cl_col,
+                           int* __restrict__ ncl_total, int maxcl, Status* st, int* sh) {
     int carry = 0;
     // a thread scans 16 consecutive cells per pass: one workgroup scan per
-    // 16,384 cells (a tube of 16 long boxes has ~50,000)
+    // 16 NT cells (a tube of 16 long boxes has ~50,000)
     constexpr int PER = 16;
-    for (int base = 0; base < ncell; base += 1024 * PER) {
+    for (int base = 0; base < ncell; base += NT * PER) {
         const int c0 = base + threadIdx.x * PER;
         int v[PER], sum = 0;
 #pragma unroll
@@ -237,7 +232,7 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
             sum += v[k];
         }
         int tot;
-        int run = carry + block_exscan<1024>(sum, sh, tot);
+        int run = carry + block_exscan<NT>(sum, sh, tot);
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             if (c0 + k < ncell) {
@@ -251,7 +246,7 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
     if (threadIdx.x == 0) cell_start[ncell] = carry;
     __syncthreads();
     carry = 0;
-    for (int base = 0; base < ncol; base += 1024) {
+    for (int base = 0; base < ncol; base += NT) {
         int g = base + threadIdx.x;
         int n = 0;
         if (g < ncol) {
@@ -259,7 +254,7 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
             n = (cnt + MSPMS_CL - 1) / MSPMS_CL;
         }
         int tot;
-        int ex = block_exscan<1024>(n, sh, tot);
+        int ex = block_exscan<NT>(n, sh, tot);
         if (g < ncol) col_cl0[g] = carry + ex;
         carry += tot;
     }
@@ -270,10 +265,43 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* __restr
     }
     __syncthreads();
     // cluster -> column map
-    for (int g = threadIdx.x; g < ncol; g += 1024) {
+    for (int g = threadIdx.x; g < ncol; g += NT) {
         int a = col_cl0[g], b = col_cl0[g + 1];
         for (int c = a; c < b && c < maxcl; c++) cl_col[c] = g;
     }
+}
+
+// Per atom: wrap into the box, integer cell, count (bin_atom).  The last
+// workgroup to finish (arrival count) scans the cells and columns
+// (scan_cells): one launch instead of two.
+__global__ void __launch_bounds__(256) k_bin_scan(
+    int natom, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ atom_site,
+    const int* __restrict__ atom_mol, const SiteDev* __restrict__ sites,
+    const MolDev* __restrict__ mols, const RepDev* __restrict__ reps, int* __restrict__ atom_cell,
+    int* __restrict__ atom_slot, double* __restrict__ atom_wz, int* cell_count,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    int* __restrict__ atom_gslot, Status* st, const int* lstate, int ncell, int ncol,
+    int* __restrict__ cell_start, const int* __restrict__ col_cell0, int* __restrict__ col_cl0,
+    int* __restrict__ cl_col, int* __restrict__ ncl_total, int maxcl, int* ctr) {
+    if (!lstate[LS_NEED]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < natom)
+        bin_atom(i, x, y, z, atom_site, atom_mol, sites, mols, reps, atom_cell, atom_slot, atom_wz,
+                 cell_count, fx, fy, fz, atom_gslot, st);
+    // arrival: this workgroup's counts are visible device-wide before its
+    // count
+    __shared__ int last;
+    __shared__ int sh[32];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(ctr, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) *ctr = 0;  // for the next build
+    scan_cells<256>(ncell, ncol, cell_count, cell_start, col_cell0, col_cl0, cl_col, ncl_total,
+                    maxcl, st, sh);
 }
 
 __global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
@@ -318,7 +346,7 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
     int local = c - R.cell0;
     int col = R.col0 + local / R.ncz;
     int slot = MSPMS_CL * col_cl0[col] + (a - cell_start[col_cell0[col]]) + rank;
-    if (slot >= MSPMS_CL * maxcl) {  // overflow flagged by k_scan
+    if (slot >= MSPMS_CL * maxcl) {  // overflow flagged by k_bin_scan
         atom_gslot[i] = -1;
         return;
     }
@@ -462,7 +490,7 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
     }
     const double rl2 = rl * rl;
     // the z window in cells: [p1, q1], and [p2, ncz - 1] when it wraps.  An
-    // atom's cell is floor(wz / h) (k_bin), monotone in wz, so the cells of
+    // atom's cell is floor(wz / h) (bin_atom), monotone in wz, so the cells of
     // the window's ends bound the cells of the atoms inside it; the margin mz
     // covers the FP32 widening of the cluster extents and the rounding of
     // the window's ends.
@@ -623,12 +651,11 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
                        b.slot_x64, b.slot_off, b.cl_box, b.fslot, b.lstate, half_skin2, force,
                        b.half, b.hist);
-    hipLaunchKernelGGL(k_bin, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
+    hipLaunchKernelGGL(k_bin_scan, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
-                       b.cell_count, fx, fy, fz, b.atom_gslot, c.status, b.lstate);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, b.ncell, b.ncol, b.cell_count,
-                       b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total, b.maxcl,
-                       c.status, b.lstate);
+                       b.cell_count, fx, fy, fz, b.atom_gslot, c.status, b.lstate, b.ncell,
+                       b.ncol, b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total,
+                       b.maxcl, b.bin_ctr);
     hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
                        b.atom_slot, b.cell_start, b.cell_members, b.lstate);
     hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,

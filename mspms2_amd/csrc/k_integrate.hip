@@ -530,7 +530,8 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
     }
     double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
     for (int ll = 0; ll < nstep_inner; ll++) {
-        __syncthreads();  // the previous step's copies have been read
+        // the drift writes only this lane's site, which the previous step's
+        // terms read before the barrier that ended them
         if (own) {
             vxo += (deltsby2 * fxo * r_mass);
             vyo += (deltsby2 * fyo * r_mass);
@@ -539,13 +540,15 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
             ly[sl] = ly[sl] + delts * vyo;
             lz[sl] = lz[sl] + delts * vzo;
         }
+        __syncthreads();  // the positions are in; every copy has been read
+        // a lane's entries of its wave's copy are written by that lane only:
+        // cleared and then accumulated in program order
 #pragma unroll
         for (int k = 0; k < NW; k++) {
             cfx[w][k * 64 + mi] = 0.0;
             cfy[w][k * 64 + mi] = 0.0;
             cfz[w][k * 64 + mi] = 0.0;
         }
-        __syncthreads();
         e[0] = e[1] = e[2] = e[3] = 0.0;
         vir = 0.0;
         if (term) {
