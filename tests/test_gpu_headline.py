@@ -37,6 +37,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(c, w) for c in (6.0, 8.0, 12.0, 16.0) for w in (1, 2, 4, 8)]
 SKIN_A = 2.5
+BENCH_SKIN_A = 5.0
 FIXTURE = "water12288_ewald"
 
 _cache = {}
@@ -195,12 +196,14 @@ def _respa_run(s, prec, shape, skin_a, nsteps, seed=5):
     return out
 
 
-def test_graph_skin_fp64_matches_rebuild_every_step():
-    """FP64, headline launch shape (6 sites per cell, 2 waves): 12 steps
-    with the 2.5 A buffer against 12 steps rebuilding every step."""
+@pytest.mark.parametrize("skin", [SKIN_A, BENCH_SKIN_A], ids=lambda x: f"skin{x:g}")
+def test_graph_skin_fp64_matches_rebuild_every_step(skin):
+    """FP64, launch shape (6 sites per cell, 2 waves): 12 steps with a
+    2.5 A or 5 A buffer (the bench's) against 12 steps rebuilding every
+    step."""
     s = _water12288()[0]
     pf, ff, lf, nf = _respa_run(s, "fp64", (6.0, 2), 0.0, 12)
-    pb, fb, lb, nb = _respa_run(s, "fp64", (6.0, 2), SKIN_A, 12)
+    pb, fb, lb, nb = _respa_run(s, "fp64", (6.0, 2), skin, 12)
     print(f"builds: fresh {nf}, buffered {nb}", flush=True)
     assert nb < nf
     assert rel_rms(pb, pf) <= 1e-12
@@ -208,15 +211,18 @@ def test_graph_skin_fp64_matches_rebuild_every_step():
     assert np.allclose(lb, lf, rtol=1e-9, atol=0, equal_nan=True)
 
 
-@pytest.mark.parametrize("shape", [(6.0, 2), (8.0, 4), (12.0, 8), (16.0, 1)],
-                         ids=lambda x: f"{x[0]:g}x{x[1]}")
-def test_graph_skin_mixed_matches_oracle(shape):
-    """Mixed, 2.5 A buffer, graph-replayed: after 12 steps the device's long
-    forces and terms equal the oracle's on the same positions (1e-5), and
-    equal a run rebuilding every step to FP32 rounding."""
+@pytest.mark.parametrize("shape,skin", [((6.0, 2), SKIN_A), ((8.0, 4), SKIN_A),
+                                        ((12.0, 8), SKIN_A), ((16.0, 1), SKIN_A),
+                                        ((12.0, 4), BENCH_SKIN_A)],
+                         ids=lambda x: f"{x[0]:g}x{x[1]}" if isinstance(x, tuple) else f"skin{x:g}")
+def test_graph_skin_mixed_matches_oracle(shape, skin):
+    """Mixed, 2.5 A buffer (and the bench's 5 A at its usual shape),
+    graph-replayed: after 12 steps the device's long forces and terms equal
+    the oracle's on the same positions (1e-5), and equal a run rebuilding
+    every step to FP32 rounding."""
     from mspms2_amd import abi
     s = _water12288()[0]
-    pb, fb, lb, nb = _respa_run(s, "mixed", shape, SKIN_A, 12)
+    pb, fb, lb, nb = _respa_run(s, "mixed", shape, skin, 12)
     pf, ff, lf, nf = _respa_run(s, "mixed", shape, 0.0, 12)
     print(f"shape {shape}: builds fresh {nf}, buffered {nb}", flush=True)
     assert nb < nf
