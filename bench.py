@@ -823,6 +823,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skin", type=float, default=5.0,
                     help="pair-list buffer in Angstrom (lists reused until a site moves half of it)")
+    ap.add_argument("--skin-fp64", type=float, default=2.5,
+                    help="pair-list buffer in Angstrom of the FP64 leg")
     ap.add_argument("--replicas", type=int, default=256,
                     help="boxes in the batched-replica leg (0: skip it)")
     ap.add_argument("--tube-loadings", type=int, default=16,
@@ -875,8 +877,10 @@ def main():
     progress("12,288-site water, mixed")
     mixed = run_gpu(system, "mixed", args.steps, args.warmup, skin=skin)
     progress("12,288-site water, FP64")
+    # the exact kernel evaluates every entry of its lists (no pruning), so
+    # it takes the shorter buffer
     fp64 = run_gpu(system, "fp64", args.fp64_steps, min(args.warmup, 5), profile=False,
-                   skin=skin)
+                   skin=args.skin_fp64 / system.params["sigma_base"])
 
     ms = mixed["ms_per_step"]
     ns_day = DT_FS * 1e-6 * 86400e3 / ms
@@ -961,7 +965,7 @@ def main():
                    "precision": "mixed",
                    "launch_shape": {k: mixed["tune"][k] for k in ("cell_atoms",
                                                                   "waves_per_cluster")},
-                   "list_skin_A": args.skin},
+                   "list_skin_A": args.skin, "fp64_list_skin_A": args.skin_fp64},
         "list_builds_per_step": mixed["list_builds_per_step"],
         "autotune_candidates_ms": mixed["tune"]["candidates"],
         "dropin_frclong_ms_pcie_inclusive": mixed["host_frclong_ms"],

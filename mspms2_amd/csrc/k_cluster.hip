@@ -12,7 +12,9 @@
 // and, only when a rebuild is requested (the others return at once):
 //   k_bin_scan per atom: wrap into the box, integer cell (cx,cy,cz), count;
 //              the last workgroup: exclusive scans over cells and columns
-//              (cells of a column are contiguous, ordered by cz)
+//              (cells of a column are contiguous, ordered by cz); grids of
+//              more than 16,384 cells: k_scan, one workgroup of 1024
+//              threads, does the scans
 //   k_scatter  per atom: cell member lists
 //   k_place    per atom: exact rank inside its cell by (wrapped z, index)
 //              -> slot in its column; a column is cut into clusters of 8
@@ -289,6 +291,7 @@ __global__ void __launch_bounds__(256) k_bin_scan(
     if (i < natom)
         bin_atom(i, x, y, z, atom_site, atom_mol, sites, mols, reps, atom_cell, atom_slot, atom_wz,
                  cell_count, fx, fy, fz, atom_gslot, st);
+    if (!ctr) return;  // the cells are scanned by k_scan
     // arrival: this workgroup's counts are visible device-wide before its
     // count
     __shared__ int last;
@@ -302,6 +305,20 @@ __global__ void __launch_bounds__(256) k_bin_scan(
     if (threadIdx.x == 0) *ctr = 0;  // for the next build
     scan_cells<256>(ncell, ncol, cell_count, cell_start, col_cell0, col_cl0, cl_col, ncl_total,
                     maxcl, st, sh);
+}
+
+// The cell and column scans by one workgroup of 1024 threads, for grids of
+// many cells (a workgroup of 256 at the end of k_bin_scan would take long).
+__global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* cell_count,
+                                               int* __restrict__ cell_start,
+                                               const int* __restrict__ col_cell0,
+                                               int* __restrict__ col_cl0, int* __restrict__ cl_col,
+                                               int* __restrict__ ncl_total, int maxcl, Status* st,
+                                               const int* lstate) {
+    if (!lstate[LS_NEED]) return;
+    __shared__ int sh[32];
+    scan_cells<1024>(ncell, ncol, cell_count, cell_start, col_cell0, col_cl0, cl_col, ncl_total,
+                     maxcl, st, sh);
 }
 
 __global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
@@ -651,11 +668,18 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                        b.atom_x0, b.ncl_total, b.maxcl, b.slot_orig, b.slot_w64, b.cl_center,
                        b.slot_x64, b.slot_off, b.cl_box, b.fslot, b.lstate, half_skin2, force,
                        b.half, b.hist);
+    // up to 16,384 cells the last binning workgroup scans them (one launch
+    // less per step); larger grids take a workgroup of 1024 threads
+    const bool scan_here = b.ncell <= 16384;
     hipLaunchKernelGGL(k_bin_scan, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, c.atom_site,
                        c.atom_mol, c.sites, c.mols, c.reps, b.atom_cell, b.atom_slot, b.atom_wz,
                        b.cell_count, fx, fy, fz, b.atom_gslot, c.status, b.lstate, b.ncell,
                        b.ncol, b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total,
-                       b.maxcl, b.bin_ctr);
+                       b.maxcl, scan_here ? b.bin_ctr : nullptr);
+    if (!scan_here)
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c.stream, b.ncell, b.ncol, b.cell_count,
+                           b.cell_start, b.col_cell0, b.col_cl0, b.cl_col, b.ncl_total, b.maxcl,
+                           c.status, b.lstate);
     hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
                        b.atom_slot, b.cell_start, b.cell_members, b.lstate);
     hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,
