@@ -68,9 +68,10 @@ template <int ELEC, bool FP64, bool GRAD, int WPC>
 __global__ void __launch_bounds__(64 * WPC) k_pair(
     const int* __restrict__ ncl_total, int maxcl, const RepDev* __restrict__ reps,
     const SiteDev* __restrict__ sites, const double4* __restrict__ cl_center,
-    const float4* __restrict__ plist, const int* __restrict__ plist_n, int cap,
-    const int* __restrict__ slot_orig, const int2* __restrict__ slot_info,
-    const double4* __restrict__ slot_x64, const float4* __restrict__ slot_off,
+    const float4* __restrict__ cl_box, const float4* __restrict__ plist,
+    const int* __restrict__ plist_n, int cap, const int* __restrict__ slot_orig,
+    const int2* __restrict__ slot_info, const double4* __restrict__ slot_x64,
+    const float4* __restrict__ slot_off,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ epart, unsigned long long* __restrict__ ipart, int want_hash,
     double* __restrict__ gq, double* __restrict__ ge, double* __restrict__ gs) {
@@ -102,11 +103,26 @@ __global__ void __launch_bounds__(64 * WPC) k_pair(
     const int nrow = plist_n[ci];
     const float4* row = plist + (size_t)ci * cap;
 
+    // row pruning as in k_pair_mx: an entry whose current cluster boxes are
+    // farther apart than the cutoff (with a margin of 1e-3 of it above FP32
+    // rounding) holds no pair in range; the test is the same for every lane
+    // of the wave, and the pairs that interact are still decided per pair
+    const float rprune = (float)fmax(R.rc_lj, R.rc_el) * 1.001f;
+    const float rprune2 = rprune * rprune;
+    const float4 bim = cl_box[2 * ci], bih = cl_box[2 * ci + 1];
     {
         const double4 pi4 = slot_x64[islot];
         for (int e = w; e < nrow; e += WPC) {
-            const uint32_t ent = __float_as_uint(row[e].w);
+            const float4 E = row[e];
+            const uint32_t ent = __float_as_uint(E.w);
             const int cj = (int)(ent >> 7);
+            if (cj != ci && !(ent & 64u)) {
+                const float4 bjm = cl_box[2 * cj], bjh = cl_box[2 * cj + 1];
+                const float gx = fmaxf(fabsf(E.x + (bim.x - bjm.x)) - (bih.x + bjh.x), 0.f);
+                const float gy = fmaxf(fabsf(E.y + (bim.y - bjm.y)) - (bih.y + bjh.y), 0.f);
+                const float gz = fmaxf(fabsf(E.z + (bim.z - bjm.z)) - (bih.z + bjh.z), 0.f);
+                if (!(gx * gx + gy * gy + gz * gz < rprune2)) continue;
+            }
             const int jslot = cj * MSPMS_CL + jsub;
             const int2 infoj = slot_info[jslot];
             if (!(vi && infoj.y >= 0 && infoj.x != infoi.x)) continue;
@@ -613,7 +629,7 @@ void launch_64(const LaunchCtx& c, const ClusterBufs& b, double* fx, double* fy,
                double* gs) {
     hipLaunchKernelGGL((k_pair<ELEC, true, GRAD, WPC>), dim3((b.maxcl + 7) / 8 * 8),
                        dim3(64 * WPC), 0, c.stream, b.ncl_total, b.maxcl, c.reps, c.sites,
-                       b.cl_center, b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_info,
+                       b.cl_center, b.cl_box, b.plist, b.plist_n, b.cap, b.slot_orig, b.slot_info,
                        b.slot_x64, b.slot_off, fx, fy, fz, epart, ipart, want_hash, gq, ge, gs);
 }
 
