@@ -134,7 +134,11 @@ __global__ void __launch_bounds__(256) k_list_check(
             cl_box[2 * c + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    if (__ballot(need) && (threadIdx.x & 63) == 0) atomicOr(&lstate[LS_NEED], 1);
+    // one atomic per wave, and none once the request is seen set: atomics
+    // on one address from every wave of a large system serialize
+    if (__ballot(need) && (threadIdx.x & 63) == 0 &&
+        !__hip_atomic_load(&lstate[LS_NEED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicOr(&lstate[LS_NEED], 1);
 }
 
 __device__ inline double wrap(double x, double L, double invL) {
@@ -651,7 +655,11 @@ __global__ void __launch_bounds__(256) k_plist(const int* __restrict__ ncl_total
             atomicOr(&st->flags, ST_OVERFLOW);
             lstate[LS_VALID] = 0;  // truncated row: never reuse
         }
-        atomicMax(&st->max_list, n);
+        // the largest row: an atomic only when this row beats the largest
+        // seen so far (every row's atomic on one address serialized the
+        // build of a 120,000-row table)
+        if (n > __hip_atomic_load(&st->max_list, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&st->max_list, n);
     }
 }
 
