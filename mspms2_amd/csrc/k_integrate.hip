@@ -405,7 +405,9 @@ __device__ inline double rsqrt_nr(double x) {
 // bonded_term for the inner loop's latency chain: the harmonic bond and the
 // harmonic angle with reciprocal square roots in place of sqrt and division
 // (the same expressions otherwise; results differ from bonded_term's in the
-// last bits); every other term is bonded_term's.
+// last bits); every other term is bonded_term's.  The caller's force copy
+// holds this one term only and starts from zero, so the two fast terms
+// store their forces (0 + f is f) instead of reading the copy back.
 __device__ inline int bonded_term_inner(const TermDev& T, int base, const double* xx,
                                         const double* yy, const double* zz, double* fxs,
                                         double* fys, double* fzs, double* u, double* vir,
@@ -420,12 +422,12 @@ __device__ inline int bonded_term_inner(const TermDev& T, int base, const double
         *u = 0.5 * pr[0] * dr * dr;
         const double fij = -pr[0] * dr * ri;
         *vir = fij * rsq;
-        fxs[i1] += fij * rx;
-        fys[i1] += fij * ry;
-        fzs[i1] += fij * rz;
-        fxs[i2] -= fij * rx;
-        fys[i2] -= fij * ry;
-        fzs[i2] -= fij * rz;
+        fxs[i1] = fij * rx;
+        fys[i1] = fij * ry;
+        fzs[i1] = fij * rz;
+        fxs[i2] = -(fij * rx);
+        fys[i2] = -(fij * ry);
+        fzs[i2] = -(fij * rz);
         return 0;
     }
     if (T.kind == TERM_ANGLE && T.type == MSPMS_ANGLE_HARMONIC) {
@@ -454,15 +456,15 @@ __device__ inline int bonded_term_inner(const TermDev& T, int base, const double
         const double fxc = gam * (xab - xbc * cost) * rrbc, fyc = gam * (yab - ybc * cost) * rrbc,
                      fzc = gam * (zab - zbc * cost) * rrbc;
         *vir = 0.0;
-        fxs[ia] += fxa;
-        fys[ia] += fya;
-        fzs[ia] += fza;
-        fxs[ib] = fxs[ib] - fxa - fxc;
-        fys[ib] = fys[ib] - fya - fyc;
-        fzs[ib] = fzs[ib] - fza - fzc;
-        fxs[ic] += fxc;
-        fys[ic] += fyc;
-        fzs[ic] += fzc;
+        fxs[ia] = fxa;
+        fys[ia] = fya;
+        fzs[ia] = fza;
+        fxs[ib] = -fxa - fxc;
+        fys[ib] = -fya - fyc;
+        fzs[ib] = -fza - fzc;
+        fxs[ic] = fxc;
+        fys[ic] = fyc;
+        fzs[ic] = fzc;
         return 1;
     }
     return bonded_term(T, base, xx, yy, zz, fxs, fys, fzs, u, vir, singular, stride);

@@ -515,10 +515,11 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                 // staged in LDS: the eight lanes of a j hold the same sum.
                 // Atomics issued here, between one entry's j loads and the
                 // next's, would sit under the wait for those loads.
+                // (the force on i; j's is its negative, taken at the flush)
                 if (isub == 0) {
-                    sh_jf[w][kk][0][jsub] = -v[0];
-                    sh_jf[w][kk][1][jsub] = -v[1];
-                    sh_jf[w][kk][2][jsub] = -v[2];
+                    sh_jf[w][kk][0][jsub] = v[0];
+                    sh_jf[w][kk][1][jsub] = v[1];
+                    sh_jf[w][kk][2][jsub] = v[2];
                 }
                 if (lane == 0) sh_jc[w][kk] = cj;
             }
@@ -547,9 +548,9 @@ __global__ void __launch_bounds__(64 * WPC) k_pair_mx(
                 // j atoms without a pair in range (and padding slots) add nothing
                 if (f0 != 0.f || f1 != 0.f || f2 != 0.f) {
                     const int sl = sh_jc[w][e] * MSPMS_CL + a;
-                    atomicAdd(&fslot[sl], (double)f0);
-                    atomicAdd(&fslot[(size_t)nslot + sl], (double)f1);
-                    atomicAdd(&fslot[2 * (size_t)nslot + sl], (double)f2);
+                    atomicAdd(&fslot[sl], -(double)f0);
+                    atomicAdd(&fslot[(size_t)nslot + sl], -(double)f1);
+                    atomicAdd(&fslot[2 * (size_t)nslot + sl], -(double)f2);
                 }
             }
             __builtin_amdgcn_wave_barrier();
