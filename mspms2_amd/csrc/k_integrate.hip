@@ -531,6 +531,18 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
         fzo = fzs[i];
     }
     double e[4] = {0.0, 0.0, 0.0, 0.0}, vir = 0.0;
+    // a lane's entries of its wave's copy are written by that lane only.
+    // The fast terms (harmonic bond and angle) store the same entries every
+    // step, so the copy is cleared once; other terms accumulate into it and
+    // clear it every step (in program order, no barrier needed)
+    const bool stores = term && ((T.kind == TERM_BOND && T.type == MSPMS_BOND_HARMONIC) ||
+                                 (T.kind == TERM_ANGLE && T.type == MSPMS_ANGLE_HARMONIC));
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        cfx[w][k * 64 + mi] = 0.0;
+        cfy[w][k * 64 + mi] = 0.0;
+        cfz[w][k * 64 + mi] = 0.0;
+    }
     for (int ll = 0; ll < nstep_inner; ll++) {
         // the drift writes only this lane's site, which the previous step's
         // terms read before the barrier that ended them
@@ -543,13 +555,13 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
             lz[sl] = lz[sl] + delts * vzo;
         }
         __syncthreads();  // the positions are in; every copy has been read
-        // a lane's entries of its wave's copy are written by that lane only:
-        // cleared and then accumulated in program order
+        if (!stores && ll > 0) {
 #pragma unroll
-        for (int k = 0; k < NW; k++) {
-            cfx[w][k * 64 + mi] = 0.0;
-            cfy[w][k * 64 + mi] = 0.0;
-            cfz[w][k * 64 + mi] = 0.0;
+            for (int k = 0; k < NW; k++) {
+                cfx[w][k * 64 + mi] = 0.0;
+                cfy[w][k * 64 + mi] = 0.0;
+                cfz[w][k * 64 + mi] = 0.0;
+            }
         }
         e[0] = e[1] = e[2] = e[3] = 0.0;
         vir = 0.0;
