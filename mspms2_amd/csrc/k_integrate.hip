@@ -512,6 +512,7 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
     const int sl = w * 64 + mi;
     double usf_last = 0.0;
     double r_mass = 0.0, vxo = 0.0, vyo = 0.0, vzo = 0.0, fxo = 0.0, fyo = 0.0, fzo = 0.0;
+    double pxo = 0.0, pyo = 0.0, pzo = 0.0;  // the own site's position (LDS holds a copy)
     if (own && snap_wanted(SN, st)) {
         const int i = a0 + w;
         snap_atom(SN, i, x[i], y[i], z[i], vx[i], vy[i], vz[i]);
@@ -520,9 +521,12 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
     if (own) {
         const int i = a0 + w;
         r_mass = 1.0 / sites[site0 + w].aw;
-        lx[sl] = x[i];
-        ly[sl] = y[i];
-        lz[sl] = z[i];
+        pxo = x[i];
+        pyo = y[i];
+        pzo = z[i];
+        lx[sl] = pxo;
+        ly[sl] = pyo;
+        lz[sl] = pzo;
         vxo = vx[i] + (deltby2 * fxl[i] * r_mass);
         vyo = vy[i] + (deltby2 * fyl[i] * r_mass);
         vzo = vz[i] + (deltby2 * fzl[i] * r_mass);
@@ -550,9 +554,12 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
             vxo += (deltsby2 * fxo * r_mass);
             vyo += (deltsby2 * fyo * r_mass);
             vzo += (deltsby2 * fzo * r_mass);
-            lx[sl] = lx[sl] + delts * vxo;
-            ly[sl] = ly[sl] + delts * vyo;
-            lz[sl] = lz[sl] + delts * vzo;
+            pxo = pxo + delts * vxo;
+            pyo = pyo + delts * vyo;
+            pzo = pzo + delts * vzo;
+            lx[sl] = pxo;
+            ly[sl] = pyo;
+            lz[sl] = pzo;
         }
         __syncthreads();  // the positions are in; every copy has been read
         if (!stores && ll > 0) {
@@ -587,8 +594,8 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
             fyo = gy;
             fzo = gz;
             if (W.on)
-                usf_last = sf_inner_site(W, sites[site0 + w], rep, lx[sl], ly[sl], lz[sl], fxo, fyo,
-                                         fzo, st);
+                usf_last = sf_inner_site(W, sites[site0 + w], rep, pxo, pyo, pzo, fxo, fyo, fzo,
+                                         st);
             vxo += (deltsby2 * fxo * r_mass);
             vyo += (deltsby2 * fyo * r_mass);
             vzo += (deltsby2 * fzo * r_mass);
@@ -596,9 +603,9 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
     }
     if (own) {
         const int i = a0 + w;
-        x[i] = lx[sl];
-        y[i] = ly[sl];
-        z[i] = lz[sl];
+        x[i] = pxo;
+        y[i] = pyo;
+        z[i] = pzo;
         vx[i] = vxo;
         vy[i] = vyo;
         vz[i] = vzo;
