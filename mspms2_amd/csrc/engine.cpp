@@ -226,6 +226,7 @@ struct mspms_ctx {
     DevBuf<RanmarDev> d_rng;
     DevBuf<HmcRep> d_hmc;
     DevBuf<double> d_aw_atom, d_pos_old;
+    DevBuf<double> d_rmass;  // [natom] 1 / site mass (the kicks' factor)
     bool hmc_ready = false;
     DevBuf<double> d_hmc_delt;          // [nrep] each chain's outer step
     bool hmc_rep_delt = false;          // chains step with d_hmc_delt
@@ -617,6 +618,7 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
     c->d_rng.release();
     c->d_hmc.release();
     c->d_aw_atom.release();
+    c->d_rmass.release();
     c->d_pos_old.release();
     c->d_hmc_delt.release();
     c->d_nvt.release();
@@ -637,6 +639,14 @@ extern "C" int mspms_destroy(mspms_ctx* c) {
 namespace {
 
 // Builds every device table from sp / nmole / params.
+// each atom's 1 / mass, as the kicks divide (the same division, once)
+int upload_rmass(mspms_ctx* c, const std::vector<SiteDev>& sites,
+                 const std::vector<int>& atom_site, hipStream_t s) {
+    std::vector<double> rm(atom_site.size());
+    for (size_t a = 0; a < atom_site.size(); a++) rm[a] = 1.0 / sites[atom_site[a]].aw;
+    return c->d_rmass.upload(rm, s);
+}
+
 int build(mspms_ctx* c) {
     HIPC(hipSetDevice(c->device));
     drop_graph(c);  // buffers may move
@@ -969,6 +979,7 @@ int build(mspms_ctx* c) {
     if ((rc = c->d_sites.upload(sites, s)) || (rc = c->d_species.upload(spd, s)) ||
         (rc = c->d_terms.upload(terms, s)) || (rc = c->d_mols.upload(mols, s)) ||
         (rc = c->d_atom_mol.upload(atom_mol, s)) || (rc = c->d_atom_site.upload(atom_site, s)) ||
+        (rc = upload_rmass(c, sites, atom_site, s)) ||
         (rc = c->d_reps.upload(reps, s)))
         return rc;
 
@@ -1542,7 +1553,8 @@ int enqueue_vver(mspms_ctx* c, double* pos, double* vel, double* fl, double* fs,
                            deltby2, delts, deltsby2, nstep_inner, c->d_ep_short.p,
                            sf_inner ? &c->sf : nullptr, c->d_ep_sf.p);
     }
-    const Kick kick{vel, vel + n, vel + 2 * n, c->d_kin.p, deltby2, L.rep_delt, L.atom_mol, L.mols};
+    const Kick kick{vel,        vel + n,    vel + 2 * n, c->d_kin.p, deltby2,
+                    L.rep_delt, L.atom_mol, L.mols,      c->d_rmass.p};
     bool kicked = false, deferred = false;
     int rc = enqueue_long(c, pos, pos + n, pos + 2 * n, fl, fl + n, fl + 2 * n, lterms, s,
                           nullptr, false, &kick, &kicked, &deferred);

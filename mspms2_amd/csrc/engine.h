@@ -342,6 +342,7 @@ struct Kick {
     const double* rep_delt;
     const int* atom_mol;
     const MolDev* mols;
+    const double* rmass;  // [natom] 1 / site mass
 };
 // A RESPA step's per-atom closing pass run by its finalize: fx += frec (the
 // side stream's reciprocal and intramolecular forces) and the half kick,

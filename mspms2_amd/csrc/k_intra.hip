@@ -146,10 +146,10 @@ __device__ inline void fold_slot(const SlotForces& S, int i, double& fx, double&
 }
 
 // closing half kick of atom i (k_respa_final's arithmetic)
-__device__ inline void kick_atom(const Kick& K, int i, double aw, const double* fx,
-                                 const double* fy, const double* fz) {
+__device__ inline void kick_atom(const Kick& K, int i, const double* fx, const double* fy,
+                                 const double* fz) {
 #pragma clang fp contract(off)
-    const double r_mass = 1.0 / aw;
+    const double r_mass = K.rmass[i];  // 1.0 / aw, computed once
     const double deltby2 = K.rep_delt ? K.rep_delt[K.mols[K.atom_mol[i]].rep] / 2.0 : K.deltby2;
     const double a = K.vx[i] + (deltby2 * fx[i] * r_mass);
     const double b = K.vy[i] + (deltby2 * fy[i] * r_mass);
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
     // this thread owns the molecule's atoms: their long forces are final
     if (K.vx)
         for (int k = 0; k < M.natom; k++)
-            kick_atom(K, base + k, sites[S.site0 + k].aw, fx, fy, fz);
+            kick_atom(K, base + k, fx, fy, fz);
 }
 
 __global__ void __launch_bounds__(256) k_bonded(int nmol, const MolDev* __restrict__ mols,
@@ -426,7 +426,7 @@ __global__ void __launch_bounds__(FIN_T) k_finalize(
             A.fx[a] = ax + A.frec[a];
             A.fy[a] = ay + A.frec[n + a];
             A.fz[a] = az + A.frec[2 * n + a];
-            if (A.K.vx) kick_atom(A.K, a, sites[atom_site[a]].aw, A.fx, A.fy, A.fz);
+            if (A.K.vx) kick_atom(A.K, a, A.fx, A.fy, A.fz);
         }
     }
     if (fb.on && !*fb.on) {
@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(256) k_add_frec(int n, double* __restrict__ fx
     fx[i] = ax + frec[i];
     fy[i] = ay + frec[(size_t)n + i];
     fz[i] = az + frec[2 * (size_t)n + i];
-    if (K.vx) kick_atom(K, i, sites[atom_site[i]].aw, fx, fy, fz);
+    if (K.vx) kick_atom(K, i, fx, fy, fz);
 }
 
 __global__ void __launch_bounds__(256) k_fold_slots(int n, double* __restrict__ fx,
