@@ -16,9 +16,10 @@
 //              more than 16,384 cells: k_scan, one workgroup of 1024
 //              threads, does the scans
 //   k_scatter  per atom: cell member lists
-//   k_place    per atom: exact rank inside its cell by (wrapped z, index)
-//              -> slot in its column; a column is cut into clusters of 8
-//   k_bbox     per cluster: bounding box, FP32 offsets from the centre
+//   k_place_bbox  one workgroup per column: exact rank of each atom inside
+//              its cell by (wrapped z, index) -> slot in its column (a column
+//              is cut into clusters of 8); then per cluster: bounding box,
+//              FP32 offsets from the centre
 //   k_plist    one wave per i-cluster: j-clusters whose boxes come within
 //              rc_list, compacted with a wave ballot (deterministic order)
 // The ranks make the final order independent of atomic arrival order.
@@ -338,36 +339,32 @@ __global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
 
 // Rank inside the cell by (wrapped z, atom index); place into the column's
 // slot range, which is 8 * (first cluster of the column) onwards.
-__global__ void k_place(int natom, const double* __restrict__ x, const double* __restrict__ y,
-                        const double* __restrict__ z, const int* __restrict__ atom_cell,
-                        const double* __restrict__ atom_wz, const int* __restrict__ cell_start,
-                        const int* __restrict__ cell_members, const int* __restrict__ atom_mol,
-                        const int* __restrict__ atom_site, const MolDev* __restrict__ mols,
-                        const RepDev* __restrict__ reps, const int* __restrict__ col_cell0,
-                        const int* __restrict__ col_cl0, int* __restrict__ slot_orig,
-                        int2* __restrict__ slot_info, double4* __restrict__ slot_x64,
-                        double4* __restrict__ slot_w64, double4* __restrict__ atom_x0,
-                        int* __restrict__ atom_gslot, const SiteDev* __restrict__ sites,
-                        int maxcl, const int* lstate) {
-    if (!lstate[LS_NEED]) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natom) return;
-    int c = atom_cell[i];
-    if (c < 0) return;
+__device__ inline void place_atom(int i, int col, int colstart, const double* __restrict__ x,
+                                  const double* __restrict__ y, const double* __restrict__ z,
+                                  const int* __restrict__ atom_cell,
+                                  const double* __restrict__ atom_wz,
+                                  const int* __restrict__ cell_start,
+                                  const int* __restrict__ cell_members,
+                                  const int* __restrict__ atom_mol,
+                                  const int* __restrict__ atom_site,
+                                  const MolDev* __restrict__ mols, const RepDev* __restrict__ reps,
+                                  const int* __restrict__ col_cl0, int* __restrict__ slot_orig,
+                                  int2* __restrict__ slot_info, double4* __restrict__ slot_x64,
+                                  double4* slot_w64, double4* __restrict__ atom_x0,
+                                  int* __restrict__ atom_gslot, const SiteDev* __restrict__ sites,
+                                  int maxcl) {
+    const int c = atom_cell[i];
     const RepDev& R = reps[mols[atom_mol[i]].rep];
-    double wz = atom_wz[i];
-    int a = cell_start[c], b = cell_start[c + 1];
+    const double wz = atom_wz[i];
+    const int a = cell_start[c], b = cell_start[c + 1];
     int rank = 0;
     for (int m = a; m < b; m++) {
-        int j = cell_members[m];
-        double wj = atom_wz[j];
+        const int j = cell_members[m];
+        const double wj = atom_wz[j];
         rank += (wj < wz) || (wj == wz && j < i);
     }
-    // column of this cell (cells of a replica: cell0 + col_local*ncz + cz)
-    int local = c - R.cell0;
-    int col = R.col0 + local / R.ncz;
-    int slot = MSPMS_CL * col_cl0[col] + (a - cell_start[col_cell0[col]]) + rank;
-    if (slot >= MSPMS_CL * maxcl) {  // overflow flagged by k_bin_scan
+    const int slot = MSPMS_CL * col_cl0[col] + (a - colstart) + rank;
+    if (slot >= MSPMS_CL * maxcl) {  // overflow flagged by the scan
         atom_gslot[i] = -1;
         return;
     }
@@ -382,32 +379,22 @@ __global__ void k_place(int natom, const double* __restrict__ x, const double* _
     atom_x0[i] = make_double4(x[i], y[i], z[i], 0.0);
 }
 
-// 8 lanes per cluster: bounding box, centre, FP32 offsets; pads marked.
-__global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* __restrict__ cl_col,
-                       const int* __restrict__ col_cl0, const int* __restrict__ col_cell0,
-                       const int* __restrict__ cell_start, const int* __restrict__ col_rep,
-                       const SiteDev* __restrict__ sites, int* __restrict__ slot_orig,
-                       int2* __restrict__ slot_info, const double4* __restrict__ slot_w64,
-                       float4* __restrict__ slot_off, float4* __restrict__ slot_par,
-                       double4* __restrict__ cl_center,
-                       float4* __restrict__ cl_ext, float4* __restrict__ cl_box, int* lstate) {
-    if (!lstate[LS_NEED]) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int c = t >> 3, k = t & 7;
-    int ncl = min(*ncl_total, maxcl);
-    if (t == 0) {
-        // usable for reuse unless the clusters overflowed (k_plist clears it
-        // again on a row overflow)
-        lstate[LS_VALID] = *ncl_total <= maxcl;
-        lstate[LS_BUILDS] += 1;
-    }
-    bool live = c < ncl;
-    int col = live ? cl_col[c] : 0;
-    int cnt = live ? (cell_start[col_cell0[col + 1]] - cell_start[col_cell0[col]]) : 0;
-    int in_cl = live ? min(MSPMS_CL, cnt - MSPMS_CL * (c - col_cl0[col])) : 0;
-    bool valid = live && k < in_cl;
-    int slot = MSPMS_CL * c + k;
-    double4 w = valid ? slot_w64[slot] : make_double4(0, 0, 0, 0);
+// 8 lanes per cluster (slot t = 8 c + k; every lane of the wave takes part
+// in the shuffles; c < c_end live): bounding box, centre, FP32 offsets;
+// pads marked.
+__device__ inline void bbox_slot(int t, int c_end, int col, int cnt, int cl0,
+                                 const int* __restrict__ col_rep,
+                                 const SiteDev* __restrict__ sites, int* __restrict__ slot_orig,
+                                 int2* __restrict__ slot_info, const double4* slot_w64,
+                                 float4* __restrict__ slot_off, float4* __restrict__ slot_par,
+                                 double4* __restrict__ cl_center, float4* __restrict__ cl_ext,
+                                 float4* __restrict__ cl_box) {
+    const int c = t >> 3, k = t & 7;
+    const bool live = c < c_end;
+    const int in_cl = live ? min(MSPMS_CL, cnt - MSPMS_CL * (c - cl0)) : 0;
+    const bool valid = live && k < in_cl;
+    const int slot = MSPMS_CL * c + k;
+    const double4 w = valid ? slot_w64[slot] : make_double4(0, 0, 0, 0);
     double lo[3], hi[3];
     const double BIG = 1e300;
     lo[0] = valid ? w.x : BIG;
@@ -429,7 +416,8 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
                                    (float)(0.5 * (hi[1] - lo[1])) * 1.000001f + 1e-6f,
                                    (float)(0.5 * (hi[2] - lo[2])) * 1.000001f + 1e-6f, 0.f);
     if (!live) return;
-    double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]),
+                 cz = 0.5 * (lo[2] + hi[2]);
     if (k == 0) {
         cl_center[c] = make_double4(cx, cy, cz, (double)col_rep[col]);
         cl_ext[c] = ext;
@@ -437,7 +425,7 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
         cl_box[2 * c + 1] = ext;
     }
     if (valid) {
-        int s = slot_info[slot].y;
+        const int s = slot_info[slot].y;
         const int mol = slot_info[slot].x;
         const SiteDev& S = sites[s];
         const bool lj = S.ghost != MSPMS_GHOST_LJ;
@@ -451,6 +439,44 @@ __global__ void k_bbox(const int* __restrict__ ncl_total, int maxcl, const int* 
         slot_off[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         slot_par[slot] = make_float4(0.f, 0.f, __int_as_float(-1), __int_as_float(0));
     }
+}
+
+// One workgroup per column: its atoms (the member lists of its cells, which
+// are contiguous) take their slots (place_atom), then, after a workgroup
+// barrier, its clusters take their boxes (bbox_slot).  A column's slots and
+// clusters are its own, so no other workgroup's writes are read.
+__global__ void __launch_bounds__(256) k_place_bbox(
+    int ncol, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ atom_cell,
+    const double* __restrict__ atom_wz, const int* __restrict__ cell_start,
+    const int* __restrict__ cell_members, const int* __restrict__ atom_mol,
+    const int* __restrict__ atom_site, const MolDev* __restrict__ mols,
+    const RepDev* __restrict__ reps, const int* __restrict__ col_cell0,
+    const int* __restrict__ col_cl0, const int* __restrict__ col_rep,
+    const int* __restrict__ ncl_total, int* __restrict__ slot_orig, int2* __restrict__ slot_info,
+    double4* __restrict__ slot_x64, double4* slot_w64, double4* __restrict__ atom_x0,
+    int* __restrict__ atom_gslot, const SiteDev* __restrict__ sites, float4* __restrict__ slot_off,
+    float4* __restrict__ slot_par, double4* __restrict__ cl_center, float4* __restrict__ cl_ext,
+    float4* __restrict__ cl_box, int maxcl, int* lstate) {
+    if (!lstate[LS_NEED]) return;
+    const int col = blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // usable for reuse unless the clusters overflowed (k_plist clears it
+        // again on a row overflow)
+        lstate[LS_VALID] = *ncl_total <= maxcl;
+        lstate[LS_BUILDS] += 1;
+    }
+    const int m0 = cell_start[col_cell0[col]], m1 = cell_start[col_cell0[col + 1]];
+    for (int m = m0 + threadIdx.x; m < m1; m += blockDim.x)
+        place_atom(cell_members[m], col, m0, x, y, z, atom_cell, atom_wz, cell_start,
+                   cell_members, atom_mol, atom_site, mols, reps, col_cl0, slot_orig, slot_info,
+                   slot_x64, slot_w64, atom_x0, atom_gslot, sites, maxcl);
+    __syncthreads();
+    const int cl0 = col_cl0[col], c_end = min(col_cl0[col + 1], maxcl);
+    // whole waves per pass, so the shuffles see all 64 lanes
+    for (int t0 = MSPMS_CL * cl0 + (threadIdx.x & ~63); t0 < MSPMS_CL * c_end; t0 += blockDim.x)
+        bbox_slot(t0 + (threadIdx.x & 63), c_end, col, m1 - m0, cl0, col_rep, sites, slot_orig,
+                  slot_info, slot_w64, slot_off, slot_par, cl_center, cl_ext, cl_box);
 }
 
 // One wave per i-cluster.  Candidate columns cover the x/y range of the
@@ -692,15 +718,12 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                            c.status, b.lstate);
     hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
                        b.atom_slot, b.cell_start, b.cell_members, b.lstate);
-    hipLaunchKernelGGL(k_place, dim3(gA), dim3(T), 0, c.stream, c.natom, x, y, z, b.atom_cell,
-                       b.atom_wz, b.cell_start, b.cell_members, c.atom_mol, c.atom_site, c.mols,
-                       c.reps, b.col_cell0, b.col_cl0, b.slot_orig, b.slot_info, b.slot_x64,
-                       b.slot_w64, b.atom_x0, b.atom_gslot, c.sites, b.maxcl, b.lstate);
-    int gB = (b.maxcl * MSPMS_CL + T - 1) / T;
-    hipLaunchKernelGGL(k_bbox, dim3(gB), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
-                       b.col_cl0, b.col_cell0, b.cell_start, b.col_rep, c.sites, b.slot_orig,
-                       b.slot_info, b.slot_w64, b.slot_off, b.slot_par, b.cl_center, b.cl_ext,
-                       b.cl_box, b.lstate);
+    hipLaunchKernelGGL(k_place_bbox, dim3(b.ncol), dim3(T), 0, c.stream, b.ncol, x, y, z,
+                       b.atom_cell, b.atom_wz, b.cell_start, b.cell_members, c.atom_mol,
+                       c.atom_site, c.mols, c.reps, b.col_cell0, b.col_cl0, b.col_rep,
+                       b.ncl_total, b.slot_orig, b.slot_info, b.slot_x64, b.slot_w64, b.atom_x0,
+                       b.atom_gslot, c.sites, b.slot_off, b.slot_par, b.cl_center, b.cl_ext,
+                       b.cl_box, b.maxcl, b.lstate);
     int gP = (b.maxcl * 64 + T - 1) / T;
     hipLaunchKernelGGL(k_plist, dim3(gP), dim3(T), 0, c.stream, b.ncl_total, b.maxcl, b.cl_col,
                        b.col_cl0, b.col_rep, b.col_cell0, b.cell_start, c.reps, b.cl_center,
