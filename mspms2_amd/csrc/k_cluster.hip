@@ -218,36 +218,53 @@ __device__ int block_exscan(int v, int* sh, int& total) {
     return ex;
 }
 
+// Cells per pass of scan_cells: NT * SCAN_PER<NT> counts staged in LDS.
+template <int NT>
+constexpr int SCAN_PER = NT >= 1024 ? 8 : 16;
+// LDS index of staged cell q: one pad word per 32, so that the PER
+// consecutive words a thread reads fall in different banks across the wave
+__device__ inline int scan_pad(int q) { return q + (q >> 5); }
+template <int NT>
+constexpr int SCAN_LDS = NT * SCAN_PER<NT> + NT * SCAN_PER<NT> / 32;
+
 // By one workgroup of NT threads: cell_start = exscan(cell_count),
 // cell_count := 0; per column the atom count, cluster count, cluster start
-// and cl_col.
+// and cl_col.  stage: SCAN_LDS<NT> ints of LDS.
 template <int NT>
 __device__ void scan_cells(int ncell, int ncol, int* __restrict__ cell_count,
                            int* __restrict__ cell_start, const int* __restrict__ col_cell0,
                            int* __restrict__ col_cl0, int* __restrict__ cl_col,
-                           int* __restrict__ ncl_total, int maxcl, Status* st, int* sh) {
+                           int* __restrict__ ncl_total, int maxcl, Status* st, int* sh,
+                           int* stage) {
     int carry = 0;
-    // a thread scans 16 consecutive cells per pass: one workgroup scan per
-    // 16 NT cells (a tube of 16 long boxes has ~50,000)
-    constexpr int PER = 16;
+    // per pass NT * PER cells: read into LDS with coalesced loads, scanned
+    // PER consecutive cells per thread, written back with coalesced stores
+    // (a tube of 16 long boxes has ~200,000 cells)
+    constexpr int PER = SCAN_PER<NT>;
+    const int t = threadIdx.x;
     for (int base = 0; base < ncell; base += NT * PER) {
-        const int c0 = base + threadIdx.x * PER;
+        const int n = min(NT * PER, ncell - base);
+        for (int q = t; q < NT * PER; q += NT) stage[scan_pad(q)] = q < n ? cell_count[base + q] : 0;
+        __syncthreads();
         int v[PER], sum = 0;
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-            v[k] = c0 + k < ncell ? cell_count[c0 + k] : 0;
+            v[k] = stage[scan_pad(t * PER + k)];
             sum += v[k];
         }
         int tot;
-        int run = carry + block_exscan<NT>(sum, sh, tot);
+        int run = carry + block_exscan<NT>(sum, sh, tot);  // its barriers order the reads
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-            if (c0 + k < ncell) {
-                cell_start[c0 + k] = run;
-                cell_count[c0 + k] = 0;
-            }
+            stage[scan_pad(t * PER + k)] = run;
             run += v[k];
         }
+        __syncthreads();
+        for (int q = t; q < n; q += NT) {
+            cell_start[base + q] = stage[scan_pad(q)];
+            cell_count[base + q] = 0;
+        }
+        __syncthreads();  // the next pass refills the stage
         carry += tot;
     }
     if (threadIdx.x == 0) cell_start[ncell] = carry;
@@ -301,6 +318,7 @@ __global__ void __launch_bounds__(256) k_bin_scan(
     // count
     __shared__ int last;
     __shared__ int sh[32];
+    __shared__ int stage[SCAN_LDS<256>];
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(ctr, 1) == (int)gridDim.x - 1;
@@ -309,7 +327,7 @@ __global__ void __launch_bounds__(256) k_bin_scan(
     __threadfence();
     if (threadIdx.x == 0) *ctr = 0;  // for the next build
     scan_cells<256>(ncell, ncol, cell_count, cell_start, col_cell0, col_cl0, cl_col, ncl_total,
-                    maxcl, st, sh);
+                    maxcl, st, sh, stage);
 }
 
 // The cell and column scans by one workgroup of 1024 threads, for grids of
@@ -322,8 +340,9 @@ __global__ void __launch_bounds__(1024) k_scan(int ncell, int ncol, int* cell_co
                                                const int* lstate) {
     if (!lstate[LS_NEED]) return;
     __shared__ int sh[32];
+    __shared__ int stage[SCAN_LDS<1024>];
     scan_cells<1024>(ncell, ncol, cell_count, cell_start, col_cell0, col_cl0, cl_col, ncl_total,
-                     maxcl, st, sh);
+                     maxcl, st, sh, stage);
 }
 
 __global__ void k_scatter(int natom, const int* __restrict__ atom_cell,
@@ -445,7 +464,7 @@ __device__ inline void bbox_slot(int t, int c_end, int col, int cnt, int cl0,
 // are contiguous) take their slots (place_atom), then, after a workgroup
 // barrier, its clusters take their boxes (bbox_slot).  A column's slots and
 // clusters are its own, so no other workgroup's writes are read.
-__global__ void __launch_bounds__(256) k_place_bbox(
+__global__ void __launch_bounds__(1024) k_place_bbox(
     int ncol, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int* __restrict__ atom_cell,
     const double* __restrict__ atom_wz, const int* __restrict__ cell_start,
@@ -718,7 +737,8 @@ void launch_cluster_build(const LaunchCtx& c, ClusterBufs& b, const double* x, c
                            c.status, b.lstate);
     hipLaunchKernelGGL(k_scatter, dim3(gA), dim3(T), 0, c.stream, c.natom, b.atom_cell,
                        b.atom_slot, b.cell_start, b.cell_members, b.lstate);
-    hipLaunchKernelGGL(k_place_bbox, dim3(b.ncol), dim3(T), 0, c.stream, b.ncol, x, y, z,
+    // 1024 threads: a column of a long box holds thousands of atoms
+    hipLaunchKernelGGL(k_place_bbox, dim3(b.ncol), dim3(1024), 0, c.stream, b.ncol, x, y, z,
                        b.atom_cell, b.atom_wz, b.cell_start, b.cell_members, c.atom_mol,
                        c.atom_site, c.mols, c.reps, b.col_cell0, b.col_cl0, b.col_rep,
                        b.ncl_total, b.slot_orig, b.slot_info, b.slot_x64, b.slot_w64, b.atom_x0,
