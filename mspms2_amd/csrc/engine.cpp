@@ -794,6 +794,12 @@ int build(mspms_ctx* c) {
                 M.species = k;
                 M.rep = r;
                 M.natom = q.natom;
+                M.site0 = spd[k].site0;
+                M.term0 = spd[k].term0;
+                M.nintra = spd[k].nintra;
+                M.bterm0 = spd[k].bterm0;
+                M.nbterm = spd[k].nbterm;
+                M.pad[0] = M.pad[1] = M.pad[2] = 0;
                 for (int a = 0; a < q.natom; a++) {
                     atom_mol.push_back((int)mols.size());
                     atom_site.push_back(spd[k].site0 + a);

@@ -5,7 +5,7 @@
 //   per atom      xx,yy,zz / fxl.. FP64 SoA (caller's arrays), atom_site,
 //                 atom_mol (int), scratch for binning
 //   per site      SiteDev table (a few hundred entries at most)
-//   per molecule  MolDev {atom0, species, rep, first site}
+//   per molecule  MolDev {atom0, species, rep, natom, the species' site and term ranges}
 //   per replica   RepDev: box, cutoffs, electrostatics constants, cell grid
 //   per cluster   8 consecutive slots of the spatially sorted arrays:
 //                 slot_orig (atom id), slot_info (mol, site),
@@ -40,6 +40,11 @@ struct SiteDev {
 
 struct MolDev {
     int atom0, species, rep, natom;
+    // the species' template fields, copied per molecule so that the
+    // per-molecule kernels reach a molecule's terms and sites in one
+    // dependent load less (no species lookup in between)
+    int site0, term0, nintra, bterm0, nbterm;
+    int pad[3];
 };
 
 struct RepDev {

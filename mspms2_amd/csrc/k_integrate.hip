@@ -498,13 +498,12 @@ __global__ void __launch_bounds__(64 * NW) k_respa_inner_tw(
     TermDev T{};
     if (live) {
         const MolDev M = mols[m];
-        const SpeciesDev S = species[M.species];
         na = M.natom;
         a0 = M.atom0;
-        site0 = S.site0;
-        nbt = S.nbterm;
+        site0 = M.site0;
+        nbt = M.nbterm;
         rep = M.rep;
-        if (w < nbt) T = terms[S.bterm0 + w];  // this wave's term, for every substep
+        if (w < nbt) T = terms[M.bterm0 + w];  // this wave's term, for every substep
         rep_steps(rep_delt, M.rep, nstep_inner, deltby2, delts, deltsby2);
     }
     const bool own = live && w < na;      // site w of molecule m

@@ -176,7 +176,6 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
     const MolDev M = mols[m];
-    const SpeciesDev S = species[M.species];
     const RepDev& R = reps[M.rep];
     IntraAcc acc = {0.0, 0.0, 0.0, 0.0};
     bool same = true;  // isSameMole carried into the 1-4 section
@@ -197,10 +196,10 @@ __global__ void __launch_bounds__(256) k_intra(int nmol, const MolDev* __restric
             fz[base + k] += frec[2 * n + base + k];
         }
     }
-    for (int t = S.term0; t < S.term0 + S.nintra; t++) {
+    for (int t = M.term0; t < M.term0 + M.nintra; t++) {
         const TermDev T = terms[t];
         const int mm = T.idx[0], nn = T.idx[1];
-        const SiteDev a = sites[S.site0 + mm], b = sites[S.site0 + nn];
+        const SiteDev a = sites[M.site0 + mm], b = sites[M.site0 + nn];
         const int ii = base + mm, jj = base + nn;
         double rx, ry, rz;
         {
